@@ -1,0 +1,117 @@
+/* CPU oracle of the PlanetSim implicit heat-transport path — C ABI.
+ *
+ * TEST INFRASTRUCTURE, NOT PRODUCT. Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg load liboracle.so, and only as the checker
+ * (or the timed CPU baseline). The product path never calls it.
+ *
+ * The oracle restates the reference algorithm in plain serial C++ with the
+ * reference's block structure and operation order (no contraction, no
+ * fast-math), so that its outputs are bit-identical to the reference's:
+ *   mesh           sim/planet/mesh/icosphere.cpp:200-419
+ *   basis          sim/planet/coordinates/local_tangent_basis.cpp:8-29
+ *   FV operators   sim/planet/operators/finite_volume.cpp:23-111
+ *   graph          sim/planet/surface/heat_transport.cpp:433-452
+ *   multigrid      sim/planet/surface/heat_transport.cpp:54-326,545-577
+ *   BiCGSTAB       sim/planet/surface/heat_transport.cpp:820-897
+ *   diffusion      sim/planet/surface/heat_transport.cpp:28-36,579-593
+ *   RNG            sim/core/random/counter_rng.hpp:19-45
+ * and the synthetic heat-step workload (this project's input generator).
+ *
+ * Every function returns 0 on success, non-zero on error (message from
+ * orc_last_error()), unless it returns a handle (NULL on error).
+ */
+#ifndef PSIM_ORACLE_H
+#define PSIM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* orc_last_error(void);
+
+uint64_t orc_mix_random_key(uint64_t value);
+uint64_t orc_keyed_random_u64(uint64_t seed, uint32_t stream, int64_t tick, uint32_t cell,
+                              uint32_t sample);
+double orc_keyed_random_unit_double(uint64_t seed, uint32_t stream, int64_t tick, uint32_t cell,
+                                    uint32_t sample);
+
+/* Mesh + its transport graph. */
+void* orc_mesh_build(uint32_t level, double radius);
+void orc_mesh_free(void* mesh);
+void orc_mesh_counts(const void* mesh, uint64_t* cells, uint64_t* edges, uint64_t* directed);
+void orc_mesh_cells(const void* mesh, double* center, double* east, double* north, double* area,
+                    uint32_t* edge_offset, uint8_t* edge_count);
+void orc_mesh_cell_edges(const void* mesh, uint32_t* neighbor, uint32_t* edge,
+                         float* normal_east, float* normal_north);
+void orc_mesh_edges(const void* mesh, uint32_t* first, uint32_t* second, double* length,
+                    double* distance);
+void orc_graph(const void* mesh, double* area, uint64_t* offset, uint64_t* neighbour,
+               double* weight, double* weight_sum);
+
+/* FV operators (reference templates for float and double). */
+int orc_gradient_f64(const void* mesh, const double* scalar, double* east, double* north,
+                     uint64_t workers);
+int orc_gradient_f32(const void* mesh, const float* scalar, float* east, float* north,
+                     uint64_t workers);
+int orc_divergence_f64(const void* mesh, const double* flux, double* out, uint64_t workers);
+int orc_divergence_f32(const void* mesh, const float* flux, float* out, uint64_t workers);
+int orc_laplacian_f64(const void* mesh, const double* scalar, double* out, uint64_t workers);
+int orc_laplacian_f32(const void* mesh, const float* scalar, float* out, uint64_t workers);
+int orc_edge_orientation(const void* mesh, uint32_t cell, uint32_t edge, double* out);
+int orc_diffusion_source(const void* mesh, double conductance, const double* temperature,
+                         double* out, uint64_t workers);
+
+/* Multigrid hierarchy structure on the unit-conductance graph. */
+void* orc_hierarchy_build(const void* mesh);
+void orc_hierarchy_free(void* hierarchy);
+uint64_t orc_hierarchy_depth(const void* hierarchy);
+void orc_hierarchy_level_counts(const void* hierarchy, uint64_t j, uint64_t* size, uint64_t* nnz);
+void orc_hierarchy_level(const void* hierarchy, uint64_t j, uint64_t* offset, uint64_t* column,
+                         uint64_t* aggregate, uint64_t* target, double* diagonal, double* value);
+
+/* GraphMultigrid: set_matrix + one symmetric Gauss–Seidel V-cycle. */
+void* orc_mg_create(const void* mesh);
+void orc_mg_free(void* mg);
+int orc_mg_set_matrix(void* mg, const double* diagonal, uint64_t n, const double* off,
+                      uint64_t nnz);
+int orc_mg_precondition(const void* mg, const double* rhs, double* out, uint64_t n,
+                        uint64_t workers);
+/* Coarsest level of the last set_matrix: size and its dense Cholesky factor
+ * (row-major lower triangle, as the reference stores it). */
+uint64_t orc_mg_coarsest_size(const void* mg);
+void orc_mg_coarsest_factor(const void* mg, double* factor);
+/* Galerkin values of level j after set_matrix. */
+void orc_mg_level_values(const void* mg, uint64_t j, double* diagonal, double* value);
+
+/* Right-preconditioned BiCGSTAB (reference form) on diag + off, from x = 0. */
+int orc_bicgstab(const void* mesh, const void* mg, const double* diagonal, const double* off,
+                 const double* b, double* x, double tolerance, int max_iterations,
+                 uint64_t workers, int* iterations);
+
+/* Synthetic heat-step workload (the project's seeded input generator).
+ *   instance_seed = orc_mix_random_key(base_seed ^ instance)
+ *   conductance_m2_s[E], capacity_area[n] = C_i A_i, land[n] in {0,1},
+ *   temperature[n] = T_old for step 0. k_base: 1e6 for the literal
+ *   configurations; sigma: log-normal spread of K_e. */
+int orc_heat_instance(const void* mesh, uint64_t instance_seed, double k_base, double sigma,
+                      double* conductance_m2_s, double* capacity_area, uint8_t* land,
+                      double* temperature);
+/* S_i for a step: 100 cos(lat) + 10 z. */
+int orc_heat_forcing(const void* mesh, uint64_t instance_seed, uint32_t step, double* forcing);
+/* diag = C A + dt Σ G, off = −dt G, G_k = K_e(k) w_k. */
+int orc_heat_matrix(const void* mesh, const double* conductance_m2_s, const double* capacity_area,
+                    double dt, double* diagonal, double* off);
+/* b = C A T + dt A S. */
+int orc_heat_rhs(const void* mesh, const double* capacity_area, const double* temperature,
+                 const double* forcing, double dt, double* rhs);
+/* True relative residual ||b − A x|| / ||b|| (block-ordered dots). */
+double orc_relative_residual(const void* mesh, const double* diagonal, const double* off,
+                             const double* b, const double* x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

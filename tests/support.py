@@ -1,0 +1,322 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) and of the compiled
+reference shim (oracle/_ref/libref.so) for the tests.
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import os
+from functools import lru_cache
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+RADIUS = 6371000.0
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_u64 = C.c_uint64
+_vp = C.c_void_p
+
+
+def _declare(lib, prefix):
+    """Give both libraries' common entry points their signatures."""
+    p = prefix
+
+    def fn(name, res, *args):
+        f = getattr(lib, p + name)
+        f.restype = res
+        f.argtypes = list(args)
+
+    fn("last_error", C.c_char_p)
+    fn("mix_random_key", _u64, _u64)
+    fn("keyed_random_u64", _u64, _u64, C.c_uint32, C.c_int64, C.c_uint32, C.c_uint32)
+    fn("keyed_random_unit_double", C.c_double, _u64, C.c_uint32, C.c_int64, C.c_uint32, C.c_uint32)
+    fn("mesh_build", _vp, C.c_uint32, C.c_double)
+    fn("mesh_free", None, _vp)
+    fn("mesh_counts", None, _vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64))
+    fn("mesh_cells", None, _vp, _dp, _dp, _dp, _dp, _u32p, _u8p)
+    fn("mesh_cell_edges", None, _vp, _u32p, _u32p, _fp, _fp)
+    fn("mesh_edges", None, _vp, _u32p, _u32p, _dp, _dp)
+    fn("graph", None, _vp, _dp, _u64p, _u64p, _dp, _dp)
+    fn("gradient_f64", C.c_int, _vp, _dp, _dp, _dp, _u64)
+    fn("gradient_f32", C.c_int, _vp, _fp, _fp, _fp, _u64)
+    fn("divergence_f64", C.c_int, _vp, _dp, _dp, _u64)
+    fn("divergence_f32", C.c_int, _vp, _fp, _fp, _u64)
+    fn("laplacian_f64", C.c_int, _vp, _dp, _dp, _u64)
+    fn("laplacian_f32", C.c_int, _vp, _fp, _fp, _u64)
+    fn("edge_orientation", C.c_int, _vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_double))
+    fn("diffusion_source", C.c_int, _vp, C.c_double, _dp, _dp, _u64)
+    fn("hierarchy_build", _vp, _vp)
+    fn("hierarchy_free", None, _vp)
+    fn("hierarchy_depth", _u64, _vp)
+    fn("hierarchy_level_counts", None, _vp, _u64, C.POINTER(_u64), C.POINTER(_u64))
+    fn("hierarchy_level", None, _vp, _u64, _u64p, _u64p, _u64p, _u64p, _dp, _dp)
+    fn("mg_create", _vp, _vp)
+    fn("mg_free", None, _vp)
+    fn("mg_set_matrix", C.c_int, _vp, _dp, _u64, _dp, _u64)
+    fn("mg_precondition", C.c_int, _vp, _dp, _dp, _u64, _u64)
+    fn("bicgstab", C.c_int, _vp, _vp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _u64,
+       C.POINTER(C.c_int))
+
+
+class Lib:
+    """One of the two CPU libraries (oracle or reference shim)."""
+
+    def __init__(self, path, prefix):
+        self.lib = C.CDLL(path)
+        self.p = prefix
+        _declare(self.lib, prefix)
+        if prefix == "orc_":
+            f = self.lib
+            f.orc_heat_instance.restype = C.c_int
+            f.orc_heat_instance.argtypes = [_vp, _u64, C.c_double, C.c_double, _dp, _dp, _u8p, _dp]
+            f.orc_heat_forcing.restype = C.c_int
+            f.orc_heat_forcing.argtypes = [_vp, _u64, C.c_uint32, _dp]
+            f.orc_heat_matrix.restype = C.c_int
+            f.orc_heat_matrix.argtypes = [_vp, _dp, _dp, C.c_double, _dp, _dp]
+            f.orc_heat_rhs.restype = C.c_int
+            f.orc_heat_rhs.argtypes = [_vp, _dp, _dp, _dp, C.c_double, _dp]
+            f.orc_relative_residual.restype = C.c_double
+            f.orc_relative_residual.argtypes = [_vp, _dp, _dp, _dp, _dp]
+            f.orc_mg_coarsest_size.restype = _u64
+            f.orc_mg_coarsest_size.argtypes = [_vp]
+            f.orc_mg_coarsest_factor.restype = None
+            f.orc_mg_coarsest_factor.argtypes = [_vp, _dp]
+            f.orc_mg_level_values.restype = None
+            f.orc_mg_level_values.argtypes = [_vp, _u64, _dp, _dp]
+        else:
+            f = self.lib
+            f.ref_coupled_linear_solve.restype = C.c_int
+            f.ref_coupled_linear_solve.argtypes = [_vp, _dp, _dp, _dp, C.c_double, C.c_double, _dp,
+                                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]
+
+    def __getattr__(self, name):
+        return getattr(self.lib, self.p + name)
+
+    def check(self, status):
+        if status != 0:
+            raise RuntimeError(self.last_error().decode())
+
+
+@lru_cache(maxsize=None)
+def oracle() -> Lib:
+    if not os.path.exists(ORACLE_SO):
+        raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+    return Lib(ORACLE_SO, "orc_")
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_SO)
+
+
+@lru_cache(maxsize=None)
+def ref() -> Lib:
+    return Lib(REF_SO, "ref_")
+
+
+class MeshArrays:
+    """A mesh built by one library, with its arrays copied out."""
+
+    def __init__(self, lib: Lib, level: int, radius: float = RADIUS):
+        self.lib = lib
+        self.level = level
+        self.handle = lib.mesh_build(level, radius)
+        if not self.handle:
+            raise RuntimeError(lib.last_error().decode())
+        n, e, d = _u64(), _u64(), _u64()
+        lib.mesh_counts(self.handle, C.byref(n), C.byref(e), C.byref(d))
+        self.n, self.n_edges, self.n_directed = n.value, e.value, d.value
+        self.center = np.zeros(3 * self.n)
+        self.east = np.zeros(3 * self.n)
+        self.north = np.zeros(3 * self.n)
+        self.area = np.zeros(self.n)
+        self.edge_offset = np.zeros(self.n, np.uint32)
+        self.edge_count = np.zeros(self.n, np.uint8)
+        lib.mesh_cells(self.handle, self.center, self.east, self.north, self.area,
+                       self.edge_offset, self.edge_count)
+        self.neighbor = np.zeros(self.n_directed, np.uint32)
+        self.edge = np.zeros(self.n_directed, np.uint32)
+        self.normal_east = np.zeros(self.n_directed, np.float32)
+        self.normal_north = np.zeros(self.n_directed, np.float32)
+        lib.mesh_cell_edges(self.handle, self.neighbor, self.edge, self.normal_east,
+                            self.normal_north)
+        self.first = np.zeros(self.n_edges, np.uint32)
+        self.second = np.zeros(self.n_edges, np.uint32)
+        self.length = np.zeros(self.n_edges)
+        self.distance = np.zeros(self.n_edges)
+        lib.mesh_edges(self.handle, self.first, self.second, self.length, self.distance)
+        self.g_area = np.zeros(self.n)
+        self.offset = np.zeros(self.n + 1, np.uint64)
+        self.g_neighbour = np.zeros(self.n_directed, np.uint64)
+        self.weight = np.zeros(self.n_directed)
+        self.weight_sum = np.zeros(self.n)
+        lib.graph(self.handle, self.g_area, self.offset, self.g_neighbour, self.weight,
+                  self.weight_sum)
+
+    def __del__(self):
+        try:
+            self.lib.mesh_free(self.handle)
+        except Exception:
+            pass
+
+    def arrays(self):
+        return {
+            "center": self.center, "east": self.east, "north": self.north, "area": self.area,
+            "edge_offset": self.edge_offset, "edge_count": self.edge_count,
+            "neighbor": self.neighbor, "edge": self.edge, "normal_east": self.normal_east,
+            "normal_north": self.normal_north, "first": self.first, "second": self.second,
+            "length": self.length, "distance": self.distance, "offset": self.offset,
+            "weight": self.weight, "weight_sum": self.weight_sum,
+        }
+
+
+_mesh_cache: dict = {}
+
+
+def mesh(lib_name: str, level: int) -> MeshArrays:
+    key = (lib_name, level)
+    if key not in _mesh_cache:
+        lib = oracle() if lib_name == "oracle" else ref()
+        _mesh_cache[key] = MeshArrays(lib, level)
+    return _mesh_cache[key]
+
+
+def sha(array: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(array).tobytes()).hexdigest()
+
+
+def crc32c(data: bytes) -> int:
+    """CRC-32C (Castagnoli, reflected 0x82F63B78), table-driven."""
+    table = _crc_table()
+    crc = 0xFFFFFFFF
+    arr = np.frombuffer(data, np.uint8)
+    for b in arr.tolist():
+        crc = table[(crc ^ b) & 0xFF] ^ (crc >> 8)
+    return crc ^ 0xFFFFFFFF
+
+
+@lru_cache(maxsize=1)
+def _crc_table():
+    table = []
+    for i in range(256):
+        r = i
+        for _ in range(8):
+            r = (r >> 1) ^ (0x82F63B78 if r & 1 else 0)
+        table.append(r)
+    return table
+
+
+def hierarchy(lib: Lib, mesh_arrays: MeshArrays):
+    """Levels of the unit-conductance hierarchy as dicts of arrays."""
+    h = lib.hierarchy_build(mesh_arrays.handle)
+    try:
+        depth = lib.hierarchy_depth(h)
+        levels = []
+        for j in range(depth):
+            size, nnz = _u64(), _u64()
+            lib.hierarchy_level_counts(h, j, C.byref(size), C.byref(nnz))
+            s, z = size.value, nnz.value
+            last = j + 1 == depth
+            lv = {
+                "size": s, "nnz": z,
+                "offset": np.zeros(s + 1, np.uint64), "column": np.zeros(max(z, 1), np.uint64),
+                "aggregate": np.zeros(max(0 if last else s, 1), np.uint64),
+                "target": np.zeros(max(0 if last else z, 1), np.uint64),
+                "diagonal": np.zeros(s), "value": np.zeros(max(z, 1)),
+            }
+            lib.hierarchy_level(h, j, lv["offset"], lv["column"], lv["aggregate"], lv["target"],
+                                lv["diagonal"], lv["value"])
+            lv["column"] = lv["column"][:z]
+            lv["value"] = lv["value"][:z]
+            lv["aggregate"] = lv["aggregate"][: (0 if last else s)]
+            lv["target"] = lv["target"][: (0 if last else z)]
+            levels.append(lv)
+        return levels
+    finally:
+        lib.hierarchy_free(h)
+
+
+class Multigrid:
+    def __init__(self, lib: Lib, mesh_arrays: MeshArrays):
+        self.lib = lib
+        self.mesh = mesh_arrays
+        self.handle = lib.mg_create(mesh_arrays.handle)
+
+    def set_matrix(self, diag, off):
+        self.lib.check(self.lib.mg_set_matrix(self.handle, diag, len(diag), off, len(off)))
+
+    def precondition(self, rhs, workers=1):
+        out = np.zeros(self.mesh.n)
+        self.lib.check(self.lib.mg_precondition(self.handle, rhs, out, self.mesh.n, workers))
+        return out
+
+    def bicgstab(self, diag, off, b, tol=1e-10, max_it=1000, workers=1):
+        x = np.zeros(self.mesh.n)
+        it = C.c_int()
+        self.lib.check(self.lib.bicgstab(self.mesh.handle, self.handle, diag, off, b, x, tol,
+                                         max_it, workers, C.byref(it)))
+        return x, it.value
+
+    def __del__(self):
+        try:
+            self.lib.mg_free(self.handle)
+        except Exception:
+            pass
+
+
+BASE_SEED = 20261203
+DT = 2.63e6
+
+
+def instance_seed(index: int, base: int = BASE_SEED) -> int:
+    return oracle().mix_random_key(base ^ index)
+
+
+def heat_instance(m: MeshArrays, seed: int, k_base=1e6, sigma=0.3):
+    """Oracle workload for one instance: conductance, C·A, land, T_old."""
+    o = oracle()
+    cond = np.zeros(m.n_edges)
+    ca = np.zeros(m.n)
+    land = np.zeros(m.n, np.uint8)
+    temp = np.zeros(m.n)
+    o.check(o.lib.orc_heat_instance(m.handle, seed, k_base, sigma, cond, ca, land, temp))
+    return cond, ca, land, temp
+
+
+def heat_forcing(m: MeshArrays, seed: int, step: int):
+    o = oracle()
+    f = np.zeros(m.n)
+    o.check(o.lib.orc_heat_forcing(m.handle, seed, step, f))
+    return f
+
+
+def heat_matrix(m: MeshArrays, cond, ca, dt=DT):
+    o = oracle()
+    diag = np.zeros(m.n)
+    off = np.zeros(m.n_directed)
+    o.check(o.lib.orc_heat_matrix(m.handle, cond, ca, dt, diag, off))
+    return diag, off
+
+
+def heat_rhs(m: MeshArrays, ca, temp, forcing, dt=DT):
+    o = oracle()
+    b = np.zeros(m.n)
+    o.check(o.lib.orc_heat_rhs(m.handle, ca, temp, forcing, dt, b))
+    return b
+
+
+def dense_matrix(m: MeshArrays, diag, off):
+    A = np.diag(diag.copy())
+    for c in range(m.n):
+        for k in range(int(m.offset[c]), int(m.offset[c + 1])):
+            A[c, int(m.neighbor[k])] += off[k]
+    return A
