@@ -1,0 +1,203 @@
+/* libpsim — MI355X (gfx950) implementation of PlanetSim's implicit
+ * diffusive heat-transport path, C ABI.
+ *
+ * This is the boundary the C++20 drop-in headers under include/planetsim/
+ * call, and the one a ctypes/cffi/FFI binding would bind. Plain pointers
+ * and sizes only. Every function returns PSIM_OK (0) or an error code, and
+ * leaves a message for psim_last_error() (thread-local) on failure, unless
+ * it is documented otherwise.
+ *
+ * Memory: functions that take a `where` argument read and write host
+ * memory (PSIM_HOST: synchronous, the call returns when outputs are
+ * written) or device memory (PSIM_DEVICE: pointers from psim_device_alloc,
+ * work is queued on the context's stream; psim_ctx_synchronize waits).
+ * Device state (meshes, hierarchies, matrices, vectors) is owned by the
+ * library and lives in HBM until its handle is destroyed.
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * file:line of fgianni/planet_game.
+ */
+#ifndef PSIM_H
+#define PSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum psim_status {
+    PSIM_OK = 0,
+    PSIM_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+    PSIM_LOGIC_ERROR = 2,      /* std::logic_error (internal invariant) */
+    PSIM_DEVICE_ERROR = 3,     /* HIP runtime error or no device */
+    PSIM_OUT_OF_MEMORY = 4,    /* device allocation / admission refused */
+    PSIM_NOT_CONVERGED = 5,    /* solver stopped above its tolerance */
+    PSIM_RUNTIME_ERROR = 6     /* std::runtime_error (numerical breakdown) */
+};
+
+enum psim_where { PSIM_HOST = 0, PSIM_DEVICE = 1 };
+enum psim_dtype { PSIM_F64 = 0, PSIM_F32 = 1 };
+/* Solver precision: PSIM_PLAIN_FP64 keeps everything in FP64;
+ * PSIM_MIXED runs the inner BiCGSTAB vectors, SpMV and V-cycle in FP32
+ * inside FP64 residual refinement (FP64 residuals, inner products and
+ * coarsest solve). */
+enum psim_precision { PSIM_PLAIN_FP64 = 0, PSIM_MIXED = 1 };
+
+const char* psim_last_error(void);
+int psim_version(void);
+
+/* ------------------------------------------------------------- host mesh
+ * make_icosphere (sim/planet/mesh/planet_mesh.hpp:15, icosphere.cpp:200-419),
+ * bit-identical; PlanetMesh accessors (planet_mesh.hpp:84-125). Host only. */
+typedef struct psim_mesh psim_mesh;
+int psim_mesh_create(uint32_t subdivision, double radius_m, psim_mesh** out);
+void psim_mesh_destroy(psim_mesh* mesh);
+int psim_mesh_counts(const psim_mesh* mesh, uint64_t* cells, uint64_t* edges, uint64_t* directed_edges);
+/* center/east/north: 3 doubles (x, y, z) per cell. */
+int psim_mesh_cells(const psim_mesh* mesh, double* center, double* east, double* north, double* area_m2,
+                    uint32_t* edge_offset, uint8_t* edge_count);
+int psim_mesh_cell_edges(const psim_mesh* mesh, uint32_t* neighbor, uint32_t* edge, float* outward_normal_east,
+                         float* outward_normal_north);
+int psim_mesh_edges(const psim_mesh* mesh, uint32_t* first_cell, uint32_t* second_cell, double* length_m,
+                    double* centroid_distance_m);
+/* mesh_transport_graph (heat_transport.cpp:433-452): area, offset[n+1],
+ * neighbour[nnz], weight = l/d, weight_sum. */
+int psim_mesh_graph(const psim_mesh* mesh, double* area_m2, uint64_t* offset, uint64_t* neighbour, double* weight,
+                    double* weight_sum);
+
+/* --------------------------------------------------- hierarchy (host view)
+ * build_hierarchy (heat_transport.cpp:175-200): the aggregation hierarchy's
+ * structure and unit-conductance Galerkin values, bit-identical. Exposed for
+ * parity checks; the device copy is built by psim_graph_create. */
+typedef struct psim_hierarchy psim_hierarchy;
+int psim_hierarchy_create(uint64_t n, const uint64_t* offset, const uint64_t* neighbour, const double* weight,
+                          const double* weight_sum, psim_hierarchy** out);
+void psim_hierarchy_destroy(psim_hierarchy* h);
+uint64_t psim_hierarchy_depth(const psim_hierarchy* h);
+int psim_hierarchy_level_counts(const psim_hierarchy* h, uint64_t level, uint64_t* size, uint64_t* nnz);
+/* aggregate[size] and target[nnz] (all-ones = coarse diagonal) are left
+ * untouched on the coarsest level. */
+int psim_hierarchy_level(const psim_hierarchy* h, uint64_t level, uint64_t* offset, uint64_t* column,
+                         uint64_t* aggregate, uint64_t* target, double* diagonal, double* value);
+
+/* ------------------------------------------------------------- workload
+ * The seeded heat-step inputs of the benchmark configurations (no
+ * reference counterpart; draws use counter_rng.hpp:19-45). */
+uint64_t psim_instance_seed(uint64_t base_seed, uint64_t index);
+int psim_heat_instance(const psim_mesh* mesh, uint64_t seed, double k_base, double sigma,
+                       double* conductance_per_edge, double* capacity_area, uint8_t* land, double* temperature);
+int psim_heat_forcing(const psim_mesh* mesh, uint64_t seed, uint32_t step, double* forcing);
+
+/* ------------------------------------------------------------- device */
+typedef struct psim_ctx psim_ctx;
+/* One context per GPU: a HIP stream (default priority) and a workspace.
+ * Fails with PSIM_DEVICE_ERROR where no GPU is visible. */
+int psim_ctx_create(int device, psim_ctx** out);
+void psim_ctx_destroy(psim_ctx* ctx);
+int psim_ctx_synchronize(psim_ctx* ctx);
+int psim_device_alloc(psim_ctx* ctx, uint64_t bytes, void** out);
+int psim_device_free(psim_ctx* ctx, void* ptr);
+int psim_copy_to_device(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int psim_copy_to_host(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ------------------------------------------------- FV operators on device
+ * finite_volume.hpp:21-44 / finite_volume.cpp:23-123. The mesh is uploaded
+ * once; each call sweeps `fields` fields stored one after another (cell
+ * fields: fields x cells, edge fields: fields x edges). Results are
+ * bit-identical to the reference for both dtypes. */
+typedef struct psim_fv psim_fv;
+int psim_fv_create(psim_ctx* ctx, const psim_mesh* mesh, psim_fv** out);
+/* The same from a PlanetMesh's arrays (the drop-in wrappers' route). */
+int psim_fv_create_from_arrays(psim_ctx* ctx, uint64_t cells, uint64_t edges, const double* area_m2,
+                               const uint32_t* edge_offset, const uint8_t* edge_count, const uint32_t* neighbor,
+                               const uint32_t* edge, const float* outward_normal_east,
+                               const float* outward_normal_north, const uint32_t* first_cell,
+                               const uint32_t* second_cell, const double* length_m,
+                               const double* centroid_distance_m, psim_fv** out);
+void psim_fv_destroy(psim_fv* fv);
+/* edge_orientation (finite_volume.cpp:23-32): +1, -1, or
+ * PSIM_INVALID_ARGUMENT "cell is not incident to edge". */
+int psim_edge_orientation(const psim_fv* fv, uint32_t cell, uint32_t edge, double* out);
+int psim_gradient(psim_fv* fv, int dtype, uint64_t fields, const void* scalar, void* east, void* north, int where);
+int psim_divergence(psim_fv* fv, int dtype, uint64_t fields, const void* edge_flux, void* result, int where);
+int psim_laplacian(psim_fv* fv, int dtype, uint64_t fields, const void* scalar, void* result, int where);
+
+/* ------------------------------------------- transport graph on device
+ * TransportGraph (heat_transport.hpp:27-36) plus the multigrid hierarchy
+ * structure built from it (GraphMultigrid's constructor,
+ * heat_transport.cpp:550-553). Shared by every system on the graph. */
+typedef struct psim_graph psim_graph;
+int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area_m2, const uint64_t* offset,
+                      const uint64_t* neighbour, const double* weight, const double* weight_sum, psim_graph** out);
+void psim_graph_destroy(psim_graph* graph);
+uint64_t psim_graph_size(const psim_graph* graph);
+uint64_t psim_graph_entries(const psim_graph* graph);
+uint64_t psim_graph_depth(const psim_graph* graph);
+/* diffusion_source (heat_transport.cpp:579-593), bit-identical. */
+int psim_diffusion_source(psim_graph* graph, double conductance_W_K, const double* temperature_K,
+                          double* source_W_m2, int where);
+
+/* ---------------------------------------------- linear system on a graph
+ * GraphMultigrid::set_matrix / precondition (heat_transport.hpp:60-76) and
+ * the right-preconditioned BiCGSTAB of heat_transport.cpp:820-897, run on
+ * the device. The V-cycle smooths with Chebyshev-weighted Jacobi (not the
+ * reference's Gauss–Seidel), so preconditioner outputs differ from the
+ * reference's while solutions agree to the solver tolerance. */
+typedef struct psim_system psim_system;
+typedef struct psim_solve_info {
+    int32_t iterations;         /* BiCGSTAB iterations, summed over restarts */
+    int32_t restarts;           /* true-residual restarts taken */
+    int32_t status;             /* PSIM_OK or PSIM_NOT_CONVERGED */
+    int32_t reserved;
+    double relative_residual;   /* true ||b - A x|| / ||b|| in FP64 */
+} psim_solve_info;
+
+int psim_system_create(psim_graph* graph, int precision, psim_system** out);
+void psim_system_destroy(psim_system* system);
+/* Workgroups that run one solve (the group walks the tiles and meets at
+ * device-wide barriers); default min(tiles, co-resident capacity). Results
+ * are bit-identical for any value. No reference counterpart (the
+ * reference's worker_count plays this role, deterministic_executor.hpp:14). */
+int psim_system_set_workgroups(psim_system* system, int32_t workgroups);
+int32_t psim_system_workgroups(const psim_system* system);
+/* FP64 Galerkin values of a multigrid level after set_matrix (the
+ * reference's Level::diagonal / Level::value, heat_transport.cpp:54-67). */
+int psim_system_level_values(psim_system* system, uint64_t level, double* diagonal, double* off_diagonal);
+/* diagonal[n], off_diagonal[nnz] in the graph's CSR order; throws the
+ * reference's "multigrid matrix does not match the graph" on size
+ * mismatch (heat_transport.cpp:562-564). */
+int psim_system_set_matrix(psim_system* system, const double* diagonal, uint64_t n, const double* off_diagonal,
+                           uint64_t nnz, int where);
+/* One V-cycle: out ~= A^-1 rhs. */
+int psim_system_precondition(psim_system* system, const double* rhs, double* out, int where);
+/* out = A x (diag * x, then entries in CSR order: heat_transport.cpp:1204-1214). */
+int psim_system_apply(psim_system* system, const double* x, double* out, int where);
+/* x = A^-1 b from x = 0 to true relative residual <= tolerance. */
+int psim_system_solve(psim_system* system, const double* b, double* x, double tolerance, int32_t max_iterations,
+                      psim_solve_info* info, int where);
+
+/* ------------------------------------------------ implicit heat step
+ * The benchmark's unit of work: (diag(C_i A_i) + dt L_K) T = C_i A_i T_old
+ * + dt A_i S_i, with L_K the two-point Laplacian with entry weights
+ * K_e l_e / d_e. In the reference's terms this is the linear-response form
+ * of ADR-0009's Newton system (heat_transport.cpp:328-336) solved with the
+ * BiCGSTAB + V-cycle of heat_transport.cpp:820-897.
+ *
+ * psim_heat_set_matrix assembles diag = C A + dt sum_k G_k, off_k = -dt G_k,
+ * G_k = K_k w_k on the device and builds the multigrid values;
+ * conductance_per_entry is K_e of each CSR entry's edge (nnz).
+ * psim_heat_step assembles b = C A T_old + dt A S on the device, solves, and
+ * writes T. */
+int psim_heat_set_matrix(psim_system* system, const double* conductance_per_entry, const double* capacity_area,
+                         double dt_s, int where);
+int psim_heat_step(psim_system* system, const double* temperature_old, const double* forcing_W_m2, double dt_s,
+                   double tolerance, int32_t max_iterations, double* temperature_new, psim_solve_info* info,
+                   int where);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSIM_H */

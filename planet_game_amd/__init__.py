@@ -1,0 +1,416 @@
+"""planet_game_amd — MI355X (gfx950) implementation of PlanetSim's implicit
+diffusive heat-transport path.
+
+The product is the C ABI of ``include/psim.h`` implemented by the in-tree
+``libpsim.so`` (host C++20 + hand-written HIP kernels). This module is a thin
+ctypes binding of that ABI for Python callers (tests, bench.py). It does no
+numerical work itself, and it raises at import when the library is missing
+instead of falling back to anything.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "LIB_PATH", "lib", "PsimError", "Mesh", "Context", "FV", "Graph", "System", "SolveInfo",
+    "HOST", "DEVICE", "F64", "F32", "PLAIN_FP64", "MIXED", "instance_seed", "heat_instance",
+    "heat_forcing", "Hierarchy",
+]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpsim.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C planet_game_amd)")
+
+lib = C.CDLL(LIB_PATH)
+
+HOST, DEVICE = 0, 1
+F64, F32 = 0, 1
+PLAIN_FP64, MIXED = 0, 1
+STATUS = {0: "PSIM_OK", 1: "PSIM_INVALID_ARGUMENT", 2: "PSIM_LOGIC_ERROR", 3: "PSIM_DEVICE_ERROR",
+          4: "PSIM_OUT_OF_MEMORY", 5: "PSIM_NOT_CONVERGED", 6: "PSIM_RUNTIME_ERROR"}
+
+_vp = C.c_void_p
+_u64 = C.c_uint64
+_i32 = C.c_int32
+_dbl = C.c_double
+_pd = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_pf = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_pu32 = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_pu64 = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_pu8 = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("restarts", C.c_int32), ("status", C.c_int32),
+                ("reserved", C.c_int32), ("relative_residual", C.c_double)]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+
+
+_sig("psim_last_error", C.c_char_p)
+_sig("psim_version", C.c_int)
+_sig("psim_mesh_create", C.c_int, C.c_uint32, _dbl, C.POINTER(_vp))
+_sig("psim_mesh_destroy", None, _vp)
+_sig("psim_mesh_counts", C.c_int, _vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64))
+_sig("psim_mesh_cells", C.c_int, _vp, _pd, _pd, _pd, _pd, _pu32, _pu8)
+_sig("psim_mesh_cell_edges", C.c_int, _vp, _pu32, _pu32, _pf, _pf)
+_sig("psim_mesh_edges", C.c_int, _vp, _pu32, _pu32, _pd, _pd)
+_sig("psim_mesh_graph", C.c_int, _vp, _pd, _pu64, _pu64, _pd, _pd)
+_sig("psim_hierarchy_create", C.c_int, _u64, _pu64, _pu64, _pd, _pd, C.POINTER(_vp))
+_sig("psim_hierarchy_destroy", None, _vp)
+_sig("psim_hierarchy_depth", _u64, _vp)
+_sig("psim_hierarchy_level_counts", C.c_int, _vp, _u64, C.POINTER(_u64), C.POINTER(_u64))
+_sig("psim_hierarchy_level", C.c_int, _vp, _u64, _pu64, _pu64, _pu64, _pu64, _pd, _pd)
+_sig("psim_instance_seed", _u64, _u64, _u64)
+_sig("psim_heat_instance", C.c_int, _vp, _u64, _dbl, _dbl, _pd, _pd, _pu8, _pd)
+_sig("psim_heat_forcing", C.c_int, _vp, _u64, C.c_uint32, _pd)
+_sig("psim_ctx_create", C.c_int, C.c_int, C.POINTER(_vp))
+_sig("psim_ctx_destroy", None, _vp)
+_sig("psim_ctx_synchronize", C.c_int, _vp)
+_sig("psim_device_alloc", C.c_int, _vp, _u64, C.POINTER(_vp))
+_sig("psim_device_free", C.c_int, _vp, _vp)
+_sig("psim_copy_to_device", C.c_int, _vp, _vp, _vp, _u64)
+_sig("psim_copy_to_host", C.c_int, _vp, _vp, _vp, _u64)
+_sig("psim_fv_create", C.c_int, _vp, _vp, C.POINTER(_vp))
+_sig("psim_fv_create_from_arrays", C.c_int, _vp, _u64, _u64, _pd, _pu32, _pu8, _pu32, _pu32, _pf, _pf,
+     _pu32, _pu32, _pd, _pd, C.POINTER(_vp))
+_sig("psim_fv_destroy", None, _vp)
+_sig("psim_edge_orientation", C.c_int, _vp, C.c_uint32, C.c_uint32, C.POINTER(_dbl))
+_sig("psim_gradient", C.c_int, _vp, C.c_int, _u64, _vp, _vp, _vp, C.c_int)
+_sig("psim_divergence", C.c_int, _vp, C.c_int, _u64, _vp, _vp, C.c_int)
+_sig("psim_laplacian", C.c_int, _vp, C.c_int, _u64, _vp, _vp, C.c_int)
+_sig("psim_graph_create", C.c_int, _vp, _u64, _pd, _pu64, _pu64, _pd, _pd, C.POINTER(_vp))
+_sig("psim_graph_destroy", None, _vp)
+_sig("psim_graph_size", _u64, _vp)
+_sig("psim_graph_entries", _u64, _vp)
+_sig("psim_graph_depth", _u64, _vp)
+_sig("psim_diffusion_source", C.c_int, _vp, _dbl, _vp, _vp, C.c_int)
+_sig("psim_system_create", C.c_int, _vp, C.c_int, C.POINTER(_vp))
+_sig("psim_system_destroy", None, _vp)
+_sig("psim_system_set_workgroups", C.c_int, _vp, _i32)
+_sig("psim_system_workgroups", _i32, _vp)
+_sig("psim_system_level_values", C.c_int, _vp, _u64, _vp, _vp)
+_sig("psim_system_set_matrix", C.c_int, _vp, _vp, _u64, _vp, _u64, C.c_int)
+_sig("psim_system_precondition", C.c_int, _vp, _vp, _vp, C.c_int)
+_sig("psim_system_apply", C.c_int, _vp, _vp, _vp, C.c_int)
+_sig("psim_system_solve", C.c_int, _vp, _vp, _vp, _dbl, _i32, C.POINTER(SolveInfo), C.c_int)
+_sig("psim_heat_set_matrix", C.c_int, _vp, _vp, _vp, _dbl, C.c_int)
+_sig("psim_heat_step", C.c_int, _vp, _vp, _vp, _dbl, _dbl, _i32, _vp, C.POINTER(SolveInfo), C.c_int)
+
+
+class PsimError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+def check(status):
+    if status != 0:
+        raise PsimError(status, lib.psim_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp) if isinstance(a, np.ndarray) else a
+
+
+class Mesh:
+    """make_icosphere(subdivision, radius) on the host, with its arrays."""
+
+    def __init__(self, subdivision: int, radius: float = 6371000.0):
+        h = _vp()
+        check(lib.psim_mesh_create(subdivision, radius, C.byref(h)))
+        self.handle = h
+        self.subdivision = subdivision
+        self.radius = radius
+        n, e, d = _u64(), _u64(), _u64()
+        check(lib.psim_mesh_counts(h, C.byref(n), C.byref(e), C.byref(d)))
+        self.cells, self.edges, self.directed = n.value, e.value, d.value
+        c = self.cells
+        self.center, self.east, self.north = (np.zeros(3 * c) for _ in range(3))
+        self.area = np.zeros(c)
+        self.edge_offset = np.zeros(c, np.uint32)
+        self.edge_count = np.zeros(c, np.uint8)
+        check(lib.psim_mesh_cells(h, self.center, self.east, self.north, self.area,
+                                  self.edge_offset, self.edge_count))
+        self.neighbor = np.zeros(self.directed, np.uint32)
+        self.edge = np.zeros(self.directed, np.uint32)
+        self.normal_east = np.zeros(self.directed, np.float32)
+        self.normal_north = np.zeros(self.directed, np.float32)
+        check(lib.psim_mesh_cell_edges(h, self.neighbor, self.edge, self.normal_east,
+                                       self.normal_north))
+        self.first = np.zeros(self.edges, np.uint32)
+        self.second = np.zeros(self.edges, np.uint32)
+        self.length = np.zeros(self.edges)
+        self.distance = np.zeros(self.edges)
+        check(lib.psim_mesh_edges(h, self.first, self.second, self.length, self.distance))
+        self.offset = np.zeros(c + 1, np.uint64)
+        self.graph_neighbour = np.zeros(self.directed, np.uint64)
+        self.weight = np.zeros(self.directed)
+        self.weight_sum = np.zeros(c)
+        g_area = np.zeros(c)
+        check(lib.psim_mesh_graph(h, g_area, self.offset, self.graph_neighbour, self.weight,
+                                  self.weight_sum))
+
+    def entry_values(self, per_edge):
+        """Expand a per-edge array to the CSR entries (cell-edge order)."""
+        return np.ascontiguousarray(per_edge[self.edge])
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.psim_mesh_destroy(self.handle)
+            self.handle = None
+
+
+class Hierarchy:
+    """Host view of build_hierarchy for parity checks."""
+
+    def __init__(self, mesh: Mesh):
+        h = _vp()
+        check(lib.psim_hierarchy_create(mesh.cells, mesh.offset, mesh.graph_neighbour, mesh.weight,
+                                        mesh.weight_sum, C.byref(h)))
+        try:
+            depth = lib.psim_hierarchy_depth(h)
+            self.levels = []
+            for j in range(depth):
+                s, z = _u64(), _u64()
+                check(lib.psim_hierarchy_level_counts(h, j, C.byref(s), C.byref(z)))
+                s, z = s.value, z.value
+                last = j + 1 == depth
+                lv = {"size": s, "nnz": z, "offset": np.zeros(s + 1, np.uint64),
+                      "column": np.zeros(max(z, 1), np.uint64),
+                      "aggregate": np.zeros(max(0 if last else s, 1), np.uint64),
+                      "target": np.zeros(max(0 if last else z, 1), np.uint64),
+                      "diagonal": np.zeros(s), "value": np.zeros(max(z, 1))}
+                check(lib.psim_hierarchy_level(h, j, lv["offset"], lv["column"], lv["aggregate"],
+                                               lv["target"], lv["diagonal"], lv["value"]))
+                lv["column"] = lv["column"][:z]
+                lv["value"] = lv["value"][:z]
+                lv["aggregate"] = lv["aggregate"][: (0 if last else s)]
+                lv["target"] = lv["target"][: (0 if last else z)]
+                self.levels.append(lv)
+        finally:
+            lib.psim_hierarchy_destroy(h)
+
+
+def instance_seed(index: int, base: int = 20261203) -> int:
+    return lib.psim_instance_seed(base, index)
+
+
+def heat_instance(mesh: Mesh, seed: int, k_base: float = 1.0e6, sigma: float = 0.3):
+    """(conductance per edge, C*A, land mask, T_old) of one seeded instance."""
+    cond = np.zeros(mesh.edges)
+    ca = np.zeros(mesh.cells)
+    land = np.zeros(mesh.cells, np.uint8)
+    temp = np.zeros(mesh.cells)
+    check(lib.psim_heat_instance(mesh.handle, seed, k_base, sigma, cond, ca, land, temp))
+    return cond, ca, land, temp
+
+
+def heat_forcing(mesh: Mesh, seed: int, step: int):
+    f = np.zeros(mesh.cells)
+    check(lib.psim_heat_forcing(mesh.handle, seed, step, f))
+    return f
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = _vp()
+        check(lib.psim_ctx_create(device, C.byref(h)))
+        self.handle = h
+
+    def synchronize(self):
+        check(lib.psim_ctx_synchronize(self.handle))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.psim_ctx_destroy(self.handle)
+            self.handle = None
+
+
+class DeviceArray:
+    """A device buffer owned by the caller (resident inputs/outputs)."""
+
+    def __init__(self, ctx: Context, count: int, dtype=np.float64):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.count = count
+        self.nbytes = count * self.dtype.itemsize
+        p = _vp()
+        check(lib.psim_device_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def of(cls, ctx, array):
+        a = np.ascontiguousarray(array)
+        d = cls(ctx, a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array, dtype=self.dtype)
+        assert a.size == self.count
+        check(lib.psim_copy_to_device(self.ctx.handle, self.ptr, a.ctypes.data_as(_vp), self.nbytes))
+
+    def download(self):
+        a = np.empty(self.count, self.dtype)
+        check(lib.psim_copy_to_host(self.ctx.handle, a.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return a
+
+    def offset(self, elements: int):
+        return _vp(self.ptr.value + elements * self.dtype.itemsize)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib.psim_device_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+
+class FV:
+    """Device FV operators on one mesh (finite_volume.hpp:21-44)."""
+
+    def __init__(self, ctx: Context, mesh: Mesh):
+        h = _vp()
+        check(lib.psim_fv_create(ctx.handle, mesh.handle, C.byref(h)))
+        self.handle, self.ctx, self.mesh = h, ctx, mesh
+
+    @staticmethod
+    def _dtype(a):
+        return F64 if a.dtype == np.float64 else F32
+
+    def gradient(self, scalar):
+        s = np.ascontiguousarray(scalar)
+        fields = s.size // self.mesh.cells
+        e, n = np.empty_like(s), np.empty_like(s)
+        check(lib.psim_gradient(self.handle, self._dtype(s), fields, _ptr(s), _ptr(e), _ptr(n), HOST))
+        return e, n
+
+    def divergence(self, flux):
+        f = np.ascontiguousarray(flux)
+        fields = f.size // self.mesh.edges
+        out = np.empty(fields * self.mesh.cells, f.dtype)
+        check(lib.psim_divergence(self.handle, self._dtype(f), fields, _ptr(f), _ptr(out), HOST))
+        return out
+
+    def laplacian(self, scalar):
+        s = np.ascontiguousarray(scalar)
+        fields = s.size // self.mesh.cells
+        out = np.empty_like(s)
+        check(lib.psim_laplacian(self.handle, self._dtype(s), fields, _ptr(s), _ptr(out), HOST))
+        return out
+
+    def orientation(self, cell, edge):
+        out = _dbl()
+        check(lib.psim_edge_orientation(self.handle, cell, edge, C.byref(out)))
+        return out.value
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.psim_fv_destroy(self.handle)
+            self.handle = None
+
+
+class Graph:
+    """Device transport graph + multigrid hierarchy structure."""
+
+    def __init__(self, ctx: Context, mesh: Mesh):
+        h = _vp()
+        check(lib.psim_graph_create(ctx.handle, mesh.cells, mesh.area, mesh.offset,
+                                    mesh.graph_neighbour, mesh.weight, mesh.weight_sum, C.byref(h)))
+        self.handle, self.ctx, self.mesh = h, ctx, mesh
+        self.size = lib.psim_graph_size(h)
+        self.entries = lib.psim_graph_entries(h)
+        self.depth = lib.psim_graph_depth(h)
+
+    def diffusion_source(self, conductance, temperature):
+        t = np.ascontiguousarray(temperature, np.float64)
+        out = np.empty_like(t)
+        check(lib.psim_diffusion_source(self.handle, conductance, _ptr(t), _ptr(out), HOST))
+        return out
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.psim_graph_destroy(self.handle)
+            self.handle = None
+
+
+class System:
+    """A linear system on a graph: GraphMultigrid + BiCGSTAB on the device."""
+
+    def __init__(self, graph: Graph, precision: int = PLAIN_FP64):
+        h = _vp()
+        check(lib.psim_system_create(graph.handle, precision, C.byref(h)))
+        self.handle, self.graph, self.precision = h, graph, precision
+
+    @property
+    def workgroups(self):
+        return lib.psim_system_workgroups(self.handle)
+
+    @workgroups.setter
+    def workgroups(self, value):
+        check(lib.psim_system_set_workgroups(self.handle, value))
+
+    def set_matrix(self, diag, off):
+        d = np.ascontiguousarray(diag, np.float64)
+        o = np.ascontiguousarray(off, np.float64)
+        check(lib.psim_system_set_matrix(self.handle, _ptr(d), d.size, _ptr(o), o.size, HOST))
+
+    def level_values(self, level):
+        n = self.graph.mesh.cells if level == 0 else None
+        # sizes come from the host hierarchy view
+        return self._level_values(level)
+
+    def _level_values(self, level):
+        if not hasattr(self, "_hier"):
+            self._hier = Hierarchy(self.graph.mesh)
+        lv = self._hier.levels[level]
+        d = np.zeros(lv["size"])
+        o = np.zeros(max(lv["nnz"], 1))
+        check(lib.psim_system_level_values(self.handle, level, _ptr(d), _ptr(o)))
+        return d, o[: lv["nnz"]]
+
+    def precondition(self, rhs):
+        r = np.ascontiguousarray(rhs, np.float64)
+        out = np.empty_like(r)
+        check(lib.psim_system_precondition(self.handle, _ptr(r), _ptr(out), HOST))
+        return out
+
+    def apply(self, x):
+        v = np.ascontiguousarray(x, np.float64)
+        out = np.empty_like(v)
+        check(lib.psim_system_apply(self.handle, _ptr(v), _ptr(out), HOST))
+        return out
+
+    def solve(self, b, tolerance=1e-10, max_iterations=1000):
+        bb = np.ascontiguousarray(b, np.float64)
+        x = np.empty_like(bb)
+        info = SolveInfo()
+        check(lib.psim_system_solve(self.handle, _ptr(bb), _ptr(x), tolerance, max_iterations,
+                                    C.byref(info), HOST))
+        return x, info
+
+    def heat_set_matrix(self, conductance_per_entry, capacity_area, dt=2.63e6):
+        k = np.ascontiguousarray(conductance_per_entry, np.float64)
+        ca = np.ascontiguousarray(capacity_area, np.float64)
+        check(lib.psim_heat_set_matrix(self.handle, _ptr(k), _ptr(ca), dt, HOST))
+
+    def heat_step(self, t_old, forcing, dt=2.63e6, tolerance=1e-10, max_iterations=1000):
+        t = np.ascontiguousarray(t_old, np.float64)
+        f = np.ascontiguousarray(forcing, np.float64)
+        out = np.empty_like(t)
+        info = SolveInfo()
+        check(lib.psim_heat_step(self.handle, _ptr(t), _ptr(f), dt, tolerance, max_iterations,
+                                 _ptr(out), C.byref(info), HOST))
+        return out, info
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.psim_system_destroy(self.handle)
+            self.handle = None

@@ -1,0 +1,588 @@
+// C ABI: device context, transport graphs, linear systems and the heat step.
+#include "device.hpp"
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+using psim::guarded;
+using psim::dev::Instance;
+using psim::dev::kMaxLevels;
+using psim::dev::kPartials;
+using psim::dev::kTile;
+using psim::dev::Launch;
+using psim::dev::LevelView;
+using psim::dev::SolveResult;
+
+namespace psim {
+
+void hip_check(hipError_t status, const char* what) {
+    if (status == hipSuccess) return;
+    const std::string text = std::string(what) + ": " + hipGetErrorString(status);
+    if (status == hipErrorOutOfMemory) throw DeviceOutOfMemory(text);
+    throw DeviceFailure(text);
+}
+
+namespace {
+
+// One device allocation carved into 256-byte aligned arrays. Sizes are
+// recorded first (plan), then the block is allocated and the pointers set.
+class Arena {
+  public:
+    template <class T> void plan(T** slot, size_t count) {
+        slots_.push_back({reinterpret_cast<void**>(slot), bytes_});
+        bytes_ += (std::max<size_t>(count, 1) * sizeof(T) + 255) / 256 * 256;
+    }
+    void allocate() {
+        hip_check(hipMalloc(&block_, bytes_), "device allocation");
+        for (auto& s : slots_) *s.slot = static_cast<char*>(block_) + s.offset;
+    }
+    void release() {
+        if (block_ != nullptr) (void)hipFree(block_);
+        block_ = nullptr;
+    }
+    size_t bytes() const { return bytes_; }
+    void* block() const { return block_; }
+
+  private:
+    struct Slot {
+        void** slot;
+        size_t offset;
+    };
+    std::vector<Slot> slots_;
+    size_t bytes_ = 0;
+    void* block_ = nullptr;
+};
+
+void put(void* dst, const void* src, size_t bytes) {
+    if (bytes != 0) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "upload");
+}
+
+void check_where(int where) {
+    if (where != PSIM_HOST && where != PSIM_DEVICE) throw std::invalid_argument("unknown memory location");
+}
+
+}  // namespace
+}  // namespace psim
+
+struct psim_graph {
+    psim_ctx* ctx = nullptr;
+    psim::HostHierarchy host;
+    int32_t n = 0, nnz = 0, depth = 0;
+    psim::Arena arena;
+    double* area = nullptr;
+    double* weight = nullptr;
+    LevelView views[kMaxLevels];
+    ~psim_graph() { arena.release(); }
+};
+
+struct psim_system {
+    psim_graph* graph = nullptr;
+    int precision = PSIM_PLAIN_FP64;
+    int32_t workgroups = 1;
+    int32_t map_capacity = 1;
+    psim::Arena arena;
+    Instance host;                 // the uploaded descriptor
+    Instance* d_inst = nullptr;
+    int32_t* d_map = nullptr;      // workgroup -> instance (all 0)
+    SolveResult* d_result = nullptr;
+    double* entry_conductance = nullptr;
+    double* capacity_area = nullptr;
+    double* stage_a = nullptr;     // host-call staging (n)
+    double* stage_b = nullptr;
+    double* stage_c = nullptr;
+    bool matrix_set = false;
+    bool heat_matrix = false;
+    ~psim_system() { arena.release(); }
+};
+
+namespace {
+
+void launch(psim_system* s, const Launch& p) {
+    psim_ctx* ctx = s->graph->ctx;
+    const bool cooperative = s->workgroups > 1;
+    if (cooperative && !ctx->cooperative) throw psim::DeviceFailure("cooperative launch unsupported on this device");
+    psim::hip_check(psim::dev::launch_program(s->d_inst, s->d_map, nullptr, p, s->workgroups, ctx->stream, cooperative),
+                    "solver launch");
+}
+
+SolveResult fetch_result(psim_system* s) {
+    SolveResult r{};
+    psim_ctx* ctx = s->graph->ctx;
+    psim::hip_check(hipMemcpyAsync(&r, s->d_result, sizeof r, hipMemcpyDeviceToHost, ctx->stream), "result copy");
+    psim::hip_check(hipStreamSynchronize(ctx->stream), "solve");
+    if (r.barrier_timeouts != 0) {
+        // A barrier gave up waiting: the result is unusable and the words
+        // must be cleared before the next launch.
+        psim::hip_check(hipMemset(s->host.barrier, 0, 2 * sizeof(uint32_t)), "barrier reset");
+        psim::hip_check(hipMemset(s->d_result, 0, sizeof(SolveResult)), "result reset");
+        throw psim::DeviceFailure("solver group barrier timed out");
+    }
+    return r;
+}
+
+void fill_info(const SolveResult& r, psim_solve_info* info) {
+    if (info == nullptr) return;
+    info->iterations = r.iterations;
+    info->restarts = r.restarts;
+    info->status = r.status;
+    info->reserved = 0;
+    info->relative_residual = r.relative_residual;
+}
+
+Launch solve_params(const psim_system* s, int op, double tolerance, int32_t max_iterations) {
+    Launch p{};
+    p.op = op;
+    p.tolerance = tolerance;
+    p.max_iterations = max_iterations;
+    p.max_restarts = s->precision == PSIM_MIXED ? 40 : 8;
+    p.inner_relative = s->precision == PSIM_MIXED ? 1.0e-5 : 0.0;
+    return p;
+}
+
+void require_matrix(const psim_system* s) {
+    if (s == nullptr) throw std::invalid_argument("null system");
+    if (!s->matrix_set) throw std::logic_error("set_matrix has not been called");
+}
+
+}  // namespace
+
+extern "C" {
+
+int psim_ctx_create(int device, psim_ctx** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::invalid_argument("null output handle");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw psim::DeviceFailure("no GPU is visible");
+        if (device < 0 || device >= count) throw std::invalid_argument("device index out of range");
+        auto ctx = std::make_unique<psim_ctx>();
+        ctx->device = device;
+        psim::hip_check(hipSetDevice(device), "hipSetDevice");
+        psim::hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream create");
+        hipDeviceProp_t prop{};
+        psim::hip_check(hipGetDeviceProperties(&prop, device), "device properties");
+        ctx->compute_units = prop.multiProcessorCount;
+        ctx->cooperative = prop.cooperativeLaunch != 0;
+        ctx->blocks_per_cu = psim::dev::program_blocks_per_cu();
+        *out = ctx.release();
+    });
+}
+
+void psim_ctx_destroy(psim_ctx* ctx) {
+    if (ctx == nullptr) return;
+    if (ctx->stream != nullptr) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+int psim_ctx_synchronize(psim_ctx* ctx) {
+    return guarded([&] {
+        if (ctx == nullptr) throw std::invalid_argument("null context");
+        psim::hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+    });
+}
+
+int psim_device_alloc(psim_ctx* ctx, uint64_t bytes, void** out) {
+    return guarded([&] {
+        if (ctx == nullptr || out == nullptr) throw std::invalid_argument("null argument");
+        psim::hip_check(hipMalloc(out, std::max<uint64_t>(bytes, 1)), "device allocation");
+    });
+}
+
+int psim_device_free(psim_ctx* ctx, void* ptr) {
+    return guarded([&] {
+        if (ctx == nullptr) throw std::invalid_argument("null context");
+        psim::hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+        psim::hip_check(hipFree(ptr), "device free");
+    });
+}
+
+int psim_copy_to_device(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded([&] {
+        if (ctx == nullptr) throw std::invalid_argument("null context");
+        psim::hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "copy to device");
+        psim::hip_check(hipStreamSynchronize(ctx->stream), "copy to device");
+    });
+}
+
+int psim_copy_to_host(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded([&] {
+        if (ctx == nullptr) throw std::invalid_argument("null context");
+        psim::hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
+        psim::hip_check(hipStreamSynchronize(ctx->stream), "copy to host");
+    });
+}
+
+int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint64_t* offset, const uint64_t* neighbour,
+                      const double* weight, const double* weight_sum, psim_graph** out) {
+    return guarded([&] {
+        if (ctx == nullptr || out == nullptr || !area || !offset || !neighbour || !weight || !weight_sum)
+            throw std::invalid_argument("null argument");
+        if (n == 0 || n >= (1ULL << 30)) throw std::invalid_argument("graph size out of range");
+        psim::HostGraph g;
+        g.n = static_cast<int32_t>(n);
+        g.row.resize(n + 1);
+        for (uint64_t i = 0; i <= n; ++i) g.row[i] = static_cast<int32_t>(offset[i]);
+        const uint64_t nnz = offset[n];
+        g.col.resize(nnz);
+        for (uint64_t k = 0; k < nnz; ++k) {
+            if (neighbour[k] >= n) throw std::invalid_argument("graph neighbour out of range");
+            g.col[k] = static_cast<int32_t>(neighbour[k]);
+        }
+        g.area.assign(area, area + n);
+        g.weight.assign(weight, weight + nnz);
+        g.weight_sum.assign(weight_sum, weight_sum + n);
+        auto gr = std::make_unique<psim_graph>();
+        gr->ctx = ctx;
+        gr->host = psim::build_hierarchy(g);
+        gr->n = g.n;
+        gr->nnz = static_cast<int32_t>(nnz);
+        gr->depth = static_cast<int32_t>(gr->host.levels.size());
+        if (gr->depth > kMaxLevels) throw std::logic_error("hierarchy deeper than supported");
+        const auto& lv = gr->host.levels;
+        if (lv.back().n > psim::dev::kMaxCoarse) throw std::logic_error("coarsest level larger than supported");
+        psim::hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        // plan
+        auto& A = gr->arena;
+        A.plan(&gr->area, n);
+        A.plan(&gr->weight, nnz);
+        int32_t* ptrs[kMaxLevels][10] = {};
+        for (int j = 0; j < gr->depth; ++j) {
+            const psim::HostLevel& L = lv[j];
+            const bool last = j + 1 == gr->depth;
+            A.plan(&ptrs[j][0], L.row.size());
+            A.plan(&ptrs[j][1], L.col.size());
+            if (!last) {
+                A.plan(&ptrs[j][2], L.aggregate.size());
+                A.plan(&ptrs[j][3], L.agg_start.size());
+                A.plan(&ptrs[j][4], L.agg_rows.size());
+                A.plan(&ptrs[j][5], L.tile_start.size());
+                A.plan(&ptrs[j][6], L.g_diag_start.size());
+                A.plan(&ptrs[j][7], L.g_diag.size());
+                A.plan(&ptrs[j][8], L.g_off_start.size());
+                A.plan(&ptrs[j][9], L.g_off.size());
+            }
+        }
+        A.allocate();
+        psim::put(gr->area, area, n * 8);
+        psim::put(gr->weight, weight, nnz * 8);
+        for (int j = 0; j < gr->depth; ++j) {
+            const psim::HostLevel& L = lv[j];
+            const bool last = j + 1 == gr->depth;
+            LevelView& v = gr->views[j];
+            v.n = L.n;
+            v.nnz = L.nnz;
+            psim::put(ptrs[j][0], L.row.data(), L.row.size() * 4);
+            psim::put(ptrs[j][1], L.col.data(), L.col.size() * 4);
+            v.row = ptrs[j][0];
+            v.col = ptrs[j][1];
+            if (!last) {
+                const std::vector<int32_t>* src[8] = {&L.aggregate, &L.agg_start, &L.agg_rows, &L.tile_start,
+                                                      &L.g_diag_start, &L.g_diag, &L.g_off_start, &L.g_off};
+                for (int q = 0; q < 8; ++q) psim::put(ptrs[j][2 + q], src[q]->data(), src[q]->size() * 4);
+                v.agg = ptrs[j][2];
+                v.agg_start = ptrs[j][3];
+                v.agg_rows = ptrs[j][4];
+                v.tile_start = ptrs[j][5];
+                v.down_tiles = static_cast<int32_t>(L.tile_start.size()) - 1;
+                v.gd_start = ptrs[j][6];
+                v.gd = ptrs[j][7];
+                v.go_start = ptrs[j][8];
+                v.go = ptrs[j][9];
+                v.n_next = lv[j + 1].n;
+            }
+        }
+        *out = gr.release();
+    });
+}
+
+void psim_graph_destroy(psim_graph* graph) { delete graph; }
+uint64_t psim_graph_size(const psim_graph* g) { return g ? static_cast<uint64_t>(g->n) : 0; }
+uint64_t psim_graph_entries(const psim_graph* g) { return g ? static_cast<uint64_t>(g->nnz) : 0; }
+uint64_t psim_graph_depth(const psim_graph* g) { return g ? static_cast<uint64_t>(g->depth) : 0; }
+
+int psim_diffusion_source(psim_graph* g, double conductance, const double* temperature, double* source, int where) {
+    return guarded([&] {
+        if (g == nullptr || temperature == nullptr || source == nullptr) throw std::invalid_argument("null argument");
+        psim::check_where(where);
+        hipStream_t stream = g->ctx->stream;
+        const size_t bytes = static_cast<size_t>(g->n) * 8;
+        double* t = const_cast<double*>(temperature);
+        double* o = source;
+        if (where == PSIM_HOST) {
+            psim::hip_check(hipMalloc(&t, 2 * bytes), "staging allocation");
+            o = t + g->n;
+            psim::hip_check(hipMemcpy(t, temperature, bytes, hipMemcpyHostToDevice), "staging copy");
+        }
+        psim::dev::launch_diffusion(g->n, g->views[0].row, g->views[0].col, g->weight, g->area, conductance, t, o,
+                                    stream);
+        if (where == PSIM_HOST) {
+            const hipError_t e1 = hipMemcpyAsync(source, o, bytes, hipMemcpyDeviceToHost, stream);
+            const hipError_t e2 = hipStreamSynchronize(stream);
+            (void)hipFree(t);
+            psim::hip_check(e1, "diffusion copy");
+            psim::hip_check(e2, "diffusion_source");
+        }
+    });
+}
+
+int psim_system_create(psim_graph* g, int precision, psim_system** out) {
+    return guarded([&] {
+        if (g == nullptr || out == nullptr) throw std::invalid_argument("null argument");
+        if (precision != PSIM_PLAIN_FP64 && precision != PSIM_MIXED) throw std::invalid_argument("unknown precision");
+        psim_ctx* ctx = g->ctx;
+        psim::hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        auto s = std::make_unique<psim_system>();
+        s->graph = g;
+        s->precision = precision;
+        const int32_t n = g->n, tiles = (n + kTile - 1) / kTile;
+        const int32_t cap = std::max(1, ctx->compute_units * ctx->blocks_per_cu);
+        s->workgroups = std::min(tiles, cap);
+        Instance& I = s->host;
+        I.precision = precision == PSIM_MIXED ? 1 : 0;
+        I.depth = g->depth;
+        I.n = n;
+        I.nnz = g->nnz;
+        I.tiles = tiles;
+        I.group_first = 0;
+        I.group_size = s->workgroups;
+        for (int j = 0; j < g->depth; ++j) I.lv[j] = g->views[j];
+        I.area = g->area;
+        I.weight = g->weight;
+        auto& A = s->arena;
+        const bool mixed = precision == PSIM_MIXED;
+        const size_t vt = mixed ? 4 : 8;
+        for (int j = 0; j < g->depth; ++j) {
+            const size_t nj = static_cast<size_t>(g->views[j].n), zj = static_cast<size_t>(g->views[j].nnz);
+            A.plan(&I.diag64[j], nj);
+            A.plan(&I.dinv64[j], nj);
+            A.plan(&I.off64[j], zj);
+            if (mixed) {
+                A.plan(&I.diag32[j], nj);
+                A.plan(&I.dinv32[j], nj);
+                A.plan(&I.off32[j], zj);
+            }
+            if (j > 0) {
+                A.plan(reinterpret_cast<char**>(&I.lvl_b[j]), nj * vt);
+                A.plan(reinterpret_cast<char**>(&I.lvl_e[j]), nj * vt);
+            }
+        }
+        const size_t nc = static_cast<size_t>(g->views[g->depth - 1].n);
+        A.plan(&I.omega, kMaxLevels);
+        A.plan(&I.cinv, nc * nc);
+        A.plan(&I.b, n);
+        A.plan(&I.x, n);
+        void** vecs[8] = {&I.r, &I.r0, &I.p, &I.v, &I.ph, &I.s, &I.sh, &I.t};
+        for (void** v : vecs) A.plan(reinterpret_cast<char**>(v), static_cast<size_t>(n) * vt);
+        A.plan(&I.partial, 2 * static_cast<size_t>(tiles) * kPartials);
+        A.plan(&I.barrier, 2);
+        A.plan(&I.result, 1);
+        A.plan(&s->d_inst, 1);
+        s->map_capacity = cap;
+        A.plan(&s->d_map, static_cast<size_t>(cap));
+        A.plan(&s->entry_conductance, static_cast<size_t>(g->nnz));
+        A.plan(&s->capacity_area, n);
+        A.plan(&s->stage_a, n);
+        A.plan(&s->stage_b, n);
+        A.plan(&s->stage_c, n);
+        A.allocate();
+        s->d_result = I.result;
+        psim::hip_check(hipMemset(A.block(), 0, A.bytes()), "system clear");
+        psim::put(s->d_inst, &I, sizeof(Instance));
+        *out = s.release();
+    });
+}
+
+void psim_system_destroy(psim_system* s) {
+    if (s == nullptr) return;
+    (void)hipStreamSynchronize(s->graph->ctx->stream);
+    delete s;
+}
+
+int psim_system_set_workgroups(psim_system* s, int32_t workgroups) {
+    return guarded([&] {
+        if (s == nullptr) throw std::invalid_argument("null system");
+        const psim_ctx* ctx = s->graph->ctx;
+        const int32_t cap = std::max(1, ctx->compute_units * ctx->blocks_per_cu);
+        if (workgroups < 1 || workgroups > cap) throw std::invalid_argument("workgroup count out of range");
+        if (workgroups > 1 && !ctx->cooperative) throw std::invalid_argument("device cannot run a multi-workgroup group");
+        psim::hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+        // the workgroup map holds one entry per workgroup: grow it if needed
+        if (workgroups > s->map_capacity) throw std::invalid_argument("workgroup count above the system's capacity");
+        s->workgroups = workgroups;
+        s->host.group_size = workgroups;
+        psim::put(s->d_inst, &s->host, sizeof(Instance));
+    });
+}
+
+int32_t psim_system_workgroups(const psim_system* s) { return s ? s->workgroups : 0; }
+
+int psim_system_level_values(psim_system* s, uint64_t level, double* diagonal, double* off) {
+    return guarded([&] {
+        require_matrix(s);
+        if (level >= static_cast<uint64_t>(s->graph->depth)) throw std::invalid_argument("no such level");
+        hipStream_t stream = s->graph->ctx->stream;
+        const LevelView& v = s->graph->views[level];
+        if (diagonal)
+            psim::hip_check(hipMemcpyAsync(diagonal, s->host.diag64[level], static_cast<size_t>(v.n) * 8,
+                                           hipMemcpyDeviceToHost, stream), "copy");
+        if (off)
+            psim::hip_check(hipMemcpyAsync(off, s->host.off64[level], static_cast<size_t>(v.nnz) * 8,
+                                           hipMemcpyDeviceToHost, stream), "copy");
+        psim::hip_check(hipStreamSynchronize(stream), "level values");
+    });
+}
+
+int psim_system_set_matrix(psim_system* s, const double* diagonal, uint64_t n, const double* off, uint64_t nnz,
+                           int where) {
+    return guarded([&] {
+        if (s == nullptr || diagonal == nullptr || off == nullptr) throw std::invalid_argument("null argument");
+        psim::check_where(where);
+        if (n != static_cast<uint64_t>(s->graph->n) || nnz != static_cast<uint64_t>(s->graph->nnz))
+            throw std::invalid_argument("multigrid matrix does not match the graph");
+        hipStream_t stream = s->graph->ctx->stream;
+        const hipMemcpyKind kind = where == PSIM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+        psim::hip_check(hipMemcpyAsync(s->host.diag64[0], diagonal, n * 8, kind, stream), "matrix copy");
+        psim::hip_check(hipMemcpyAsync(s->host.off64[0], off, nnz * 8, kind, stream), "matrix copy");
+        Launch p{};
+        p.op = psim::dev::kOpSetMatrix;
+        launch(s, p);
+        if (where == PSIM_HOST) psim::hip_check(hipStreamSynchronize(stream), "set_matrix");
+        s->matrix_set = true;
+        s->heat_matrix = false;
+    });
+}
+
+int psim_system_precondition(psim_system* s, const double* rhs, double* out, int where) {
+    return guarded([&] {
+        require_matrix(s);
+        psim::check_where(where);
+        if (rhs == nullptr || out == nullptr) throw std::invalid_argument("null argument");
+        hipStream_t stream = s->graph->ctx->stream;
+        const size_t bytes = static_cast<size_t>(s->graph->n) * 8;
+        Launch p{};
+        p.op = psim::dev::kOpPrecondition;
+        p.in = rhs;
+        p.out = out;
+        if (where == PSIM_HOST) {
+            psim::hip_check(hipMemcpyAsync(s->stage_a, rhs, bytes, hipMemcpyHostToDevice, stream), "copy");
+            p.in = s->stage_a;
+            p.out = s->stage_b;
+        }
+        launch(s, p);
+        if (where == PSIM_HOST) {
+            psim::hip_check(hipMemcpyAsync(out, s->stage_b, bytes, hipMemcpyDeviceToHost, stream), "copy");
+            psim::hip_check(hipStreamSynchronize(stream), "precondition");
+        }
+    });
+}
+
+int psim_system_apply(psim_system* s, const double* x, double* out, int where) {
+    return guarded([&] {
+        require_matrix(s);
+        psim::check_where(where);
+        if (x == nullptr || out == nullptr) throw std::invalid_argument("null argument");
+        hipStream_t stream = s->graph->ctx->stream;
+        const size_t bytes = static_cast<size_t>(s->graph->n) * 8;
+        Launch p{};
+        p.op = psim::dev::kOpApply;
+        p.in = x;
+        p.out = out;
+        if (where == PSIM_HOST) {
+            psim::hip_check(hipMemcpyAsync(s->stage_a, x, bytes, hipMemcpyHostToDevice, stream), "copy");
+            p.in = s->stage_a;
+            p.out = s->stage_b;
+        }
+        launch(s, p);
+        if (where == PSIM_HOST) {
+            psim::hip_check(hipMemcpyAsync(out, s->stage_b, bytes, hipMemcpyDeviceToHost, stream), "copy");
+            psim::hip_check(hipStreamSynchronize(stream), "apply");
+        }
+    });
+}
+
+int psim_system_solve(psim_system* s, const double* b, double* x, double tolerance, int32_t max_iterations,
+                      psim_solve_info* info, int where) {
+    return guarded([&] {
+        require_matrix(s);
+        psim::check_where(where);
+        if (b == nullptr || x == nullptr) throw std::invalid_argument("null argument");
+        if (!(tolerance >= 0.0) || max_iterations < 0) throw std::invalid_argument("invalid solver settings");
+        hipStream_t stream = s->graph->ctx->stream;
+        const size_t bytes = static_cast<size_t>(s->graph->n) * 8;
+        Launch p = solve_params(s, psim::dev::kOpSolve, tolerance, max_iterations);
+        p.in = b;
+        if (where == PSIM_HOST) {
+            psim::hip_check(hipMemcpyAsync(s->stage_a, b, bytes, hipMemcpyHostToDevice, stream), "copy");
+            p.in = s->stage_a;
+        }
+        launch(s, p);
+        psim::hip_check(hipMemcpyAsync(where == PSIM_HOST ? x : x, s->host.x, bytes,
+                                       where == PSIM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, stream),
+                        "copy");
+        if (where == PSIM_HOST || info != nullptr) fill_info(fetch_result(s), info);
+    });
+}
+
+int psim_heat_set_matrix(psim_system* s, const double* conductance_per_entry, const double* capacity_area, double dt,
+                         int where) {
+    return guarded([&] {
+        if (s == nullptr || conductance_per_entry == nullptr || capacity_area == nullptr)
+            throw std::invalid_argument("null argument");
+        psim::check_where(where);
+        if (!(dt > 0.0) || !std::isfinite(dt)) throw std::invalid_argument("time step must be finite and positive");
+        hipStream_t stream = s->graph->ctx->stream;
+        const hipMemcpyKind kind = where == PSIM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+        psim::hip_check(hipMemcpyAsync(s->entry_conductance, conductance_per_entry, static_cast<size_t>(s->graph->nnz) * 8,
+                                       kind, stream),
+                        "copy");
+        psim::hip_check(hipMemcpyAsync(s->capacity_area, capacity_area, static_cast<size_t>(s->graph->n) * 8, kind, stream),
+                        "copy");
+        Launch p{};
+        p.op = psim::dev::kOpSetMatrix | psim::dev::kOpHeatAssemble;
+        p.dt = dt;
+        p.entry_conductance = s->entry_conductance;
+        p.capacity_area = s->capacity_area;
+        launch(s, p);
+        if (where == PSIM_HOST) psim::hip_check(hipStreamSynchronize(stream), "heat set_matrix");
+        s->matrix_set = true;
+        s->heat_matrix = true;
+    });
+}
+
+int psim_heat_step(psim_system* s, const double* t_old, const double* forcing, double dt, double tolerance,
+                   int32_t max_iterations, double* t_new, psim_solve_info* info, int where) {
+    return guarded([&] {
+        require_matrix(s);
+        if (!s->heat_matrix) throw std::logic_error("psim_heat_set_matrix has not been called");
+        psim::check_where(where);
+        if (t_old == nullptr || forcing == nullptr || t_new == nullptr) throw std::invalid_argument("null argument");
+        if (!(dt > 0.0) || !std::isfinite(dt)) throw std::invalid_argument("time step must be finite and positive");
+        if (!(tolerance >= 0.0) || max_iterations < 0) throw std::invalid_argument("invalid solver settings");
+        hipStream_t stream = s->graph->ctx->stream;
+        const size_t bytes = static_cast<size_t>(s->graph->n) * 8;
+        Launch p = solve_params(s, psim::dev::kOpSolve | psim::dev::kOpHeatRhs, tolerance, max_iterations);
+        p.dt = dt;
+        p.capacity_area = s->capacity_area;
+        p.t_old = t_old;
+        p.forcing = forcing;
+        p.t_new = t_new;
+        if (where == PSIM_HOST) {
+            psim::hip_check(hipMemcpyAsync(s->stage_a, t_old, bytes, hipMemcpyHostToDevice, stream), "copy");
+            psim::hip_check(hipMemcpyAsync(s->stage_b, forcing, bytes, hipMemcpyHostToDevice, stream), "copy");
+            p.t_old = s->stage_a;
+            p.forcing = s->stage_b;
+            p.t_new = s->stage_c;
+        }
+        launch(s, p);
+        if (where == PSIM_HOST)
+            psim::hip_check(hipMemcpyAsync(t_new, s->stage_c, bytes, hipMemcpyDeviceToHost, stream), "copy");
+        if (where == PSIM_HOST || info != nullptr) fill_info(fetch_result(s), info);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+// Device-side data layout shared by the HIP sources of libpsim.
+//
+// HBM layout, per mesh level (psim_graph, shared by all systems on it):
+//   level j of the multigrid hierarchy: int32 CSR (row, col) of the
+//   off-diagonals, aggregate map, aggregate-major row order with its
+//   restriction tiles, and the Galerkin gather lists of level j+1.
+// Per system (psim_system, one heat-transport instance):
+//   FP64 diagonal / inverse diagonal / off-diagonals of every level, FP32
+//   copies for the mixed-precision form, the coarsest level's dense inverse
+//   (FP64), the solver vectors, per-tile reduction partials, a barrier word
+//   pair and a result record.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+
+namespace psim::dev {
+
+constexpr int kTile = 256;        // rows per tile = threads per workgroup = the reference's block
+constexpr int kMaxLevels = 8;
+constexpr int kPartials = 4;      // reduction slots per tile
+constexpr int kMaxCoarse = 512;   // dense coarsest level (the hierarchy stops at <= 400 rows)
+constexpr int32_t kNotConverged = 5;   // PSIM_NOT_CONVERGED
+
+enum Op : int32_t {
+    kOpSetMatrix = 1,     // Galerkin values, smoother weights, coarse inverse
+    kOpHeatAssemble = 2,  // diag/off from entry conductances, C A and dt (with kOpSetMatrix)
+    kOpSolve = 4,         // BiCGSTAB with true-residual restarts on b
+    kOpHeatRhs = 8,       // b = C A T_old + dt A S (with kOpSolve); T_new = x
+    kOpPrecondition = 16, // out = V-cycle(in)
+    kOpApply = 32         // out = A in (FP64)
+};
+
+// Structure of one multigrid level (per graph).
+struct LevelView {
+    int32_t n = 0, nnz = 0, n_next = 0, down_tiles = 0;
+    const int32_t* row = nullptr;
+    const int32_t* col = nullptr;
+    const int32_t* agg = nullptr;         // n (not on the coarsest level)
+    const int32_t* agg_start = nullptr;   // n_next + 1
+    const int32_t* agg_rows = nullptr;    // n
+    const int32_t* tile_start = nullptr;  // down_tiles + 1 (coarse-row ranges)
+    const int32_t* gd_start = nullptr;    // Galerkin lists of level j+1
+    const int32_t* gd = nullptr;
+    const int32_t* go_start = nullptr;
+    const int32_t* go = nullptr;
+};
+
+struct SolveResult {
+    int32_t iterations;
+    int32_t restarts;
+    int32_t status;
+    int32_t barrier_timeouts;
+    double relative_residual;
+    double b_norm;
+};
+
+// Everything one instance's program touches. Lives in device memory; every
+// workgroup of the instance's group reads it (uniform, scalar loads).
+struct Instance {
+    int32_t precision = 0;       // 0 = FP64 inner vectors, 1 = FP32
+    int32_t depth = 0;
+    int32_t n = 0, nnz = 0, tiles = 0;   // fine level
+    int32_t group_first = 0, group_size = 1;   // workgroups [first, first+size)
+    LevelView lv[kMaxLevels];
+    // values (per system)
+    double* diag64[kMaxLevels] = {};
+    double* dinv64[kMaxLevels] = {};
+    double* off64[kMaxLevels] = {};
+    float* diag32[kMaxLevels] = {};
+    float* dinv32[kMaxLevels] = {};
+    float* off32[kMaxLevels] = {};
+    double* omega = nullptr;     // smoother weight per level
+    double* cinv = nullptr;      // dense inverse of the coarsest level, n_c x n_c
+    // graph data
+    const double* area = nullptr;
+    const double* weight = nullptr;
+    // vectors
+    double* b = nullptr;
+    double* x = nullptr;
+    void* r = nullptr;
+    void* r0 = nullptr;
+    void* p = nullptr;
+    void* v = nullptr;
+    void* ph = nullptr;
+    void* s = nullptr;
+    void* sh = nullptr;
+    void* t = nullptr;
+    void* lvl_b[kMaxLevels] = {};   // V-cycle right-hand sides, levels >= 1
+    void* lvl_e[kMaxLevels] = {};   // V-cycle corrections, levels >= 1
+    double* pin = nullptr;          // FP64 staging for precondition in FP32 (mixed)
+    double* partial = nullptr;      // 2 x tiles x kPartials (double-buffered)
+    uint32_t* barrier = nullptr;    // [0] arrivals, [1] generation
+    SolveResult* result = nullptr;
+};
+
+// What one launch asks of one instance (passed by value for a single
+// instance, as a device array for a batch). Plain data: value-initialise
+// with Launch{}.
+struct Launch {
+    int32_t op;
+    int32_t max_iterations, max_restarts;
+    int32_t pad;
+    double tolerance, inner_relative;
+    double dt;
+    const double* entry_conductance;   // K per CSR entry
+    const double* capacity_area;       // C A per cell
+    const double* t_old;
+    const double* forcing;
+    double* t_new;
+    const double* in;   // b for kOpSolve, input of precondition / apply
+    double* out;        // output of precondition / apply
+};
+
+// Launch the persistent program. Workgroup w serves instance
+// block_instance[w]; its parameters are launches[that] when `launches` is
+// non-null, else `single`.
+hipError_t launch_program(const Instance* instances, const int32_t* block_instance, const Launch* launches,
+                          const Launch& single, int32_t blocks, hipStream_t stream, bool cooperative);
+
+// Co-resident program workgroups per compute unit.
+int program_blocks_per_cu();
+
+void launch_diffusion(int n, const int32_t* row, const int32_t* col, const double* weight, const double* area,
+                      double conductance, const double* t, double* out, hipStream_t stream);
+
+}  // namespace psim::dev
+
+// Host-side context shared by the device entry points.
+struct psim_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int compute_units = 0;
+    int blocks_per_cu = 0;      // co-resident program workgroups per CU
+    bool cooperative = false;   // cooperative launch supported
+};
+
+namespace psim {
+// Throws DeviceFailure (PSIM_DEVICE_ERROR) or DeviceOutOfMemory with the
+// HIP error text when `status` is not hipSuccess.
+void hip_check(hipError_t status, const char* what);
+}  // namespace psim
