@@ -101,6 +101,14 @@ int psim_device_alloc(psim_ctx* ctx, uint64_t bytes, void** out);
 int psim_device_free(psim_ctx* ctx, void* ptr);
 int psim_copy_to_device(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int psim_copy_to_host(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* HIP events on the context's stream, for timing device work in place:
+ * psim_ctx_events makes `count` events, psim_ctx_record records event
+ * `index` on the stream, psim_ctx_elapsed waits for event `to` and returns
+ * the milliseconds between `from` and `to`. */
+int psim_ctx_events(psim_ctx* ctx, uint32_t count);
+int psim_ctx_record(psim_ctx* ctx, uint32_t index);
+int psim_ctx_elapsed(psim_ctx* ctx, uint32_t from, uint32_t to, float* ms);
+
 
 /* ------------------------------------------------- FV operators on device
  * finite_volume.hpp:21-44 / finite_volume.cpp:23-123. The mesh is uploaded
@@ -150,7 +158,7 @@ typedef struct psim_solve_info {
     int32_t iterations;         /* BiCGSTAB iterations, summed over restarts */
     int32_t restarts;           /* true-residual restarts taken */
     int32_t status;             /* PSIM_OK or PSIM_NOT_CONVERGED */
-    int32_t reserved;
+    int32_t v_cycles;           /* preconditioner applications (half-iterations) */
     double relative_residual;   /* true ||b - A x|| / ||b|| in FP64 */
 } psim_solve_info;
 
@@ -190,6 +198,10 @@ int psim_system_solve(psim_system* system, const double* b, double* x, double to
  * conductance_per_entry is K_e of each CSR entry's edge (nnz).
  * psim_heat_step assembles b = C A T_old + dt A S on the device, solves, and
  * writes T. */
+/* Result record of the last solve on the system, once its work has
+ * completed (synchronises the stream). */
+int psim_system_last_info(psim_system* system, psim_solve_info* info);
+
 int psim_heat_set_matrix(psim_system* system, const double* conductance_per_entry, const double* capacity_area,
                          double dt_s, int where);
 int psim_heat_step(psim_system* system, const double* temperature_old, const double* forcing_W_m2, double dt_s,

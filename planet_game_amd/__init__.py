@@ -46,7 +46,7 @@ _pu8 = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("restarts", C.c_int32), ("status", C.c_int32),
-                ("reserved", C.c_int32), ("relative_residual", C.c_double)]
+                ("v_cycles", C.c_int32), ("relative_residual", C.c_double)]
 
 
 def _sig(name, res, *args):
@@ -79,6 +79,10 @@ _sig("psim_device_alloc", C.c_int, _vp, _u64, C.POINTER(_vp))
 _sig("psim_device_free", C.c_int, _vp, _vp)
 _sig("psim_copy_to_device", C.c_int, _vp, _vp, _vp, _u64)
 _sig("psim_copy_to_host", C.c_int, _vp, _vp, _vp, _u64)
+_sig("psim_ctx_events", C.c_int, _vp, C.c_uint32)
+_sig("psim_ctx_record", C.c_int, _vp, C.c_uint32)
+_sig("psim_ctx_elapsed", C.c_int, _vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_float))
+_sig("psim_system_last_info", C.c_int, _vp, C.POINTER(SolveInfo))
 _sig("psim_fv_create", C.c_int, _vp, _vp, C.POINTER(_vp))
 _sig("psim_fv_create_from_arrays", C.c_int, _vp, _u64, _u64, _pd, _pu32, _pu8, _pu32, _pu32, _pf, _pf,
      _pu32, _pu32, _pd, _pd, C.POINTER(_vp))
@@ -228,6 +232,17 @@ class Context:
 
     def synchronize(self):
         check(lib.psim_ctx_synchronize(self.handle))
+
+    def events(self, count):
+        check(lib.psim_ctx_events(self.handle, count))
+
+    def record(self, index):
+        check(lib.psim_ctx_record(self.handle, index))
+
+    def elapsed_ms(self, start, end):
+        ms = C.c_float()
+        check(lib.psim_ctx_elapsed(self.handle, start, end, C.byref(ms)))
+        return ms.value
 
     def __del__(self):
         if getattr(self, "handle", None):
@@ -400,6 +415,21 @@ class System:
         k = np.ascontiguousarray(conductance_per_entry, np.float64)
         ca = np.ascontiguousarray(capacity_area, np.float64)
         check(lib.psim_heat_set_matrix(self.handle, _ptr(k), _ptr(ca), dt, HOST))
+
+    def heat_set_matrix_device(self, conductance_per_entry: "DeviceArray", capacity_area: "DeviceArray",
+                               dt=2.63e6):
+        check(lib.psim_heat_set_matrix(self.handle, conductance_per_entry.ptr, capacity_area.ptr, dt,
+                                       DEVICE))
+
+    def heat_step_device(self, t_old, forcing, t_new, dt=2.63e6, tolerance=1e-10, max_iterations=1000):
+        """Queue one solve on device-resident vectors (no synchronisation)."""
+        check(lib.psim_heat_step(self.handle, t_old, forcing, dt, tolerance, max_iterations, t_new,
+                                 None, DEVICE))
+
+    def last_info(self):
+        info = SolveInfo()
+        check(lib.psim_system_last_info(self.handle, C.byref(info)))
+        return info
 
     def heat_step(self, t_old, forcing, dt=2.63e6, tolerance=1e-10, max_iterations=1000):
         t = np.ascontiguousarray(t_old, np.float64)

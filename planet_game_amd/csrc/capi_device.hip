@@ -128,7 +128,7 @@ void fill_info(const SolveResult& r, psim_solve_info* info) {
     info->iterations = r.iterations;
     info->restarts = r.restarts;
     info->status = r.status;
-    info->reserved = 0;
+    info->v_cycles = r.v_cycles;
     info->relative_residual = r.relative_residual;
 }
 
@@ -172,6 +172,7 @@ int psim_ctx_create(int device, psim_ctx** out) {
 
 void psim_ctx_destroy(psim_ctx* ctx) {
     if (ctx == nullptr) return;
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     if (ctx->stream != nullptr) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
@@ -183,6 +184,33 @@ int psim_ctx_synchronize(psim_ctx* ctx) {
     return guarded([&] {
         if (ctx == nullptr) throw std::invalid_argument("null context");
         psim::hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+    });
+}
+
+int psim_ctx_events(psim_ctx* ctx, uint32_t count) {
+    return guarded([&] {
+        if (ctx == nullptr) throw std::invalid_argument("null context");
+        if (count > (1U << 20)) throw std::invalid_argument("too many events");
+        psim::hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+        for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+        ctx->events.assign(count, nullptr);
+        for (auto& e : ctx->events) psim::hip_check(hipEventCreate(&e), "event create");
+    });
+}
+
+int psim_ctx_record(psim_ctx* ctx, uint32_t index) {
+    return guarded([&] {
+        if (ctx == nullptr || index >= ctx->events.size()) throw std::invalid_argument("no such event");
+        psim::hip_check(hipEventRecord(ctx->events[index], ctx->stream), "event record");
+    });
+}
+
+int psim_ctx_elapsed(psim_ctx* ctx, uint32_t from, uint32_t to, float* ms) {
+    return guarded([&] {
+        if (ctx == nullptr || ms == nullptr || from >= ctx->events.size() || to >= ctx->events.size())
+            throw std::invalid_argument("no such event");
+        psim::hip_check(hipEventSynchronize(ctx->events[to]), "event wait");
+        psim::hip_check(hipEventElapsedTime(ms, ctx->events[from], ctx->events[to]), "event elapsed");
     });
 }
 
@@ -525,6 +553,13 @@ int psim_system_solve(psim_system* s, const double* b, double* x, double toleran
                                        where == PSIM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, stream),
                         "copy");
         if (where == PSIM_HOST || info != nullptr) fill_info(fetch_result(s), info);
+    });
+}
+
+int psim_system_last_info(psim_system* s, psim_solve_info* info) {
+    return guarded([&] {
+        if (s == nullptr || info == nullptr) throw std::invalid_argument("null argument");
+        fill_info(fetch_result(s), info);
     });
 }
 

@@ -15,6 +15,7 @@
 
 #include <cstdint>
 #include <stdexcept>
+#include <vector>
 
 namespace psim::dev {
 
@@ -55,6 +56,8 @@ struct SolveResult {
     int32_t barrier_timeouts;
     double relative_residual;
     double b_norm;
+    int32_t v_cycles;
+    int32_t pad;
 };
 
 // Everything one instance's program touches. Lives in device memory; every
@@ -132,6 +135,7 @@ void launch_diffusion(int n, const int32_t* row, const int32_t* col, const doubl
 struct psim_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> events;
     int compute_units = 0;
     int blocks_per_cu = 0;      // co-resident program workgroups per CU
     bool cooperative = false;   // cooperative launch supported

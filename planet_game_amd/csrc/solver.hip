@@ -442,7 +442,7 @@ __device__ void solve(const Instance& I, const Launch& P, int g, int G, Lds& lds
     const double b_norm = sqrt(tot2[0]);
     double rho1 = tot2[1];
     double r_norm = b_norm;
-    int iterations = 0, restarts = 0;
+    int iterations = 0, restarts = 0, v_cycles = 0;
     int status = 0;
     const double target = P.tolerance * b_norm;
     bool first = true;
@@ -500,6 +500,7 @@ __device__ void solve(const Instance& I, const Launch& P, int g, int G, Lds& lds
                 }
                 group_sync(I, timeouts);
                 vcycle<VT>(I, V, p, ph, g, G, lds, timeouts);
+                ++v_cycles;
                 // v = A ph, (r0, v)
                 part = next_partials();
                 for (int tile = g; tile < tiles; tile += G) {
@@ -548,6 +549,7 @@ __device__ void solve(const Instance& I, const Launch& P, int g, int G, Lds& lds
                     break;
                 }
                 vcycle<VT>(I, V, s, sh, g, G, lds, timeouts);
+                ++v_cycles;
                 // t = A sh, (t, t), (t, s)
                 part = next_partials();
                 for (int tile = g; tile < tiles; tile += G) {
@@ -612,6 +614,7 @@ __device__ void solve(const Instance& I, const Launch& P, int g, int G, Lds& lds
         I.result->status = status;
         I.result->relative_residual = b_norm > 0.0 ? r_norm / b_norm : 0.0;
         I.result->b_norm = b_norm;
+        I.result->v_cycles = v_cycles;
     }
 }
 
