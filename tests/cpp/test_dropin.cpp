@@ -1,0 +1,211 @@
+// Drop-in check of the C++20 layer (include/planetsim/): the reference's
+// call sites compile unchanged against it and behave as the reference's
+// tests expect (tests/conservation/test_operator_conservation.cpp,
+// tests/regression/test_operator_determinism.cpp,
+// tests/unit/test_heat_transport.cpp). `cpu` runs the host-only part,
+// `gpu` the part that needs a device.
+#include "planetsim/planetsim.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numbers>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+int failures = 0;
+
+#define EXPECT(condition)                                                              \
+    do {                                                                               \
+        if (!(condition)) {                                                            \
+            std::fprintf(stderr, "%s:%d: expectation failed: %s\n", __FILE__, __LINE__, #condition); \
+            ++failures;                                                                \
+        }                                                                              \
+    } while (false)
+
+template <class E, class F> bool throws_with(F&& f, const char* message) {
+    try {
+        f();
+    } catch (const E& e) {
+        return message == nullptr || std::string(e.what()).find(message) != std::string::npos;
+    } catch (...) {
+        return false;
+    }
+    return false;
+}
+
+// splitmix-based uniform numbers for test fields (not the RNG under test)
+double uniform(std::uint64_t key) {
+    key += 0x9e3779b97f4a7c15ULL;
+    key = (key ^ (key >> 30U)) * 0xbf58476d1ce4e5b9ULL;
+    key = (key ^ (key >> 27U)) * 0x94d049bb133111ebULL;
+    key ^= key >> 31U;
+    return static_cast<double>(key >> 11U) * (1.0 / 9007199254740992.0);
+}
+
+void host_part() {
+    for (std::uint32_t level = 0; level <= 5; ++level) {
+        const auto mesh = planetsim::make_icosphere(level, 6'371'000.0);
+        EXPECT(mesh.cell_count() == 10U * (1U << (2U * level)) + 2U);
+        EXPECT(mesh.edge_count() == 30U * (1U << (2U * level)));
+        std::size_t pentagons = 0;
+        double total = 0.0, carry = 0.0;
+        for (const auto& cell : mesh.cells()) {
+            pentagons += cell.is_pentagon() ? 1U : 0U;
+            const double piece = cell.area_m2 - carry;
+            const double next = total + piece;
+            carry = (next - total) - piece;
+            total = next;
+        }
+        EXPECT(pentagons == 12U);
+        const double expected = 4.0 * std::numbers::pi_v<double> * 6'371'000.0 * 6'371'000.0;
+        EXPECT(std::abs(total - expected) / expected <= 5e-14);
+        // reciprocal edges (icosphere.cpp:383-387)
+        for (const auto& cell : mesh.cells())
+            for (const auto& ce : mesh.cell_edges(cell.id)) {
+                const auto& edge = mesh.edge(ce.edge);
+                EXPECT((edge.first_cell == cell.id && edge.second_cell == ce.neighbor) ||
+                       (edge.second_cell == cell.id && edge.first_cell == ce.neighbor));
+            }
+        const auto graph = planetsim::mesh_transport_graph(mesh);
+        EXPECT(graph.size() == mesh.cell_count() && graph.offset.back() == mesh.directed_edge_count());
+    }
+    const auto mesh = planetsim::make_icosphere(2, 6'371'000.0);
+    EXPECT(throws_with<std::invalid_argument>([&] { (void)planetsim::make_icosphere(2, -5.0); },
+                                              "icosphere radius must be finite and positive"));
+    EXPECT(throws_with<std::invalid_argument>(
+        [&] { (void)planetsim::edge_orientation(mesh, planetsim::CellId{0}, planetsim::EdgeId{mesh.edge_count() - 1}); },
+        "cell is not incident to edge"));
+    planetsim::Field2D<double> wrong(mesh.cell_count() + 1), out(mesh.cell_count());
+    EXPECT(throws_with<std::invalid_argument>([&] { planetsim::laplacian(mesh, wrong, out); },
+                                              "cell field size does not match the mesh"));
+    planetsim::EdgeField<double> wrong_flux(mesh.edge_count() - 1);
+    EXPECT(throws_with<std::invalid_argument>([&] { planetsim::divergence(mesh, wrong_flux, out); },
+                                              "edge field size does not match the mesh"));
+}
+
+void device_part() {
+    const auto mesh = planetsim::make_icosphere(5, 6'371'000.0);
+    const std::size_t n = mesh.cell_count();
+    // test_operator_conservation.cpp:36-57: constant fields give exactly 0.
+    {
+        const planetsim::Field2D<double> constant(n, 287.5);
+        planetsim::Field2D<double> east(n, 1.0), north(n, 1.0), lap(n, 1.0), div(n, 1.0);
+        planetsim::gradient(mesh, constant, east, north);
+        planetsim::laplacian(mesh, constant, lap);
+        planetsim::divergence(mesh, planetsim::EdgeField<double>(mesh.edge_count(), 0.0), div);
+        bool zero = true;
+        for (std::size_t c = 0; c < n; ++c) zero = zero && east[c] == 0.0 && north[c] == 0.0 && lap[c] == 0.0 && div[c] == 0.0;
+        EXPECT(zero);
+    }
+    // test_operator_determinism.cpp: bit-identical for any worker count;
+    // the Laplacian is conservative and dissipative.
+    {
+        planetsim::Field2D<float> scalar(n);
+        for (std::size_t c = 0; c < n; ++c) scalar[c] = static_cast<float>(uniform(11 * 1000003ULL + c));
+        planetsim::EdgeField<float> flux(mesh.edge_count());
+        for (std::size_t e = 0; e < mesh.edge_count(); ++e)
+            flux[planetsim::EdgeId{static_cast<std::uint32_t>(e)}] = static_cast<float>(uniform(7 * 1000003ULL + e));
+        planetsim::Field2D<float> e1(n), n1(n), d1(n), l1(n), e2(n), n2(n), d2(n), l2(n);
+        planetsim::gradient(mesh, scalar, e1, n1, 1U);
+        planetsim::divergence(mesh, flux, d1, 1U);
+        planetsim::laplacian(mesh, scalar, l1, 1U);
+        planetsim::gradient(mesh, scalar, e2, n2, 16U);
+        planetsim::divergence(mesh, flux, d2, 16U);
+        planetsim::laplacian(mesh, scalar, l2, 16U);
+        EXPECT(std::memcmp(e1.values().data(), e2.values().data(), n * 4) == 0);
+        EXPECT(std::memcmp(d1.values().data(), d2.values().data(), n * 4) == 0);
+        EXPECT(std::memcmp(l1.values().data(), l2.values().data(), n * 4) == 0);
+        planetsim::Field2D<double> phi(n), lap(n);
+        for (std::size_t c = 0; c < n; ++c) phi[c] = uniform(3 * 1000003ULL + c) - 0.5;
+        planetsim::laplacian(mesh, phi, lap);
+        double sum = 0.0, abs_sum = 0.0, dissipation = 0.0;
+        for (std::size_t c = 0; c < n; ++c) {
+            sum += mesh.cells()[c].area_m2 * lap[c];
+            abs_sum += mesh.cells()[c].area_m2 * std::abs(lap[c]);
+            dissipation += mesh.cells()[c].area_m2 * phi[c] * lap[c];
+        }
+        EXPECT(std::abs(sum) <= 1e-13 * abs_sum);
+        EXPECT(dissipation <= 0.0);
+    }
+    // GraphMultigrid and the heat step on the transport graph.
+    {
+        const auto graph = planetsim::mesh_transport_graph(mesh);
+        const double dt = 2.63e6;
+        std::vector<double> capacity(n), conductance(graph.neighbour.size()), diag(n), off(graph.neighbour.size());
+        for (std::size_t c = 0; c < n; ++c) capacity[c] = (uniform(c) < 0.3 ? 1.0e7 : 4.0e8) * graph.area_m2[c];
+        for (std::size_t c = 0; c < n; ++c) {
+            double s = 0.0;
+            for (std::size_t k = graph.offset[c]; k < graph.offset[c + 1U]; ++k) {
+                conductance[k] = 0.5 * 6'371'000.0 * 6'371'000.0;   // stiff: Laplacian-dominated
+                const double g = conductance[k] * graph.weight[k];
+                s += g;
+                off[k] = -dt * g;
+            }
+            diag[c] = capacity[c] + dt * s;
+        }
+        planetsim::GraphMultigrid multigrid(graph);
+        EXPECT(throws_with<std::invalid_argument>([&] { multigrid.set_matrix(std::span(diag).first(n - 1), off); },
+                                                  "multigrid matrix does not match the graph"));
+        multigrid.set_matrix(diag, off);
+        std::vector<double> rhs(n), z;
+        for (std::size_t c = 0; c < n; ++c) rhs[c] = uniform(99 + c) - 0.5;
+        multigrid.precondition(rhs, z, 4U);
+        double rz = 0.0;
+        for (std::size_t c = 0; c < n; ++c) rz += rhs[c] * z[c];
+        EXPECT(z.size() == n && rz > 0.0);
+
+        planetsim::Field2D<double> t_old(n), forcing(n), t_new;
+        for (std::size_t c = 0; c < n; ++c) {
+            t_old[c] = 288.0 - 40.0 * mesh.cells()[c].center_unit.z * mesh.cells()[c].center_unit.z;
+            forcing[c] = 100.0 * std::sqrt(1.0 - mesh.cells()[c].center_unit.z * mesh.cells()[c].center_unit.z);
+        }
+        planetsim::HeatStepSettings settings;
+        settings.relative_tolerance = 1e-10;
+        const auto result = planetsim::solve_heat_step(graph, conductance, capacity, t_old, forcing, t_new, settings);
+        EXPECT(result.converged && result.relative_residual <= 1e-10 && t_new.size() == n);
+        // the same residual, recomputed here in FP64
+        double rr = 0.0, bb = 0.0;
+        for (std::size_t c = 0; c < n; ++c) {
+            double ax = diag[c] * t_new[c];
+            for (std::size_t k = graph.offset[c]; k < graph.offset[c + 1U]; ++k) ax += off[k] * t_new[graph.neighbour[k]];
+            const double b = capacity[c] * t_old[c] + dt * graph.area_m2[c] * forcing[c];
+            rr += (b - ax) * (b - ax);
+            bb += b * b;
+        }
+        EXPECT(std::sqrt(rr / bb) <= 1.5e-10);
+        // energy: sum (C A (T - T_old)) = dt sum A S (the Laplacian conserves)
+        double stored = 0.0, supplied = 0.0;
+        for (std::size_t c = 0; c < n; ++c) {
+            stored += capacity[c] * (t_new[c] - t_old[c]);
+            supplied += dt * graph.area_m2[c] * forcing[c];
+        }
+        EXPECT(std::abs(stored - supplied) <= 1e-8 * std::abs(supplied));
+        planetsim::Field2D<double> source;
+        planetsim::diffusion_source(graph, 3.5e12, t_new, source);
+        double net = 0.0, gross = 0.0;
+        for (std::size_t c = 0; c < n; ++c) {
+            net += graph.area_m2[c] * source[c];
+            gross += graph.area_m2[c] * std::abs(source[c]);
+        }
+        EXPECT(std::abs(net) <= 1e-13 * gross);
+    }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const std::string part = argc > 1 ? argv[1] : "cpu";
+    try {
+        if (part == "cpu") host_part();
+        if (part == "gpu") device_part();
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "unexpected exception: %s\n", e.what());
+        return 2;
+    }
+    std::printf("%s part: %d failure(s)\n", part.c_str(), failures);
+    return failures == 0 ? 0 : 1;
+}
