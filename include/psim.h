@@ -170,6 +170,12 @@ void psim_system_destroy(psim_system* system);
  * reference's worker_count plays this role, deterministic_executor.hpp:14). */
 int psim_system_set_workgroups(psim_system* system, int32_t workgroups);
 int32_t psim_system_workgroups(const psim_system* system);
+/* Phase trace (diagnostics): with capacity > 0, every later launch on the
+ * system stamps the device wall clock (100 MHz) when it starts and after
+ * each device-wide barrier; psim_system_trace_read returns the stamps of
+ * the last launch (synchronises). capacity 0 turns tracing off. */
+int psim_system_trace(psim_system* system, int32_t capacity);
+int psim_system_trace_read(psim_system* system, uint64_t* ticks, int32_t capacity, int32_t* count);
 /* FP64 Galerkin values of a multigrid level after set_matrix (the
  * reference's Level::diagonal / Level::value, heat_transport.cpp:54-67). */
 int psim_system_level_values(psim_system* system, uint64_t level, double* diagonal, double* off_diagonal);

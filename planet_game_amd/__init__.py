@@ -102,6 +102,9 @@ _sig("psim_system_destroy", None, _vp)
 _sig("psim_system_set_workgroups", C.c_int, _vp, _i32)
 _sig("psim_system_workgroups", _i32, _vp)
 _sig("psim_system_level_values", C.c_int, _vp, _u64, _vp, _vp)
+_sig("psim_system_trace", C.c_int, _vp, _i32)
+_sig("psim_system_trace_read", C.c_int, _vp, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS"), _i32,
+     C.POINTER(_i32))
 _sig("psim_system_set_matrix", C.c_int, _vp, _vp, _u64, _vp, _u64, C.c_int)
 _sig("psim_system_precondition", C.c_int, _vp, _vp, _vp, C.c_int)
 _sig("psim_system_apply", C.c_int, _vp, _vp, _vp, C.c_int)
@@ -371,6 +374,15 @@ class System:
     @workgroups.setter
     def workgroups(self, value):
         check(lib.psim_system_set_workgroups(self.handle, value))
+
+    def trace(self, capacity):
+        check(lib.psim_system_trace(self.handle, capacity))
+
+    def trace_read(self, capacity=4096):
+        ticks = np.zeros(capacity, np.uint64)
+        count = _i32()
+        check(lib.psim_system_trace_read(self.handle, ticks, capacity, C.byref(count)))
+        return ticks[: min(count.value, capacity)]
 
     def set_matrix(self, diag, off):
         d = np.ascontiguousarray(diag, np.float64)

@@ -83,6 +83,7 @@ struct psim_system {
     int precision = PSIM_PLAIN_FP64;
     int32_t workgroups = 1;
     int32_t map_capacity = 1;
+    size_t tri_bytes = 0;          // packed coarsest matrix
     psim::Arena arena;
     Instance host;                 // the uploaded descriptor
     Instance* d_inst = nullptr;
@@ -95,7 +96,11 @@ struct psim_system {
     double* stage_c = nullptr;
     bool matrix_set = false;
     bool heat_matrix = false;
-    ~psim_system() { arena.release(); }
+    void* trace_block = nullptr;   // uint32 count + uint64 stamps
+    ~psim_system() {
+        arena.release();
+        if (trace_block != nullptr) (void)hipFree(trace_block);
+    }
 };
 
 namespace {
@@ -104,7 +109,9 @@ void launch(psim_system* s, const Launch& p) {
     psim_ctx* ctx = s->graph->ctx;
     const bool cooperative = s->workgroups > 1;
     if (cooperative && !ctx->cooperative) throw psim::DeviceFailure("cooperative launch unsupported on this device");
-    psim::hip_check(psim::dev::launch_program(s->d_inst, s->d_map, nullptr, p, s->workgroups, ctx->stream, cooperative),
+    const size_t lds = (p.op & psim::dev::kOpSetMatrix) && s->host.tri_lds ? s->tri_bytes : 0;
+    psim::hip_check(psim::dev::launch_program(s->d_inst, s->d_map, nullptr, p, s->workgroups, lds, ctx->stream,
+                                              cooperative),
                     "solver launch");
 }
 
@@ -116,7 +123,8 @@ SolveResult fetch_result(psim_system* s) {
     if (r.barrier_timeouts != 0) {
         // A barrier gave up waiting: the result is unusable and the words
         // must be cleared before the next launch.
-        psim::hip_check(hipMemset(s->host.barrier, 0, 2 * sizeof(uint32_t)), "barrier reset");
+        psim::hip_check(hipMemset(s->host.barrier, 0, psim::dev::kBarrierWordCount * sizeof(uint32_t)),
+                        "barrier reset");
         psim::hip_check(hipMemset(s->d_result, 0, sizeof(SolveResult)), "result reset");
         throw psim::DeviceFailure("solver group barrier timed out");
     }
@@ -165,7 +173,9 @@ int psim_ctx_create(int device, psim_ctx** out) {
         psim::hip_check(hipGetDeviceProperties(&prop, device), "device properties");
         ctx->compute_units = prop.multiProcessorCount;
         ctx->cooperative = prop.cooperativeLaunch != 0;
-        ctx->blocks_per_cu = psim::dev::program_blocks_per_cu();
+        ctx->blocks_per_cu = psim::dev::program_blocks_per_cu(0);
+        // leave room for the kernel's static LDS (under 8 KB)
+        ctx->lds_per_block = prop.sharedMemPerBlock > 12288 ? prop.sharedMemPerBlock - 12288 : 0;
         *out = ctx.release();
     });
 }
@@ -368,7 +378,17 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         s->graph = g;
         s->precision = precision;
         const int32_t n = g->n, tiles = (n + kTile - 1) / kTile;
-        const int32_t cap = std::max(1, ctx->compute_units * ctx->blocks_per_cu);
+        const size_t nc_rows = static_cast<size_t>(g->views[g->depth - 1].n);
+        s->tri_bytes = nc_rows * (nc_rows + 1) / 2 * sizeof(double);
+        int per_cu = ctx->blocks_per_cu;
+        if (s->tri_bytes <= ctx->lds_per_block) {
+            const int with_lds = psim::dev::program_blocks_per_cu(s->tri_bytes);
+            if (with_lds > 0) {
+                s->host.tri_lds = 1;
+                per_cu = std::min(per_cu, with_lds);
+            }
+        }
+        const int32_t cap = std::max(1, ctx->compute_units * per_cu);
         s->workgroups = std::min(tiles, cap);
         Instance& I = s->host;
         I.precision = precision == PSIM_MIXED ? 1 : 0;
@@ -407,7 +427,8 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         void** vecs[8] = {&I.r, &I.r0, &I.p, &I.v, &I.ph, &I.s, &I.sh, &I.t};
         for (void** v : vecs) A.plan(reinterpret_cast<char**>(v), static_cast<size_t>(n) * vt);
         A.plan(&I.partial, 2 * static_cast<size_t>(tiles) * kPartials);
-        A.plan(&I.barrier, 2);
+        A.plan(&I.barrier, psim::dev::kBarrierWordCount);
+        if (!I.tri_lds) A.plan(&I.tri, nc_rows * (nc_rows + 1) / 2);
         A.plan(&I.result, 1);
         A.plan(&s->d_inst, 1);
         s->map_capacity = cap;
@@ -435,7 +456,7 @@ int psim_system_set_workgroups(psim_system* s, int32_t workgroups) {
     return guarded([&] {
         if (s == nullptr) throw std::invalid_argument("null system");
         const psim_ctx* ctx = s->graph->ctx;
-        const int32_t cap = std::max(1, ctx->compute_units * ctx->blocks_per_cu);
+        const int32_t cap = s->map_capacity;
         if (workgroups < 1 || workgroups > cap) throw std::invalid_argument("workgroup count out of range");
         if (workgroups > 1 && !ctx->cooperative) throw std::invalid_argument("device cannot run a multi-workgroup group");
         psim::hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
@@ -448,6 +469,41 @@ int psim_system_set_workgroups(psim_system* s, int32_t workgroups) {
 }
 
 int32_t psim_system_workgroups(const psim_system* s) { return s ? s->workgroups : 0; }
+
+int psim_system_trace(psim_system* s, int32_t capacity) {
+    return guarded([&] {
+        if (s == nullptr || capacity < 0 || capacity > (1 << 20)) throw std::invalid_argument("invalid trace capacity");
+        psim::hip_check(hipStreamSynchronize(s->graph->ctx->stream), "synchronize");
+        if (s->trace_block != nullptr) psim::hip_check(hipFree(s->trace_block), "trace free");
+        s->trace_block = nullptr;
+        s->host.trace = nullptr;
+        s->host.trace_count = nullptr;
+        s->host.trace_capacity = 0;
+        if (capacity > 0) {
+            psim::hip_check(hipMalloc(&s->trace_block, 256 + static_cast<size_t>(capacity) * 8), "trace allocation");
+            psim::hip_check(hipMemset(s->trace_block, 0, 256), "trace clear");
+            s->host.trace_count = static_cast<uint32_t*>(s->trace_block);
+            s->host.trace = reinterpret_cast<uint64_t*>(static_cast<char*>(s->trace_block) + 256);
+            s->host.trace_capacity = capacity;
+        }
+        psim::put(s->d_inst, &s->host, sizeof(Instance));
+    });
+}
+
+int psim_system_trace_read(psim_system* s, uint64_t* ticks, int32_t capacity, int32_t* count) {
+    return guarded([&] {
+        if (s == nullptr || ticks == nullptr || count == nullptr) throw std::invalid_argument("null argument");
+        if (s->trace_block == nullptr) throw std::logic_error("tracing is off");
+        hipStream_t stream = s->graph->ctx->stream;
+        uint32_t n = 0;
+        psim::hip_check(hipMemcpyAsync(&n, s->host.trace_count, 4, hipMemcpyDeviceToHost, stream), "copy");
+        psim::hip_check(hipStreamSynchronize(stream), "trace");
+        const uint32_t m = std::min<uint32_t>({n, static_cast<uint32_t>(capacity),
+                                               static_cast<uint32_t>(s->host.trace_capacity)});
+        if (m > 0) psim::hip_check(hipMemcpy(ticks, s->host.trace, m * 8ULL, hipMemcpyDeviceToHost), "copy");
+        *count = static_cast<int32_t>(n);
+    });
+}
 
 int psim_system_level_values(psim_system* s, uint64_t level, double* diagonal, double* off) {
     return guarded([&] {

@@ -93,10 +93,14 @@ struct Instance {
     void* t = nullptr;
     void* lvl_b[kMaxLevels] = {};   // V-cycle right-hand sides, levels >= 1
     void* lvl_e[kMaxLevels] = {};   // V-cycle corrections, levels >= 1
-    double* pin = nullptr;          // FP64 staging for precondition in FP32 (mixed)
+    double* tri = nullptr;          // packed coarsest matrix when it does not fit in LDS
+    int32_t tri_lds = 0;            // 1: set_matrix launches carry it in dynamic LDS
     double* partial = nullptr;      // 2 x tiles x kPartials (double-buffered)
-    uint32_t* barrier = nullptr;    // [0] arrivals, [1] generation
+    uint32_t* barrier = nullptr;    // group barrier words (kBarrierWords, zeroed)
     SolveResult* result = nullptr;
+    uint64_t* trace = nullptr;      // phase timestamps (wall clock), or null
+    uint32_t* trace_count = nullptr;
+    int32_t trace_capacity = 0;
 };
 
 // What one launch asks of one instance (passed by value for a single
@@ -121,10 +125,14 @@ struct Launch {
 // block_instance[w]; its parameters are launches[that] when `launches` is
 // non-null, else `single`.
 hipError_t launch_program(const Instance* instances, const int32_t* block_instance, const Launch* launches,
-                          const Launch& single, int32_t blocks, hipStream_t stream, bool cooperative);
+                          const Launch& single, int32_t blocks, size_t dynamic_lds, hipStream_t stream,
+                          bool cooperative);
 
-// Co-resident program workgroups per compute unit.
-int program_blocks_per_cu();
+// Co-resident program workgroups per compute unit with `dynamic_lds`
+// bytes of dynamic LDS (0 if the kernel cannot launch with that much).
+int program_blocks_per_cu(size_t dynamic_lds);
+
+constexpr int kBarrierWordCount = 64 + 32 * 8;   // generation, root, 8 leaves (128-byte apart)
 
 void launch_diffusion(int n, const int32_t* row, const int32_t* col, const double* weight, const double* area,
                       double conductance, const double* t, double* out, hipStream_t stream);
@@ -138,6 +146,7 @@ struct psim_ctx {
     std::vector<hipEvent_t> events;
     int compute_units = 0;
     int blocks_per_cu = 0;      // co-resident program workgroups per CU
+    size_t lds_per_block = 0;   // largest dynamic LDS a program workgroup may use
     bool cooperative = false;   // cooperative launch supported
 };
 

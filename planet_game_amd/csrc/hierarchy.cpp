@@ -148,6 +148,44 @@ void restriction_tiles(HostLevel& fine, int32_t coarse_n) {
     fine.tile_start.push_back(coarse_n);
 }
 
+// Slot-major ELL columns of a level (padding -1).
+void ell_columns(HostLevel& lv) {
+    lv.width = 0;
+    for (int32_t i = 0; i < lv.n; ++i) lv.width = std::max(lv.width, lv.row[i + 1] - lv.row[i]);
+    lv.col_ell.assign(static_cast<size_t>(lv.width) * static_cast<size_t>(lv.n), -1);
+    for (int32_t i = 0; i < lv.n; ++i)
+        for (int32_t k = lv.row[i]; k < lv.row[i + 1]; ++k) lv.col_ell[ell_position(lv, i, k)] = lv.col[k];
+}
+
+// The Galerkin lists of `fine` in ELL positions (fine entries) and indexed
+// by the coarse level's ELL positions; same summation order.
+void ell_galerkin(HostLevel& fine, const HostLevel& coarse) {
+    std::vector<int32_t> fine_row(static_cast<size_t>(fine.nnz));
+    for (int32_t i = 0; i < fine.n; ++i)
+        for (int32_t k = fine.row[i]; k < fine.row[i + 1]; ++k) fine_row[k] = i;
+    fine.d_diag.resize(fine.g_diag.size());
+    for (size_t q = 0; q < fine.g_diag.size(); ++q) {
+        const int32_t code = fine.g_diag[q];
+        fine.d_diag[q] = code < 0 ? code : ell_position(fine, fine_row[code], code);
+    }
+    const size_t slots = static_cast<size_t>(coarse.width) * static_cast<size_t>(coarse.n);
+    std::vector<std::vector<int32_t>> lists(slots);
+    for (int32_t c = 0; c < coarse.n; ++c)
+        for (int32_t t = coarse.row[c]; t < coarse.row[c + 1]; ++t) {
+            auto& list = lists[static_cast<size_t>(ell_position(coarse, c, t))];
+            for (int32_t q = fine.g_off_start[t]; q < fine.g_off_start[t + 1]; ++q) {
+                const int32_t k = fine.g_off[q];
+                list.push_back(ell_position(fine, fine_row[k], k));
+            }
+        }
+    fine.d_off_start.assign(slots + 1, 0);
+    fine.d_off.clear();
+    for (size_t p = 0; p < slots; ++p) {
+        fine.d_off.insert(fine.d_off.end(), lists[p].begin(), lists[p].end());
+        fine.d_off_start[p + 1] = static_cast<int32_t>(fine.d_off.size());
+    }
+}
+
 }  // namespace
 
 HostHierarchy build_hierarchy(const HostGraph& g) {
@@ -177,6 +215,8 @@ HostHierarchy build_hierarchy(const HostGraph& g) {
         restriction_tiles(top, coarse.n);
         h.levels.push_back(std::move(coarse));
     }
+    for (HostLevel& lv : h.levels) ell_columns(lv);
+    for (size_t j = 0; j + 1 < h.levels.size(); ++j) ell_galerkin(h.levels[j], h.levels[j + 1]);
     return h;
 }
 

@@ -112,7 +112,19 @@ struct HostLevel {
     // for coarse entry t: fine entries g_off[g_off_start[t] ..).
     std::vector<int32_t> g_diag_start, g_diag;
     std::vector<int32_t> g_off_start, g_off;
+    // Device layout: off-diagonals in slot-major ELL (entry k of row i, the
+    // s-th of that row, lives at s * n + i; padding has column -1), and the
+    // Galerkin lists with fine entries named by ELL position: the diagonal
+    // lists keep the diagonal codes, the off-diagonal lists are indexed by
+    // the NEXT level's ELL positions (empty for padding).
+    int32_t width = 0;
+    std::vector<int32_t> col_ell;
+    std::vector<int32_t> d_diag;                 // g_diag in ELL positions
+    std::vector<int32_t> d_off_start, d_off;     // per next-level ELL position
 };
+
+// ELL position of CSR entry k of row i.
+inline int32_t ell_position(const HostLevel& lv, int32_t i, int32_t k) { return (k - lv.row[i]) * lv.n + i; }
 
 struct HostHierarchy {
     std::vector<HostLevel> levels;

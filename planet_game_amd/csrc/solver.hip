@@ -48,9 +48,14 @@ struct Lds {
     double red[kPartials][4];   // per-wave partials
     double bcast[kPartials];
     double tile[kTile];         // restriction staging
-    double rowk[kMaxCoarse];    // Gauss-Jordan pivot row
-    double colk[kMaxCoarse];    // Gauss-Jordan pivot column
+    double colk[kMaxCoarse];    // sweep pivot column
 };
+
+// Dynamic LDS: the packed coarsest matrix during set_matrix launches.
+__device__ inline double* dynamic_lds() {
+    extern __shared__ double dyn[];
+    return dyn;
+}
 
 // Sum of K per-thread values over the workgroup (one 256-row tile), fixed
 // order: wave butterflies, then waves 0..3 left to right. Thread 0 returns
@@ -113,31 +118,65 @@ __device__ inline double group_max(const double* partial, int tiles, Lds& lds) {
 
 // ------------------------------------------------------------ group barrier
 
-// Sense-counting barrier over the instance's workgroups. Every thread
-// releases its writes at agent scope; thread 0 arrives and waits on the
-// generation word. The wait is bounded by the 100 MHz wall clock (about
-// one second); a timeout is recorded and is sticky, so a broken group
-// drains within seconds and the host reports the failure.
+// Two-level counting barrier over the instance's workgroups. Workgroup g
+// arrives on leaf counter g % kLeaves (separate 128-byte lines, so the
+// arrivals do not serialise on one address); the last arrival of a leaf
+// arrives at the root; the last root arrival bumps the generation word that
+// every thread 0 polls. Writes are published with one agent-scope release
+// fence per workgroup and read after one acquire fence; the poll itself
+// uses relaxed (coherent, non-invalidating) loads. The wait is bounded by
+// the 100 MHz wall clock (about one second); a timeout is recorded and is
+// sticky, so a broken group drains within seconds and the host reports it.
+//
+// Barrier words (uint32, zeroed at creation, left zeroed by each episode):
+//   [0]       generation
+//   [32]      root arrivals
+//   [64 + 32 l] arrivals of leaf l
 constexpr long long kBarrierTicks = 100000000;
+constexpr int kLeaves = 8;
+constexpr int kBarrierWords = 64 + 32 * kLeaves;
+
+// Phase trace: workgroup 0 of a traced instance stamps the wall clock at
+// every group barrier (after it completes) into I.trace.
+__device__ inline void trace_stamp(const Instance& I) {
+    if (I.trace != nullptr && threadIdx.x == 0 && static_cast<int>(blockIdx.x) == I.group_first) {
+        const uint32_t slot = I.trace_count[0];
+        if (slot < static_cast<uint32_t>(I.trace_capacity)) I.trace[slot] = static_cast<uint64_t>(wall_clock64());
+        I.trace_count[0] = slot + 1;
+    }
+}
 
 __device__ void group_sync(const Instance& I, int& timeouts) {
-    if (I.group_size == 1) {
-        __syncthreads();
+    __syncthreads();
+    const int G = I.group_size;
+    if (G == 1) {
+        trace_stamp(I);
         return;
     }
-    __threadfence();
-    __syncthreads();
     if (threadIdx.x == 0 && timeouts == 0) {
-        uint32_t* arrive = I.barrier;
-        uint32_t* gen = I.barrier + 1;
-        const uint32_t seen = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t before = __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (before + 1 == static_cast<uint32_t>(I.group_size)) {
-            __hip_atomic_store(arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(gen, seen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+        uint32_t* gen = I.barrier;
+        uint32_t* root = I.barrier + 32;
+        const int g = static_cast<int>(blockIdx.x) - I.group_first;
+        const int leaf = g % kLeaves;
+        const int leaves = G < kLeaves ? G : kLeaves;
+        // workgroups on this leaf: g' = leaf, leaf + kLeaves, ... < G
+        const uint32_t leaf_size = static_cast<uint32_t>((G - leaf + kLeaves - 1) / kLeaves);
+        uint32_t* mine = I.barrier + 64 + 32 * leaf;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const uint32_t seen = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool last = false;
+        if (__hip_atomic_fetch_add(mine, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1 == leaf_size) {
+            __hip_atomic_store(mine, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_fetch_add(root, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1 ==
+                static_cast<uint32_t>(leaves)) {
+                __hip_atomic_store(root, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(gen, seen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                last = true;
+            }
+        }
+        if (!last) {
             const long long start = wall_clock64();
-            while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == seen) {
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == seen) {
                 __builtin_amdgcn_s_sleep(1);
                 if (wall_clock64() - start > kBarrierTicks) {
                     timeouts = 1;
@@ -145,8 +184,9 @@ __device__ void group_sync(const Instance& I, int& timeouts) {
                 }
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
+    trace_stamp(I);
     __syncthreads();
 }
 
@@ -269,39 +309,62 @@ __device__ void vcycle(const Instance& I, const Values<VT>& V, const VT* rhs0, V
 
 // ------------------------------------------------------------ set_matrix
 
-// Gauss-Jordan inverse of the coarsest level (SPD, no pivoting needed),
-// in place in I.cinv, by one workgroup.
-__device__ void coarse_inverse(const Instance& I, Lds& lds) {
+// Inverse of the coarsest level by the symmetric sweep operator (one
+// workgroup). SWEEP(k) maps a symmetric A to B with
+//   B_kk = -1/A_kk, B_ik = A_ik / A_kk, B_ij = A_ij - A_ik A_kj / A_kk,
+// keeps symmetry, and sweeping every pivot of an SPD matrix gives -A^-1
+// (no pivoting needed). Only the packed lower triangle is stored, in
+// `tri` (LDS when it fits, else global scratch); the result is written
+// negated and fully expanded, column k contiguous, to I.cinv.
+__device__ inline int tri_index(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+
+__device__ void coarse_inverse(const Instance& I, double* tri, Lds& lds) {
     const int j = I.depth - 1;
     const LevelView& L = I.lv[j];
     const int n = L.n;
-    double* a = I.cinv;
+    const int packed = n * (n + 1) / 2;
     const double* diag = I.diag64[j];
     const double* off = I.off64[j];
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) a[e] = 0.0;
+    for (int e = threadIdx.x; e < packed; e += blockDim.x) tri[e] = 0.0;
     __syncthreads();
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        a[static_cast<size_t>(r) * n + r] = diag[r];
-        for (int k = L.row[r]; k < L.row[r + 1]; ++k) a[static_cast<size_t>(r) * n + L.col[k]] = off[k];
+        tri[tri_index(r, r)] = diag[r];
+        for (int k = L.row[r]; k < L.row[r + 1]; ++k)
+            if (L.col[k] < r) tri[tri_index(r, L.col[k])] = off[k];
     }
     __syncthreads();
+    // each thread owns a contiguous run of packed entries
+    const int per = (packed + blockDim.x - 1) / blockDim.x;
+    const int e0 = min(packed, static_cast<int>(threadIdx.x) * per), e1 = min(packed, e0 + per);
+    int i0 = static_cast<int>((sqrt(8.0 * e0 + 1.0) - 1.0) * 0.5);
+    while (i0 * (i0 + 1) / 2 > e0) --i0;
+    while ((i0 + 1) * (i0 + 2) / 2 <= e0) ++i0;
+    const int j0 = e0 - i0 * (i0 + 1) / 2;
     for (int k = 0; k < n; ++k) {
-        const double pivot_inv = 1.0 / a[static_cast<size_t>(k) * n + k];
-        for (int c = threadIdx.x; c < n; c += blockDim.x) {
-            lds.rowk[c] = (c == k ? 1.0 : a[static_cast<size_t>(k) * n + c]) * pivot_inv;
-            lds.colk[c] = a[static_cast<size_t>(c) * n + k];
-        }
+        for (int c = threadIdx.x; c < n; c += blockDim.x) lds.colk[c] = tri[tri_index(c, k)];
         __syncthreads();
-        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-            const int r = e / n, c = e - r * n;
-            if (r == k) {
-                a[e] = lds.rowk[c];
-            } else {
-                const double base = c == k ? 0.0 : a[e];
-                a[e] = base - lds.colk[r] * lds.rowk[c];
+        const double d = lds.colk[k];
+        const double dinv = 1.0 / d;
+        int i = i0, jj = j0;
+        for (int e = e0; e < e1; ++e) {
+            if (i == k && jj == k)
+                tri[e] = -dinv;
+            else if (i == k)
+                tri[e] = lds.colk[jj] * dinv;
+            else if (jj == k)
+                tri[e] = lds.colk[i] * dinv;
+            else
+                tri[e] -= lds.colk[i] * lds.colk[jj] * dinv;
+            if (++jj > i) {
+                ++i;
+                jj = 0;
             }
         }
         __syncthreads();
+    }
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int col = e / n, row = e - col * n;
+        I.cinv[e] = -tri[tri_index(row, col)];
     }
 }
 
@@ -379,7 +442,7 @@ __device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds
         const double lo = fmax(1.0 - rho, 0.25 * hi);
         if (g == 0 && threadIdx.x == 0) I.omega[j] = 2.0 / (lo + hi);
     }
-    if (g == 0) coarse_inverse(I, lds);
+    if (g == 0) coarse_inverse(I, I.tri_lds ? dynamic_lds() : I.tri, lds);
     group_sync(I, timeouts);
 }
 
@@ -633,6 +696,10 @@ __global__ __launch_bounds__(kTile) void program_kernel(const Instance* instance
     const int G = I.group_size;
     int timeouts = 0;
     int parity = 0;
+    if (I.trace != nullptr && threadIdx.x == 0 && g == 0) {
+        I.trace_count[0] = 1;
+        I.trace[0] = static_cast<uint64_t>(wall_clock64());
+    }
     if (P.op & kOpSetMatrix) set_matrix(I, P, g, G, lds, timeouts, parity);
     if (P.op & kOpPrecondition) {
         if (I.precision == 0) {
@@ -675,25 +742,37 @@ __global__ __launch_bounds__(kTile) void program_kernel(const Instance* instance
 }  // namespace
 
 hipError_t launch_program(const Instance* instances, const int32_t* block_instance, const Launch* launches,
-                          const Launch& single, int32_t blocks, hipStream_t stream, bool cooperative) {
+                          const Launch& single, int32_t blocks, size_t dynamic_lds, hipStream_t stream,
+                          bool cooperative) {
+    if (dynamic_lds > 0) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&program_kernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 static_cast<int>(dynamic_lds));
+        if (e != hipSuccess) return e;
+    }
     if (cooperative) {
         Launch copy = single;
         void* args[] = {const_cast<Instance**>(&instances), const_cast<int32_t**>(&block_instance),
                         const_cast<Launch**>(&launches), &copy};
         return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&program_kernel), dim3(blocks), dim3(kTile),
-                                          args, 0, stream);
+                                          args, static_cast<unsigned>(dynamic_lds), stream);
     }
-    hipLaunchKernelGGL(program_kernel, dim3(blocks), dim3(kTile), 0, stream, instances, block_instance, launches,
-                       single);
+    hipLaunchKernelGGL(program_kernel, dim3(blocks), dim3(kTile), dynamic_lds, stream, instances, block_instance,
+                       launches, single);
     return hipGetLastError();
 }
 
 }  // namespace psim::dev
 
 namespace psim::dev {
-int program_blocks_per_cu() {
+int program_blocks_per_cu(size_t dynamic_lds) {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, program_kernel, kTile, 0) != hipSuccess) return 1;
-    return blocks > 0 ? blocks : 1;
+    if (dynamic_lds > 0 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&program_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(dynamic_lds)) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, program_kernel, kTile, dynamic_lds) != hipSuccess)
+        return 0;
+    return blocks;
 }
 }  // namespace psim::dev
