@@ -97,6 +97,17 @@ typedef struct psim_ctx psim_ctx;
 int psim_ctx_create(int device, psim_ctx** out);
 void psim_ctx_destroy(psim_ctx* ctx);
 int psim_ctx_synchronize(psim_ctx* ctx);
+typedef struct psim_device_info {
+    int32_t compute_units;
+    int32_t program_blocks_per_cu;   /* co-resident solver workgroups per CU */
+    int32_t cooperative;             /* device-wide groups supported */
+    int32_t reserved;
+    uint64_t lds_per_block;
+    uint64_t lds_per_cu;
+    uint64_t hbm_bytes;
+    double wall_clock_hz;
+} psim_device_info;
+int psim_ctx_info(const psim_ctx* ctx, psim_device_info* info);
 int psim_device_alloc(psim_ctx* ctx, uint64_t bytes, void** out);
 int psim_device_free(psim_ctx* ctx, void* ptr);
 int psim_copy_to_device(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes);
@@ -160,6 +171,7 @@ typedef struct psim_solve_info {
     int32_t status;             /* PSIM_OK or PSIM_NOT_CONVERGED */
     int32_t v_cycles;           /* preconditioner applications (half-iterations) */
     double relative_residual;   /* true ||b - A x|| / ||b|| in FP64 */
+    double device_ms;           /* the solve's duration on the device clock */
 } psim_solve_info;
 
 int psim_system_create(psim_graph* graph, int precision, psim_system** out);
@@ -213,6 +225,17 @@ int psim_heat_set_matrix(psim_system* system, const double* conductance_per_entr
 int psim_heat_step(psim_system* system, const double* temperature_old, const double* forcing_W_m2, double dt_s,
                    double tolerance, int32_t max_iterations, double* temperature_new, psim_solve_info* info,
                    int where);
+
+/* A chain of `steps` heat steps in one device launch (config 1's monthly
+ * chain): step k solves with forcings[k * n ..] and the previous step's T,
+ * and the last step's T is written to temperature_new. When
+ * conductance_per_entry and capacity_area are both given, the same launch
+ * first assembles the matrix as psim_heat_set_matrix does. step_info, if
+ * not NULL, receives one record per step (synchronises). */
+int psim_heat_chain(psim_system* system, const double* conductance_per_entry, const double* capacity_area,
+                    const double* temperature_old, const double* forcings, int32_t steps, double dt_s,
+                    double tolerance, int32_t max_iterations, double* temperature_new, psim_solve_info* step_info,
+                    int where);
 
 #ifdef __cplusplus
 }

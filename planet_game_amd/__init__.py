@@ -44,9 +44,15 @@ _pu64 = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 _pu8 = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 
 
+class DeviceInfo(C.Structure):
+    _fields_ = [("compute_units", C.c_int32), ("program_blocks_per_cu", C.c_int32), ("cooperative", C.c_int32),
+                ("reserved", C.c_int32), ("lds_per_block", C.c_uint64), ("lds_per_cu", C.c_uint64),
+                ("hbm_bytes", C.c_uint64), ("wall_clock_hz", C.c_double)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("restarts", C.c_int32), ("status", C.c_int32),
-                ("v_cycles", C.c_int32), ("relative_residual", C.c_double)]
+                ("v_cycles", C.c_int32), ("relative_residual", C.c_double), ("device_ms", C.c_double)]
 
 
 def _sig(name, res, *args):
@@ -79,6 +85,7 @@ _sig("psim_device_alloc", C.c_int, _vp, _u64, C.POINTER(_vp))
 _sig("psim_device_free", C.c_int, _vp, _vp)
 _sig("psim_copy_to_device", C.c_int, _vp, _vp, _vp, _u64)
 _sig("psim_copy_to_host", C.c_int, _vp, _vp, _vp, _u64)
+_sig("psim_ctx_info", C.c_int, _vp, C.POINTER(DeviceInfo))
 _sig("psim_ctx_events", C.c_int, _vp, C.c_uint32)
 _sig("psim_ctx_record", C.c_int, _vp, C.c_uint32)
 _sig("psim_ctx_elapsed", C.c_int, _vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_float))
@@ -111,6 +118,7 @@ _sig("psim_system_apply", C.c_int, _vp, _vp, _vp, C.c_int)
 _sig("psim_system_solve", C.c_int, _vp, _vp, _vp, _dbl, _i32, C.POINTER(SolveInfo), C.c_int)
 _sig("psim_heat_set_matrix", C.c_int, _vp, _vp, _vp, _dbl, C.c_int)
 _sig("psim_heat_step", C.c_int, _vp, _vp, _vp, _dbl, _dbl, _i32, _vp, C.POINTER(SolveInfo), C.c_int)
+_sig("psim_heat_chain", C.c_int, _vp, _vp, _vp, _vp, _vp, _i32, _dbl, _dbl, _i32, _vp, _vp, C.c_int)
 
 
 class PsimError(RuntimeError):
@@ -235,6 +243,11 @@ class Context:
 
     def synchronize(self):
         check(lib.psim_ctx_synchronize(self.handle))
+
+    def info(self):
+        d = DeviceInfo()
+        check(lib.psim_ctx_info(self.handle, C.byref(d)))
+        return {k: getattr(d, k) for k, _ in DeviceInfo._fields_ if k != "reserved"}
 
     def events(self, count):
         check(lib.psim_ctx_events(self.handle, count))
@@ -437,6 +450,30 @@ class System:
         """Queue one solve on device-resident vectors (no synchronisation)."""
         check(lib.psim_heat_step(self.handle, t_old, forcing, dt, tolerance, max_iterations, t_new,
                                  None, DEVICE))
+
+    def heat_chain(self, t_old, forcings, dt=2.63e6, tolerance=1e-10, max_iterations=1000,
+                   conductance_per_entry=None, capacity_area=None):
+        """Chained steps on the device in one launch; forcings is steps x n.
+        With conductances and C*A given, the launch also sets the matrix."""
+        t = np.ascontiguousarray(t_old, np.float64)
+        f = np.ascontiguousarray(forcings, np.float64)
+        steps = f.size // t.size
+        out = np.empty_like(t)
+        infos = (SolveInfo * steps)()
+        k = None if conductance_per_entry is None else np.ascontiguousarray(conductance_per_entry, np.float64)
+        ca = None if capacity_area is None else np.ascontiguousarray(capacity_area, np.float64)
+        check(lib.psim_heat_chain(self.handle, _ptr(k) if k is not None else None,
+                                  _ptr(ca) if ca is not None else None, _ptr(t), _ptr(f), steps, dt, tolerance,
+                                  max_iterations, _ptr(out), C.cast(infos, _vp), HOST))
+        return out, list(infos)
+
+    def heat_chain_device(self, t_old, forcings, steps, t_new, dt=2.63e6, tolerance=1e-10,
+                          max_iterations=1000, step_infos=None, conductance_per_entry=None, capacity_area=None):
+        """Queue a chain on device-resident vectors (DeviceArray pointers);
+        synchronises only when step_infos (a ctypes SolveInfo array) is given."""
+        check(lib.psim_heat_chain(self.handle, conductance_per_entry, capacity_area, t_old, forcings, steps, dt,
+                                  tolerance, max_iterations, t_new,
+                                  C.cast(step_infos, _vp) if step_infos is not None else None, DEVICE))
 
     def last_info(self):
         info = SolveInfo()

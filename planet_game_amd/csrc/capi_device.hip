@@ -74,6 +74,7 @@ struct psim_graph {
     psim::Arena arena;
     double* area = nullptr;
     double* weight = nullptr;
+    int32_t* csr_col = nullptr;   // level-0 CSR columns (the diffusion sweep reads CSR order)
     LevelView views[kMaxLevels];
     ~psim_graph() { arena.release(); }
 };
@@ -94,6 +95,7 @@ struct psim_system {
     double* stage_a = nullptr;     // host-call staging (n)
     double* stage_b = nullptr;
     double* stage_c = nullptr;
+    double* csr_stage = nullptr;   // CSR-ordered off-diagonals for set_matrix
     bool matrix_set = false;
     bool heat_matrix = false;
     void* trace_block = nullptr;   // uint32 count + uint64 stamps
@@ -131,8 +133,9 @@ SolveResult fetch_result(psim_system* s) {
     return r;
 }
 
-void fill_info(const SolveResult& r, psim_solve_info* info) {
+void fill_info(const SolveResult& r, psim_solve_info* info, double clock_hz = 1.0e8) {
     if (info == nullptr) return;
+    info->device_ms = static_cast<double>(r.t_end - r.t_begin) / clock_hz * 1e3;
     info->iterations = r.iterations;
     info->restarts = r.restarts;
     info->status = r.status;
@@ -174,8 +177,13 @@ int psim_ctx_create(int device, psim_ctx** out) {
         ctx->compute_units = prop.multiProcessorCount;
         ctx->cooperative = prop.cooperativeLaunch != 0;
         ctx->blocks_per_cu = psim::dev::program_blocks_per_cu(0);
-        // leave room for the kernel's static LDS (under 8 KB)
-        ctx->lds_per_block = prop.sharedMemPerBlock > 12288 ? prop.sharedMemPerBlock - 12288 : 0;
+        int clock_khz = 0;
+        if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && clock_khz > 0)
+            ctx->wall_clock_hz = 1.0e3 * clock_khz;
+        ctx->static_lds = psim::dev::program_static_lds();
+        ctx->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
+        ctx->lds_per_block =
+            prop.sharedMemPerBlock > ctx->static_lds ? prop.sharedMemPerBlock - ctx->static_lds : 0;
         *out = ctx.release();
     });
 }
@@ -194,6 +202,22 @@ int psim_ctx_synchronize(psim_ctx* ctx) {
     return guarded([&] {
         if (ctx == nullptr) throw std::invalid_argument("null context");
         psim::hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+    });
+}
+
+int psim_ctx_info(const psim_ctx* ctx, psim_device_info* info) {
+    return guarded([&] {
+        if (ctx == nullptr || info == nullptr) throw std::invalid_argument("null argument");
+        hipDeviceProp_t prop{};
+        psim::hip_check(hipGetDeviceProperties(&prop, ctx->device), "device properties");
+        info->compute_units = ctx->compute_units;
+        info->program_blocks_per_cu = ctx->blocks_per_cu;
+        info->cooperative = ctx->cooperative ? 1 : 0;
+        info->reserved = 0;
+        info->lds_per_block = static_cast<uint64_t>(prop.sharedMemPerBlock);
+        info->lds_per_cu = static_cast<uint64_t>(prop.maxSharedMemoryPerMultiProcessor);
+        info->hbm_bytes = static_cast<uint64_t>(prop.totalGlobalMem);
+        info->wall_clock_hz = ctx->wall_clock_hz;
     });
 }
 
@@ -288,39 +312,46 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
         auto& A = gr->arena;
         A.plan(&gr->area, n);
         A.plan(&gr->weight, nnz);
-        int32_t* ptrs[kMaxLevels][10] = {};
+        A.plan(&gr->csr_col, nnz);
+        int32_t* ptrs[kMaxLevels][12] = {};
         for (int j = 0; j < gr->depth; ++j) {
             const psim::HostLevel& L = lv[j];
             const bool last = j + 1 == gr->depth;
             A.plan(&ptrs[j][0], L.row.size());
-            A.plan(&ptrs[j][1], L.col.size());
+            A.plan(&ptrs[j][1], L.col_ell.size());
             if (!last) {
                 A.plan(&ptrs[j][2], L.aggregate.size());
                 A.plan(&ptrs[j][3], L.agg_start.size());
                 A.plan(&ptrs[j][4], L.agg_rows.size());
                 A.plan(&ptrs[j][5], L.tile_start.size());
                 A.plan(&ptrs[j][6], L.g_diag_start.size());
-                A.plan(&ptrs[j][7], L.g_diag.size());
-                A.plan(&ptrs[j][8], L.g_off_start.size());
-                A.plan(&ptrs[j][9], L.g_off.size());
+                A.plan(&ptrs[j][7], L.d_diag.size());
+                A.plan(&ptrs[j][8], L.d_off_start.size());
+                A.plan(&ptrs[j][9], L.d_off.size());
+            }
+            if (!L.need_start.empty()) {
+                A.plan(&ptrs[j][10], L.need_start.size());
+                A.plan(&ptrs[j][11], L.need.size());
             }
         }
         A.allocate();
         psim::put(gr->area, area, n * 8);
         psim::put(gr->weight, weight, nnz * 8);
+        psim::put(gr->csr_col, g.col.data(), nnz * 4);
         for (int j = 0; j < gr->depth; ++j) {
             const psim::HostLevel& L = lv[j];
             const bool last = j + 1 == gr->depth;
             LevelView& v = gr->views[j];
             v.n = L.n;
             v.nnz = L.nnz;
+            v.width = L.width;
             psim::put(ptrs[j][0], L.row.data(), L.row.size() * 4);
-            psim::put(ptrs[j][1], L.col.data(), L.col.size() * 4);
+            psim::put(ptrs[j][1], L.col_ell.data(), L.col_ell.size() * 4);
             v.row = ptrs[j][0];
             v.col = ptrs[j][1];
             if (!last) {
                 const std::vector<int32_t>* src[8] = {&L.aggregate, &L.agg_start, &L.agg_rows, &L.tile_start,
-                                                      &L.g_diag_start, &L.g_diag, &L.g_off_start, &L.g_off};
+                                                      &L.g_diag_start, &L.d_diag, &L.d_off_start, &L.d_off};
                 for (int q = 0; q < 8; ++q) psim::put(ptrs[j][2 + q], src[q]->data(), src[q]->size() * 4);
                 v.agg = ptrs[j][2];
                 v.agg_start = ptrs[j][3];
@@ -332,6 +363,12 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
                 v.go_start = ptrs[j][8];
                 v.go = ptrs[j][9];
                 v.n_next = lv[j + 1].n;
+            }
+            if (!L.need_start.empty()) {
+                psim::put(ptrs[j][10], L.need_start.data(), L.need_start.size() * 4);
+                psim::put(ptrs[j][11], L.need.data(), L.need.size() * 4);
+                v.need_start = ptrs[j][10];
+                v.need = ptrs[j][11];
             }
         }
         *out = gr.release();
@@ -356,8 +393,7 @@ int psim_diffusion_source(psim_graph* g, double conductance, const double* tempe
             o = t + g->n;
             psim::hip_check(hipMemcpy(t, temperature, bytes, hipMemcpyHostToDevice), "staging copy");
         }
-        psim::dev::launch_diffusion(g->n, g->views[0].row, g->views[0].col, g->weight, g->area, conductance, t, o,
-                                    stream);
+        psim::dev::launch_diffusion(g->n, g->views[0].row, g->csr_col, g->weight, g->area, conductance, t, o, stream);
         if (where == PSIM_HOST) {
             const hipError_t e1 = hipMemcpyAsync(source, o, bytes, hipMemcpyDeviceToHost, stream);
             const hipError_t e2 = hipStreamSynchronize(stream);
@@ -382,10 +418,13 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         s->tri_bytes = nc_rows * (nc_rows + 1) / 2 * sizeof(double);
         int per_cu = ctx->blocks_per_cu;
         if (s->tri_bytes <= ctx->lds_per_block) {
-            const int with_lds = psim::dev::program_blocks_per_cu(s->tri_bytes);
-            if (with_lds > 0) {
+            // set_matrix launches carry the packed coarsest matrix in dynamic
+            // LDS; co-residency then follows from the CU's LDS
+            const size_t per_block = ctx->static_lds + s->tri_bytes;
+            const int by_lds = static_cast<int>(ctx->lds_per_cu / per_block);
+            if (by_lds > 0) {
                 s->host.tri_lds = 1;
-                per_cu = std::min(per_cu, with_lds);
+                per_cu = std::min(per_cu, by_lds);
             }
         }
         const int32_t cap = std::max(1, ctx->compute_units * per_cu);
@@ -405,14 +444,15 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         const bool mixed = precision == PSIM_MIXED;
         const size_t vt = mixed ? 4 : 8;
         for (int j = 0; j < g->depth; ++j) {
-            const size_t nj = static_cast<size_t>(g->views[j].n), zj = static_cast<size_t>(g->views[j].nnz);
+            const size_t nj = static_cast<size_t>(g->views[j].n);
+            const size_t slots = nj * static_cast<size_t>(g->views[j].width);
             A.plan(&I.diag64[j], nj);
             A.plan(&I.dinv64[j], nj);
-            A.plan(&I.off64[j], zj);
+            A.plan(&I.off64[j], slots);
             if (mixed) {
                 A.plan(&I.diag32[j], nj);
                 A.plan(&I.dinv32[j], nj);
-                A.plan(&I.off32[j], zj);
+                A.plan(&I.off32[j], slots);
             }
             if (j > 0) {
                 A.plan(reinterpret_cast<char**>(&I.lvl_b[j]), nj * vt);
@@ -423,10 +463,13 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         A.plan(&I.omega, kMaxLevels);
         A.plan(&I.cinv, nc * nc);
         A.plan(&I.b, n);
-        A.plan(&I.x, n);
-        void** vecs[8] = {&I.r, &I.r0, &I.p, &I.v, &I.ph, &I.s, &I.sh, &I.t};
+        A.plan(&I.x[0], n);
+        A.plan(&I.x[1], n);
+        void** vecs[9] = {&I.r, &I.r0, &I.p[0], &I.p[1], &I.v, &I.ph, &I.s, &I.sh, &I.t};
         for (void** v : vecs) A.plan(reinterpret_cast<char**>(v), static_cast<size_t>(n) * vt);
-        A.plan(&I.partial, 2 * static_cast<size_t>(tiles) * kPartials);
+        I.partial_slots = std::max(tiles, g->depth > 1 ? g->views[0].down_tiles : 0);
+        A.plan(&I.partial, 2 * static_cast<size_t>(I.partial_slots) * kPartials);
+        A.plan(&s->csr_stage, static_cast<size_t>(g->nnz));
         A.plan(&I.barrier, psim::dev::kBarrierWordCount);
         if (!I.tri_lds) A.plan(&I.tri, nc_rows * (nc_rows + 1) / 2);
         A.plan(&I.result, 1);
@@ -511,12 +554,18 @@ int psim_system_level_values(psim_system* s, uint64_t level, double* diagonal, d
         if (level >= static_cast<uint64_t>(s->graph->depth)) throw std::invalid_argument("no such level");
         hipStream_t stream = s->graph->ctx->stream;
         const LevelView& v = s->graph->views[level];
+        const psim::HostLevel& H = s->graph->host.levels[level];
         if (diagonal)
             psim::hip_check(hipMemcpyAsync(diagonal, s->host.diag64[level], static_cast<size_t>(v.n) * 8,
                                            hipMemcpyDeviceToHost, stream), "copy");
-        if (off)
-            psim::hip_check(hipMemcpyAsync(off, s->host.off64[level], static_cast<size_t>(v.nnz) * 8,
-                                           hipMemcpyDeviceToHost, stream), "copy");
+        if (off) {
+            std::vector<double> ell(static_cast<size_t>(v.n) * static_cast<size_t>(v.width));
+            psim::hip_check(hipMemcpyAsync(ell.data(), s->host.off64[level], ell.size() * 8, hipMemcpyDeviceToHost,
+                                           stream), "copy");
+            psim::hip_check(hipStreamSynchronize(stream), "level values");
+            for (int32_t i = 0; i < H.n; ++i)
+                for (int32_t k = H.row[i]; k < H.row[i + 1]; ++k) off[k] = ell[psim::ell_position(H, i, k)];
+        }
         psim::hip_check(hipStreamSynchronize(stream), "level values");
     });
 }
@@ -531,9 +580,10 @@ int psim_system_set_matrix(psim_system* s, const double* diagonal, uint64_t n, c
         hipStream_t stream = s->graph->ctx->stream;
         const hipMemcpyKind kind = where == PSIM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
         psim::hip_check(hipMemcpyAsync(s->host.diag64[0], diagonal, n * 8, kind, stream), "matrix copy");
-        psim::hip_check(hipMemcpyAsync(s->host.off64[0], off, nnz * 8, kind, stream), "matrix copy");
+        psim::hip_check(hipMemcpyAsync(s->csr_stage, off, nnz * 8, kind, stream), "matrix copy");
         Launch p{};
-        p.op = psim::dev::kOpSetMatrix;
+        p.op = psim::dev::kOpSetMatrix | psim::dev::kOpCsrValues;
+        p.in = s->csr_stage;
         launch(s, p);
         if (where == PSIM_HOST) psim::hip_check(hipStreamSynchronize(stream), "set_matrix");
         s->matrix_set = true;
@@ -600,22 +650,23 @@ int psim_system_solve(psim_system* s, const double* b, double* x, double toleran
         const size_t bytes = static_cast<size_t>(s->graph->n) * 8;
         Launch p = solve_params(s, psim::dev::kOpSolve, tolerance, max_iterations);
         p.in = b;
+        p.out = x;
         if (where == PSIM_HOST) {
             psim::hip_check(hipMemcpyAsync(s->stage_a, b, bytes, hipMemcpyHostToDevice, stream), "copy");
             p.in = s->stage_a;
+            p.out = s->stage_b;
         }
         launch(s, p);
-        psim::hip_check(hipMemcpyAsync(where == PSIM_HOST ? x : x, s->host.x, bytes,
-                                       where == PSIM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, stream),
-                        "copy");
-        if (where == PSIM_HOST || info != nullptr) fill_info(fetch_result(s), info);
+        if (where == PSIM_HOST)
+            psim::hip_check(hipMemcpyAsync(x, s->stage_b, bytes, hipMemcpyDeviceToHost, stream), "copy");
+        if (where == PSIM_HOST || info != nullptr) fill_info(fetch_result(s), info, s->graph->ctx->wall_clock_hz);
     });
 }
 
 int psim_system_last_info(psim_system* s, psim_solve_info* info) {
     return guarded([&] {
         if (s == nullptr || info == nullptr) throw std::invalid_argument("null argument");
-        fill_info(fetch_result(s), info);
+        fill_info(fetch_result(s), info, s->graph->ctx->wall_clock_hz);
     });
 }
 
@@ -672,7 +723,89 @@ int psim_heat_step(psim_system* s, const double* t_old, const double* forcing, d
         launch(s, p);
         if (where == PSIM_HOST)
             psim::hip_check(hipMemcpyAsync(t_new, s->stage_c, bytes, hipMemcpyDeviceToHost, stream), "copy");
-        if (where == PSIM_HOST || info != nullptr) fill_info(fetch_result(s), info);
+        if (where == PSIM_HOST || info != nullptr) fill_info(fetch_result(s), info, s->graph->ctx->wall_clock_hz);
+    });
+}
+
+int psim_heat_chain(psim_system* s, const double* conductance_per_entry, const double* capacity_area,
+                    const double* t_old, const double* forcings, int32_t steps, double dt, double tolerance,
+                    int32_t max_iterations, double* t_new, psim_solve_info* step_info, int where) {
+    return guarded([&] {
+        if (s == nullptr) throw std::invalid_argument("null system");
+        const bool assemble = conductance_per_entry != nullptr && capacity_area != nullptr;
+        if (!assemble) {
+            require_matrix(s);
+            if (!s->heat_matrix) throw std::logic_error("psim_heat_set_matrix has not been called");
+        }
+        psim::check_where(where);
+        if (t_old == nullptr || forcings == nullptr || t_new == nullptr) throw std::invalid_argument("null argument");
+        if (steps < 1) throw std::invalid_argument("a chain needs at least one step");
+        if (!(dt > 0.0) || !std::isfinite(dt)) throw std::invalid_argument("time step must be finite and positive");
+        if (!(tolerance >= 0.0) || max_iterations < 0) throw std::invalid_argument("invalid solver settings");
+        hipStream_t stream = s->graph->ctx->stream;
+        const size_t n = static_cast<size_t>(s->graph->n), bytes = n * 8;
+        int op = psim::dev::kOpSolve | psim::dev::kOpHeatRhs | psim::dev::kOpChain;
+        if (assemble) {
+            const hipMemcpyKind kind = where == PSIM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+            psim::hip_check(hipMemcpyAsync(s->entry_conductance, conductance_per_entry,
+                                           static_cast<size_t>(s->graph->nnz) * 8, kind, stream), "copy");
+            psim::hip_check(hipMemcpyAsync(s->capacity_area, capacity_area, bytes, kind, stream), "copy");
+            op |= psim::dev::kOpSetMatrix | psim::dev::kOpHeatAssemble;
+        }
+        Launch p = solve_params(s, op, tolerance, max_iterations);
+        p.steps = steps;
+        p.dt = dt;
+        p.entry_conductance = s->entry_conductance;
+        p.capacity_area = s->capacity_area;
+        p.t_old = t_old;
+        p.forcing = forcings;
+        p.t_new = t_new;
+        std::vector<void*> staged;
+        struct Free {
+            std::vector<void*>& v;
+            ~Free() {
+                for (void* d : v) (void)hipFree(d);
+            }
+        } release{staged};
+        const auto stage = [&](size_t b) {
+            void* d = nullptr;
+            psim::hip_check(hipMalloc(&d, b), "staging allocation");
+            staged.push_back(d);
+            return d;
+        };
+        if (where == PSIM_HOST) {
+            double* d_t = static_cast<double*>(stage(bytes));
+            double* d_f = static_cast<double*>(stage(bytes * static_cast<size_t>(steps)));
+            psim::hip_check(hipMemcpyAsync(d_t, t_old, bytes, hipMemcpyHostToDevice, stream), "copy");
+            psim::hip_check(hipMemcpyAsync(d_f, forcings, bytes * static_cast<size_t>(steps), hipMemcpyHostToDevice,
+                                           stream), "copy");
+            p.t_old = d_t;
+            p.forcing = d_f;
+            p.t_new = s->stage_c;
+        }
+        SolveResult* d_steps = nullptr;
+        if (step_info != nullptr) {
+            d_steps = static_cast<SolveResult*>(stage(sizeof(SolveResult) * static_cast<size_t>(steps)));
+            p.step_results = d_steps;
+        }
+        launch(s, p);
+        if (assemble) {
+            s->matrix_set = true;
+            s->heat_matrix = true;
+        }
+        if (where == PSIM_HOST)
+            psim::hip_check(hipMemcpyAsync(t_new, s->stage_c, bytes, hipMemcpyDeviceToHost, stream), "copy");
+        if (step_info != nullptr) {
+            std::vector<SolveResult> host(static_cast<size_t>(steps));
+            psim::hip_check(hipMemcpyAsync(host.data(), d_steps, host.size() * sizeof(SolveResult),
+                                           hipMemcpyDeviceToHost, stream), "copy");
+            (void)fetch_result(s);
+            for (int32_t k = 0; k < steps; ++k)
+                fill_info(host[static_cast<size_t>(k)], step_info + k, s->graph->ctx->wall_clock_hz);
+        } else if (!staged.empty()) {
+            // staged buffers must outlive the launch
+            (void)fetch_result(s);
+        }
     });
 }
 

@@ -27,26 +27,32 @@ constexpr int32_t kNotConverged = 5;   // PSIM_NOT_CONVERGED
 
 enum Op : int32_t {
     kOpSetMatrix = 1,     // Galerkin values, smoother weights, coarse inverse
-    kOpHeatAssemble = 2,  // diag/off from entry conductances, C A and dt (with kOpSetMatrix)
+    kOpHeatAssemble = 2,  // level-0 values from entry conductances, C A and dt (with kOpSetMatrix)
     kOpSolve = 4,         // BiCGSTAB with true-residual restarts on b
     kOpHeatRhs = 8,       // b = C A T_old + dt A S (with kOpSolve); T_new = x
     kOpPrecondition = 16, // out = V-cycle(in)
-    kOpApply = 32         // out = A in (FP64)
+    kOpApply = 32,        // out = A in (FP64)
+    kOpCsrValues = 64,    // level-0 off-diagonals given in CSR order in `in` (with kOpSetMatrix)
+    kOpChain = 128        // `steps` heat steps, each T feeding the next (with kOpSolve | kOpHeatRhs)
 };
 
-// Structure of one multigrid level (per graph).
+// Structure of one multigrid level (per graph). Off-diagonals are stored
+// slot-major ELL: entry s of row i at s * n + i, in the row's CSR order,
+// padding marked by column -1.
 struct LevelView {
-    int32_t n = 0, nnz = 0, n_next = 0, down_tiles = 0;
-    const int32_t* row = nullptr;
-    const int32_t* col = nullptr;
+    int32_t n = 0, nnz = 0, width = 0, n_next = 0, down_tiles = 0;
+    const int32_t* row = nullptr;         // CSR offsets, n + 1 (CSR-ordered inputs)
+    const int32_t* col = nullptr;         // ELL columns, width * n
     const int32_t* agg = nullptr;         // n (not on the coarsest level)
     const int32_t* agg_start = nullptr;   // n_next + 1
-    const int32_t* agg_rows = nullptr;    // n
+    const int32_t* agg_rows = nullptr;    // n, aggregate-major
     const int32_t* tile_start = nullptr;  // down_tiles + 1 (coarse-row ranges)
-    const int32_t* gd_start = nullptr;    // Galerkin lists of level j+1
-    const int32_t* gd = nullptr;
-    const int32_t* go_start = nullptr;
+    const int32_t* gd_start = nullptr;    // Galerkin lists of level j+1's diagonal
+    const int32_t* gd = nullptr;          //   (ELL positions, or -(row+1) for a fine diagonal)
+    const int32_t* go_start = nullptr;    // ... and of level j+1's ELL positions
     const int32_t* go = nullptr;
+    const int32_t* need_start = nullptr;  // level above the coarsest: coarse rows each tile reads
+    const int32_t* need = nullptr;
 };
 
 struct SolveResult {
@@ -58,6 +64,7 @@ struct SolveResult {
     double b_norm;
     int32_t v_cycles;
     int32_t pad;
+    uint64_t t_begin, t_end;     // device wall clock at the solve's start and end
 };
 
 // Everything one instance's program touches. Lives in device memory; every
@@ -65,10 +72,11 @@ struct SolveResult {
 struct Instance {
     int32_t precision = 0;       // 0 = FP64 inner vectors, 1 = FP32
     int32_t depth = 0;
-    int32_t n = 0, nnz = 0, tiles = 0;   // fine level
+    int32_t n = 0, nnz = 0, tiles = 0;   // fine level; tiles of 256 rows
+    int32_t partial_slots = 0;   // tiles per partial buffer (>= tiles, >= level-0 restriction tiles)
     int32_t group_first = 0, group_size = 1;   // workgroups [first, first+size)
     LevelView lv[kMaxLevels];
-    // values (per system)
+    // values (per system), off-diagonals in ELL
     double* diag64[kMaxLevels] = {};
     double* dinv64[kMaxLevels] = {};
     double* off64[kMaxLevels] = {};
@@ -76,16 +84,16 @@ struct Instance {
     float* dinv32[kMaxLevels] = {};
     float* off32[kMaxLevels] = {};
     double* omega = nullptr;     // smoother weight per level
-    double* cinv = nullptr;      // dense inverse of the coarsest level, n_c x n_c
+    double* cinv = nullptr;      // dense inverse of the coarsest level, column k contiguous
     // graph data
     const double* area = nullptr;
-    const double* weight = nullptr;
+    const double* weight = nullptr;   // CSR order
     // vectors
     double* b = nullptr;
-    double* x = nullptr;
+    double* x[2] = {};           // FP64 iterate, double-buffered for fused updates
     void* r = nullptr;
     void* r0 = nullptr;
-    void* p = nullptr;
+    void* p[2] = {};             // search direction, double-buffered
     void* v = nullptr;
     void* ph = nullptr;
     void* s = nullptr;
@@ -95,8 +103,8 @@ struct Instance {
     void* lvl_e[kMaxLevels] = {};   // V-cycle corrections, levels >= 1
     double* tri = nullptr;          // packed coarsest matrix when it does not fit in LDS
     int32_t tri_lds = 0;            // 1: set_matrix launches carry it in dynamic LDS
-    double* partial = nullptr;      // 2 x tiles x kPartials (double-buffered)
-    uint32_t* barrier = nullptr;    // group barrier words (kBarrierWords, zeroed)
+    double* partial = nullptr;      // 2 x partial_slots x kPartials (double-buffered)
+    uint32_t* barrier = nullptr;    // group barrier words (kBarrierWordCount, zeroed)
     SolveResult* result = nullptr;
     uint64_t* trace = nullptr;      // phase timestamps (wall clock), or null
     uint32_t* trace_count = nullptr;
@@ -109,16 +117,17 @@ struct Instance {
 struct Launch {
     int32_t op;
     int32_t max_iterations, max_restarts;
-    int32_t pad;
+    int32_t steps;               // kOpChain: number of chained steps
     double tolerance, inner_relative;
     double dt;
     const double* entry_conductance;   // K per CSR entry
     const double* capacity_area;       // C A per cell
     const double* t_old;
-    const double* forcing;
+    const double* forcing;             // kOpChain: steps x n, step-major
     double* t_new;
-    const double* in;   // b for kOpSolve, input of precondition / apply
+    const double* in;   // b for kOpSolve, input of precondition / apply, CSR values
     double* out;        // output of precondition / apply
+    SolveResult* step_results;         // kOpChain: one record per step, or null
 };
 
 // Launch the persistent program. Workgroup w serves instance
@@ -131,6 +140,8 @@ hipError_t launch_program(const Instance* instances, const int32_t* block_instan
 // Co-resident program workgroups per compute unit with `dynamic_lds`
 // bytes of dynamic LDS (0 if the kernel cannot launch with that much).
 int program_blocks_per_cu(size_t dynamic_lds);
+// The program kernel's static LDS bytes.
+size_t program_static_lds();
 
 constexpr int kBarrierWordCount = 64 + 32 * 8;   // generation, root, 8 leaves (128-byte apart)
 
@@ -147,6 +158,9 @@ struct psim_ctx {
     int compute_units = 0;
     int blocks_per_cu = 0;      // co-resident program workgroups per CU
     size_t lds_per_block = 0;   // largest dynamic LDS a program workgroup may use
+    size_t lds_per_cu = 0;
+    size_t static_lds = 0;      // the program kernel's static LDS
+    double wall_clock_hz = 1.0e8;
     bool cooperative = false;   // cooperative launch supported
 };
 

@@ -217,6 +217,23 @@ HostHierarchy build_hierarchy(const HostGraph& g) {
     }
     for (HostLevel& lv : h.levels) ell_columns(lv);
     for (size_t j = 0; j + 1 < h.levels.size(); ++j) ell_galerkin(h.levels[j], h.levels[j + 1]);
+    if (h.levels.size() >= 2) {
+        HostLevel& lv = h.levels[h.levels.size() - 2];
+        const int32_t tiles = (lv.n + tile_rows - 1) / tile_rows;
+        lv.need_start.assign(static_cast<size_t>(tiles) + 1, 0);
+        lv.need.clear();
+        for (int32_t t = 0; t < tiles; ++t) {
+            std::vector<int32_t> rows;
+            for (int32_t i = t * tile_rows; i < std::min(lv.n, (t + 1) * tile_rows); ++i) {
+                rows.push_back(lv.aggregate[i]);
+                for (int32_t k = lv.row[i]; k < lv.row[i + 1]; ++k) rows.push_back(lv.aggregate[lv.col[k]]);
+            }
+            std::sort(rows.begin(), rows.end());
+            rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+            lv.need.insert(lv.need.end(), rows.begin(), rows.end());
+            lv.need_start[t + 1] = static_cast<int32_t>(lv.need.size());
+        }
+    }
     return h;
 }
 

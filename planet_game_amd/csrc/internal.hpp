@@ -121,6 +121,10 @@ struct HostLevel {
     std::vector<int32_t> col_ell;
     std::vector<int32_t> d_diag;                 // g_diag in ELL positions
     std::vector<int32_t> d_off_start, d_off;     // per next-level ELL position
+    // On the level just above the coarsest: for each 256-row tile, the
+    // coarse rows its prolongation reads (aggregates of its rows and of
+    // their neighbours), sorted.
+    std::vector<int32_t> need_start, need;
 };
 
 // ELL position of CSR entry k of row i.

@@ -1,31 +1,36 @@
 // The persistent solver program for gfx950.
 //
 // One launch runs whole operations (set_matrix, a V-cycle, a complete
-// BiCGSTAB solve with its true-residual check) for a batch of instances.
-// Each instance owns a group of workgroups; the group walks the
-// instance's phases (RHS assembly, SpMV, V-cycle levels, vector updates)
-// tile by tile, 256 rows per tile and one row per thread, and meets at a
-// group barrier between phases that read what the previous one wrote. With
-// one workgroup per instance the barrier is a plain __syncthreads.
+// BiCGSTAB solve with its true-residual check, or a chain of heat steps)
+// for a batch of instances. Each instance owns a group of workgroups; the
+// group walks the instance's phases tile by tile (256 rows per tile, one
+// row per thread) and meets at a group barrier between phases that read
+// what the previous one wrote. With one workgroup per instance the barrier
+// is a plain __syncthreads.
 //
 // The algorithm is the reference's (heat_transport.cpp:820-897 BiCGSTAB,
 // right-preconditioned, from x = 0; V(1,1) aggregation multigrid of
-// :288-326 with Galerkin values of :157-171), with these device forms:
+// :288-326 with Galerkin values of :157-171), in these device forms:
 //   * smoothing is Chebyshev-weighted Jacobi (weight 2/(a+b) on the
-//     Gershgorin interval of D^-1 A, lower end clipped at b/4) instead of
-//     block Gauss-Seidel, and the pre-smoothing, residual and restriction
-//     of a level are one sweep: residuals land in LDS in aggregate-major
-//     order and each coarse row sums its fine rows in ascending order;
-//   * prolongation is fused with post-smoothing;
-//   * the coarsest level is solved with its dense inverse (Gauss-Jordan
-//     at set_matrix time) as a GEMV;
-//   * inner products reduce each 256-row tile with a fixed butterfly, then
-//     the tile partials in a fixed order, so every workgroup of a group
-//     computes the same scalars and results do not depend on the group
-//     size or on the batch;
-//   * after BiCGSTAB stops, the FP64 residual b - A x is recomputed and the
-//     solve restarts from x if it is above tolerance (the mixed-precision
-//     form relies on this refinement).
+//     Gershgorin interval [a, b] of D^-1 A, a clipped at b/4) instead of
+//     block Gauss-Seidel; pre-smoothing from zero, the residual and the
+//     restriction of a level are one sweep: residuals are staged in LDS in
+//     aggregate-major order and each coarse row sums its fine rows in
+//     ascending order (the reference's restriction order);
+//   * prolongation is fused with post-smoothing, and the coarsest level's
+//     dense solve (a GEMV with its inverse, built at set_matrix time by the
+//     symmetric sweep operator) is recomputed into LDS by each workgroup of
+//     the last prolongation sweep instead of costing a phase of its own;
+//   * the vector updates feeding a V-cycle (p = r + beta (p - omega v),
+//     s = r - alpha v) are evaluated inside its first sweep (neighbours
+//     recompute them; each row stores its own), and the x update after a
+//     half-step exit is fused with the FP64 true-residual SpMV;
+//   * inner products reduce each tile with a fixed butterfly, then the
+//     tile partials in a fixed order, so every workgroup computes the same
+//     scalars and results do not depend on the group size or the batch;
+//   * BiCGSTAB stops on the recursive residual as the reference does, then
+//     the FP64 residual b - A x is recomputed; above tolerance the solve
+//     restarts from x (the mixed-precision form relies on this refinement).
 #include "device.hpp"
 
 namespace psim::dev {
@@ -45,10 +50,12 @@ __device__ inline double wave_max(double v) {
 }
 
 struct Lds {
-    double red[kPartials][4];   // per-wave partials
+    double red[kPartials][4];        // per-wave partials
     double bcast[kPartials];
-    double tile[kTile];         // restriction staging
-    double colk[kMaxCoarse];    // sweep pivot column
+    double tile[kTile];              // restriction staging
+    double cvec[kMaxCoarse];         // coarse right-hand side, then solution
+    double crhs[kMaxCoarse];         // coarse right-hand side (FP64 staging)
+    double colk[kMaxCoarse];         // sweep pivot column (set_matrix)
 };
 
 // Dynamic LDS: the packed coarsest matrix during set_matrix launches.
@@ -57,9 +64,8 @@ __device__ inline double* dynamic_lds() {
     return dyn;
 }
 
-// Sum of K per-thread values over the workgroup (one 256-row tile), fixed
-// order: wave butterflies, then waves 0..3 left to right. Thread 0 returns
-// the totals.
+// Sum of K per-thread values over the workgroup, fixed order: wave
+// butterflies, then waves 0..3 left to right. Thread 0 gets the totals.
 template <int K> __device__ inline void tile_sums(const double (&v)[K], double (&out)[K], Lds& lds) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -75,6 +81,15 @@ template <int K> __device__ inline void tile_sums(const double (&v)[K], double (
     __syncthreads();
 }
 
+template <int K> __device__ inline void store_partials(double* part, int tile, const double (&v)[K], Lds& lds) {
+    double tot[K];
+    tile_sums<K>(v, tot, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) part[tile * kPartials + q] = tot[q];
+    }
+}
+
 __device__ inline double tile_max(double v, Lds& lds) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const double w = wave_max(v);
@@ -86,7 +101,7 @@ __device__ inline double tile_max(double v, Lds& lds) {
     return out;
 }
 
-// Total of slot q over all tiles (fixed order), broadcast to the workgroup.
+// Totals of slots 0..K-1 over `tiles` tiles (fixed order), to every thread.
 template <int K> __device__ inline void group_totals(const double* partial, int tiles, double (&out)[K], Lds& lds) {
     if (threadIdx.x < 64) {
 #pragma unroll
@@ -118,6 +133,16 @@ __device__ inline double group_max(const double* partial, int tiles, Lds& lds) {
 
 // ------------------------------------------------------------ group barrier
 
+// Phase trace: workgroup 0 of a traced instance stamps the wall clock at
+// every group barrier (after it completes) into I.trace.
+__device__ inline void trace_stamp(const Instance& I) {
+    if (I.trace != nullptr && threadIdx.x == 0 && static_cast<int>(blockIdx.x) == I.group_first) {
+        const uint32_t slot = I.trace_count[0];
+        if (slot < static_cast<uint32_t>(I.trace_capacity)) I.trace[slot] = static_cast<uint64_t>(wall_clock64());
+        I.trace_count[0] = slot + 1;
+    }
+}
+
 // Two-level counting barrier over the instance's workgroups. Workgroup g
 // arrives on leaf counter g % kLeaves (separate 128-byte lines, so the
 // arrivals do not serialise on one address); the last arrival of a leaf
@@ -129,22 +154,12 @@ __device__ inline double group_max(const double* partial, int tiles, Lds& lds) {
 // sticky, so a broken group drains within seconds and the host reports it.
 //
 // Barrier words (uint32, zeroed at creation, left zeroed by each episode):
-//   [0]       generation
-//   [32]      root arrivals
+//   [0]         generation
+//   [32]        root arrivals
 //   [64 + 32 l] arrivals of leaf l
 constexpr long long kBarrierTicks = 100000000;
 constexpr int kLeaves = 8;
-constexpr int kBarrierWords = 64 + 32 * kLeaves;
-
-// Phase trace: workgroup 0 of a traced instance stamps the wall clock at
-// every group barrier (after it completes) into I.trace.
-__device__ inline void trace_stamp(const Instance& I) {
-    if (I.trace != nullptr && threadIdx.x == 0 && static_cast<int>(blockIdx.x) == I.group_first) {
-        const uint32_t slot = I.trace_count[0];
-        if (slot < static_cast<uint32_t>(I.trace_capacity)) I.trace[slot] = static_cast<uint64_t>(wall_clock64());
-        I.trace_count[0] = slot + 1;
-    }
-}
+static_assert(64 + 32 * kLeaves == kBarrierWordCount, "barrier layout");
 
 __device__ void group_sync(const Instance& I, int& timeouts) {
     __syncthreads();
@@ -159,7 +174,6 @@ __device__ void group_sync(const Instance& I, int& timeouts) {
         const int g = static_cast<int>(blockIdx.x) - I.group_first;
         const int leaf = g % kLeaves;
         const int leaves = G < kLeaves ? G : kLeaves;
-        // workgroups on this leaf: g' = leaf, leaf + kLeaves, ... < G
         const uint32_t leaf_size = static_cast<uint32_t>((G - leaf + kLeaves - 1) / kLeaves);
         uint32_t* mine = I.barrier + 64 + 32 * leaf;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -205,16 +219,36 @@ template <> struct Values<float> {
     __device__ const float* dinv(int j) const { return I.dinv32[j]; }
     __device__ const float* off(int j) const { return I.off32[j]; }
 };
-template <class VT> __device__ inline Values<VT> values_of(const Instance& I) { return Values<VT>{I}; }
+
+// The level-0 input of a V-cycle, evaluated where it is read:
+//   kind 0: a;  kind 1: a + c1 (b - c2 c)  (p update);  kind 2: a - c1 b  (s)
+// in FP64 and rounded to VT, exactly as the stand-alone update would.
+template <class VT> struct Source {
+    const VT* a;
+    const VT* b;
+    const VT* c;
+    double c1, c2;
+    int kind;
+    __device__ VT at(int k) const {
+        if (kind == 0) return a[k];
+        if (kind == 1)
+            return static_cast<VT>(static_cast<double>(a[k]) +
+                                   c1 * (static_cast<double>(b[k]) - c2 * static_cast<double>(c[k])));
+        return static_cast<VT>(static_cast<double>(a[k]) - c1 * static_cast<double>(b[k]));
+    }
+};
 
 // ------------------------------------------------------------ V-cycle phases
 
 // Pre-smoothing from zero, residual and restriction of level j in one sweep
-// over the level's restriction tiles.
+// over the level's restriction tiles. On level 0 the input comes from a
+// Source; each row stores its value in `keep` (if set) and adds its square
+// to the tile partial (if `part` is set).
 template <class VT>
-__device__ void down_phase(const Instance& I, const Values<VT>& V, int j, const VT* rhs, VT* coarse_rhs, int g, int G,
-                           Lds& lds) {
+__device__ void down_phase(const Instance& I, const Values<VT>& V, int j, const Source<VT>& src, VT* keep,
+                           VT* coarse_rhs, double* part, int g, int G, Lds& lds) {
     const LevelView& L = I.lv[j];
+    const int n = L.n, W = L.width;
     const VT om = static_cast<VT>(I.omega[j]);
     const VT* diag = V.diag(j);
     const VT* dinv = V.dinv(j);
@@ -222,87 +256,153 @@ __device__ void down_phase(const Instance& I, const Values<VT>& V, int j, const 
     for (int tile = g; tile < L.down_tiles; tile += G) {
         const int c0 = L.tile_start[tile], c1 = L.tile_start[tile + 1];
         const int base = L.agg_start[c0], count = L.agg_start[c1] - base;
+        double sq[1] = {0.0};
         if (static_cast<int>(threadIdx.x) < count) {
             const int i = L.agg_rows[base + threadIdx.x];
-            const VT xi = om * dinv[i] * rhs[i];
-            VT acc = rhs[i] - diag[i] * xi;
-            for (int k = L.row[i]; k < L.row[i + 1]; ++k) {
-                const int c = L.col[k];
-                acc -= off[k] * (om * dinv[c] * rhs[c]);
+            const VT wi = src.at(i);
+            if (keep != nullptr) keep[i] = wi;
+            sq[0] = static_cast<double>(wi) * static_cast<double>(wi);
+            const VT xi = om * dinv[i] * wi;
+            VT acc = wi - diag[i] * xi;
+            for (int s = 0; s < W; ++s) {
+                const int c = L.col[s * n + i];
+                if (c >= 0) acc -= off[s * n + i] * (om * dinv[c] * src.at(c));
             }
             lds.tile[threadIdx.x] = static_cast<double>(acc);
         }
+        if (part != nullptr) store_partials<1>(part, tile, sq, lds);
         __syncthreads();
         if (static_cast<int>(threadIdx.x) < c1 - c0) {
             const int c = c0 + threadIdx.x;
-            double s = 0.0;
-            for (int q = L.agg_start[c] - base; q < L.agg_start[c + 1] - base; ++q) s += lds.tile[q];
-            coarse_rhs[c] = static_cast<VT>(s);
+            double sum = 0.0;
+            for (int q = L.agg_start[c] - base; q < L.agg_start[c + 1] - base; ++q) sum += lds.tile[q];
+            coarse_rhs[c] = static_cast<VT>(sum);
         }
         __syncthreads();
     }
 }
 
-// Dense solve of the coarsest level with its inverse (FP64 GEMV).
+// Rows [list[0..count)) of the coarsest level's solution A_c^-1 rhs into
+// lds.cvec (indexed by coarse row), in FP64. One wave per row: lanes split
+// the columns (row-major reads of the symmetric inverse, coalesced) and a
+// butterfly sums them, so every workgroup computing a row gets the same
+// bits. `list` null means rows first .. first + count - 1.
 template <class VT>
-__device__ void coarse_phase(const Instance& I, int j, const VT* rhs, VT* out, int g, int G) {
-    const int n = I.lv[j].n;
+__device__ void coarse_rows_lds(const Instance& I, const VT* rhs, const int32_t* list, int first, int count,
+                                Lds& lds) {
+    const int n = I.lv[I.depth - 1].n;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) lds.crhs[k] = static_cast<double>(rhs[k]);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const double* inv = I.cinv;
-    const int tiles = (n + kTile - 1) / kTile;
-    for (int tile = g; tile < tiles; tile += G) {
-        const int i = tile * kTile + threadIdx.x;
-        if (i < n) {
-            double s = 0.0;
-            for (int k = 0; k < n; ++k) s += inv[static_cast<size_t>(k) * n + i] * static_cast<double>(rhs[k]);
-            out[i] = static_cast<VT>(s);
+    constexpr int kRows = 4;   // rows in flight per wave
+    for (int q0 = wave * kRows; q0 < count; q0 += 4 * kRows) {
+        double acc[kRows];
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            acc[u] = 0.0;
+            const int q = q0 + u;
+            if (q < count) {
+                const int a = list != nullptr ? list[q] : first + q;
+                const double* row = inv + static_cast<size_t>(a) * n;
+                for (int k = lane; k < n; k += 64) acc[u] += row[k] * lds.crhs[k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const double total = wave_sum(acc[u]);
+            const int q = q0 + u;
+            if (lane == 0 && q < count) lds.cvec[list != nullptr ? list[q] : first + q] = total;
         }
     }
+    __syncthreads();
 }
 
-// Prolongation of the coarse correction fused with post-smoothing.
+// Prolongation of the coarse correction fused with post-smoothing on level
+// j. The correction comes from `corr`, or, on the level above the coarsest,
+// from the coarse solve each workgroup recomputes into LDS.
 template <class VT>
 __device__ void up_phase(const Instance& I, const Values<VT>& V, int j, const VT* rhs, const VT* corr, VT* out, int g,
-                         int G) {
+                         int G, Lds& lds) {
     const LevelView& L = I.lv[j];
+    const int n = L.n, W = L.width;
     const VT om = static_cast<VT>(I.omega[j]);
     const VT* diag = V.diag(j);
     const VT* dinv = V.dinv(j);
     const VT* off = V.off(j);
-    const int tiles = (L.n + kTile - 1) / kTile;
+    const bool fused = j + 2 == I.depth;
+    const int tiles = (n + kTile - 1) / kTile;
     for (int tile = g; tile < tiles; tile += G) {
+        if (fused)
+            coarse_rows_lds<VT>(I, static_cast<const VT*>(I.lvl_b[j + 1]), L.need + L.need_start[tile], 0,
+                                L.need_start[tile + 1] - L.need_start[tile], lds);
         const int i = tile * kTile + threadIdx.x;
-        if (i < L.n) {
-            const VT x2 = om * dinv[i] * rhs[i] + corr[L.agg[i]];
+        if (i < n) {
+            const auto correction = [&](int a) { return fused ? static_cast<VT>(lds.cvec[a]) : corr[a]; };
+            const VT x2 = om * dinv[i] * rhs[i] + correction(L.agg[i]);
             VT acc = rhs[i] - diag[i] * x2;
-            for (int k = L.row[i]; k < L.row[i + 1]; ++k) {
-                const int c = L.col[k];
-                acc -= off[k] * (om * dinv[c] * rhs[c] + corr[L.agg[c]]);
+            for (int s = 0; s < W; ++s) {
+                const int c = L.col[s * n + i];
+                if (c >= 0) acc -= off[s * n + i] * (om * dinv[c] * rhs[c] + correction(L.agg[c]));
             }
             out[i] = x2 + om * dinv[i] * acc;
         }
     }
 }
 
+// First sweep of a V-cycle: level 0's down phase (input evaluated from
+// `src`, stored into `keep`, squares into `part`); on a one-level hierarchy
+// the input is only materialised. Ends with a barrier.
 template <class VT>
-__device__ void vcycle(const Instance& I, const Values<VT>& V, const VT* rhs0, VT* out0, int g, int G, Lds& lds,
-                       int& timeouts) {
+__device__ void vcycle_first(const Instance& I, const Values<VT>& V, const Source<VT>& src, VT* keep, double* part,
+                             int g, int G, Lds& lds, int& timeouts) {
+    if (I.depth == 1) {
+        for (int tile = g; tile < I.tiles; tile += G) {
+            const int i = tile * kTile + threadIdx.x;
+            double sq[1] = {0.0};
+            if (i < I.n) {
+                const VT wi = src.at(i);
+                keep[i] = wi;
+                sq[0] = static_cast<double>(wi) * static_cast<double>(wi);
+            }
+            if (part != nullptr) store_partials<1>(part, tile, sq, lds);
+        }
+    } else {
+        down_phase<VT>(I, V, 0, src, keep, static_cast<VT*>(I.lvl_b[1]), part, g, G, lds);
+    }
+    group_sync(I, timeouts);
+}
+
+// Tiles the first sweep's partials cover.
+__device__ inline int first_sweep_tiles(const Instance& I) { return I.depth == 1 ? I.tiles : I.lv[0].down_tiles; }
+
+// The rest of the V-cycle after vcycle_first; w0 is the materialised level-0
+// input, out0 the level-0 output. Ends with a barrier.
+template <class VT>
+__device__ void vcycle_rest(const Instance& I, const Values<VT>& V, const VT* w0, VT* out0, int g, int G, Lds& lds,
+                            int& timeouts) {
     const int J = I.depth;
     if (J == 1) {
-        coarse_phase<VT>(I, 0, rhs0, out0, g, G);
+        for (int tile = g; tile < I.tiles; tile += G) {
+            const int first = tile * kTile, rows = min(kTile, I.n - first);
+            coarse_rows_lds<VT>(I, w0, nullptr, first, rows, lds);
+            const int i = first + threadIdx.x;
+            if (i < I.n) out0[i] = static_cast<VT>(lds.cvec[i]);
+        }
         group_sync(I, timeouts);
         return;
     }
-    for (int j = 0; j + 1 < J; ++j) {
-        const VT* rhs = j == 0 ? rhs0 : static_cast<const VT*>(I.lvl_b[j]);
-        down_phase<VT>(I, V, j, rhs, static_cast<VT*>(I.lvl_b[j + 1]), g, G, lds);
+    const Source<VT> none{nullptr, nullptr, nullptr, 0.0, 0.0, 0};
+    for (int j = 1; j + 1 < J; ++j) {
+        Source<VT> src = none;
+        src.a = static_cast<const VT*>(I.lvl_b[j]);
+        down_phase<VT>(I, V, j, src, nullptr, static_cast<VT*>(I.lvl_b[j + 1]), nullptr, g, G, lds);
         group_sync(I, timeouts);
     }
-    coarse_phase<VT>(I, J - 1, static_cast<const VT*>(I.lvl_b[J - 1]), static_cast<VT*>(I.lvl_e[J - 1]), g, G);
-    group_sync(I, timeouts);
     for (int j = J - 2; j >= 0; --j) {
-        const VT* rhs = j == 0 ? rhs0 : static_cast<const VT*>(I.lvl_b[j]);
+        const VT* rhs = j == 0 ? w0 : static_cast<const VT*>(I.lvl_b[j]);
         VT* out = j == 0 ? out0 : static_cast<VT*>(I.lvl_e[j]);
-        up_phase<VT>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G);
+        up_phase<VT>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds);
         group_sync(I, timeouts);
     }
 }
@@ -321,7 +421,7 @@ __device__ inline int tri_index(int i, int j) { return i >= j ? i * (i + 1) / 2 
 __device__ void coarse_inverse(const Instance& I, double* tri, Lds& lds) {
     const int j = I.depth - 1;
     const LevelView& L = I.lv[j];
-    const int n = L.n;
+    const int n = L.n, W = L.width;
     const int packed = n * (n + 1) / 2;
     const double* diag = I.diag64[j];
     const double* off = I.off64[j];
@@ -329,36 +429,33 @@ __device__ void coarse_inverse(const Instance& I, double* tri, Lds& lds) {
     __syncthreads();
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
         tri[tri_index(r, r)] = diag[r];
-        for (int k = L.row[r]; k < L.row[r + 1]; ++k)
-            if (L.col[k] < r) tri[tri_index(r, L.col[k])] = off[k];
+        for (int s = 0; s < W; ++s) {
+            const int c = L.col[s * n + r];
+            if (c >= 0 && c < r) tri[tri_index(r, c)] = off[s * n + r];
+        }
     }
     __syncthreads();
-    // each thread owns a contiguous run of packed entries
-    const int per = (packed + blockDim.x - 1) / blockDim.x;
-    const int e0 = min(packed, static_cast<int>(threadIdx.x) * per), e1 = min(packed, e0 + per);
-    int i0 = static_cast<int>((sqrt(8.0 * e0 + 1.0) - 1.0) * 0.5);
-    while (i0 * (i0 + 1) / 2 > e0) --i0;
-    while ((i0 + 1) * (i0 + 2) / 2 <= e0) ++i0;
-    const int j0 = e0 - i0 * (i0 + 1) / 2;
+    // Entries are strided over the threads (entry e = t + 256 m), so every
+    // thread's loads and stores are independent and consecutive threads
+    // touch consecutive entries; the row of entry e is recovered from
+    // e = r (r + 1) / 2 + c.
     for (int k = 0; k < n; ++k) {
         for (int c = threadIdx.x; c < n; c += blockDim.x) lds.colk[c] = tri[tri_index(c, k)];
         __syncthreads();
-        const double d = lds.colk[k];
-        const double dinv = 1.0 / d;
-        int i = i0, jj = j0;
-        for (int e = e0; e < e1; ++e) {
-            if (i == k && jj == k)
-                tri[e] = -dinv;
-            else if (i == k)
-                tri[e] = lds.colk[jj] * dinv;
-            else if (jj == k)
-                tri[e] = lds.colk[i] * dinv;
+        const double dinv = 1.0 / lds.colk[k];
+        for (int e = threadIdx.x; e < packed; e += blockDim.x) {
+            int r = static_cast<int>((sqrtf(8.0f * static_cast<float>(e) + 1.0f) - 1.0f) * 0.5f);
+            if (r * (r + 1) / 2 > e) --r;
+            if ((r + 1) * (r + 2) / 2 <= e) ++r;
+            const int c = e - r * (r + 1) / 2;
+            double value;
+            if (r == k)
+                value = c == k ? -dinv : lds.colk[c] * dinv;
+            else if (c == k)
+                value = lds.colk[r] * dinv;
             else
-                tri[e] -= lds.colk[i] * lds.colk[jj] * dinv;
-            if (++jj > i) {
-                ++i;
-                jj = 0;
-            }
+                value = tri[e] - lds.colk[r] * lds.colk[c] * dinv;
+            tri[e] = value;
         }
         __syncthreads();
     }
@@ -370,40 +467,54 @@ __device__ void coarse_inverse(const Instance& I, double* tri, Lds& lds) {
 
 __device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts, int& parity) {
     const int J = I.depth;
-    if (P.op & kOpHeatAssemble) {
-        // diag = C A + dt sum_k G_k, off_k = -dt G_k, G_k = K_k w_k.
+    {
+        // Level-0 values into ELL: assembled from the heat inputs
+        // (diag = C A + dt sum_k G_k, off_k = -dt G_k, G_k = K_k w_k, in CSR
+        // order), or copied from CSR-ordered off-diagonals.
         const LevelView& L = I.lv[0];
+        const int n = L.n, W = L.width;
         for (int tile = g; tile < I.tiles; tile += G) {
             const int i = tile * kTile + threadIdx.x;
-            if (i < I.n) {
-                double s = 0.0;
-                for (int k = L.row[i]; k < L.row[i + 1]; ++k) {
-                    const double gk = P.entry_conductance[k] * I.weight[k];
-                    s += gk;
-                    I.off64[0][k] = -P.dt * gk;
+            if (i < n) {
+                const int k0 = L.row[i], deg = L.row[i + 1] - k0;
+                double sum = 0.0;
+                for (int s = 0; s < W; ++s) {
+                    double value = 0.0;
+                    if (s < deg) {
+                        if (P.op & kOpHeatAssemble) {
+                            const double gk = P.entry_conductance[k0 + s] * I.weight[k0 + s];
+                            sum += gk;
+                            value = -P.dt * gk;
+                        } else {
+                            value = P.in[k0 + s];
+                        }
+                    }
+                    I.off64[0][s * n + i] = value;
                 }
-                I.diag64[0][i] = P.capacity_area[i] + P.dt * s;
+                if (P.op & kOpHeatAssemble) I.diag64[0][i] = P.capacity_area[i] + P.dt * sum;
             }
         }
         group_sync(I, timeouts);
     }
     for (int j = 0; j < J; ++j) {
         const LevelView& L = I.lv[j];
-        double* part = I.partial + parity * I.tiles * kPartials;
+        const int n = L.n, W = L.width;
+        double* part = I.partial + parity * I.partial_slots * kPartials;
         parity ^= 1;
-        const int tiles = (L.n + kTile - 1) / kTile;
+        const int tiles = (n + kTile - 1) / kTile;
         // inverse diagonal, FP32 copies, Gershgorin ratio sum |off| / diag
         for (int tile = g; tile < tiles; tile += G) {
             const int i = tile * kTile + threadIdx.x;
             double ratio = 0.0;
-            if (i < L.n) {
+            if (i < n) {
                 const double d = I.diag64[j][i];
                 const double di = 1.0 / d;
                 I.dinv64[j][i] = di;
                 double s = 0.0;
-                for (int k = L.row[i]; k < L.row[i + 1]; ++k) {
-                    s += fabs(I.off64[j][k]);
-                    if (I.off32[j] != nullptr) I.off32[j][k] = static_cast<float>(I.off64[j][k]);
+                for (int q = 0; q < W; ++q) {
+                    const double o = I.off64[j][q * n + i];
+                    s += fabs(o);
+                    if (I.off32[j] != nullptr) I.off32[j][q * n + i] = static_cast<float>(o);
                 }
                 ratio = s * di;
                 if (I.diag32[j] != nullptr) {
@@ -416,22 +527,40 @@ __device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds
         }
         // Galerkin values of level j+1 from level j's, in the reference's order
         if (j + 1 < J) {
-            const int nc = L.n_next, zc = I.lv[j + 1].nnz;
-            const int work_tiles = (nc + zc + kTile - 1) / kTile;
+            const LevelView& C = I.lv[j + 1];
+            const int nc = C.n, slots = C.width * C.n;
+            const int work_tiles = (nc + slots + kTile - 1) / kTile;
             for (int tile = g; tile < work_tiles; tile += G) {
                 const int e = tile * kTile + threadIdx.x;
+                // sums in list order; operands fetched four at a time
                 if (e < nc) {
                     double s = 0.0;
-                    for (int q = L.gd_start[e]; q < L.gd_start[e + 1]; ++q) {
-                        const int code = L.gd[q];
-                        s += code < 0 ? I.diag64[j][-code - 1] : I.off64[j][code];
+                    const int q1 = L.gd_start[e + 1];
+                    for (int q = L.gd_start[e]; q < q1; q += 4) {
+                        double v[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int code = q + u < q1 ? L.gd[q + u] : 0;
+                            v[u] = q + u < q1 ? (code < 0 ? I.diag64[j][-code - 1] : I.off64[j][code]) : 0.0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (q + u < q1) s += v[u];
                     }
                     I.diag64[j + 1][e] = s;
-                } else if (e < nc + zc) {
-                    const int t = e - nc;
+                } else if (e < nc + slots) {
+                    const int p = e - nc;
                     double s = 0.0;
-                    for (int q = L.go_start[t]; q < L.go_start[t + 1]; ++q) s += I.off64[j][L.go[q]];
-                    I.off64[j + 1][t] = s;
+                    const int q1 = L.go_start[p + 1];
+                    for (int q = L.go_start[p]; q < q1; q += 4) {
+                        double v[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) v[u] = q + u < q1 ? I.off64[j][L.go[q + u]] : 0.0;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (q + u < q1) s += v[u];
+                    }
+                    I.off64[j + 1][p] = s;
                 }
             }
         }
@@ -448,55 +577,69 @@ __device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds
 
 // ------------------------------------------------------------ BiCGSTAB
 
+// FP64 true residual row of x over level 0 (ELL): b_i - (diag_i x_i + sum).
+__device__ inline double residual_row(const Instance& I, const double* x, int i) {
+    const LevelView& L = I.lv[0];
+    const int n = L.n, W = L.width;
+    double ax = I.diag64[0][i] * x[i];
+    for (int s = 0; s < W; ++s) {
+        const int c = L.col[s * n + i];
+        if (c >= 0) ax += I.off64[0][s * n + i] * x[c];
+    }
+    return I.b[i] - ax;
+}
+
 template <class VT>
-__device__ void solve(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts, int& parity) {
-    const Values<VT> V = values_of<VT>(I);
+__device__ SolveResult solve(const Instance& I, const Launch& P, const double* t_old, const double* forcing,
+                             double* t_new, int g, int G, Lds& lds, int& timeouts, int& parity, int& xi) {
+    const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
+    const Values<VT> V{I};
     const LevelView& L0 = I.lv[0];
-    const int n = I.n, tiles = I.tiles;
+    const int n = I.n, W = L0.width, tiles = I.tiles;
     VT* r = static_cast<VT*>(I.r);
     VT* r0 = static_cast<VT*>(I.r0);
-    VT* p = static_cast<VT*>(I.p);
     VT* v = static_cast<VT*>(I.v);
     VT* ph = static_cast<VT*>(I.ph);
     VT* s = static_cast<VT*>(I.s);
     VT* sh = static_cast<VT*>(I.sh);
     VT* t = static_cast<VT*>(I.t);
-    double* x = I.x;
     double* b = I.b;
     const VT* diag = V.diag(0);
     const VT* off = V.off(0);
     const auto next_partials = [&]() {
-        double* part = I.partial + parity * tiles * kPartials;
+        double* part = I.partial + parity * I.partial_slots * kPartials;
         parity ^= 1;
         return part;
     };
-
-    // b (assembled or given), x = 0, r = r0 = b, p = v = 0.
-    double* part = next_partials();
-    for (int tile = g; tile < tiles; tile += G) {
-        const int i = tile * kTile + threadIdx.x;
-        double q[2] = {0.0, 0.0};
-        if (i < n) {
-            double bi;
-            if (P.op & kOpHeatRhs)
-                bi = P.capacity_area[i] * P.t_old[i] + P.dt * I.area[i] * P.forcing[i];
-            else
-                bi = P.in[i];
-            b[i] = bi;
-            x[i] = 0.0;
-            const VT ri = static_cast<VT>(bi);
-            r[i] = ri;
-            r0[i] = ri;
-            p[i] = VT(0);
-            v[i] = VT(0);
-            q[0] = bi * bi;
-            q[1] = static_cast<double>(ri) * static_cast<double>(ri);
+    const auto spmv_row = [&](const VT* in, int i) {
+        VT acc = diag[i] * in[i];
+        for (int q = 0; q < W; ++q) {
+            const int c = L0.col[q * n + i];
+            if (c >= 0) acc += off[q * n + i] * in[c];
         }
-        double tot[2];
-        tile_sums<2>(q, tot, lds);
-        if (threadIdx.x == 0) {
-            part[tile * kPartials] = tot[0];
-            part[tile * kPartials + 1] = tot[1];
+        return acc;
+    };
+
+    // b (assembled or given), x = 0, r = r0 = b.
+    double* part = next_partials();
+    {
+        double* x = I.x[xi];
+        for (int tile = g; tile < tiles; tile += G) {
+            const int i = tile * kTile + threadIdx.x;
+            double q[2] = {0.0, 0.0};
+            if (i < n) {
+                const double bi = (P.op & kOpHeatRhs)
+                                      ? P.capacity_area[i] * t_old[i] + P.dt * I.area[i] * forcing[i]
+                                      : P.in[i];
+                b[i] = bi;
+                x[i] = 0.0;
+                const VT ri = static_cast<VT>(bi);
+                r[i] = ri;
+                r0[i] = ri;
+                q[0] = bi * bi;
+                q[1] = static_cast<double>(ri) * static_cast<double>(ri);
+            }
+            store_partials<2>(part, tile, q, lds);
         }
     }
     group_sync(I, timeouts);
@@ -508,39 +651,36 @@ __device__ void solve(const Instance& I, const Launch& P, int g, int G, Lds& lds
     int iterations = 0, restarts = 0, v_cycles = 0;
     int status = 0;
     const double target = P.tolerance * b_norm;
-    bool first = true;
+    bool first = true;            // first pass: r = b exactly
+    bool have_residual = false;   // the true residual of x is in r, r_norm, rho1
+    int pi = 0;                   // current search-direction buffer
 
     if (b_norm > 0.0) {
         for (;;) {
             if (!first) {
-                // FP64 true residual r = b - A x; restart from x if needed.
-                part = next_partials();
-                for (int tile = g; tile < tiles; tile += G) {
-                    const int i = tile * kTile + threadIdx.x;
-                    double q[2] = {0.0, 0.0};
-                    if (i < n) {
-                        double ax = I.diag64[0][i] * x[i];
-                        for (int k = L0.row[i]; k < L0.row[i + 1]; ++k) ax += I.off64[0][k] * x[L0.col[k]];
-                        const double ri = b[i] - ax;
-                        const VT rv = static_cast<VT>(ri);
-                        r[i] = rv;
-                        r0[i] = rv;
-                        p[i] = VT(0);
-                        v[i] = VT(0);
-                        q[0] = ri * ri;
-                        q[1] = static_cast<double>(rv) * static_cast<double>(rv);
+                if (!have_residual) {
+                    // FP64 true residual r = b - A x; restart from x if needed.
+                    part = next_partials();
+                    const double* x = I.x[xi];
+                    for (int tile = g; tile < tiles; tile += G) {
+                        const int i = tile * kTile + threadIdx.x;
+                        double q[2] = {0.0, 0.0};
+                        if (i < n) {
+                            const double ri = residual_row(I, x, i);
+                            const VT rv = static_cast<VT>(ri);
+                            r[i] = rv;
+                            r0[i] = rv;
+                            q[0] = ri * ri;
+                            q[1] = static_cast<double>(rv) * static_cast<double>(rv);
+                        }
+                        store_partials<2>(part, tile, q, lds);
                     }
-                    double tot[2];
-                    tile_sums<2>(q, tot, lds);
-                    if (threadIdx.x == 0) {
-                        part[tile * kPartials] = tot[0];
-                        part[tile * kPartials + 1] = tot[1];
-                    }
+                    group_sync(I, timeouts);
+                    group_totals<2>(part, tiles, tot2, lds);
+                    r_norm = sqrt(tot2[0]);
+                    rho1 = tot2[1];
                 }
-                group_sync(I, timeouts);
-                group_totals<2>(part, tiles, tot2, lds);
-                r_norm = sqrt(tot2[0]);
-                rho1 = tot2[1];
+                have_residual = false;
                 if (r_norm <= target) break;
                 if (iterations >= P.max_iterations || restarts >= P.max_restarts) {
                     status = kNotConverged;
@@ -551,67 +691,77 @@ __device__ void solve(const Instance& I, const Launch& P, int g, int G, Lds& lds
             first = false;
             const double inner_target = fmax(target, P.inner_relative * r_norm);
             double rho = 1.0, alpha = 1.0, omega = 1.0;
+            bool first_iteration = true;
             while (iterations < P.max_iterations) {
                 if (rho1 == 0.0 || omega == 0.0) break;
                 const double beta = rho1 / rho * (alpha / omega);
                 rho = rho1;
-                for (int tile = g; tile < tiles; tile += G) {
-                    const int i = tile * kTile + threadIdx.x;
-                    if (i < n)
-                        p[i] = static_cast<VT>(static_cast<double>(r[i]) +
-                                               beta * (static_cast<double>(p[i]) - omega * static_cast<double>(v[i])));
-                }
-                group_sync(I, timeouts);
-                vcycle<VT>(I, V, p, ph, g, G, lds, timeouts);
+                // V-cycle on p = r + beta (p - omega v); p = r on a fresh start
+                Source<VT> src{r, nullptr, nullptr, 0.0, 0.0, 0};
+                if (!first_iteration) src = Source<VT>{r, static_cast<const VT*>(I.p[pi]), v, beta, omega, 1};
+                VT* p_new = static_cast<VT*>(I.p[pi ^ 1]);
+                pi ^= 1;
+                vcycle_first<VT>(I, V, src, p_new, nullptr, g, G, lds, timeouts);
+                vcycle_rest<VT>(I, V, p_new, ph, g, G, lds, timeouts);
                 ++v_cycles;
+                first_iteration = false;
                 // v = A ph, (r0, v)
                 part = next_partials();
                 for (int tile = g; tile < tiles; tile += G) {
                     const int i = tile * kTile + threadIdx.x;
                     double q[1] = {0.0};
                     if (i < n) {
-                        VT acc = diag[i] * ph[i];
-                        for (int k = L0.row[i]; k < L0.row[i + 1]; ++k) acc += off[k] * ph[L0.col[k]];
+                        const VT acc = spmv_row(ph, i);
                         v[i] = acc;
                         q[0] = static_cast<double>(r0[i]) * static_cast<double>(acc);
                     }
-                    double tot[1];
-                    tile_sums<1>(q, tot, lds);
-                    if (threadIdx.x == 0) part[tile * kPartials] = tot[0];
+                    store_partials<1>(part, tile, q, lds);
                 }
                 group_sync(I, timeouts);
                 double r0v[1];
                 group_totals<1>(part, tiles, r0v, lds);
                 if (r0v[0] == 0.0) break;
                 alpha = rho / r0v[0];
-                // s = r - alpha v, (s, s)
+                // s = r - alpha v inside the next V-cycle's first sweep, (s, s)
                 part = next_partials();
-                for (int tile = g; tile < tiles; tile += G) {
-                    const int i = tile * kTile + threadIdx.x;
-                    double q[1] = {0.0};
-                    if (i < n) {
-                        const double si = static_cast<double>(r[i]) - alpha * static_cast<double>(v[i]);
-                        const VT sv = static_cast<VT>(si);
-                        s[i] = sv;
-                        q[0] = static_cast<double>(sv) * static_cast<double>(sv);
-                    }
-                    double tot[1];
-                    tile_sums<1>(q, tot, lds);
-                    if (threadIdx.x == 0) part[tile * kPartials] = tot[0];
-                }
-                group_sync(I, timeouts);
+                vcycle_first<VT>(I, V, Source<VT>{r, v, nullptr, alpha, 0.0, 2}, s, part, g, G, lds, timeouts);
                 double ss[1];
-                group_totals<1>(part, tiles, ss, lds);
+                group_totals<1>(part, first_sweep_tiles(I), ss, lds);
                 ++iterations;
                 if (sqrt(ss[0]) <= inner_target) {
+                    // x' = x + alpha ph fused with the FP64 true residual of x'
+                    part = next_partials();
+                    const double* x = I.x[xi];
+                    double* xn = I.x[xi ^ 1];
                     for (int tile = g; tile < tiles; tile += G) {
                         const int i = tile * kTile + threadIdx.x;
-                        if (i < n) x[i] += alpha * static_cast<double>(ph[i]);
+                        double q[2] = {0.0, 0.0};
+                        if (i < n) {
+                            const double x_new = x[i] + alpha * static_cast<double>(ph[i]);
+                            xn[i] = x_new;
+                            double ax = I.diag64[0][i] * x_new;
+                            for (int k = 0; k < W; ++k) {
+                                const int c = L0.col[k * n + i];
+                                if (c >= 0) ax += I.off64[0][k * n + i] * (x[c] + alpha * static_cast<double>(ph[c]));
+                            }
+                            const double ri = b[i] - ax;
+                            const VT rv = static_cast<VT>(ri);
+                            r[i] = rv;
+                            r0[i] = rv;
+                            q[0] = ri * ri;
+                            q[1] = static_cast<double>(rv) * static_cast<double>(rv);
+                        }
+                        store_partials<2>(part, tile, q, lds);
                     }
                     group_sync(I, timeouts);
+                    xi ^= 1;
+                    group_totals<2>(part, tiles, tot2, lds);
+                    r_norm = sqrt(tot2[0]);
+                    rho1 = tot2[1];
+                    have_residual = true;
                     break;
                 }
-                vcycle<VT>(I, V, s, sh, g, G, lds, timeouts);
+                vcycle_rest<VT>(I, V, s, sh, g, G, lds, timeouts);
                 ++v_cycles;
                 // t = A sh, (t, t), (t, s)
                 part = next_partials();
@@ -619,19 +769,13 @@ __device__ void solve(const Instance& I, const Launch& P, int g, int G, Lds& lds
                     const int i = tile * kTile + threadIdx.x;
                     double q[2] = {0.0, 0.0};
                     if (i < n) {
-                        VT acc = diag[i] * sh[i];
-                        for (int k = L0.row[i]; k < L0.row[i + 1]; ++k) acc += off[k] * sh[L0.col[k]];
+                        const VT acc = spmv_row(sh, i);
                         t[i] = acc;
                         const double td = static_cast<double>(acc);
                         q[0] = td * td;
                         q[1] = td * static_cast<double>(s[i]);
                     }
-                    double tot[2];
-                    tile_sums<2>(q, tot, lds);
-                    if (threadIdx.x == 0) {
-                        part[tile * kPartials] = tot[0];
-                        part[tile * kPartials + 1] = tot[1];
-                    }
+                    store_partials<2>(part, tile, q, lds);
                 }
                 group_sync(I, timeouts);
                 double tt[2];
@@ -639,22 +783,20 @@ __device__ void solve(const Instance& I, const Launch& P, int g, int G, Lds& lds
                 omega = tt[0] > 0.0 ? tt[1] / tt[0] : 0.0;
                 // x += alpha ph + omega sh, r = s - omega t, (r, r), (r0, r)
                 part = next_partials();
-                for (int tile = g; tile < tiles; tile += G) {
-                    const int i = tile * kTile + threadIdx.x;
-                    double q[2] = {0.0, 0.0};
-                    if (i < n) {
-                        x[i] += alpha * static_cast<double>(ph[i]) + omega * static_cast<double>(sh[i]);
-                        const double ri = static_cast<double>(s[i]) - omega * static_cast<double>(t[i]);
-                        const VT rv = static_cast<VT>(ri);
-                        r[i] = rv;
-                        q[0] = static_cast<double>(rv) * static_cast<double>(rv);
-                        q[1] = static_cast<double>(r0[i]) * static_cast<double>(rv);
-                    }
-                    double tot[2];
-                    tile_sums<2>(q, tot, lds);
-                    if (threadIdx.x == 0) {
-                        part[tile * kPartials] = tot[0];
-                        part[tile * kPartials + 1] = tot[1];
+                {
+                    double* x = I.x[xi];
+                    for (int tile = g; tile < tiles; tile += G) {
+                        const int i = tile * kTile + threadIdx.x;
+                        double q[2] = {0.0, 0.0};
+                        if (i < n) {
+                            x[i] += alpha * static_cast<double>(ph[i]) + omega * static_cast<double>(sh[i]);
+                            const double ri = static_cast<double>(s[i]) - omega * static_cast<double>(t[i]);
+                            const VT rv = static_cast<VT>(ri);
+                            r[i] = rv;
+                            q[0] = static_cast<double>(rv) * static_cast<double>(rv);
+                            q[1] = static_cast<double>(r0[i]) * static_cast<double>(rv);
+                        }
+                        store_partials<2>(part, tile, q, lds);
                     }
                 }
                 group_sync(I, timeouts);
@@ -665,23 +807,55 @@ __device__ void solve(const Instance& I, const Launch& P, int g, int G, Lds& lds
             }
         }
     }
-    if (P.op & kOpHeatRhs) {
+    if (t_new != nullptr) {
+        const double* x = I.x[xi];
         for (int tile = g; tile < tiles; tile += G) {
             const int i = tile * kTile + threadIdx.x;
-            if (i < n) P.t_new[i] = x[i];
+            if (i < n) t_new[i] = x[i];
         }
     }
-    if (g == 0 && threadIdx.x == 0) {
-        I.result->iterations = iterations;
-        I.result->restarts = restarts;
-        I.result->status = status;
-        I.result->relative_residual = b_norm > 0.0 ? r_norm / b_norm : 0.0;
-        I.result->b_norm = b_norm;
-        I.result->v_cycles = v_cycles;
-    }
+    SolveResult out{};
+    out.iterations = iterations;
+    out.restarts = restarts;
+    out.status = status;
+    out.relative_residual = b_norm > 0.0 ? r_norm / b_norm : 0.0;
+    out.b_norm = b_norm;
+    out.v_cycles = v_cycles;
+    out.t_begin = t_begin;
+    out.t_end = static_cast<uint64_t>(wall_clock64());
+    return out;
 }
 
 // ------------------------------------------------------------ the kernel
+
+template <class VT>
+__device__ void run_solve(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts, int& parity) {
+    int xi = 0;
+    const int steps = (P.op & kOpChain) ? P.steps : 1;
+    SolveResult last{};
+    for (int step = 0; step < steps; ++step) {
+        const double* t_old = P.t_old;
+        const double* forcing = P.forcing;
+        double* t_new = (P.op & kOpHeatRhs) ? P.t_new : P.out;
+        if (P.op & kOpChain) {
+            // step k reads the previous step's x (row by row, before this
+            // step zeroes it) and writes the final T only at the end
+            t_old = step == 0 ? P.t_old : I.x[xi];
+            forcing = P.forcing + static_cast<size_t>(step) * I.n;
+            t_new = step + 1 == steps ? P.t_new : nullptr;
+        }
+        last = solve<VT>(I, P, t_old, forcing, t_new, g, G, lds, timeouts, parity, xi);
+        if (g == 0 && threadIdx.x == 0 && P.step_results != nullptr) P.step_results[step] = last;
+    }
+    if (g == 0 && threadIdx.x == 0) {
+        I.result->iterations = last.iterations;
+        I.result->restarts = last.restarts;
+        I.result->status = last.status;
+        I.result->relative_residual = last.relative_residual;
+        I.result->b_norm = last.b_norm;
+        I.result->v_cycles = last.v_cycles;
+    }
+}
 
 __global__ __launch_bounds__(kTile) void program_kernel(const Instance* instances, const int32_t* block_instance,
                                                         const Launch* launches, Launch single) {
@@ -703,16 +877,21 @@ __global__ __launch_bounds__(kTile) void program_kernel(const Instance* instance
     if (P.op & kOpSetMatrix) set_matrix(I, P, g, G, lds, timeouts, parity);
     if (P.op & kOpPrecondition) {
         if (I.precision == 0) {
-            vcycle<double>(I, values_of<double>(I), P.in, P.out, g, G, lds, timeouts);
+            const Source<double> src{P.in, nullptr, nullptr, 0.0, 0.0, 0};
+            double* keep = static_cast<double*>(I.p[0]);
+            vcycle_first<double>(I, Values<double>{I}, src, keep, nullptr, g, G, lds, timeouts);
+            vcycle_rest<double>(I, Values<double>{I}, keep, P.out, g, G, lds, timeouts);
         } else {
-            float* pin = static_cast<float*>(I.p);
+            float* pin = static_cast<float*>(I.p[0]);
             float* pout = static_cast<float*>(I.ph);
             for (int tile = g; tile < I.tiles; tile += G) {
                 const int i = tile * kTile + threadIdx.x;
                 if (i < I.n) pin[i] = static_cast<float>(P.in[i]);
             }
             group_sync(I, timeouts);
-            vcycle<float>(I, values_of<float>(I), pin, pout, g, G, lds, timeouts);
+            const Source<float> src{pin, nullptr, nullptr, 0.0, 0.0, 0};
+            vcycle_first<float>(I, Values<float>{I}, src, static_cast<float*>(I.p[1]), nullptr, g, G, lds, timeouts);
+            vcycle_rest<float>(I, Values<float>{I}, static_cast<float*>(I.p[1]), pout, g, G, lds, timeouts);
             for (int tile = g; tile < I.tiles; tile += G) {
                 const int i = tile * kTile + threadIdx.x;
                 if (i < I.n) P.out[i] = static_cast<double>(pout[i]);
@@ -721,20 +900,24 @@ __global__ __launch_bounds__(kTile) void program_kernel(const Instance* instance
     }
     if (P.op & kOpApply) {
         const LevelView& L = I.lv[0];
+        const int n = L.n, W = L.width;
         for (int tile = g; tile < I.tiles; tile += G) {
             const int i = tile * kTile + threadIdx.x;
-            if (i < I.n) {
+            if (i < n) {
                 double acc = I.diag64[0][i] * P.in[i];
-                for (int k = L.row[i]; k < L.row[i + 1]; ++k) acc += I.off64[0][k] * P.in[L.col[k]];
+                for (int s = 0; s < W; ++s) {
+                    const int c = L.col[s * n + i];
+                    if (c >= 0) acc += I.off64[0][s * n + i] * P.in[c];
+                }
                 P.out[i] = acc;
             }
         }
     }
     if (P.op & kOpSolve) {
         if (I.precision == 0)
-            solve<double>(I, P, g, G, lds, timeouts, parity);
+            run_solve<double>(I, P, g, G, lds, timeouts, parity);
         else
-            solve<float>(I, P, g, G, lds, timeouts, parity);
+            run_solve<float>(I, P, g, G, lds, timeouts, parity);
     }
     if (timeouts != 0 && threadIdx.x == 0) atomicAdd(&I.result->barrier_timeouts, timeouts);
 }
@@ -762,9 +945,13 @@ hipError_t launch_program(const Instance* instances, const int32_t* block_instan
     return hipGetLastError();
 }
 
-}  // namespace psim::dev
+size_t program_static_lds() {
+    hipFuncAttributes attributes{};
+    if (hipFuncGetAttributes(&attributes, reinterpret_cast<const void*>(&program_kernel)) != hipSuccess)
+        return sizeof(Lds) + sizeof(Launch) + 256;
+    return attributes.sharedSizeBytes;
+}
 
-namespace psim::dev {
 int program_blocks_per_cu(size_t dynamic_lds) {
     int blocks = 0;
     if (dynamic_lds > 0 &&
@@ -775,4 +962,5 @@ int program_blocks_per_cu(size_t dynamic_lds) {
         return 0;
     return blocks;
 }
+
 }  // namespace psim::dev

@@ -311,3 +311,26 @@ def test_invalid_arguments_raise(ctx):
         fv.orientation(0, far)
     assert fv.orientation(int(m.first[0]), 0) == 1.0
     assert fv.orientation(int(m.second[0]), 0) == -1.0
+
+
+@pytest.mark.parametrize("level", [3, 6])
+def test_chain_launch_equals_separate_steps(ctx, level):
+    """psim_heat_chain (set_matrix + N steps in one launch) gives the same
+    bits as psim_heat_set_matrix followed by N psim_heat_step launches."""
+    announce("chained launch", level)
+    m, seed, cond, ca, land, t_old, _ = heat_case(level)
+    steps = 6
+    forcings = np.concatenate([P.heat_forcing(m, seed, k) for k in range(steps)])
+    g = P.Graph(ctx, m)
+    s = P.System(g)
+    chained, infos = s.heat_chain(t_old, forcings, tolerance=1e-11,
+                                  conductance_per_entry=m.entry_values(cond), capacity_area=ca)
+    assert len(infos) == steps and all(i.status == 0 and i.relative_residual <= 1e-11 for i in infos)
+    assert all(i.device_ms > 0 for i in infos)
+    s2 = P.System(g)
+    s2.heat_set_matrix(m.entry_values(cond), ca)
+    t = t_old.copy()
+    for k in range(steps):
+        t, info = s2.heat_step(t, forcings[k * m.cells:(k + 1) * m.cells], tolerance=1e-11)
+        assert info.iterations == infos[k].iterations
+    assert bits_equal(chained, t)
