@@ -417,7 +417,9 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         const size_t nc_rows = static_cast<size_t>(g->views[g->depth - 1].n);
         s->tri_bytes = nc_rows * (nc_rows + 1) / 2 * sizeof(double);
         int per_cu = ctx->blocks_per_cu;
-        if (s->tri_bytes <= ctx->lds_per_block) {
+        if (nc_rows <= static_cast<size_t>(psim::dev::kBlockedCoarse)) {
+            s->tri_bytes = 0;   // inverted in registers
+        } else if (s->tri_bytes <= ctx->lds_per_block) {
             // set_matrix launches carry the packed coarsest matrix in dynamic
             // LDS; co-residency then follows from the CU's LDS
             const size_t per_block = ctx->static_lds + s->tri_bytes;
@@ -471,7 +473,7 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         A.plan(&I.partial, 2 * static_cast<size_t>(I.partial_slots) * kPartials);
         A.plan(&s->csr_stage, static_cast<size_t>(g->nnz));
         A.plan(&I.barrier, psim::dev::kBarrierWordCount);
-        if (!I.tri_lds) A.plan(&I.tri, nc_rows * (nc_rows + 1) / 2);
+        if (!I.tri_lds && s->tri_bytes > 0) A.plan(&I.tri, nc_rows * (nc_rows + 1) / 2);
         A.plan(&I.result, 1);
         A.plan(&s->d_inst, 1);
         s->map_capacity = cap;

@@ -23,6 +23,8 @@ constexpr int kTile = 256;        // rows per tile = threads per workgroup = the
 constexpr int kMaxLevels = 8;
 constexpr int kPartials = 4;      // reduction slots per tile
 constexpr int kMaxCoarse = 512;   // dense coarsest level (the hierarchy stops at <= 400 rows)
+// Coarsest levels up to this size are inverted in registers (no scratch).
+constexpr int kBlockedCoarse = 168;
 constexpr int32_t kNotConverged = 5;   // PSIM_NOT_CONVERGED
 
 enum Op : int32_t {

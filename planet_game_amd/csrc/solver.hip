@@ -465,6 +465,116 @@ __device__ void coarse_inverse(const Instance& I, double* tri, Lds& lds) {
     }
 }
 
+// The same sweep for a coarsest level of at most kBlockedCoarse rows, held
+// in registers: the lower triangle, padded to 8-row blocks, is cut into
+// 8 x 8 blocks (bi >= bj), one per thread (21 block rows give 231 blocks).
+// Pivot k broadcasts column k through LDS, double-buffered so that one
+// barrier per pivot suffices, and each thread updates its 64 entries:
+//   a_rc <- a_rc - (a_rk / d) a_kc,  a_rk <- a_rk / d,  a_kk <- -1 / d.
+// Diagonal blocks keep both halves but only their lower entries are ever
+// read, so the result is exactly symmetric. Written negated to I.cinv.
+constexpr int kSweepBlock = 8;
+constexpr int kSweepBlockRows = kBlockedCoarse / kSweepBlock;
+static_assert(kSweepBlockRows * kSweepBlock == kBlockedCoarse, "whole sweep blocks");
+static_assert(kSweepBlockRows * (kSweepBlockRows + 1) / 2 <= kTile, "one sweep block per thread");
+static_assert(2 * 256 <= kMaxCoarse && kBlockedCoarse <= 256, "two pivot-column buffers in Lds::colk");
+
+__device__ void coarse_inverse_blocked(const Instance& I, Lds& lds) {
+    const int j = I.depth - 1;
+    const LevelView& L = I.lv[j];
+    const int n = L.n, W = L.width;
+    const int nb = (n + kSweepBlock - 1) / kSweepBlock;
+    const int q = static_cast<int>(threadIdx.x);
+    int bi = static_cast<int>((sqrtf(8.0f * static_cast<float>(q) + 1.0f) - 1.0f) * 0.5f);
+    if (bi * (bi + 1) / 2 > q) --bi;
+    if ((bi + 1) * (bi + 2) / 2 <= q) ++bi;
+    const int bj = q - bi * (bi + 1) / 2;
+    const bool active = bi < nb;
+    const int r0 = bi * kSweepBlock, c0 = bj * kSweepBlock;
+    double a[kSweepBlock][kSweepBlock];
+#pragma unroll
+    for (int u = 0; u < kSweepBlock; ++u)
+#pragma unroll
+        for (int v = 0; v < kSweepBlock; ++v) a[u][v] = 0.0;
+    if (active) {
+        // lower-triangle entries (c <= r) from row r of the ELL level
+#pragma unroll
+        for (int u = 0; u < kSweepBlock; ++u) {
+            const int r = r0 + u;
+            if (r < n) {
+                const double d = I.diag64[j][r];
+#pragma unroll
+                for (int v = 0; v < kSweepBlock; ++v)
+                    if (c0 + v == r) a[u][v] = d;
+                for (int s = 0; s < W; ++s) {
+                    const int c = L.col[s * n + r];
+                    const double o = I.off64[j][s * n + r];
+#pragma unroll
+                    for (int v = 0; v < kSweepBlock; ++v)
+                        if (c == c0 + v && c < r) a[u][v] = o;
+                }
+            }
+        }
+    }
+    for (int k = 0; k < n; ++k) {
+        double* colk = lds.colk + (k & 1) * 256;
+        const int K = k / kSweepBlock, kk = k % kSweepBlock;
+        if (active) {
+#pragma unroll
+            for (int KK = 0; KK < kSweepBlock; ++KK) {
+                if (KK != kk) continue;
+                if (bj == K) {
+                    // rows >= 8K: column KK of the block (lower half on the diagonal)
+#pragma unroll
+                    for (int u = 0; u < kSweepBlock; ++u) colk[r0 + u] = (bi == K && u < KK) ? a[KK][u] : a[u][KK];
+                } else if (bi == K) {
+                    // rows < 8K: row KK of the block, by symmetry
+#pragma unroll
+                    for (int v = 0; v < kSweepBlock; ++v) colk[c0 + v] = a[KK][v];
+                }
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const double dinv = 1.0 / colk[k];
+            double rs[kSweepBlock], cv[kSweepBlock];
+#pragma unroll
+            for (int u = 0; u < kSweepBlock; ++u) rs[u] = colk[r0 + u] * dinv;
+#pragma unroll
+            for (int v = 0; v < kSweepBlock; ++v) cv[v] = colk[c0 + v];
+#pragma unroll
+            for (int u = 0; u < kSweepBlock; ++u)
+#pragma unroll
+                for (int v = 0; v < kSweepBlock; ++v) a[u][v] = fma(-rs[u], cv[v], a[u][v]);
+#pragma unroll
+            for (int KK = 0; KK < kSweepBlock; ++KK) {
+                if (KK != kk) continue;
+                if (bi == K) {
+#pragma unroll
+                    for (int v = 0; v < kSweepBlock; ++v) a[KK][v] = cv[v] * dinv;
+                }
+                if (bj == K) {
+#pragma unroll
+                    for (int u = 0; u < kSweepBlock; ++u) a[u][KK] = rs[u];
+                }
+                if (bi == K && bj == K) a[KK][KK] = -dinv;
+            }
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int u = 0; u < kSweepBlock; ++u)
+#pragma unroll
+            for (int v = 0; v < kSweepBlock; ++v) {
+                const int r = r0 + u, c = c0 + v;
+                if (r < n && c < n && (bi > bj || u >= v)) {
+                    I.cinv[c * n + r] = -a[u][v];
+                    I.cinv[r * n + c] = -a[u][v];
+                }
+            }
+    }
+}
+
 __device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts, int& parity) {
     const int J = I.depth;
     {
@@ -571,7 +681,12 @@ __device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds
         const double lo = fmax(1.0 - rho, 0.25 * hi);
         if (g == 0 && threadIdx.x == 0) I.omega[j] = 2.0 / (lo + hi);
     }
-    if (g == 0) coarse_inverse(I, I.tri_lds ? dynamic_lds() : I.tri, lds);
+    if (g == 0) {
+        if (I.lv[J - 1].n <= kBlockedCoarse)
+            coarse_inverse_blocked(I, lds);
+        else
+            coarse_inverse(I, I.tri_lds ? dynamic_lds() : I.tri, lds);
+    }
     group_sync(I, timeouts);
 }
 
