@@ -112,6 +112,11 @@ int psim_device_alloc(psim_ctx* ctx, uint64_t bytes, void** out);
 int psim_device_free(psim_ctx* ctx, void* ptr);
 int psim_copy_to_device(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int psim_copy_to_host(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Device-to-device copy through a plain streaming kernel, queued on the
+ * context's stream (no counterpart in the reference: the kernel of known
+ * traffic that HBM byte counters are calibrated against). 16-byte aligned
+ * pointers and size. */
+int psim_device_copy(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 /* HIP events on the context's stream, for timing device work in place:
  * psim_ctx_events makes `count` events, psim_ctx_record records event
  * `index` on the stream, psim_ctx_elapsed waits for event `to` and returns

@@ -85,6 +85,7 @@ _sig("psim_device_alloc", C.c_int, _vp, _u64, C.POINTER(_vp))
 _sig("psim_device_free", C.c_int, _vp, _vp)
 _sig("psim_copy_to_device", C.c_int, _vp, _vp, _vp, _u64)
 _sig("psim_copy_to_host", C.c_int, _vp, _vp, _vp, _u64)
+_sig("psim_device_copy", C.c_int, _vp, _vp, _vp, _u64)
 _sig("psim_ctx_info", C.c_int, _vp, C.POINTER(DeviceInfo))
 _sig("psim_ctx_events", C.c_int, _vp, C.c_uint32)
 _sig("psim_ctx_record", C.c_int, _vp, C.c_uint32)
@@ -254,6 +255,10 @@ class Context:
 
     def record(self, index):
         check(lib.psim_ctx_record(self.handle, index))
+
+    def copy(self, dst, src, nbytes):
+        """Queue a device-to-device copy through the library's streaming kernel."""
+        check(lib.psim_device_copy(self.handle, dst, src, nbytes))
 
     def elapsed_ms(self, start, end):
         ms = C.c_float()

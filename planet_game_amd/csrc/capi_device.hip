@@ -107,6 +107,15 @@ struct psim_system {
 
 namespace {
 
+// Plain streaming copy (16 B per lane, grid-stride): the HBM reference
+// kernel whose bytes per launch are known exactly.
+__global__ __launch_bounds__(256) void copy_words_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                                         uint64_t words) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; w < words; w += stride)
+        dst[w] = src[w];
+}
+
 void launch(psim_system* s, const Launch& p) {
     psim_ctx* ctx = s->graph->ctx;
     const bool cooperative = s->workgroups > 1;
@@ -115,6 +124,29 @@ void launch(psim_system* s, const Launch& p) {
     psim::hip_check(psim::dev::launch_program(s->d_inst, s->d_map, nullptr, p, s->workgroups, lds, ctx->stream,
                                               cooperative),
                     "solver launch");
+}
+
+// Heat-assembly inputs of a launch: host arrays are copied into the
+// system's buffers first; device arrays are read in place by the assembly,
+// which keeps C A in the system for the steps that follow.
+void stage_heat_inputs(psim_system* s, const double* conductance_per_entry, const double* capacity_area, int where,
+                       Launch& p) {
+    if (where == PSIM_HOST) {
+        hipStream_t stream = s->graph->ctx->stream;
+        psim::hip_check(hipMemcpyAsync(s->entry_conductance, conductance_per_entry,
+                                       static_cast<size_t>(s->graph->nnz) * 8, hipMemcpyHostToDevice, stream),
+                        "copy");
+        psim::hip_check(hipMemcpyAsync(s->capacity_area, capacity_area, static_cast<size_t>(s->graph->n) * 8,
+                                       hipMemcpyHostToDevice, stream),
+                        "copy");
+        p.entry_conductance = s->entry_conductance;
+        p.capacity_area = s->capacity_area;
+        p.capacity_keep = nullptr;
+    } else {
+        p.entry_conductance = conductance_per_entry;
+        p.capacity_area = capacity_area;
+        p.capacity_keep = s->capacity_area;
+    }
 }
 
 SolveResult fetch_result(psim_system* s) {
@@ -276,6 +308,20 @@ int psim_copy_to_host(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes)
         if (ctx == nullptr) throw std::invalid_argument("null context");
         psim::hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
         psim::hip_check(hipStreamSynchronize(ctx->stream), "copy to host");
+    });
+}
+
+int psim_device_copy(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded([&] {
+        if (ctx == nullptr || dst == nullptr || src == nullptr) throw std::invalid_argument("null argument");
+        if (bytes % 16 != 0 || reinterpret_cast<uintptr_t>(dst) % 16 != 0 || reinterpret_cast<uintptr_t>(src) % 16 != 0)
+            throw std::invalid_argument("device copy needs 16-byte aligned pointers and size");
+        if (bytes == 0) return;
+        const uint64_t words = bytes / 16;
+        const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((words + 255) / 256, 8192));
+        hipLaunchKernelGGL(copy_words_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                           static_cast<uint4*>(dst), static_cast<const uint4*>(src), words);
+        psim::hip_check(hipGetLastError(), "device copy");
     });
 }
 
@@ -680,17 +726,10 @@ int psim_heat_set_matrix(psim_system* s, const double* conductance_per_entry, co
         psim::check_where(where);
         if (!(dt > 0.0) || !std::isfinite(dt)) throw std::invalid_argument("time step must be finite and positive");
         hipStream_t stream = s->graph->ctx->stream;
-        const hipMemcpyKind kind = where == PSIM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
-        psim::hip_check(hipMemcpyAsync(s->entry_conductance, conductance_per_entry, static_cast<size_t>(s->graph->nnz) * 8,
-                                       kind, stream),
-                        "copy");
-        psim::hip_check(hipMemcpyAsync(s->capacity_area, capacity_area, static_cast<size_t>(s->graph->n) * 8, kind, stream),
-                        "copy");
         Launch p{};
         p.op = psim::dev::kOpSetMatrix | psim::dev::kOpHeatAssemble;
         p.dt = dt;
-        p.entry_conductance = s->entry_conductance;
-        p.capacity_area = s->capacity_area;
+        stage_heat_inputs(s, conductance_per_entry, capacity_area, where, p);
         launch(s, p);
         if (where == PSIM_HOST) psim::hip_check(hipStreamSynchronize(stream), "heat set_matrix");
         s->matrix_set = true;
@@ -747,18 +786,12 @@ int psim_heat_chain(psim_system* s, const double* conductance_per_entry, const d
         hipStream_t stream = s->graph->ctx->stream;
         const size_t n = static_cast<size_t>(s->graph->n), bytes = n * 8;
         int op = psim::dev::kOpSolve | psim::dev::kOpHeatRhs | psim::dev::kOpChain;
-        if (assemble) {
-            const hipMemcpyKind kind = where == PSIM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
-            psim::hip_check(hipMemcpyAsync(s->entry_conductance, conductance_per_entry,
-                                           static_cast<size_t>(s->graph->nnz) * 8, kind, stream), "copy");
-            psim::hip_check(hipMemcpyAsync(s->capacity_area, capacity_area, bytes, kind, stream), "copy");
-            op |= psim::dev::kOpSetMatrix | psim::dev::kOpHeatAssemble;
-        }
+        if (assemble) op |= psim::dev::kOpSetMatrix | psim::dev::kOpHeatAssemble;
         Launch p = solve_params(s, op, tolerance, max_iterations);
         p.steps = steps;
         p.dt = dt;
-        p.entry_conductance = s->entry_conductance;
         p.capacity_area = s->capacity_area;
+        if (assemble) stage_heat_inputs(s, conductance_per_entry, capacity_area, where, p);
         p.t_old = t_old;
         p.forcing = forcings;
         p.t_new = t_new;

@@ -124,6 +124,7 @@ struct Launch {
     double dt;
     const double* entry_conductance;   // K per CSR entry
     const double* capacity_area;       // C A per cell
+    double* capacity_keep;             // kOpHeatAssemble: where C A is kept for later steps, or null
     const double* t_old;
     const double* forcing;             // kOpChain: steps x n, step-major
     double* t_new;

@@ -601,7 +601,11 @@ __device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds
                     }
                     I.off64[0][s * n + i] = value;
                 }
-                if (P.op & kOpHeatAssemble) I.diag64[0][i] = P.capacity_area[i] + P.dt * sum;
+                if (P.op & kOpHeatAssemble) {
+                    const double ca = P.capacity_area[i];
+                    I.diag64[0][i] = ca + P.dt * sum;
+                    if (P.capacity_keep != nullptr) P.capacity_keep[i] = ca;
+                }
             }
         }
         group_sync(I, timeouts);
