@@ -231,8 +231,8 @@ int psim_heat_step(psim_system* system, const double* temperature_old, const dou
                    double tolerance, int32_t max_iterations, double* temperature_new, psim_solve_info* info,
                    int where);
 
-/* A chain of `steps` heat steps in one device launch (config 1's monthly
- * chain): step k solves with forcings[k * n ..] and the previous step's T,
+/* A chain of `steps` heat steps in one device launch (e.g. a year of
+ * monthly steps): step k solves with forcings[k * n ..] and the previous step's T,
  * and the last step's T is written to temperature_new. When
  * conductance_per_entry and capacity_area are both given, the same launch
  * first assembles the matrix as psim_heat_set_matrix does. step_info, if
@@ -241,6 +241,48 @@ int psim_heat_chain(psim_system* system, const double* conductance_per_entry, co
                     const double* temperature_old, const double* forcings, int32_t steps, double dt_s,
                     double tolerance, int32_t max_iterations, double* temperature_new, psim_solve_info* step_info,
                     int where);
+
+/* ---- Batches (no counterpart in the reference, whose solves run one at a
+ * time): many systems, of any graphs and precisions on one context, solved
+ * in ONE launch of the persistent solver. Each system owns a group of
+ * workgroups for the whole launch; its results are bit-identical to its own
+ * psim_heat_step (they do not depend on the group size or the other
+ * members). A batch holds at most as many systems as the device has
+ * co-resident solver workgroups (psim_batch_size). Systems stay owned by
+ * the caller and must outlive the batch; one system may belong to several
+ * batches, whose calls are ordered on the context's stream. */
+typedef struct psim_batch psim_batch;
+
+int psim_batch_create(psim_ctx* ctx, psim_system* const* systems, int32_t count, psim_batch** out);
+void psim_batch_destroy(psim_batch* batch);
+/* Number of systems and co-resident workgroups (either output may be NULL). */
+int psim_batch_size(const psim_batch* batch, int32_t* count, int32_t* capacity);
+/* Workgroups per system (count entries). */
+int psim_batch_groups(const psim_batch* batch, int32_t* groups);
+/* Set workgroups per system (each >= 1, sum <= capacity); NULL restores the
+ * default allocation (tiles per workgroup levelled greedily). */
+int psim_batch_set_groups(psim_batch* batch, const int32_t* groups);
+
+/* One heat step of every system in one launch. All arrays hold `count`
+ * DEVICE pointers. conductance_per_entry / capacity_area may be NULL, or
+ * hold NULL for a system: that system keeps its matrix (it must have one);
+ * otherwise its matrix is assembled first in the same launch. infos (host,
+ * count records) synchronises; NULL queues the work and returns (up to four
+ * calls may be in flight; psim_batch_synchronize waits for them). */
+int psim_batch_heat_step(psim_batch* batch, const double* const* conductance_per_entry,
+                         const double* const* capacity_area, const double* const* temperature_old,
+                         const double* const* forcing_W_m2, double dt_s, double tolerance, int32_t max_iterations,
+                         double* const* temperature_new, psim_solve_info* infos);
+int psim_batch_synchronize(psim_batch* batch);
+
+/* Autotuner: times candidate workgroup allocations (shares of the device and
+ * tile-count weightings) on this exact step, `repeats` launches each, and
+ * keeps the fastest (its mean launch time in best_ms). Outputs are written
+ * as by psim_batch_heat_step. */
+int psim_batch_autotune(psim_batch* batch, const double* const* conductance_per_entry,
+                        const double* const* capacity_area, const double* const* temperature_old,
+                        const double* const* forcing_W_m2, double dt_s, double tolerance, int32_t max_iterations,
+                        double* const* temperature_new, int32_t repeats, double* best_ms);
 
 #ifdef __cplusplus
 }

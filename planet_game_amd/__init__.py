@@ -17,10 +17,11 @@ import numpy as np
 __all__ = [
     "LIB_PATH", "lib", "PsimError", "Mesh", "Context", "FV", "Graph", "System", "SolveInfo",
     "HOST", "DEVICE", "F64", "F32", "PLAIN_FP64", "MIXED", "instance_seed", "heat_instance",
-    "heat_forcing", "Hierarchy",
+    "heat_forcing", "Hierarchy", "Batch", "DeviceArray",
 ]
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpsim.so")
+# PSIM_LIBRARY names another build of the same ABI (e.g. a tuning variant).
+LIB_PATH = os.environ.get("PSIM_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpsim.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C planet_game_amd)")
@@ -120,6 +121,15 @@ _sig("psim_system_solve", C.c_int, _vp, _vp, _vp, _dbl, _i32, C.POINTER(SolveInf
 _sig("psim_heat_set_matrix", C.c_int, _vp, _vp, _vp, _dbl, C.c_int)
 _sig("psim_heat_step", C.c_int, _vp, _vp, _vp, _dbl, _dbl, _i32, _vp, C.POINTER(SolveInfo), C.c_int)
 _sig("psim_heat_chain", C.c_int, _vp, _vp, _vp, _vp, _vp, _i32, _dbl, _dbl, _i32, _vp, _vp, C.c_int)
+_pp = C.POINTER(_vp)
+_sig("psim_batch_create", C.c_int, _vp, _pp, _i32, C.POINTER(_vp))
+_sig("psim_batch_destroy", None, _vp)
+_sig("psim_batch_size", C.c_int, _vp, C.POINTER(_i32), C.POINTER(_i32))
+_sig("psim_batch_groups", C.c_int, _vp, C.POINTER(_i32))
+_sig("psim_batch_set_groups", C.c_int, _vp, C.POINTER(_i32))
+_sig("psim_batch_heat_step", C.c_int, _vp, _pp, _pp, _pp, _pp, _dbl, _dbl, _i32, _pp, C.POINTER(SolveInfo))
+_sig("psim_batch_synchronize", C.c_int, _vp)
+_sig("psim_batch_autotune", C.c_int, _vp, _pp, _pp, _pp, _pp, _dbl, _dbl, _i32, _pp, _i32, C.POINTER(_dbl))
 
 
 class PsimError(RuntimeError):
@@ -497,4 +507,71 @@ class System:
     def __del__(self):
         if getattr(self, "handle", None):
             lib.psim_system_destroy(self.handle)
+            self.handle = None
+
+def _pointers(values):
+    """A ctypes array of device pointers (ints / DeviceArray / None)."""
+    out = (_vp * len(values))()
+    for i, v in enumerate(values):
+        out[i] = None if v is None else (v.ptr if isinstance(v, DeviceArray) else v)
+    return out
+
+
+class Batch:
+    """Many systems in one launch of the persistent solver (psim_batch)."""
+
+    def __init__(self, ctx: Context, systems):
+        self.systems = list(systems)
+        arr = (_vp * len(self.systems))(*[s.handle for s in self.systems])
+        h = _vp()
+        check(lib.psim_batch_create(ctx.handle, arr, len(self.systems), C.byref(h)))
+        self.handle, self.ctx = h, ctx
+
+    @property
+    def capacity(self):
+        cap = _i32()
+        check(lib.psim_batch_size(self.handle, None, C.byref(cap)))
+        return cap.value
+
+    @property
+    def groups(self):
+        g = (_i32 * len(self.systems))()
+        check(lib.psim_batch_groups(self.handle, g))
+        return [int(x) for x in g]
+
+    @groups.setter
+    def groups(self, values):
+        if values is None:
+            check(lib.psim_batch_set_groups(self.handle, None))
+        else:
+            check(lib.psim_batch_set_groups(self.handle, (_i32 * len(values))(*values)))
+
+    def _args(self, t_old, forcing, t_new, conductance, capacity):
+        k = None if conductance is None else _pointers(conductance)
+        c = None if capacity is None else _pointers(capacity)
+        return k, c, _pointers(t_old), _pointers(forcing), _pointers(t_new)
+
+    def heat_step(self, t_old, forcing, t_new, dt=2.63e6, tolerance=1e-10, max_iterations=1000,
+                  conductance=None, capacity=None, collect=True):
+        """One step of every system; per-system lists of device arrays or
+        pointers. Returns the SolveInfo records when `collect`, else queues."""
+        k, c, t0, f, t1 = self._args(t_old, forcing, t_new, conductance, capacity)
+        infos = (SolveInfo * len(self.systems))() if collect else None
+        check(lib.psim_batch_heat_step(self.handle, k, c, t0, f, dt, tolerance, max_iterations, t1, infos))
+        return list(infos) if collect else None
+
+    def synchronize(self):
+        check(lib.psim_batch_synchronize(self.handle))
+
+    def autotune(self, t_old, forcing, t_new, dt=2.63e6, tolerance=1e-10, max_iterations=1000,
+                 conductance=None, capacity=None, repeats=2):
+        k, c, t0, f, t1 = self._args(t_old, forcing, t_new, conductance, capacity)
+        best = _dbl()
+        check(lib.psim_batch_autotune(self.handle, k, c, t0, f, dt, tolerance, max_iterations, t1, repeats,
+                                      C.byref(best)))
+        return best.value
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.psim_batch_destroy(self.handle)
             self.handle = None

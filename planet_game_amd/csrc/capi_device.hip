@@ -1,6 +1,5 @@
 // C ABI: device context, transport graphs, linear systems and the heat step.
-#include "device.hpp"
-#include "internal.hpp"
+#include "system.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -27,83 +26,17 @@ void hip_check(hipError_t status, const char* what) {
 
 namespace {
 
-// One device allocation carved into 256-byte aligned arrays. Sizes are
-// recorded first (plan), then the block is allocated and the pointers set.
-class Arena {
-  public:
-    template <class T> void plan(T** slot, size_t count) {
-        slots_.push_back({reinterpret_cast<void**>(slot), bytes_});
-        bytes_ += (std::max<size_t>(count, 1) * sizeof(T) + 255) / 256 * 256;
-    }
-    void allocate() {
-        hip_check(hipMalloc(&block_, bytes_), "device allocation");
-        for (auto& s : slots_) *s.slot = static_cast<char*>(block_) + s.offset;
-    }
-    void release() {
-        if (block_ != nullptr) (void)hipFree(block_);
-        block_ = nullptr;
-    }
-    size_t bytes() const { return bytes_; }
-    void* block() const { return block_; }
-
-  private:
-    struct Slot {
-        void** slot;
-        size_t offset;
-    };
-    std::vector<Slot> slots_;
-    size_t bytes_ = 0;
-    void* block_ = nullptr;
-};
-
 void put(void* dst, const void* src, size_t bytes) {
     if (bytes != 0) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "upload");
 }
+
+}  // namespace
 
 void check_where(int where) {
     if (where != PSIM_HOST && where != PSIM_DEVICE) throw std::invalid_argument("unknown memory location");
 }
 
-}  // namespace
 }  // namespace psim
-
-struct psim_graph {
-    psim_ctx* ctx = nullptr;
-    psim::HostHierarchy host;
-    int32_t n = 0, nnz = 0, depth = 0;
-    psim::Arena arena;
-    double* area = nullptr;
-    double* weight = nullptr;
-    int32_t* csr_col = nullptr;   // level-0 CSR columns (the diffusion sweep reads CSR order)
-    LevelView views[kMaxLevels];
-    ~psim_graph() { arena.release(); }
-};
-
-struct psim_system {
-    psim_graph* graph = nullptr;
-    int precision = PSIM_PLAIN_FP64;
-    int32_t workgroups = 1;
-    int32_t map_capacity = 1;
-    size_t tri_bytes = 0;          // packed coarsest matrix
-    psim::Arena arena;
-    Instance host;                 // the uploaded descriptor
-    Instance* d_inst = nullptr;
-    int32_t* d_map = nullptr;      // workgroup -> instance (all 0)
-    SolveResult* d_result = nullptr;
-    double* entry_conductance = nullptr;
-    double* capacity_area = nullptr;
-    double* stage_a = nullptr;     // host-call staging (n)
-    double* stage_b = nullptr;
-    double* stage_c = nullptr;
-    double* csr_stage = nullptr;   // CSR-ordered off-diagonals for set_matrix
-    bool matrix_set = false;
-    bool heat_matrix = false;
-    void* trace_block = nullptr;   // uint32 count + uint64 stamps
-    ~psim_system() {
-        arena.release();
-        if (trace_block != nullptr) (void)hipFree(trace_block);
-    }
-};
 
 namespace {
 
@@ -115,6 +48,10 @@ __global__ __launch_bounds__(256) void copy_words_kernel(uint4* __restrict__ dst
     for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; w < words; w += stride)
         dst[w] = src[w];
 }
+
+}  // namespace
+
+namespace psim::sys {
 
 void launch(psim_system* s, const Launch& p) {
     psim_ctx* ctx = s->graph->ctx;
@@ -165,7 +102,7 @@ SolveResult fetch_result(psim_system* s) {
     return r;
 }
 
-void fill_info(const SolveResult& r, psim_solve_info* info, double clock_hz = 1.0e8) {
+void fill_info(const SolveResult& r, psim_solve_info* info, double clock_hz) {
     if (info == nullptr) return;
     info->device_ms = static_cast<double>(r.t_end - r.t_begin) / clock_hz * 1e3;
     info->iterations = r.iterations;
@@ -190,7 +127,9 @@ void require_matrix(const psim_system* s) {
     if (!s->matrix_set) throw std::logic_error("set_matrix has not been called");
 }
 
-}  // namespace
+}  // namespace psim::sys
+
+using namespace psim::sys;
 
 extern "C" {
 
@@ -359,7 +298,7 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
         A.plan(&gr->area, n);
         A.plan(&gr->weight, nnz);
         A.plan(&gr->csr_col, nnz);
-        int32_t* ptrs[kMaxLevels][12] = {};
+        int32_t* ptrs[kMaxLevels][13] = {};
         for (int j = 0; j < gr->depth; ++j) {
             const psim::HostLevel& L = lv[j];
             const bool last = j + 1 == gr->depth;
@@ -374,6 +313,7 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
                 A.plan(&ptrs[j][7], L.d_diag.size());
                 A.plan(&ptrs[j][8], L.d_off_start.size());
                 A.plan(&ptrs[j][9], L.d_off.size());
+                A.plan(&ptrs[j][12], L.agg_ell.size());
             }
             if (!L.need_start.empty()) {
                 A.plan(&ptrs[j][10], L.need_start.size());
@@ -399,7 +339,9 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
                 const std::vector<int32_t>* src[8] = {&L.aggregate, &L.agg_start, &L.agg_rows, &L.tile_start,
                                                       &L.g_diag_start, &L.d_diag, &L.d_off_start, &L.d_off};
                 for (int q = 0; q < 8; ++q) psim::put(ptrs[j][2 + q], src[q]->data(), src[q]->size() * 4);
+                psim::put(ptrs[j][12], L.agg_ell.data(), L.agg_ell.size() * 4);
                 v.agg = ptrs[j][2];
+                v.agg_col = ptrs[j][12];
                 v.agg_start = ptrs[j][3];
                 v.agg_rows = ptrs[j][4];
                 v.tile_start = ptrs[j][5];

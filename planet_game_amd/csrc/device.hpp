@@ -46,6 +46,7 @@ struct LevelView {
     const int32_t* row = nullptr;         // CSR offsets, n + 1 (CSR-ordered inputs)
     const int32_t* col = nullptr;         // ELL columns, width * n
     const int32_t* agg = nullptr;         // n (not on the coarsest level)
+    const int32_t* agg_col = nullptr;     // width * n: aggregate of each slot's column (-1 padding)
     const int32_t* agg_start = nullptr;   // n_next + 1
     const int32_t* agg_rows = nullptr;    // n, aggregate-major
     const int32_t* tile_start = nullptr;  // down_tiles + 1 (coarse-row ranges)

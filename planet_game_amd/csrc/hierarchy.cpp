@@ -216,6 +216,12 @@ HostHierarchy build_hierarchy(const HostGraph& g) {
         h.levels.push_back(std::move(coarse));
     }
     for (HostLevel& lv : h.levels) ell_columns(lv);
+    for (size_t j = 0; j + 1 < h.levels.size(); ++j) {
+        HostLevel& lv = h.levels[j];
+        lv.agg_ell.assign(lv.col_ell.size(), -1);
+        for (size_t p = 0; p < lv.col_ell.size(); ++p)
+            if (lv.col_ell[p] >= 0) lv.agg_ell[p] = lv.aggregate[static_cast<size_t>(lv.col_ell[p])];
+    }
     for (size_t j = 0; j + 1 < h.levels.size(); ++j) ell_galerkin(h.levels[j], h.levels[j + 1]);
     if (h.levels.size() >= 2) {
         HostLevel& lv = h.levels[h.levels.size() - 2];

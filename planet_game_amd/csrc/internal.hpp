@@ -119,6 +119,7 @@ struct HostLevel {
     // the NEXT level's ELL positions (empty for padding).
     int32_t width = 0;
     std::vector<int32_t> col_ell;
+    std::vector<int32_t> agg_ell;                // aggregate of each ELL slot's column (-1 padding)
     std::vector<int32_t> d_diag;                 // g_diag in ELL positions
     std::vector<int32_t> d_off_start, d_off;     // per next-level ELL position
     // On the level just above the coarsest: for each 256-row tile, the

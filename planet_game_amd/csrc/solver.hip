@@ -238,6 +238,41 @@ template <class VT> struct Source {
     }
 };
 
+// Row i of an ELL level: fold(off_s, gather(s, c_s)) over the row's slots
+// in order, skipping padding (c_s < 0). All column and value loads are
+// issued first, then every slot's gather (a padding slot gathers at row i,
+// a valid address, and is discarded), then the folds, so a row costs two
+// dependent memory round trips instead of two per slot. Fold order and
+// every rounding are those of the plain slot loop.
+constexpr int kMaxWidth = 8;
+
+template <class T, class Gather, class Fold>
+__device__ inline void ell_row(const int32_t* col, const T* off, int n, int W, int i, const Gather& gather,
+                               const Fold& fold) {
+    if (W <= kMaxWidth) {
+        int c[kMaxWidth];
+        T o[kMaxWidth];
+#pragma unroll
+        for (int s = 0; s < kMaxWidth; ++s)
+            if (s < W) {
+                c[s] = col[s * n + i];
+                o[s] = off[s * n + i];
+            }
+        decltype(gather(0, 0)) v[kMaxWidth];
+#pragma unroll
+        for (int s = 0; s < kMaxWidth; ++s)
+            if (s < W) v[s] = gather(s, c[s] >= 0 ? c[s] : i);
+#pragma unroll
+        for (int s = 0; s < kMaxWidth; ++s)
+            if (s < W && c[s] >= 0) fold(o[s], v[s]);
+    } else {
+        for (int s = 0; s < W; ++s) {
+            const int cs = col[s * n + i];
+            if (cs >= 0) fold(off[s * n + i], gather(s, cs));
+        }
+    }
+}
+
 // ------------------------------------------------------------ V-cycle phases
 
 // Pre-smoothing from zero, residual and restriction of level j in one sweep
@@ -245,7 +280,7 @@ template <class VT> struct Source {
 // Source; each row stores its value in `keep` (if set) and adds its square
 // to the tile partial (if `part` is set).
 template <class VT>
-__device__ void down_phase(const Instance& I, const Values<VT>& V, int j, const Source<VT>& src, VT* keep,
+__device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& V, int j, const Source<VT>& src, VT* keep,
                            VT* coarse_rhs, double* part, int g, int G, Lds& lds) {
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
@@ -264,10 +299,9 @@ __device__ void down_phase(const Instance& I, const Values<VT>& V, int j, const 
             sq[0] = static_cast<double>(wi) * static_cast<double>(wi);
             const VT xi = om * dinv[i] * wi;
             VT acc = wi - diag[i] * xi;
-            for (int s = 0; s < W; ++s) {
-                const int c = L.col[s * n + i];
-                if (c >= 0) acc -= off[s * n + i] * (om * dinv[c] * src.at(c));
-            }
+            ell_row(
+                L.col, off, n, W, i, [&](int, int c) { return om * dinv[c] * src.at(c); },
+                [&](VT o, VT v) { acc -= o * v; });
             lds.tile[threadIdx.x] = static_cast<double>(acc);
         }
         if (part != nullptr) store_partials<1>(part, tile, sq, lds);
@@ -288,7 +322,7 @@ __device__ void down_phase(const Instance& I, const Values<VT>& V, int j, const 
 // butterfly sums them, so every workgroup computing a row gets the same
 // bits. `list` null means rows first .. first + count - 1.
 template <class VT>
-__device__ void coarse_rows_lds(const Instance& I, const VT* rhs, const int32_t* list, int first, int count,
+__device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs, const int32_t* list, int first, int count,
                                 Lds& lds) {
     const int n = I.lv[I.depth - 1].n;
     for (int k = threadIdx.x; k < n; k += blockDim.x) lds.crhs[k] = static_cast<double>(rhs[k]);
@@ -322,7 +356,7 @@ __device__ void coarse_rows_lds(const Instance& I, const VT* rhs, const int32_t*
 // j. The correction comes from `corr`, or, on the level above the coarsest,
 // from the coarse solve each workgroup recomputes into LDS.
 template <class VT>
-__device__ void up_phase(const Instance& I, const Values<VT>& V, int j, const VT* rhs, const VT* corr, VT* out, int g,
+__device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V, int j, const VT* rhs, const VT* corr, VT* out, int g,
                          int G, Lds& lds) {
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
@@ -341,10 +375,15 @@ __device__ void up_phase(const Instance& I, const Values<VT>& V, int j, const VT
             const auto correction = [&](int a) { return fused ? static_cast<VT>(lds.cvec[a]) : corr[a]; };
             const VT x2 = om * dinv[i] * rhs[i] + correction(L.agg[i]);
             VT acc = rhs[i] - diag[i] * x2;
-            for (int s = 0; s < W; ++s) {
-                const int c = L.col[s * n + i];
-                if (c >= 0) acc -= off[s * n + i] * (om * dinv[c] * rhs[c] + correction(L.agg[c]));
-            }
+            // the neighbour's aggregate comes from the slot's own map, so the
+            // correction gather does not wait for the column
+            ell_row(
+                L.col, off, n, W, i,
+                [&](int s, int c) {
+                    const int a = L.agg_col[s * n + i];
+                    return om * dinv[c] * rhs[c] + correction(a >= 0 ? a : 0);
+                },
+                [&](VT o, VT v) { acc -= o * v; });
             out[i] = x2 + om * dinv[i] * acc;
         }
     }
@@ -354,7 +393,7 @@ __device__ void up_phase(const Instance& I, const Values<VT>& V, int j, const VT
 // `src`, stored into `keep`, squares into `part`); on a one-level hierarchy
 // the input is only materialised. Ends with a barrier.
 template <class VT>
-__device__ void vcycle_first(const Instance& I, const Values<VT>& V, const Source<VT>& src, VT* keep, double* part,
+__device__ __forceinline__ void vcycle_first(const Instance& I, const Values<VT>& V, const Source<VT>& src, VT* keep, double* part,
                              int g, int G, Lds& lds, int& timeouts) {
     if (I.depth == 1) {
         for (int tile = g; tile < I.tiles; tile += G) {
@@ -379,7 +418,7 @@ __device__ inline int first_sweep_tiles(const Instance& I) { return I.depth == 1
 // The rest of the V-cycle after vcycle_first; w0 is the materialised level-0
 // input, out0 the level-0 output. Ends with a barrier.
 template <class VT>
-__device__ void vcycle_rest(const Instance& I, const Values<VT>& V, const VT* w0, VT* out0, int g, int G, Lds& lds,
+__device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>& V, const VT* w0, VT* out0, int g, int G, Lds& lds,
                             int& timeouts) {
     const int J = I.depth;
     if (J == 1) {
@@ -701,15 +740,13 @@ __device__ inline double residual_row(const Instance& I, const double* x, int i)
     const LevelView& L = I.lv[0];
     const int n = L.n, W = L.width;
     double ax = I.diag64[0][i] * x[i];
-    for (int s = 0; s < W; ++s) {
-        const int c = L.col[s * n + i];
-        if (c >= 0) ax += I.off64[0][s * n + i] * x[c];
-    }
+    ell_row(
+        L.col, I.off64[0], n, W, i, [&](int, int c) { return x[c]; }, [&](double o, double v) { ax += o * v; });
     return I.b[i] - ax;
 }
 
 template <class VT>
-__device__ SolveResult solve(const Instance& I, const Launch& P, const double* t_old, const double* forcing,
+__device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P, const double* t_old, const double* forcing,
                              double* t_new, int g, int G, Lds& lds, int& timeouts, int& parity, int& xi) {
     const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
     const Values<VT> V{I};
@@ -732,10 +769,8 @@ __device__ SolveResult solve(const Instance& I, const Launch& P, const double* t
     };
     const auto spmv_row = [&](const VT* in, int i) {
         VT acc = diag[i] * in[i];
-        for (int q = 0; q < W; ++q) {
-            const int c = L0.col[q * n + i];
-            if (c >= 0) acc += off[q * n + i] * in[c];
-        }
+        ell_row(
+            L0.col, off, n, W, i, [&](int, int c) { return in[c]; }, [&](VT o, VT v) { acc += o * v; });
         return acc;
     };
 
@@ -859,10 +894,10 @@ __device__ SolveResult solve(const Instance& I, const Launch& P, const double* t
                             const double x_new = x[i] + alpha * static_cast<double>(ph[i]);
                             xn[i] = x_new;
                             double ax = I.diag64[0][i] * x_new;
-                            for (int k = 0; k < W; ++k) {
-                                const int c = L0.col[k * n + i];
-                                if (c >= 0) ax += I.off64[0][k * n + i] * (x[c] + alpha * static_cast<double>(ph[c]));
-                            }
+                            ell_row(
+                                L0.col, I.off64[0], n, W, i,
+                                [&](int, int c) { return x[c] + alpha * static_cast<double>(ph[c]); },
+                                [&](double o, double v) { ax += o * v; });
                             const double ri = b[i] - ax;
                             const VT rv = static_cast<VT>(ri);
                             r[i] = rv;
@@ -948,7 +983,7 @@ __device__ SolveResult solve(const Instance& I, const Launch& P, const double* t
 // ------------------------------------------------------------ the kernel
 
 template <class VT>
-__device__ void run_solve(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts, int& parity) {
+__device__ __forceinline__ void run_solve(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts, int& parity) {
     int xi = 0;
     const int steps = (P.op & kOpChain) ? P.steps : 1;
     SolveResult last{};
@@ -976,7 +1011,10 @@ __device__ void run_solve(const Instance& I, const Launch& P, int g, int G, Lds&
     }
 }
 
-__global__ __launch_bounds__(kTile) void program_kernel(const Instance* instances, const int32_t* block_instance,
+// Three workgroups per CU (12 waves): the register budget that keeps the
+// solver's live state out of scratch (a tighter cap spills it) while
+// leaving batches 768 co-resident workgroups.
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void program_kernel(const Instance* instances, const int32_t* block_instance,
                                                         const Launch* launches, Launch single) {
     __shared__ Lds lds;
     __shared__ Launch params;
@@ -1024,10 +1062,9 @@ __global__ __launch_bounds__(kTile) void program_kernel(const Instance* instance
             const int i = tile * kTile + threadIdx.x;
             if (i < n) {
                 double acc = I.diag64[0][i] * P.in[i];
-                for (int s = 0; s < W; ++s) {
-                    const int c = L.col[s * n + i];
-                    if (c >= 0) acc += I.off64[0][s * n + i] * P.in[c];
-                }
+                ell_row(
+                    L.col, I.off64[0], n, W, i, [&](int, int c) { return P.in[c]; },
+                    [&](double o, double v) { acc += o * v; });
                 P.out[i] = acc;
             }
         }
