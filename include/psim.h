@@ -284,6 +284,55 @@ int psim_batch_autotune(psim_batch* batch, const double* const* conductance_per_
                         const double* const* forcing_W_m2, double dt_s, double tolerance, int32_t max_iterations,
                         double* const* temperature_new, int32_t repeats, double* best_ms);
 
+/* ---- Serving (no counterpart in the reference): continuous batching of
+ * heat-step requests on one persistent launch. The server splits the
+ * device's solver workgroups into slots per graph (slot size from the
+ * graph's tiles, slot count from the expected share of requests x tiles)
+ * and keeps, per graph, a pool of systems sized by an HBM budget: the
+ * graphs' hierarchies and every live session's matrix and solver state stay
+ * resident. A session's first request assembles its matrix on a free
+ * system of its graph (admission waits for one); its later steps reuse it;
+ * its last request frees it. Requests that arrive while others run start as
+ * soon as a slot of their graph is free, and run exactly as they would
+ * alone (the same result bits). Graphs must have a coarsest level of at
+ * most 168 rows (all icosphere levels do). */
+typedef struct psim_server psim_server;
+
+typedef struct {
+    double arrival_s;          /* arrival time, seconds after the run starts (non-decreasing) */
+    int32_t graph;             /* index into the server's graphs */
+    int32_t session;           /* requests of one session share a resident system, in order */
+    int32_t first, last;       /* first: assemble from conductance/capacity; last: free the system */
+    const double* conductance_per_entry;   /* device, first request only */
+    const double* capacity_area;           /* device, first request only */
+    const double* temperature_old;         /* device (or device-visible pinned host) memory */
+    const double* forcing_W_m2;
+    double* temperature_new;
+} psim_serve_request;
+
+typedef struct {
+    double arrival_s, submitted_s, completed_s;   /* host clock, seconds after the run starts */
+    double device_ms;                             /* the solve on the device clock */
+    int32_t status, iterations, restarts, v_cycles;
+    double relative_residual;
+} psim_serve_record;
+
+/* request_share: expected fraction of requests per graph (NULL: equal).
+ * memory_budget_bytes: HBM for pooled systems, split by slots x bytes per
+ * system (at least one system per slot, at most 4096 per graph; 0 = one
+ * per slot). */
+int psim_server_create(psim_ctx* ctx, psim_graph* const* graphs, const double* request_share, int32_t graph_count,
+                       int precision, uint64_t memory_budget_bytes, psim_server** out);
+void psim_server_destroy(psim_server* server);
+/* Per graph: slots, workgroups per slot, pool systems, bytes per system
+ * (any output may be NULL). */
+int psim_server_plan(const psim_server* server, int32_t* slots, int32_t* slot_size, int32_t* pool,
+                     uint64_t* system_bytes);
+/* Serve `count` requests (sorted by arrival) in real time on one persistent
+ * launch and fill one record per request; returns when all are done. */
+int psim_server_run(psim_server* server, const psim_serve_request* requests, int32_t count, double dt_s,
+                    double tolerance, int32_t max_iterations, psim_serve_record* records);
+
 #ifdef __cplusplus
 }
 #endif

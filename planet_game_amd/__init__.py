@@ -9,15 +9,17 @@ instead of falling back to anything.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
 __all__ = [
     "LIB_PATH", "lib", "PsimError", "Mesh", "Context", "FV", "Graph", "System", "SolveInfo",
     "HOST", "DEVICE", "F64", "F32", "PLAIN_FP64", "MIXED", "instance_seed", "heat_instance",
-    "heat_forcing", "Hierarchy", "Batch", "DeviceArray",
+    "heat_forcing", "Hierarchy", "Batch", "DeviceArray", "Server", "ServeRequest", "ServeRecord",
 ]
 
 # PSIM_LIBRARY names another build of the same ABI (e.g. a tuning variant).
@@ -27,6 +29,26 @@ if not os.path.exists(LIB_PATH):
         f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C planet_game_amd)")
 
 lib = C.CDLL(LIB_PATH)
+
+# Live native objects by class. At interpreter exit they are released here,
+# dependants first, while ctypes and the HIP runtime are still whole (object
+# finalisation during interpreter teardown is too late to call into them).
+_LIVE = {}
+
+
+def _track(obj):
+    _LIVE.setdefault(type(obj).__name__, weakref.WeakSet()).add(obj)
+    return obj
+
+
+@atexit.register
+def _release_all():
+    for name in ("Server", "Batch", "System", "FV", "Graph", "DeviceArray", "Context", "Mesh"):
+        for obj in list(_LIVE.get(name, ())):
+            try:
+                obj.close()
+            except Exception:  # noqa: BLE001 - best effort at exit
+                pass
 
 HOST, DEVICE = 0, 1
 F64, F32 = 0, 1
@@ -122,6 +144,24 @@ _sig("psim_heat_set_matrix", C.c_int, _vp, _vp, _vp, _dbl, C.c_int)
 _sig("psim_heat_step", C.c_int, _vp, _vp, _vp, _dbl, _dbl, _i32, _vp, C.POINTER(SolveInfo), C.c_int)
 _sig("psim_heat_chain", C.c_int, _vp, _vp, _vp, _vp, _vp, _i32, _dbl, _dbl, _i32, _vp, _vp, C.c_int)
 _pp = C.POINTER(_vp)
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("graph", C.c_int32), ("session", C.c_int32), ("first", C.c_int32),
+                ("last", C.c_int32), ("conductance_per_entry", _vp), ("capacity_area", _vp),
+                ("temperature_old", _vp), ("forcing_W_m2", _vp), ("temperature_new", _vp)]
+
+
+class ServeRecord(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("submitted_s", C.c_double), ("completed_s", C.c_double),
+                ("device_ms", C.c_double), ("status", C.c_int32), ("iterations", C.c_int32),
+                ("restarts", C.c_int32), ("v_cycles", C.c_int32), ("relative_residual", C.c_double)]
+
+
+_sig("psim_server_create", C.c_int, _vp, _pp, C.POINTER(_dbl), _i32, C.c_int, _u64, C.POINTER(_vp))
+_sig("psim_server_destroy", None, _vp)
+_sig("psim_server_plan", C.c_int, _vp, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_u64))
+_sig("psim_server_run", C.c_int, _vp, C.POINTER(ServeRequest), _i32, _dbl, _dbl, _i32, C.POINTER(ServeRecord))
 _sig("psim_batch_create", C.c_int, _vp, _pp, _i32, C.POINTER(_vp))
 _sig("psim_batch_destroy", None, _vp)
 _sig("psim_batch_size", C.c_int, _vp, C.POINTER(_i32), C.POINTER(_i32))
@@ -154,6 +194,7 @@ class Mesh:
         h = _vp()
         check(lib.psim_mesh_create(subdivision, radius, C.byref(h)))
         self.handle = h
+        _track(self)
         self.subdivision = subdivision
         self.radius = radius
         n, e, d = _u64(), _u64(), _u64()
@@ -189,10 +230,14 @@ class Mesh:
         """Expand a per-edge array to the CSR entries (cell-edge order)."""
         return np.ascontiguousarray(per_edge[self.edge])
 
-    def __del__(self):
+    def close(self):
+        """Release the native object now (also done at exit and on collection)."""
         if getattr(self, "handle", None):
             lib.psim_mesh_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        self.close()
 
 
 class Hierarchy:
@@ -251,6 +296,7 @@ class Context:
         h = _vp()
         check(lib.psim_ctx_create(device, C.byref(h)))
         self.handle = h
+        _track(self)
 
     def synchronize(self):
         check(lib.psim_ctx_synchronize(self.handle))
@@ -275,10 +321,14 @@ class Context:
         check(lib.psim_ctx_elapsed(self.handle, start, end, C.byref(ms)))
         return ms.value
 
-    def __del__(self):
+    def close(self):
+        """Release the native object now (also done at exit and on collection)."""
         if getattr(self, "handle", None):
             lib.psim_ctx_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        self.close()
 
 
 class DeviceArray:
@@ -292,6 +342,7 @@ class DeviceArray:
         p = _vp()
         check(lib.psim_device_alloc(ctx.handle, self.nbytes, C.byref(p)))
         self.ptr = p
+        _track(self)
 
     @classmethod
     def of(cls, ctx, array):
@@ -313,10 +364,14 @@ class DeviceArray:
     def offset(self, elements: int):
         return _vp(self.ptr.value + elements * self.dtype.itemsize)
 
-    def __del__(self):
+    def close(self):
+        """Release the native object now (also done at exit and on collection)."""
         if getattr(self, "ptr", None):
             lib.psim_device_free(self.ctx.handle, self.ptr)
             self.ptr = None
+
+    def __del__(self):
+        self.close()
 
 
 class FV:
@@ -326,6 +381,7 @@ class FV:
         h = _vp()
         check(lib.psim_fv_create(ctx.handle, mesh.handle, C.byref(h)))
         self.handle, self.ctx, self.mesh = h, ctx, mesh
+        _track(self)
 
     @staticmethod
     def _dtype(a):
@@ -357,10 +413,14 @@ class FV:
         check(lib.psim_edge_orientation(self.handle, cell, edge, C.byref(out)))
         return out.value
 
-    def __del__(self):
+    def close(self):
+        """Release the native object now (also done at exit and on collection)."""
         if getattr(self, "handle", None):
             lib.psim_fv_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        self.close()
 
 
 class Graph:
@@ -371,6 +431,7 @@ class Graph:
         check(lib.psim_graph_create(ctx.handle, mesh.cells, mesh.area, mesh.offset,
                                     mesh.graph_neighbour, mesh.weight, mesh.weight_sum, C.byref(h)))
         self.handle, self.ctx, self.mesh = h, ctx, mesh
+        _track(self)
         self.size = lib.psim_graph_size(h)
         self.entries = lib.psim_graph_entries(h)
         self.depth = lib.psim_graph_depth(h)
@@ -381,10 +442,14 @@ class Graph:
         check(lib.psim_diffusion_source(self.handle, conductance, _ptr(t), _ptr(out), HOST))
         return out
 
-    def __del__(self):
+    def close(self):
+        """Release the native object now (also done at exit and on collection)."""
         if getattr(self, "handle", None):
             lib.psim_graph_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        self.close()
 
 
 class System:
@@ -394,6 +459,7 @@ class System:
         h = _vp()
         check(lib.psim_system_create(graph.handle, precision, C.byref(h)))
         self.handle, self.graph, self.precision = h, graph, precision
+        _track(self)
 
     @property
     def workgroups(self):
@@ -504,10 +570,14 @@ class System:
                                  _ptr(out), C.byref(info), HOST))
         return out, info
 
-    def __del__(self):
+    def close(self):
+        """Release the native object now (also done at exit and on collection)."""
         if getattr(self, "handle", None):
             lib.psim_system_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        self.close()
 
 def _pointers(values):
     """A ctypes array of device pointers (ints / DeviceArray / None)."""
@@ -526,6 +596,7 @@ class Batch:
         h = _vp()
         check(lib.psim_batch_create(ctx.handle, arr, len(self.systems), C.byref(h)))
         self.handle, self.ctx = h, ctx
+        _track(self)
 
     @property
     def capacity(self):
@@ -571,7 +642,50 @@ class Batch:
                                       C.byref(best)))
         return best.value
 
-    def __del__(self):
+    def close(self):
+        """Release the native object now (also done at exit and on collection)."""
         if getattr(self, "handle", None):
             lib.psim_batch_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+class Server:
+    """Continuous batching of heat-step requests on one persistent launch
+    (psim_server): slots per graph, pooled resident systems, real-time
+    arrivals. `graphs` are Graph objects; `share` the expected fraction of
+    requests per graph."""
+
+    def __init__(self, ctx: Context, graphs, share=None, precision=MIXED, memory_budget=4 << 30):
+        self.ctx, self.graphs = ctx, list(graphs)
+        arr = (_vp * len(self.graphs))(*[g.handle for g in self.graphs])
+        sh = None if share is None else (_dbl * len(self.graphs))(*share)
+        h = _vp()
+        check(lib.psim_server_create(ctx.handle, arr, sh, len(self.graphs), precision, memory_budget, C.byref(h)))
+        self.handle = h
+        _track(self)
+
+    def plan(self):
+        k = len(self.graphs)
+        slots, size, pool, nbytes = (_i32 * k)(), (_i32 * k)(), (_i32 * k)(), (_u64 * k)()
+        check(lib.psim_server_plan(self.handle, slots, size, pool, nbytes))
+        return [dict(slots=slots[q], slot_size=size[q], pool=pool[q], system_bytes=nbytes[q]) for q in range(k)]
+
+    def run(self, requests, dt=2.63e6, tolerance=1e-10, max_iterations=1000):
+        """requests: sequence of ServeRequest (sorted by arrival). Returns
+        the ServeRecord array."""
+        reqs = (ServeRequest * len(requests))(*requests)
+        recs = (ServeRecord * len(requests))()
+        check(lib.psim_server_run(self.handle, reqs, len(requests), dt, tolerance, max_iterations, recs))
+        return recs
+
+    def close(self):
+        """Release the native object now (also done at exit and on collection)."""
+        if getattr(self, "handle", None):
+            lib.psim_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()

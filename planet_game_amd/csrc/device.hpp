@@ -132,6 +132,8 @@ struct Launch {
     const double* in;   // b for kOpSolve, input of precondition / apply, CSR values
     double* out;        // output of precondition / apply
     SolveResult* step_results;         // kOpChain: one record per step, or null
+    uint64_t ticket;                   // server: echoed in the completion record
+    int64_t completion;                // server: completion record index
 };
 
 // Launch the persistent program. Workgroup w serves instance
@@ -148,6 +150,76 @@ int program_blocks_per_cu(size_t dynamic_lds);
 size_t program_static_lds();
 
 constexpr int kBarrierWordCount = 64 + 32 * 8;   // generation, root, 8 leaves (128-byte apart)
+
+// ---- server: a persistent launch whose workgroups are partitioned into
+// slots; each slot takes requests from one queue, one at a time.
+//
+// Host -> device: request rings and their `tail` counters live in pinned,
+// device-visible host memory; the host fills an entry, then publishes it by
+// storing tail + 1 (release). Slot leaders claim entries by incrementing
+// the queue's `head` in device memory. Device -> host: a completion record
+// per request in pinned memory, whose `done` word (ticket + 1) is stored
+// last (system-scope release).
+constexpr int kMaxQueues = 8;
+
+struct ServeRequest {
+    int32_t state;              // index into the server's instance table
+    int32_t op;                 // kOpSolve | kOpHeatRhs, plus kOpSetMatrix | kOpHeatAssemble to assemble
+    int32_t max_iterations, max_restarts;
+    double dt, tolerance, inner_relative;
+    const double* entry_conductance;
+    const double* capacity_area;
+    double* capacity_keep;
+    const double* t_old;
+    const double* forcing;
+    double* t_new;
+    uint64_t ticket;
+    int32_t completion;         // index of the completion record to write
+    int32_t pad;
+};
+static_assert(sizeof(ServeRequest) % 8 == 0, "requests are read as 64-bit words");
+
+struct ServeCompletion {
+    uint64_t ticket;
+    int32_t status, iterations, restarts, v_cycles;
+    double relative_residual;
+    uint64_t t_begin, t_end;    // device wall clock
+    int32_t barrier_timeouts, pad;
+    uint64_t done;              // ticket + 1 once the record is complete
+};
+static_assert(sizeof(ServeCompletion) % 8 == 0, "completions are written as 64-bit words");
+
+struct ServeQueue {
+    const ServeRequest* ring;   // pinned host memory, `capacity` entries
+    const uint64_t* tail;       // pinned host memory: entries published
+    uint64_t* head;             // device memory: entries claimed
+    int32_t capacity;
+    int32_t pad;
+};
+
+struct ServeSlot {
+    int32_t first, size, queue, pad;
+    uint32_t* barrier;          // the slot's group barrier words (zeroed)
+    uint32_t* mailbox;          // [0] generation, [32] ring position, [64] command (0 run, 1 exit)
+    SolveResult* result;        // the slot's result record
+};
+
+struct ServeParams {
+    const Instance* table;      // instance descriptors (static while serving)
+    const ServeSlot* slots;
+    const int32_t* block_slot;  // workgroup -> slot
+    ServeQueue queues[kMaxQueues];
+    ServeCompletion* completions;   // pinned host memory, indexed by ticket % capacity
+    int32_t completion_capacity;
+    int32_t pad;
+    const uint32_t* stop;       // pinned host memory: nonzero ends the launch once slots are idle
+    uint64_t lifetime;          // wall-clock ticks after its start that an idle slot gives up
+};
+
+// Launch the server (cooperative, `blocks` = sum of slot sizes).
+hipError_t launch_server(const ServeParams& params, int32_t blocks, hipStream_t stream);
+// Co-resident server workgroups per compute unit.
+int server_blocks_per_cu();
 
 void launch_diffusion(int n, const int32_t* row, const int32_t* col, const double* weight, const double* area,
                       double conductance, const double* t, double* out, hipStream_t stream);

@@ -133,13 +133,33 @@ __device__ inline double group_max(const double* partial, int tiles, Lds& lds) {
 
 // ------------------------------------------------------------ group barrier
 
-// Phase trace: workgroup 0 of a traced instance stamps the wall clock at
-// every group barrier (after it completes) into I.trace.
-__device__ inline void trace_stamp(const Instance& I) {
-    if (I.trace != nullptr && threadIdx.x == 0 && static_cast<int>(blockIdx.x) == I.group_first) {
-        const uint32_t slot = I.trace_count[0];
-        if (slot < static_cast<uint32_t>(I.trace_capacity)) I.trace[slot] = static_cast<uint64_t>(wall_clock64());
-        I.trace_count[0] = slot + 1;
+// The group this workgroup belongs to while it serves an instance: its
+// workgroups, barrier words, result record and phase trace. Set by the
+// kernel entry (from the instance descriptor, or from a server slot) and
+// read from LDS, so every use is a uniform LDS read.
+struct Group {
+    int32_t first, size;
+    uint32_t* barrier;
+    SolveResult* result;
+    uint64_t* trace;
+    uint32_t* trace_count;
+    int32_t trace_capacity;
+};
+__shared__ Group grp;
+
+__device__ inline void set_group(int32_t first, int32_t size, uint32_t* barrier, SolveResult* result, uint64_t* trace,
+                                 uint32_t* trace_count, int32_t trace_capacity) {
+    if (threadIdx.x == 0) grp = Group{first, size, barrier, result, trace, trace_count, trace_capacity};
+    __syncthreads();
+}
+
+// Phase trace: workgroup 0 of a traced group stamps the wall clock at every
+// group barrier (after it completes).
+__device__ inline void trace_stamp() {
+    if (grp.trace != nullptr && threadIdx.x == 0 && static_cast<int>(blockIdx.x) == grp.first) {
+        const uint32_t slot = grp.trace_count[0];
+        if (slot < static_cast<uint32_t>(grp.trace_capacity)) grp.trace[slot] = static_cast<uint64_t>(wall_clock64());
+        grp.trace_count[0] = slot + 1;
     }
 }
 
@@ -161,21 +181,21 @@ constexpr long long kBarrierTicks = 100000000;
 constexpr int kLeaves = 8;
 static_assert(64 + 32 * kLeaves == kBarrierWordCount, "barrier layout");
 
-__device__ void group_sync(const Instance& I, int& timeouts) {
+__device__ void group_sync(int& timeouts) {
     __syncthreads();
-    const int G = I.group_size;
+    const int G = grp.size;
     if (G == 1) {
-        trace_stamp(I);
+        trace_stamp();
         return;
     }
     if (threadIdx.x == 0 && timeouts == 0) {
-        uint32_t* gen = I.barrier;
-        uint32_t* root = I.barrier + 32;
-        const int g = static_cast<int>(blockIdx.x) - I.group_first;
+        uint32_t* gen = grp.barrier;
+        uint32_t* root = grp.barrier + 32;
+        const int g = static_cast<int>(blockIdx.x) - grp.first;
         const int leaf = g % kLeaves;
         const int leaves = G < kLeaves ? G : kLeaves;
         const uint32_t leaf_size = static_cast<uint32_t>((G - leaf + kLeaves - 1) / kLeaves);
-        uint32_t* mine = I.barrier + 64 + 32 * leaf;
+        uint32_t* mine = grp.barrier + 64 + 32 * leaf;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const uint32_t seen = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         bool last = false;
@@ -200,7 +220,7 @@ __device__ void group_sync(const Instance& I, int& timeouts) {
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    trace_stamp(I);
+    trace_stamp();
     __syncthreads();
 }
 
@@ -409,7 +429,7 @@ __device__ __forceinline__ void vcycle_first(const Instance& I, const Values<VT>
     } else {
         down_phase<VT>(I, V, 0, src, keep, static_cast<VT*>(I.lvl_b[1]), part, g, G, lds);
     }
-    group_sync(I, timeouts);
+    group_sync(timeouts);
 }
 
 // Tiles the first sweep's partials cover.
@@ -428,7 +448,7 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
             const int i = first + threadIdx.x;
             if (i < I.n) out0[i] = static_cast<VT>(lds.cvec[i]);
         }
-        group_sync(I, timeouts);
+        group_sync(timeouts);
         return;
     }
     const Source<VT> none{nullptr, nullptr, nullptr, 0.0, 0.0, 0};
@@ -436,13 +456,13 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
         Source<VT> src = none;
         src.a = static_cast<const VT*>(I.lvl_b[j]);
         down_phase<VT>(I, V, j, src, nullptr, static_cast<VT*>(I.lvl_b[j + 1]), nullptr, g, G, lds);
-        group_sync(I, timeouts);
+        group_sync(timeouts);
     }
     for (int j = J - 2; j >= 0; --j) {
         const VT* rhs = j == 0 ? w0 : static_cast<const VT*>(I.lvl_b[j]);
         VT* out = j == 0 ? out0 : static_cast<VT*>(I.lvl_e[j]);
         up_phase<VT>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds);
-        group_sync(I, timeouts);
+        group_sync(timeouts);
     }
 }
 
@@ -647,7 +667,7 @@ __device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds
                 }
             }
         }
-        group_sync(I, timeouts);
+        group_sync(timeouts);
     }
     for (int j = 0; j < J; ++j) {
         const LevelView& L = I.lv[j];
@@ -717,7 +737,7 @@ __device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds
                 }
             }
         }
-        group_sync(I, timeouts);
+        group_sync(timeouts);
         const double rho = group_max(part, tiles, lds);
         // Chebyshev degree-1 weight on [a, b] = [max(1 - rho, (1 + rho) / 4), 1 + rho]
         const double hi = 1.0 + rho;
@@ -730,7 +750,7 @@ __device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds
         else
             coarse_inverse(I, I.tri_lds ? dynamic_lds() : I.tri, lds);
     }
-    group_sync(I, timeouts);
+    group_sync(timeouts);
 }
 
 // ------------------------------------------------------------ BiCGSTAB
@@ -796,7 +816,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
             store_partials<2>(part, tile, q, lds);
         }
     }
-    group_sync(I, timeouts);
+    group_sync(timeouts);
     double tot2[2];
     group_totals<2>(part, tiles, tot2, lds);
     const double b_norm = sqrt(tot2[0]);
@@ -829,7 +849,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         }
                         store_partials<2>(part, tile, q, lds);
                     }
-                    group_sync(I, timeouts);
+                    group_sync(timeouts);
                     group_totals<2>(part, tiles, tot2, lds);
                     r_norm = sqrt(tot2[0]);
                     rho1 = tot2[1];
@@ -871,7 +891,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     }
                     store_partials<1>(part, tile, q, lds);
                 }
-                group_sync(I, timeouts);
+                group_sync(timeouts);
                 double r0v[1];
                 group_totals<1>(part, tiles, r0v, lds);
                 if (r0v[0] == 0.0) break;
@@ -907,7 +927,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         }
                         store_partials<2>(part, tile, q, lds);
                     }
-                    group_sync(I, timeouts);
+                    group_sync(timeouts);
                     xi ^= 1;
                     group_totals<2>(part, tiles, tot2, lds);
                     r_norm = sqrt(tot2[0]);
@@ -931,7 +951,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     }
                     store_partials<2>(part, tile, q, lds);
                 }
-                group_sync(I, timeouts);
+                group_sync(timeouts);
                 double tt[2];
                 group_totals<2>(part, tiles, tt, lds);
                 omega = tt[0] > 0.0 ? tt[1] / tt[0] : 0.0;
@@ -953,7 +973,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         store_partials<2>(part, tile, q, lds);
                     }
                 }
-                group_sync(I, timeouts);
+                group_sync(timeouts);
                 double rr[2];
                 group_totals<2>(part, tiles, rr, lds);
                 rho1 = rr[1];
@@ -1002,12 +1022,14 @@ __device__ __forceinline__ void run_solve(const Instance& I, const Launch& P, in
         if (g == 0 && threadIdx.x == 0 && P.step_results != nullptr) P.step_results[step] = last;
     }
     if (g == 0 && threadIdx.x == 0) {
-        I.result->iterations = last.iterations;
-        I.result->restarts = last.restarts;
-        I.result->status = last.status;
-        I.result->relative_residual = last.relative_residual;
-        I.result->b_norm = last.b_norm;
-        I.result->v_cycles = last.v_cycles;
+        grp.result->iterations = last.iterations;
+        grp.result->restarts = last.restarts;
+        grp.result->status = last.status;
+        grp.result->relative_residual = last.relative_residual;
+        grp.result->b_norm = last.b_norm;
+        grp.result->v_cycles = last.v_cycles;
+        grp.result->t_begin = last.t_begin;
+        grp.result->t_end = last.t_end;
     }
 }
 
@@ -1021,7 +1043,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
     const int index = block_instance[blockIdx.x];
     const Instance& I = instances[index];
     if (threadIdx.x == 0) params = launches != nullptr ? launches[index] : single;
-    __syncthreads();
+    set_group(I.group_first, I.group_size, I.barrier, I.result, I.trace, I.trace_count, I.trace_capacity);
     const Launch& P = params;
     const int g = static_cast<int>(blockIdx.x) - I.group_first;
     const int G = I.group_size;
@@ -1045,7 +1067,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
                 const int i = tile * kTile + threadIdx.x;
                 if (i < I.n) pin[i] = static_cast<float>(P.in[i]);
             }
-            group_sync(I, timeouts);
+            group_sync(timeouts);
             const Source<float> src{pin, nullptr, nullptr, 0.0, 0.0, 0};
             vcycle_first<float>(I, Values<float>{I}, src, static_cast<float*>(I.p[1]), nullptr, g, G, lds, timeouts);
             vcycle_rest<float>(I, Values<float>{I}, static_cast<float*>(I.p[1]), pout, g, G, lds, timeouts);
@@ -1075,10 +1097,173 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
         else
             run_solve<float>(I, P, g, G, lds, timeouts, parity);
     }
-    if (timeouts != 0 && threadIdx.x == 0) atomicAdd(&I.result->barrier_timeouts, timeouts);
+    if (timeouts != 0 && threadIdx.x == 0) atomicAdd(&grp.result->barrier_timeouts, timeouts);
+}
+
+// ------------------------------------------------------------ the server
+
+constexpr long long kMemberGrace = 1000000000;   // 10 s past the lifetime
+
+__device__ inline uint64_t load_system(const uint64_t* p) {
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Slot leader: wait for a request on the slot's queue and claim it (its
+// ring position), or return -1 once the host asks to stop or the launch's
+// lifetime is over while the slot is idle.
+__device__ int64_t claim_request(const ServeParams& S, const ServeQueue& Q, long long start) {
+    uint64_t head = __hip_atomic_load(Q.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int idle = 0;   // empty polls so far: the wait between polls doubles up to ~14 us
+    for (;;) {
+        const uint64_t tail =
+            __hip_atomic_load(const_cast<uint64_t*>(Q.tail), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+        while (head < tail) {
+            if (__hip_atomic_compare_exchange_strong(Q.head, &head, head + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT))
+                return static_cast<int64_t>(head);
+        }
+        if (__hip_atomic_load(const_cast<uint32_t*>(S.stop), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0)
+            return -1;
+        if (wall_clock64() - start > static_cast<long long>(S.lifetime)) return -1;
+        // Every poll of the tail crosses the host link; idle leaders back off
+        // so that hundreds of them do not load it while other slots solve.
+        const int naps = 1 << (idle < 4 ? idle : 4);
+        for (int k = 0; k < naps; ++k) __builtin_amdgcn_s_sleep(32);
+        ++idle;
+        head = __hip_atomic_load(Q.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// One request on the slot's group; kept out of line so that the server
+// loop's own state does not add to the solver's register pressure.
+__device__ __noinline__ void serve_one(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts) {
+    int parity = 0;
+    if (P.op & kOpSetMatrix) set_matrix(I, P, g, G, lds, timeouts, parity);
+    if (I.precision == 0)
+        run_solve<double>(I, P, g, G, lds, timeouts, parity);
+    else
+        run_solve<float>(I, P, g, G, lds, timeouts, parity);
+    // every workgroup's rows of T are written before the completion
+    group_sync(timeouts);
+}
+
+// The persistent server: workgroup w belongs to slot block_slot[w]. The
+// slot's first workgroup claims requests from the slot's queue and hands
+// each one to the others through the slot's mailbox; the slot then runs it
+// exactly as a launch of that one request would (the same phases, barriers
+// and reductions over the slot's workgroups), so results do not depend on
+// what else is being served. Every wait is bounded by the host's stop flag
+// or the launch's lifetime.
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void server_kernel(ServeParams S) {
+    __shared__ Lds lds;
+    __shared__ Launch params;
+    __shared__ int32_t state;
+    __shared__ int64_t claimed;
+    const long long start = wall_clock64();
+    const ServeSlot& slot = S.slots[S.block_slot[blockIdx.x]];
+    const ServeQueue& Q = S.queues[slot.queue];
+    const int g = static_cast<int>(blockIdx.x) - slot.first;
+    const int G = slot.size;
+    set_group(slot.first, slot.size, slot.barrier, slot.result, nullptr, nullptr, 0);
+    uint32_t generation = 0;   // mailbox generations seen (thread 0)
+    int timeouts = 0;          // sticky: a broken slot stops waiting
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int64_t position = -1;
+            if (g == 0) {
+                position = claim_request(S, Q, start);
+                if (G > 1) {
+                    __hip_atomic_store(slot.mailbox + 32, position >= 0 ? static_cast<uint32_t>(position % Q.capacity) : 0u,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(slot.mailbox + 64, position >= 0 ? 0u : 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(slot.mailbox, generation + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            } else {
+                bool alive = true;
+                while (__hip_atomic_load(slot.mailbox, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == generation) {
+                    __builtin_amdgcn_s_sleep(4);
+                    if (wall_clock64() - start > static_cast<long long>(S.lifetime) + kMemberGrace) {
+                        alive = false;
+                        break;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                if (alive &&
+                    __hip_atomic_load(slot.mailbox + 64, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+                    position = __hip_atomic_load(slot.mailbox + 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            ++generation;
+            claimed = position;
+            if (position >= 0) {
+                // the request, read word by word from pinned host memory
+                ServeRequest r;
+                const uint64_t* src = reinterpret_cast<const uint64_t*>(Q.ring + position % Q.capacity);
+                uint64_t* dst = reinterpret_cast<uint64_t*>(&r);
+                for (size_t k = 0; k < sizeof(ServeRequest) / 8; ++k) dst[k] = load_system(src + k);
+                Launch p{};
+                p.op = r.op;
+                p.max_iterations = r.max_iterations;
+                p.max_restarts = r.max_restarts;
+                p.steps = 1;
+                p.tolerance = r.tolerance;
+                p.inner_relative = r.inner_relative;
+                p.dt = r.dt;
+                p.entry_conductance = r.entry_conductance;
+                p.capacity_area = r.capacity_area;
+                p.capacity_keep = r.capacity_keep;
+                p.t_old = r.t_old;
+                p.forcing = r.forcing;
+                p.t_new = r.t_new;
+                p.ticket = r.ticket;
+                p.completion = r.completion;
+                params = p;
+                state = r.state;
+            }
+        }
+        __syncthreads();
+        if (claimed < 0) break;
+        serve_one(S.table[state], params, g, G, lds, timeouts);
+        const Launch& P = params;
+        if (g == 0 && threadIdx.x == 0) {
+            const SolveResult& res = *slot.result;
+            ServeCompletion c{};
+            c.ticket = P.ticket;
+            c.status = res.status;
+            c.iterations = res.iterations;
+            c.restarts = res.restarts;
+            c.v_cycles = res.v_cycles;
+            c.relative_residual = res.relative_residual;
+            c.t_begin = res.t_begin;
+            c.t_end = res.t_end;
+            c.barrier_timeouts = timeouts;
+            c.done = 0;
+            ServeCompletion* out = S.completions + P.completion;
+            const uint64_t* src = reinterpret_cast<const uint64_t*>(&c);
+            uint64_t* dst = reinterpret_cast<uint64_t*>(out);
+            constexpr size_t words = sizeof(ServeCompletion) / 8;
+            for (size_t k = 0; k + 1 < words; ++k)
+                __hip_atomic_store(dst + k, src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(dst + words - 1, c.ticket + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        __syncthreads();
+    }
 }
 
 }  // namespace
+
+hipError_t launch_server(const ServeParams& params, int32_t blocks, hipStream_t stream) {
+    ServeParams copy = params;
+    void* args[] = {&copy};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&server_kernel), dim3(blocks), dim3(kTile), args,
+                                      0, stream);
+}
+
+int server_blocks_per_cu() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, server_kernel, kTile, 0) != hipSuccess) return 0;
+    return blocks;
+}
 
 hipError_t launch_program(const Instance* instances, const int32_t* block_instance, const Launch* launches,
                           const Launch& single, int32_t blocks, size_t dynamic_lds, hipStream_t stream,
