@@ -87,24 +87,26 @@ def measured_traffic():
         return None
 
 
-def algorithmic_bytes(levels, v_cycles, restarts):
-    """Bytes one solve must move at least (FP64 values, int32 indices, each
-    array once per operation, neighbour gathers from cache):
-      SpMV at level j    28 n_j + 12 nnz_j + 4
-      V-cycle            sum_{j<coarsest} (3 B_A(j) + 8 n_j + 16 n_{j+1}) + 8 n_c^2
-      half-iteration     B_A(0) + B_V + 88 n (mean of the two halves' 96 n and 80 n)
-      per solve          48 n (RHS assembly and ||b||) + halves + (1 + restarts)(B_A(0) + 16 n)
+def algorithmic_bytes(levels, v_cycles, restarts, vb=8):
+    """Bytes one solve must move at least (int32 indices, each array once per
+    operation, neighbour gathers from cache), with vb-byte values in the
+    V-cycle, SpMV and inner vectors (8 plain, 4 mixed) and FP64 elsewhere:
+      SpMV at level j    (3 vb + 4) n_j + (vb + 4) nnz_j + 4
+      V-cycle            sum_{j<coarsest} (3 B_A(j) + vb n_j + 2 vb n_{j+1}) + 8 n_c^2
+      half-iteration     B_A(0) + B_V + 11 vb n (the halves' vector updates)
+      per solve          48 n (RHS assembly and ||b||) + halves
+                         + (1 + restarts)(B_A(0) in FP64 + 16 n) (true residuals)
     """
-    def b_a(n, nnz):
-        return 28 * n + 12 * nnz + 4
+    def b_a(n, nnz, w):
+        return (3 * w + 4) * n + (w + 4) * nnz + 4
 
     n0, nnz0 = levels[0]
     b_v = 8 * levels[-1][0] ** 2
     for j in range(len(levels) - 1):
         n, nnz = levels[j]
-        b_v += 3 * b_a(n, nnz) + 8 * n + 16 * levels[j + 1][0]
-    half = b_a(n0, nnz0) + b_v + 88 * n0
-    return 48 * n0 + v_cycles * half + (1 + restarts) * (b_a(n0, nnz0) + 16 * n0)
+        b_v += 3 * b_a(n, nnz, vb) + vb * n + 2 * vb * levels[j + 1][0]
+    half = b_a(n0, nnz0, vb) + b_v + 11 * vb * n0
+    return 48 * n0 + v_cycles * half + (1 + restarts) * (b_a(n0, nnz0, 8) + 16 * n0)
 
 
 def set_matrix_bytes(levels, edges):
@@ -165,6 +167,219 @@ def cpu_baseline(mesh, cond, ca, t0, forcings, seconds):
     return solves / elapsed, solves, chains, elapsed, final
 
 
+def mixed_section(P, ctx, mesh, graph, levels, with_oracle):
+    """Mixed precision at L6 on the widened log-normal spread (sigma = 1.0):
+    the 120-step chain in one launch, against the plain FP64 path on the same
+    inputs, with each step's error against the FP64 oracle solving that step
+    from the same T (the step's own error) and the trajectory's drift."""
+    seed = P.instance_seed(0)
+    cond, ca, _, t0 = P.heat_instance(mesh, seed, 1.0e6, 1.0)
+    forcings = [P.heat_forcing(mesh, seed, s) for s in range(CHAIN)]
+    d_cond = P.DeviceArray.of(ctx, mesh.entry_values(cond))
+    d_ca = P.DeviceArray.of(ctx, ca)
+    d_f = P.DeviceArray.of(ctx, np.concatenate(forcings))
+    d_t0 = P.DeviceArray.of(ctx, t0)
+    d_out = P.DeviceArray(ctx, mesh.cells)
+    out = {}
+    for name, precision in (("mixed", P.MIXED), ("plain_fp64", P.PLAIN_FP64)):
+        system = P.System(graph, precision)
+        infos = (P.SolveInfo * CHAIN)()
+        system.heat_chain_device(d_t0.ptr, d_f.ptr, CHAIN, d_out.ptr, DT, TOL, 1000, step_infos=infos,
+                                 conductance_per_entry=d_cond.ptr, capacity_area=d_ca.ptr)
+        census = list(infos)
+        ctx.events(2)
+        ctx.synchronize()
+        ctx.record(0)
+        for _ in range(3):
+            system.heat_chain_device(d_t0.ptr, d_f.ptr, CHAIN, d_out.ptr, DT, TOL, 1000,
+                                     conductance_per_entry=d_cond.ptr, capacity_area=d_ca.ptr)
+        ctx.record(1)
+        ctx.synchronize()
+        ms = ctx.elapsed_ms(0, 1) / 3
+        vb = 4 if precision == P.MIXED else 8
+        nbytes = set_matrix_bytes(levels, mesh.edges) + sum(
+            algorithmic_bytes(levels, i.v_cycles, i.restarts, vb) for i in census)
+        out[name] = {
+            "solves_per_s": round(CHAIN / ms * 1e3, 2),
+            "iterations_per_solve": float(np.mean([i.iterations for i in census])),
+            "restarts_per_solve": float(np.mean([i.restarts for i in census])),
+            "v_cycles_per_solve": float(np.mean([i.v_cycles for i in census])),
+            "max_true_residual": float(max(i.relative_residual for i in census)),
+            "converged": all(i.status == 0 for i in census),
+            "launch_GBps": round(nbytes / (ms / 1e3) / 1e9, 2),
+        }
+        if precision == P.MIXED:
+            # each step's T (one launch per step, same kernels) for the error report
+            steps = []
+            src = d_t0
+            system.heat_set_matrix_device(d_cond, d_ca, DT)
+            for k in range(CHAIN):
+                dst = P.DeviceArray(ctx, mesh.cells)
+                system.heat_step_device(src.ptr, d_f.offset(k * mesh.cells), dst.ptr, DT, TOL, 1000)
+                steps.append(dst)
+                src = dst
+            mixed_steps = [s.download() for s in steps]
+    if with_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import support as S
+
+        om = S.mesh("oracle", mesh.subdivision)
+        diag, off = S.heat_matrix(om, cond, ca)
+        mg = S.Multigrid(S.oracle(), om)
+        mg.set_matrix(diag, off)
+        errors = []
+        prev = t0
+        for k in range(CHAIN):
+            ref, _ = mg.bicgstab(diag, off, S.heat_rhs(om, ca, prev, forcings[k]), 1e-13, 1000, 1)
+            errors.append(float(np.linalg.norm(mixed_steps[k] - ref) / np.linalg.norm(ref)))
+            prev = mixed_steps[k]
+        out["mixed"]["error_vs_fp64_oracle_per_step"] = {
+            "max": max(errors), "median": float(np.median(errors)), "first": errors[0], "last": errors[-1],
+            "oracle": "FP64 oracle converged to 1e-13 from the same T_k"}
+    out["config"] = {"workload": "L6, sigma = 1.0, set_matrix + 120 chained steps per launch", "tolerance": TOL}
+    return out
+
+
+def batched_section(P, ctx, meshes, graphs, hierarchy_levels):
+    """256 independent instances, L3:L4:L5:L6 = 64 each, own seeds, mixed
+    precision, matrix assembled in the same launch; default allocation and
+    the autotuner's."""
+    members = []
+    for i, level in enumerate([3, 4, 5, 6] * 64):
+        m = meshes[level]
+        seed = P.instance_seed(1 + i)
+        cond, ca, _, t0 = P.heat_instance(m, seed, 1.0e6, 0.3)
+        members.append(dict(level=level, system=P.System(graphs[level], P.MIXED),
+                            cond=P.DeviceArray.of(ctx, m.entry_values(cond)), ca=P.DeviceArray.of(ctx, ca),
+                            t0=P.DeviceArray.of(ctx, t0), f=P.DeviceArray.of(ctx, P.heat_forcing(m, seed, 0)),
+                            out=P.DeviceArray(ctx, m.cells)))
+    batch = P.Batch(ctx, [x["system"] for x in members])
+    args = ([x["t0"] for x in members], [x["f"] for x in members], [x["out"] for x in members])
+    kw = dict(conductance=[x["cond"] for x in members], capacity=[x["ca"] for x in members])
+
+    def timed(reps=5):
+        infos = batch.heat_step(*args, DT, TOL, 1000, **kw)
+        ctx.events(2)
+        ctx.synchronize()
+        ctx.record(0)
+        for _ in range(reps):
+            batch.heat_step(*args, DT, TOL, 1000, collect=False, **kw)
+        ctx.record(1)
+        batch.synchronize()
+        return ctx.elapsed_ms(0, 1) / reps, infos
+
+    ms_default, infos = timed()
+    tuned = batch.autotune(*args, DT, TOL, 1000, repeats=2, **kw)
+    ms_tuned, infos = timed()
+    nbytes = 0
+    for x, info in zip(members, infos):
+        lv = hierarchy_levels[x["level"]]
+        nbytes += set_matrix_bytes(lv, meshes[x["level"]].edges) + algorithmic_bytes(lv, info.v_cycles,
+                                                                                      info.restarts, 4)
+    best = min(ms_default, ms_tuned)
+    groups = batch.groups
+    by_level = {}
+    for x, gsize in zip(members, groups):
+        by_level.setdefault(str(x["level"]), []).append(gsize)
+    return {
+        "config": {"workload": "256 instances, L3/L4/L5/L6 x 64, mixed precision, set_matrix + 1 solve each, "
+                               "one launch", "tolerance": TOL},
+        "solves_per_s": round(len(members) / best * 1e3, 1),
+        "ms_per_launch": round(best, 4),
+        "default_allocation_ms": round(ms_default, 4),
+        "autotuned_ms": round(ms_tuned, 4),
+        "workgroups_per_instance": {k: [min(v), max(v)] for k, v in by_level.items()},
+        "converged": all(i.status == 0 and i.relative_residual <= TOL for i in infos),
+        "launch_GBps": round(nbytes / (best / 1e3) / 1e9, 2),
+    }
+
+
+def serve_requests(P, meshes, bank, levels, shares, rate, count, rng, outs):
+    """Poisson arrivals at `rate` (inf: all at once); level by `shares`; half
+    the requests continue a live 12-step session (its resident T), the rest
+    start sessions (1 in 11 of them 12 steps long)."""
+    times = np.cumsum(rng.exponential(1.0 / rate, count)) if np.isfinite(rate) else np.zeros(count)
+    times -= times[0]
+    live, reqs, session = [], [], 0
+    for k in range(count):
+        if rng.random() < 0.5 and live:
+            s = live[int(rng.integers(len(live)))]
+        else:
+            level = levels[int(rng.choice(len(levels), p=shares))]
+            s = [session, level, int(rng.integers(len(bank[level]))), 0, None, 12 if rng.random() < 1 / 11 else 1]
+            session += 1
+            if s[5] > 1:
+                live.append(s)
+        sid, level, b, step, prev, total = s
+        inst = bank[level][b]
+        out = P.DeviceArray(inst["ctx"], meshes[level].cells)
+        outs.append(out)
+        r = P.ServeRequest()
+        r.arrival_s = float(times[k])
+        r.graph = levels.index(level)
+        r.session = sid
+        r.first = int(step == 0)
+        r.last = int(step == total - 1)
+        r.conductance_per_entry = inst["cond"].ptr if step == 0 else None
+        r.capacity_area = inst["ca"].ptr if step == 0 else None
+        r.temperature_old = (inst["t0"] if step == 0 else prev).ptr
+        r.forcing_W_m2 = inst["f"][step].ptr
+        r.temperature_new = out.ptr
+        reqs.append(r)
+        s[3], s[4] = step + 1, out
+        if s[3] >= total and s in live:
+            live.remove(s)
+    unfinished = {s[0] for s in live}
+    return [r for r in reqs if r.session not in unfinished]
+
+
+def served_section(P, ctx, meshes, graphs, plain_value):
+    """Served stream: Poisson arrivals of single-step requests, levels
+    L3:L4:L5:L6 = 40:30:20:10, half continuing a resident 12-step session,
+    mixed precision, one persistent server launch per run."""
+    levels, shares = [3, 4, 5, 6], [0.4, 0.3, 0.2, 0.1]
+    bank = {}
+    for level in levels:
+        m = meshes[level]
+        bank[level] = []
+        for b in range(8):
+            seed = P.instance_seed(50000 + 100 * level + b)
+            cond, ca, _, t0 = P.heat_instance(m, seed, 1.0e6, 0.3)
+            bank[level].append(dict(ctx=ctx, cond=P.DeviceArray.of(ctx, m.entry_values(cond)),
+                                    ca=P.DeviceArray.of(ctx, ca), t0=P.DeviceArray.of(ctx, t0),
+                                    f=[P.DeviceArray.of(ctx, P.heat_forcing(m, seed, s)) for s in range(12)]))
+
+    def run(server, lv, sh, rate, count, seed):
+        outs = []
+        reqs = serve_requests(P, meshes, bank, lv, sh, rate, count, np.random.default_rng(seed), outs)
+        rec = server.run(reqs, DT, TOL, 1000)
+        lat = np.array([(r.completed_s - r.arrival_s) * 1e3 for r in rec])
+        span = max(r.completed_s for r in rec) - min(r.arrival_s for r in rec)
+        return {
+            "offered_per_s": rate if np.isfinite(rate) else "all at t=0", "requests": len(rec),
+            "solves_per_s": round(len(rec) / span, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "converged": all(r.status == 0 and r.relative_residual <= TOL for r in rec),
+        }
+
+    server = P.Server(ctx, [graphs[L] for L in levels], share=shares, precision=P.MIXED, memory_budget=8 << 30)
+    sweep = [run(server, levels, shares, rate, count, 11 + i)
+             for i, (rate, count) in enumerate(((100, 200), (1000, 2000), (5000, 10000), (float("inf"), 10000)))]
+    plan = server.plan()
+    server.close()
+    l6 = P.Server(ctx, [graphs[6]], precision=P.MIXED, memory_budget=8 << 30, slot_sizes=[4])
+    l6_saturated = run(l6, [6], [1.0], float("inf"), 4000, 99)
+    l6.close()
+    return {
+        "config": {"workload": "Poisson single-step requests, L3:L4:L5:L6 = 40:30:20:10, half continuing a "
+                               "resident 12-step session, mixed precision, persistent server", "tolerance": TOL,
+                   "slots": {str(L): [p["slots"], p["slot_size"], p["pool"]] for L, p in zip(levels, plan)}},
+        "sweep": sweep,
+        "l6_only_saturated": l6_saturated,
+        "l6_served_mixed_vs_plain_fp64_chain": round(l6_saturated["solves_per_s"] / plain_value, 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +388,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--workgroups", type=int, default=0, help="override the solver's workgroup count")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the mixed-precision, batched and served measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (instances shard "
@@ -348,6 +565,13 @@ def main():
         tight_cpu = oracle_chain(mesh, cond, ca, t0, forcings, 1e-13)
         line["error_vs_oracle_after_120_steps_tol_1e-13"] = float(
             np.linalg.norm(tight_final - tight_cpu) / np.linalg.norm(tight_cpu))
+    if not args.no_extras:
+        meshes = {L: (mesh if L == level else P.Mesh(L)) for L in (3, 4, 5, 6)}
+        graphs = {L: (graph if L == level else P.Graph(ctx, meshes[L])) for L in meshes}
+        hierarchy_levels = {L: [(lv["size"], lv["nnz"]) for lv in P.Hierarchy(meshes[L]).levels] for L in meshes}
+        line["mixed_precision"] = mixed_section(P, ctx, mesh, graph, levels, not args.no_cpu)
+        line["batched"] = batched_section(P, ctx, meshes, graphs, hierarchy_levels)
+        line["served"] = served_section(P, ctx, meshes, graphs, value)
     print(json.dumps(line))
 
 

@@ -318,11 +318,14 @@ typedef struct {
 } psim_serve_record;
 
 /* request_share: expected fraction of requests per graph (NULL: equal).
+ * slot_sizes: workgroups per slot per graph (NULL or 0: about 12 tiles of
+ * the graph per workgroup).
  * memory_budget_bytes: HBM for pooled systems, split by slots x bytes per
  * system (at least one system per slot, at most 4096 per graph; 0 = one
  * per slot). */
-int psim_server_create(psim_ctx* ctx, psim_graph* const* graphs, const double* request_share, int32_t graph_count,
-                       int precision, uint64_t memory_budget_bytes, psim_server** out);
+int psim_server_create(psim_ctx* ctx, psim_graph* const* graphs, const double* request_share,
+                       const int32_t* slot_sizes, int32_t graph_count, int precision, uint64_t memory_budget_bytes,
+                       psim_server** out);
 void psim_server_destroy(psim_server* server);
 /* Per graph: slots, workgroups per slot, pool systems, bytes per system
  * (any output may be NULL). */

@@ -158,7 +158,7 @@ class ServeRecord(C.Structure):
                 ("restarts", C.c_int32), ("v_cycles", C.c_int32), ("relative_residual", C.c_double)]
 
 
-_sig("psim_server_create", C.c_int, _vp, _pp, C.POINTER(_dbl), _i32, C.c_int, _u64, C.POINTER(_vp))
+_sig("psim_server_create", C.c_int, _vp, _pp, C.POINTER(_dbl), C.POINTER(_i32), _i32, C.c_int, _u64, C.POINTER(_vp))
 _sig("psim_server_destroy", None, _vp)
 _sig("psim_server_plan", C.c_int, _vp, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_u64))
 _sig("psim_server_run", C.c_int, _vp, C.POINTER(ServeRequest), _i32, _dbl, _dbl, _i32, C.POINTER(ServeRecord))
@@ -658,12 +658,14 @@ class Server:
     arrivals. `graphs` are Graph objects; `share` the expected fraction of
     requests per graph."""
 
-    def __init__(self, ctx: Context, graphs, share=None, precision=MIXED, memory_budget=4 << 30):
+    def __init__(self, ctx: Context, graphs, share=None, precision=MIXED, memory_budget=4 << 30, slot_sizes=None):
         self.ctx, self.graphs = ctx, list(graphs)
         arr = (_vp * len(self.graphs))(*[g.handle for g in self.graphs])
         sh = None if share is None else (_dbl * len(self.graphs))(*share)
+        sz = None if slot_sizes is None else (_i32 * len(self.graphs))(*slot_sizes)
         h = _vp()
-        check(lib.psim_server_create(ctx.handle, arr, sh, len(self.graphs), precision, memory_budget, C.byref(h)))
+        check(lib.psim_server_create(ctx.handle, arr, sh, sz, len(self.graphs), precision, memory_budget,
+                                     C.byref(h)))
         self.handle = h
         _track(self)
 

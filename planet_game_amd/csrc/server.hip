@@ -92,13 +92,15 @@ double seconds_since(std::chrono::steady_clock::time_point t0) {
 
 // Slot plan: slot size from the graph's tiles, slot counts from each
 // graph's share of the expected work (request share x tiles).
-void plan_slots(psim_server* s, const std::vector<double>& share, int32_t capacity) {
+void plan_slots(psim_server* s, const std::vector<double>& share, const int32_t* sizes, int32_t capacity) {
     const size_t k = s->graphs.size();
     std::vector<double> work(k);
     double total = 0.0;
     for (size_t q = 0; q < k; ++q) {
         const int32_t tiles = (s->graphs[q]->n + psim::dev::kTile - 1) / psim::dev::kTile;
-        s->slot_size[q] = std::max<int32_t>(1, (tiles + kTargetTilesPerGroup - 1) / kTargetTilesPerGroup);
+        s->slot_size[q] = sizes != nullptr && sizes[q] > 0
+                              ? std::min(sizes[q], capacity)
+                              : std::max<int32_t>(1, (tiles + kTargetTilesPerGroup - 1) / kTargetTilesPerGroup);
         work[q] = share[q] * tiles;
         total += work[q];
     }
@@ -129,8 +131,8 @@ void plan_slots(psim_server* s, const std::vector<double>& share, int32_t capaci
 
 extern "C" {
 
-int psim_server_create(psim_ctx* ctx, psim_graph* const* graphs, const double* request_share, int32_t graph_count,
-                       int precision, uint64_t memory_budget, psim_server** out) {
+int psim_server_create(psim_ctx* ctx, psim_graph* const* graphs, const double* request_share, const int32_t* slot_sizes,
+                       int32_t graph_count, int precision, uint64_t memory_budget, psim_server** out) {
     return guarded([&] {
         if (ctx == nullptr || graphs == nullptr || out == nullptr) throw std::invalid_argument("null argument");
         if (graph_count < 1 || graph_count > kMaxQueues) throw std::invalid_argument("graph count out of range");
@@ -161,7 +163,7 @@ int psim_server_create(psim_ctx* ctx, psim_graph* const* graphs, const double* r
         s->footprint.assign(k, 0);
         const int per_cu = std::min(ctx->blocks_per_cu, psim::dev::server_blocks_per_cu());
         if (per_cu < 1) throw psim::DeviceFailure("the server kernel cannot be resident");
-        plan_slots(s.get(), share, ctx->compute_units * per_cu);
+        plan_slots(s.get(), share, slot_sizes, ctx->compute_units * per_cu);
         // pools: one system per slot at least, the rest of the budget split by
         // the expected resident memory (share x footprint)
         for (size_t q = 0; q < k; ++q) {
