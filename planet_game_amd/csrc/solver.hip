@@ -181,7 +181,7 @@ constexpr long long kBarrierTicks = 100000000;
 constexpr int kLeaves = 8;
 static_assert(64 + 32 * kLeaves == kBarrierWordCount, "barrier layout");
 
-__device__ void group_sync(int& timeouts) {
+__device__ __forceinline__ void group_sync(int& timeouts) {
     __syncthreads();
     const int G = grp.size;
     if (G == 1) {
@@ -634,7 +634,8 @@ __device__ void coarse_inverse_blocked(const Instance& I, Lds& lds) {
     }
 }
 
-__device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts, int& parity) {
+__device__ __forceinline__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts,
+                                           int& parity) {
     const int J = I.depth;
     {
         // Level-0 values into ELL: assembled from the heat inputs
@@ -751,6 +752,18 @@ __device__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds
             coarse_inverse(I, I.tri_lds ? dynamic_lds() : I.tri, lds);
     }
     group_sync(timeouts);
+}
+
+// set_matrix out of line (its register-blocked inverse would otherwise add
+// to the solver's register pressure), with the barrier state passed by
+// value so that it stays in registers on both sides.
+struct SyncState {
+    int timeouts, parity;
+};
+__device__ __noinline__ SyncState set_matrix_call(const Instance& I, const Launch& P, int g, int G, Lds& lds,
+                                                  SyncState st) {
+    set_matrix(I, P, g, G, lds, st.timeouts, st.parity);
+    return st;
 }
 
 // ------------------------------------------------------------ BiCGSTAB
@@ -1053,7 +1066,11 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
         I.trace_count[0] = 1;
         I.trace[0] = static_cast<uint64_t>(wall_clock64());
     }
-    if (P.op & kOpSetMatrix) set_matrix(I, P, g, G, lds, timeouts, parity);
+    if (P.op & kOpSetMatrix) {
+        const SyncState st = set_matrix_call(I, P, g, G, lds, SyncState{timeouts, parity});
+        timeouts = st.timeouts;
+        parity = st.parity;
+    }
     if (P.op & kOpPrecondition) {
         if (I.precision == 0) {
             const Source<double> src{P.in, nullptr, nullptr, 0.0, 0.0, 0};
@@ -1134,17 +1151,36 @@ __device__ int64_t claim_request(const ServeParams& S, const ServeQueue& Q, long
     }
 }
 
-// One request on the slot's group; kept out of line so that the server
-// loop's own state does not add to the solver's register pressure.
-__device__ __noinline__ void serve_one(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts) {
+// A pointer every lane holds, rebuilt in scalar registers: descriptor reads
+// through it are scalar loads (a function argument arrives in vector
+// registers otherwise).
+template <class T> __device__ inline const T* uniform_ptr(const T* p) {
+    const uint64_t bits = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits >> 32));
+    return reinterpret_cast<const T*>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// One request on the slot's group, out of line so that the server loop's
+// own state does not add to the solver's register pressure.
+__device__ __noinline__ int serve_one(const Instance* instance, const Launch& P, int g, int G, Lds& lds,
+                                      int timeouts) {
+    const Instance& I = *uniform_ptr(instance);
+    g = __builtin_amdgcn_readfirstlane(g);
+    G = __builtin_amdgcn_readfirstlane(G);
     int parity = 0;
-    if (P.op & kOpSetMatrix) set_matrix(I, P, g, G, lds, timeouts, parity);
+    if (P.op & kOpSetMatrix) {
+        const SyncState st = set_matrix_call(I, P, g, G, lds, SyncState{timeouts, parity});
+        timeouts = st.timeouts;
+        parity = st.parity;
+    }
     if (I.precision == 0)
         run_solve<double>(I, P, g, G, lds, timeouts, parity);
     else
         run_solve<float>(I, P, g, G, lds, timeouts, parity);
     // every workgroup's rows of T are written before the completion
     group_sync(timeouts);
+    return timeouts;
 }
 
 // The persistent server: workgroup w belongs to slot block_slot[w]. The
@@ -1159,16 +1195,20 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
     __shared__ Launch params;
     __shared__ int32_t state;
     __shared__ int64_t claimed;
-    const long long start = wall_clock64();
+    __shared__ long long start;          // thread 0: the launch's start on the wall clock
+    __shared__ uint32_t generation;      // thread 0: mailbox generations seen
+    if (threadIdx.x == 0) {
+        start = wall_clock64();
+        generation = 0;
+    }
     const ServeSlot& slot = S.slots[S.block_slot[blockIdx.x]];
-    const ServeQueue& Q = S.queues[slot.queue];
     const int g = static_cast<int>(blockIdx.x) - slot.first;
     const int G = slot.size;
     set_group(slot.first, slot.size, slot.barrier, slot.result, nullptr, nullptr, 0);
-    uint32_t generation = 0;   // mailbox generations seen (thread 0)
     int timeouts = 0;          // sticky: a broken slot stops waiting
     for (;;) {
         if (threadIdx.x == 0) {
+            const ServeQueue& Q = S.queues[slot.queue];
             int64_t position = -1;
             if (g == 0) {
                 position = claim_request(S, Q, start);
@@ -1223,7 +1263,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
         }
         __syncthreads();
         if (claimed < 0) break;
-        serve_one(S.table[state], params, g, G, lds, timeouts);
+        timeouts = serve_one(S.table + state, params, g, G, lds, timeouts);
         const Launch& P = params;
         if (g == 0 && threadIdx.x == 0) {
             const SolveResult& res = *slot.result;
