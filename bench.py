@@ -76,6 +76,9 @@ class Markers:
                 self.lib.roctxRangePop()
 
 
+MARKS = Markers()
+
+
 def measured_traffic():
     """HBM bytes per chained-pass launch from the committed counter profile
     (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration),
@@ -261,11 +264,12 @@ def batched_section(P, ctx, meshes, graphs, hierarchy_levels):
         infos = batch.heat_step(*args, DT, TOL, 1000, **kw)
         ctx.events(2)
         ctx.synchronize()
-        ctx.record(0)
-        for _ in range(reps):
-            batch.heat_step(*args, DT, TOL, 1000, collect=False, **kw)
-        ctx.record(1)
-        batch.synchronize()
+        with MARKS.range("batched_launch_timed"):
+            ctx.record(0)
+            for _ in range(reps):
+                batch.heat_step(*args, DT, TOL, 1000, collect=False, **kw)
+            ctx.record(1)
+            batch.synchronize()
         return ctx.elapsed_ms(0, 1) / reps, infos
 
     ms_default, infos = timed()
@@ -352,7 +356,8 @@ def served_section(P, ctx, meshes, graphs, plain_value):
     def run(server, lv, sh, rate, count, seed):
         outs = []
         reqs = serve_requests(P, meshes, bank, lv, sh, rate, count, np.random.default_rng(seed), outs)
-        rec = server.run(reqs, DT, TOL, 1000)
+        with MARKS.range("server_run"):
+            rec = server.run(reqs, DT, TOL, 1000)
         lat = np.array([(r.completed_s - r.arrival_s) * 1e3 for r in rec])
         span = max(r.completed_s for r in rec) - min(r.arrival_s for r in rec)
         return {
@@ -420,7 +425,7 @@ def main():
     d_out = P.DeviceArray(ctx, n)
     infos = (P.SolveInfo * CHAIN)()
 
-    marks = Markers()
+    marks = MARKS
 
     def run_pass(collect=True, tol=TOL):
         """One launch: set the matrix, then 120 chained steps T_k -> T_{k+1}."""
