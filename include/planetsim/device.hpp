@@ -41,6 +41,32 @@ inline psim_ctx* context() {
     return ctx;
 }
 
+// Device memory owned by the drop-in layer (staging of host fields).
+class DeviceBuffer {
+  public:
+    explicit DeviceBuffer(std::size_t bytes) { check(psim_device_alloc(context(), bytes, &ptr_)); }
+    DeviceBuffer(DeviceBuffer&& other) noexcept : ptr_(other.ptr_) { other.ptr_ = nullptr; }
+    DeviceBuffer& operator=(DeviceBuffer&& other) noexcept {
+        if (this != &other) {
+            release();
+            ptr_ = other.ptr_;
+            other.ptr_ = nullptr;
+        }
+        return *this;
+    }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void* get() const noexcept { return ptr_; }
+
+  private:
+    void release() noexcept {
+        if (ptr_ != nullptr) (void)psim_device_free(context(), ptr_);
+        ptr_ = nullptr;
+    }
+    void* ptr_ = nullptr;
+};
+
 // worker_count is accepted for signature compatibility; device results do
 // not depend on it, but 0 is still rejected as the reference rejects it
 // (deterministic_executor.hpp:17-19).

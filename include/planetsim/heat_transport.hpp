@@ -89,6 +89,9 @@ namespace detail {
 struct SystemFree {
     void operator()(psim_system* s) const noexcept { psim_system_destroy(s); }
 };
+struct BatchFree {
+    void operator()(psim_batch* b) const noexcept { psim_batch_destroy(b); }
+};
 }  // namespace detail
 
 // The aggregation-multigrid V-cycle on a transport graph. The device form
@@ -195,11 +198,80 @@ class HeatTransportSolver {
         return {info.iterations, info.restarts, info.v_cycles, info.relative_residual, info.status == PSIM_OK};
     }
 
+    std::size_t size() const noexcept { return n_; }
+    double time_step() const noexcept { return dt_s_; }
+    psim_system* device_system() const noexcept { return system_.get(); }
+
   private:
     std::size_t n_ = 0, nnz_ = 0;
     double dt_s_ = 2.63e6;
     std::shared_ptr<TransportGraph> graph_hold_;
     std::unique_ptr<psim_system, detail::SystemFree> system_;
+};
+
+// Heat steps of many solvers (any graphs, any precisions, each with its own
+// matrix set by set_conductance) in ONE device launch (new: the reference
+// solves one instance at a time). Each result equals that solver's own
+// step() bit for bit. The solvers must outlive the batch and share dt.
+class HeatStepBatch {
+  public:
+    explicit HeatStepBatch(std::vector<HeatTransportSolver*> solvers) : solvers_(std::move(solvers)) {
+        if (solvers_.empty()) throw std::invalid_argument("a batch needs at least one solver");
+        std::vector<psim_system*> systems;
+        for (HeatTransportSolver* s : solvers_) {
+            if (s == nullptr) throw std::invalid_argument("null solver");
+            systems.push_back(s->device_system());
+        }
+        psim_batch* b = nullptr;
+        gpu::check(psim_batch_create(gpu::context(), systems.data(), static_cast<int32_t>(systems.size()), &b));
+        batch_.reset(b);
+        for (HeatTransportSolver* s : solvers_) {
+            buffers_.push_back(gpu::DeviceBuffer(3 * s->size() * sizeof(double)));
+        }
+    }
+
+    std::vector<HeatStepResult> step(std::span<const Field2D<double>* const> temperatures_K,
+                                     std::span<const Field2D<double>* const> forcings_W_m2,
+                                     std::span<Field2D<double>* const> next_K, double relative_tolerance = 1.0e-10,
+                                     int max_iterations = 1000) {
+        const std::size_t count = solvers_.size();
+        if (temperatures_K.size() != count || forcings_W_m2.size() != count || next_K.size() != count)
+            throw std::invalid_argument("one field per solver is needed");
+        const double dt = solvers_[0]->time_step();
+        std::vector<const double*> t_old(count), forcing(count);
+        std::vector<double*> t_new(count);
+        for (std::size_t i = 0; i < count; ++i) {
+            const std::size_t n = solvers_[i]->size();
+            if (temperatures_K[i] == nullptr || forcings_W_m2[i] == nullptr || next_K[i] == nullptr)
+                throw std::invalid_argument("null field");
+            if (temperatures_K[i]->size() != n || forcings_W_m2[i]->size() != n)
+                throw std::invalid_argument("transport temperature does not match the mesh");
+            if (solvers_[i]->time_step() != dt) throw std::invalid_argument("batched solvers must share dt");
+            double* base = static_cast<double*>(buffers_[i].get());
+            gpu::check(psim_copy_to_device(gpu::context(), base, temperatures_K[i]->values().data(), n * 8));
+            gpu::check(psim_copy_to_device(gpu::context(), base + n, forcings_W_m2[i]->values().data(), n * 8));
+            t_old[i] = base;
+            forcing[i] = base + n;
+            t_new[i] = base + 2 * n;
+        }
+        std::vector<psim_solve_info> infos(count);
+        gpu::check(psim_batch_heat_step(batch_.get(), nullptr, nullptr, t_old.data(), forcing.data(), dt,
+                                        relative_tolerance, max_iterations, t_new.data(), infos.data()));
+        std::vector<HeatStepResult> out(count);
+        for (std::size_t i = 0; i < count; ++i) {
+            const std::size_t n = solvers_[i]->size();
+            *next_K[i] = Field2D<double>(n, 0.0);
+            gpu::check(psim_copy_to_host(gpu::context(), next_K[i]->values().data(), t_new[i], n * 8));
+            out[i] = {infos[i].iterations, infos[i].restarts, infos[i].v_cycles, infos[i].relative_residual,
+                      infos[i].status == PSIM_OK};
+        }
+        return out;
+    }
+
+  private:
+    std::vector<HeatTransportSolver*> solvers_;
+    std::unique_ptr<psim_batch, detail::BatchFree> batch_;
+    std::vector<gpu::DeviceBuffer> buffers_;
 };
 
 inline HeatStepResult solve_heat_step(const TransportGraph& graph, std::span<const double> conductance_per_entry,

@@ -197,6 +197,49 @@ void device_part() {
         }
         EXPECT(std::abs(net) <= 1e-12 * gross);
     }
+    // HeatStepBatch: solvers of different graphs and precisions in one
+    // launch give exactly what each solver's own step() gives.
+    {
+        const auto small = planetsim::make_icosphere(3, 6'371'000.0);
+        const planetsim::TransportGraph graphs[2] = {planetsim::mesh_transport_graph(mesh),
+                                                     planetsim::mesh_transport_graph(small)};
+        const double dt = 2.63e6;
+        std::vector<planetsim::HeatTransportSolver> solvers;
+        solvers.emplace_back(graphs[0], false);
+        solvers.emplace_back(graphs[1], true);
+        std::vector<planetsim::Field2D<double>> t_old, forcing, alone(2), batched(2);
+        for (int q = 0; q < 2; ++q) {
+            const auto& g = graphs[q];
+            const std::size_t m = g.size();
+            std::vector<double> capacity(m), conductance(g.neighbour.size());
+            for (std::size_t c = 0; c < m; ++c) capacity[c] = (uniform(17 * q + c) < 0.3 ? 1.0e7 : 4.0e8) * g.area_m2[c];
+            for (std::size_t k = 0; k < conductance.size(); ++k) conductance[k] = 1.0e6 * (1.0 + uniform(31 * q + k));
+            solvers[static_cast<std::size_t>(q)].set_conductance(conductance, capacity, dt);
+            planetsim::Field2D<double> t(m), f(m);
+            for (std::size_t c = 0; c < m; ++c) {
+                t[c] = 280.0 + 10.0 * uniform(41 * q + c);
+                f[c] = 50.0 + 20.0 * uniform(53 * q + c);
+            }
+            t_old.push_back(std::move(t));
+            forcing.push_back(std::move(f));
+        }
+        for (int q = 0; q < 2; ++q) {
+            const auto r = solvers[static_cast<std::size_t>(q)].step(t_old[q], forcing[q], alone[q]);
+            EXPECT(r.converged);
+        }
+        planetsim::HeatStepBatch batch({&solvers[0], &solvers[1]});
+        const planetsim::Field2D<double>* ins[2] = {&t_old[0], &t_old[1]};
+        const planetsim::Field2D<double>* fs[2] = {&forcing[0], &forcing[1]};
+        planetsim::Field2D<double>* outs[2] = {&batched[0], &batched[1]};
+        const auto results = batch.step(ins, fs, outs);
+        for (int q = 0; q < 2; ++q) {
+            EXPECT(results[static_cast<std::size_t>(q)].converged);
+            EXPECT(batched[q].size() == alone[q].size() &&
+                   std::memcmp(batched[q].values().data(), alone[q].values().data(), alone[q].size() * 8) == 0);
+        }
+        EXPECT(throws_with<std::invalid_argument>([&] { (void)batch.step(std::span(ins, 1), fs, outs); },
+                                                  "one field per solver is needed"));
+    }
 }
 
 }  // namespace
