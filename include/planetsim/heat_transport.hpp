@@ -9,9 +9,11 @@
 #include "planet_mesh.hpp"
 
 #include <cstddef>
+#include <functional>
 #include <memory>
 #include <span>
 #include <stdexcept>
+#include <exception>
 #include <vector>
 
 namespace planetsim {
@@ -142,6 +144,104 @@ inline void diffusion_source(const TransportGraph& graph, double conductance_W_K
     source_W_m2 = Field2D<double>(graph.size(), 0.0);
     gpu::check(psim_diffusion_source(graph.device(), conductance_W_K, temperature_K.values().data(),
                                      source_W_m2.values().data(), PSIM_HOST));
+}
+
+// ---------------------------------------------------------------------------
+// The implicit transport (heat_transport.hpp:78-138): Newton on
+// F(h) = h - K lap T(h) with the caller's column response, the reference's
+// line search, kink handling and result sums. The response runs on the host
+// (called at every iterate and line-search trial); the diffusion and the
+// Newton system's PCG run on the device.
+
+using TransportResponse = std::function<void(const Field2D<double>& source_W_m2, Field2D<double>& mean_K,
+                                             Field2D<double>& slope_K_m2_W)>;
+
+struct ImplicitTransportSettings {
+    int max_newton_iterations = 30;
+    int max_line_search_halvings = 12;
+    double rounding_floor_W_m2 = 1e-3;
+    int max_non_monotone_steps = 4;
+    double max_non_monotone_growth = 100.0;
+    double newton_tolerance_W_m2 = 1e-6;
+    double cg_relative_tolerance = 1e-6;
+    int max_cg_iterations = 2'000;
+    const Field2D<double>* initial_source_W_m2 = nullptr;   // accepted; the first iterate is the floor
+};
+
+inline constexpr double coupled_newton_tolerance_W_m2 = 1e-4;
+
+struct ImplicitTransportResult {
+    Field2D<double> source_W_m2;
+    Field2D<double> mean_K;
+    double consistency_residual_W_m2 = 0.0;
+    int newton_iterations = 0;
+    int cg_iterations = 0;
+    double sum_W = 0.0;
+    double absolute_sum_W = 0.0;
+    double dissipation_W_K = 0.0;
+};
+
+namespace detail {
+struct ResponseCall {
+    const TransportResponse* response;
+    std::size_t n;
+    std::exception_ptr error;
+};
+// C callback into the std::function; an exception is held and rethrown
+// once the library call returns (it cannot cross the C boundary).
+inline void response_trampoline(void* user, const double* source, double* mean, double* slope, std::uint64_t n) {
+    auto* call = static_cast<ResponseCall*>(user);
+    if (call->error) return;
+    try {
+        const Field2D<double> h(std::vector<double>(source, source + n));
+        Field2D<double> m(n, 0.0), s(n, 0.0);
+        (*call->response)(h, m, s);
+        std::copy(m.values().begin(), m.values().end(), mean);
+        std::copy(s.values().begin(), s.values().end(), slope);
+    } catch (...) {
+        call->error = std::current_exception();
+        std::fill(mean, mean + n, 0.0);
+        std::fill(slope, slope + n, 0.0);
+    }
+}
+}  // namespace detail
+
+[[nodiscard]] inline ImplicitTransportResult solve_implicit_transport(
+    const TransportGraph& graph, double conductance_W_K, const Field2D<double>& source_floor_W_m2,
+    const TransportResponse& response, const ImplicitTransportSettings& settings = {},
+    std::size_t worker_count = 1U, const std::vector<double>* latent_W_m2 = nullptr) {
+    const std::size_t n = graph.size();
+    if (source_floor_W_m2.size() != n) throw std::invalid_argument("transport floor does not match the mesh");
+    if (latent_W_m2 != nullptr && latent_W_m2->size() != n)
+        throw std::invalid_argument("transport latent heat does not match the mesh");
+    gpu::check_workers(worker_count);
+    psim_transport_settings st{};
+    st.max_newton_iterations = settings.max_newton_iterations;
+    st.max_line_search_halvings = settings.max_line_search_halvings;
+    st.rounding_floor_W_m2 = settings.rounding_floor_W_m2;
+    st.max_non_monotone_steps = settings.max_non_monotone_steps;
+    st.max_non_monotone_growth = settings.max_non_monotone_growth;
+    st.newton_tolerance_W_m2 = settings.newton_tolerance_W_m2;
+    st.cg_relative_tolerance = settings.cg_relative_tolerance;
+    st.max_cg_iterations = settings.max_cg_iterations;
+    detail::ResponseCall call{&response, n, nullptr};
+    ImplicitTransportResult result;
+    result.source_W_m2 = Field2D<double>(n, 0.0);
+    result.mean_K = Field2D<double>(n, 0.0);
+    psim_transport_result r{};
+    const int status = psim_implicit_transport(graph.device(), conductance_W_K, source_floor_W_m2.values().data(),
+                                               &detail::response_trampoline, &call, &st,
+                                               latent_W_m2 != nullptr ? latent_W_m2->data() : nullptr,
+                                               result.source_W_m2.values().data(), result.mean_K.values().data(), &r);
+    if (call.error) std::rethrow_exception(call.error);
+    gpu::check(status);
+    result.consistency_residual_W_m2 = r.consistency_residual_W_m2;
+    result.newton_iterations = r.newton_iterations;
+    result.cg_iterations = r.cg_iterations;
+    result.sum_W = r.sum_W;
+    result.absolute_sum_W = r.absolute_sum_W;
+    result.dissipation_W_K = r.dissipation_W_K;
+    return result;
 }
 
 // ---------------------------------------------------------------------------

@@ -209,6 +209,16 @@ int psim_system_apply(psim_system* system, const double* x, double* out, int whe
 int psim_system_solve(psim_system* system, const double* b, double* x, double tolerance, int32_t max_iterations,
                       psim_solve_info* info, int where);
 
+/* Preconditioned conjugate gradients with the V-cycle on the system's matrix
+ * (plain FP64 systems), the reference's Newton-system solve
+ * (heat_transport.cpp:328-429, solve_newton_system): from x = 0, stops when
+ * ||b - A x|| <= tolerance ||b|| (recursive residual), after max_iterations,
+ * or when (r, M r) <= 0. hold (n bytes, or NULL) marks rows held at 0: the
+ * caller makes them decoupled identity rows of the matrix with b = 0 there,
+ * and x stays exactly 0 on them. info->iterations counts CG iterations. */
+int psim_system_pcg(psim_system* system, const double* b, const uint8_t* hold, double* x, double tolerance,
+                    int32_t max_iterations, psim_solve_info* info, int where);
+
 /* ------------------------------------------------ implicit heat step
  * The benchmark's unit of work: (diag(C_i A_i) + dt L_K) T = C_i A_i T_old
  * + dt A_i S_i, with L_K the two-point Laplacian with entry weights
@@ -241,6 +251,37 @@ int psim_heat_chain(psim_system* system, const double* conductance_per_entry, co
                     const double* temperature_old, const double* forcings, int32_t steps, double dt_s,
                     double tolerance, int32_t max_iterations, double* temperature_new, psim_solve_info* step_info,
                     int where);
+
+/* ---- The implicit transport (heat_transport.hpp:78-138,
+ * solve_implicit_transport; heat_transport.cpp:595-772). Newton on
+ * F(h) = h - K lap T(h) (+ latent) with the reference's backtracking line
+ * search, kink handling and result sums; the response is the caller's
+ * (host callback, called at every iterate and trial); the diffusion and the
+ * Newton system's PCG run on the device. */
+typedef void (*psim_response_fn)(void* user, const double* source_W_m2, double* mean_K, double* slope_K_m2_W,
+                                 uint64_t n);
+
+typedef struct {
+    int32_t max_newton_iterations, max_line_search_halvings;
+    double rounding_floor_W_m2;
+    int32_t max_non_monotone_steps, pad;
+    double max_non_monotone_growth, newton_tolerance_W_m2, cg_relative_tolerance;
+    int32_t max_cg_iterations, pad2;
+} psim_transport_settings;
+
+typedef struct {
+    int32_t newton_iterations, cg_iterations;
+    double consistency_residual_W_m2, sum_W, absolute_sum_W, dissipation_W_K;
+} psim_transport_result;
+
+/* The reference's ImplicitTransportSettings defaults. */
+void psim_transport_settings_default(psim_transport_settings* settings);
+/* source_floor_W_m2, latent_W_m2 (or NULL), source_W_m2 and mean_K: host,
+ * n = psim_graph_size. settings NULL: the defaults. */
+int psim_implicit_transport(psim_graph* graph, double conductance_W_K, const double* source_floor_W_m2,
+                            psim_response_fn response, void* user, const psim_transport_settings* settings,
+                            const double* latent_W_m2, double* source_W_m2, double* mean_K,
+                            psim_transport_result* result);
 
 /* ---- Batches (no counterpart in the reference, whose solves run one at a
  * time): many systems, of any graphs and precisions on one context, solved

@@ -1073,6 +1073,87 @@ int orc_bicgstab(const void* h, const void* mg, const double* diag, const double
     });
 }
 
+// heat_transport.cpp:328-429 (solve_newton_system): cells with slope <= 0
+// become identity rows (zero right-hand side), the others
+//   (A_c / s_c) x_c + K sum_e w_e (x_c - x_n) = rhs_c   (free neighbours only),
+// Galerkin values and the dense factor on every call, then PCG with the
+// V-cycle and block-ordered dot products from x = 0, stopping when
+// ||r|| <= tol ||r_0||, after max_it iterations, or when (r, z) <= 0.
+int newton_system(const Mesh& m, std::vector<MgLevel>& levels, double k, const double* slope,
+                  const std::vector<double>& rhs, std::vector<double>& solution, double tol, int max_it,
+                  size_t workers) {
+    MgLevel& fine = levels[0];
+    each_block(m.n, workers, [&](size_t, size_t begin, size_t end) {
+        for (size_t c = begin; c < end; ++c) {
+            const bool free = slope[c] > 0.0;
+            fine.diag[c] = free ? m.area[c] / slope[c] + k * m.g_weight_sum[c] : 1.0;
+            for (size_t q = m.g_offset[c]; q < m.g_offset[c + 1]; ++q)
+                fine.val[q] = free && slope[m.ce_neighbor[q]] > 0.0 ? -k * m.g_weight[q] : 0.0;
+        }
+    });
+    for (size_t j = 0; j + 1 < levels.size(); ++j) galerkin(levels[j], levels[j + 1]);
+    cholesky(levels.back());
+    const auto dot = [&](const std::vector<double>& u, const std::vector<double>& v) {
+        return block_sum(m.n, workers, [&](size_t begin, size_t end) {
+            double s = 0.0;
+            for (size_t i = begin; i < end; ++i) s += u[i] * v[i];
+            return s;
+        });
+    };
+    std::vector<double> x(m.n, 0.0), r(m.n, 0.0), z, product(m.n, 0.0);
+    for (size_t c = 0; c < m.n; ++c) r[c] = slope[c] > 0.0 ? rhs[c] : 0.0;
+    vcycle(levels, 0, r, z, workers);
+    std::vector<double> d = z;
+    double rho = dot(r, z);
+    const double target = tol * std::sqrt(dot(r, r));
+    int it = 0;
+    while (it < max_it && std::sqrt(dot(r, r)) > target && rho > 0.0) {
+        csr_apply(m, fine.diag.data(), fine.val.data(), d, product, workers);
+        const double alpha = rho / dot(d, product);
+        each_block(m.n, workers, [&](size_t, size_t begin, size_t end) {
+            for (size_t c = begin; c < end; ++c) {
+                x[c] += alpha * d[c];
+                r[c] -= alpha * product[c];
+            }
+        });
+        vcycle(levels, 0, r, z, workers);
+        const double next_rho = dot(r, z);
+        const double beta = next_rho / rho;
+        rho = next_rho;
+        each_block(m.n, workers, [&](size_t, size_t begin, size_t end) {
+            for (size_t c = begin; c < end; ++c) d[c] = z[c] + beta * d[c];
+        });
+        ++it;
+    }
+    solution = x;
+    return it;
+}
+
+void diffusion(const Mesh& m, double k, const std::vector<double>& t, std::vector<double>& out, size_t workers) {
+    out.assign(m.n, 0.0);
+    each_block(m.n, workers, [&](size_t, size_t begin, size_t end) {
+        for (size_t c = begin; c < end; ++c) {
+            const double own = t[c];
+            double s = 0.0;
+            for (size_t q = m.g_offset[c]; q < m.g_offset[c + 1]; ++q) s += (t[m.ce_neighbor[q]] - own) * m.g_weight[q];
+            out[c] = k * s / m.area[c];
+        }
+    });
+}
+
+double block_max_abs(size_t n, size_t workers, const std::vector<double>& v) {
+    const size_t blocks = (n + block_cells - 1) / block_cells;
+    std::vector<double> partials(blocks, 0.0);
+    each_block(n, workers, [&](size_t b, size_t begin, size_t end) {
+        double largest = 0.0;
+        for (size_t i = begin; i < end; ++i) largest = std::max(largest, std::abs(v[i]));
+        partials[b] = largest;
+    });
+    double total = 0.0;
+    for (const double p : partials) total = std::max(total, p);
+    return total;
+}
+
 int orc_heat_instance(const void* h, uint64_t seed, double k_base, double sigma, double* cond,
                       double* ca, uint8_t* land, double* temp) {
     return guard([&] { heat_instance(M(h), seed, k_base, sigma, cond, ca, land, temp); });
@@ -1107,6 +1188,120 @@ double orc_relative_residual(const void* h, const double* diag, const double* of
         return s;
     });
     return std::sqrt(rr) / std::sqrt(bb);
+}
+
+int orc_newton_system(const void* h, double k, const double* slope, const double* rhs, double tol, int max_it,
+                      uint64_t workers, double* solution, int* iterations) {
+    return guard([&] {
+        const Mesh& m = M(h);
+        auto levels = hierarchy(m);
+        std::vector<double> b(rhs, rhs + m.n), x;
+        *iterations = newton_system(m, levels, k, slope, b, x, tol, max_it, workers);
+        std::memcpy(solution, x.data(), m.n * 8);
+    });
+}
+
+// heat_transport.cpp:595-772 (solve_implicit_transport): Newton on
+// F(h) = h - K lap T(h) (+ latent), merit sum A F^2 (block order), with the
+// backtracking line search, the bounded full steps across kinks, and the
+// conservation sums of the final transport.
+int orc_solve_implicit_transport(const void* h, double k, const double* floor, orc_response_fn response, void* user,
+                                 const orc_transport_settings* settings, uint64_t workers, const double* latent,
+                                 double* source_out, double* mean_out, orc_transport_result* result) {
+    return guard([&] {
+        if (!std::isfinite(k) || !(k > 0.0))
+            throw std::invalid_argument("transport conductance must be finite and positive");
+        const Mesh& m = M(h);
+        const orc_transport_settings& st = *settings;
+        const size_t n = m.n;
+        auto levels = hierarchy(m);
+        orc_transport_result res{};
+        std::vector<double> source(n), mean(n, 0.0), slope(n, 0.0), diff(n, 0.0), residual(n, 0.0), rhs(n, 0.0),
+            delta_mean, delta_source, trial(n, 0.0), trial_mean(n, 0.0), trial_slope(n, 0.0);
+        for (size_t c = 0; c < n; ++c) source[c] = std::max(0.0, floor[c]);
+        const auto evaluate = [&](const std::vector<double>& at, std::vector<double>& at_mean,
+                                  std::vector<double>& at_slope, std::vector<double>& at_residual) {
+            response(user, at.data(), at_mean.data(), at_slope.data(), n);
+            diffusion(m, k, at_mean, diff, workers);
+            return block_sum(n, workers, [&](size_t begin, size_t end) {
+                double merit = 0.0;
+                for (size_t c = begin; c < end; ++c) {
+                    at_residual[c] = at[c] - diff[c];
+                    if (latent != nullptr) at_residual[c] += latent[c];
+                    merit += m.area[c] * at_residual[c] * at_residual[c];
+                }
+                return merit;
+            });
+        };
+        double merit = evaluate(source, mean, slope, residual);
+        int non_monotone = 0;
+        for (int iteration = 0; iteration < st.max_newton_iterations; ++iteration) {
+            if (block_max_abs(n, workers, residual) <= st.newton_tolerance) break;
+            for (size_t c = 0; c < n; ++c) rhs[c] = -residual[c] * m.area[c];
+            res.cg_iterations +=
+                newton_system(m, levels, k, slope.data(), rhs, delta_mean, st.cg_relative_tolerance,
+                              st.max_cg_iterations, workers);
+            diffusion(m, k, delta_mean, delta_source, workers);
+            for (size_t c = 0; c < n; ++c) delta_source[c] -= residual[c];
+            double step = 1.0, trial_merit = 0.0;
+            std::vector<double> trial_residual;
+            for (int halving = 0;; ++halving) {
+                for (size_t c = 0; c < n; ++c) trial[c] = std::max(source[c] + step * delta_source[c], floor[c]);
+                trial_residual.assign(n, 0.0);
+                trial_merit = evaluate(trial, trial_mean, trial_slope, trial_residual);
+                if (trial_merit < merit) {
+                    residual = trial_residual;
+                    break;
+                }
+                if (halving == st.max_line_search_halvings) {
+                    if (block_max_abs(n, workers, residual) <= st.rounding_floor ||
+                        non_monotone == st.max_non_monotone_steps) {
+                        step = 0.0;
+                    } else {
+                        ++non_monotone;
+                        step = 1.0;
+                        for (size_t c = 0; c < n; ++c) trial[c] = std::max(source[c] + delta_source[c], floor[c]);
+                        trial_merit = evaluate(trial, trial_mean, trial_slope, trial_residual);
+                        if (std::isfinite(trial_merit) && trial_merit <= st.max_non_monotone_growth * merit)
+                            residual = trial_residual;
+                        else
+                            step = 0.0;
+                    }
+                    break;
+                }
+                step *= 0.5;
+            }
+            if (step == 0.0) break;
+            std::swap(source, trial);
+            std::swap(mean, trial_mean);
+            std::swap(slope, trial_slope);
+            merit = trial_merit;
+            ++res.newton_iterations;
+        }
+        std::vector<double> out;
+        diffusion(m, k, mean, out, workers);
+        if (latent != nullptr)
+            for (size_t c = 0; c < n; ++c) out[c] -= latent[c];
+        res.consistency_residual = block_max_abs(n, workers, residual);
+        const size_t blocks = (n + block_cells - 1) / block_cells;
+        std::vector<double> ps(blocks, 0.0), pa(blocks, 0.0), pd(blocks, 0.0);
+        each_block(n, workers, [&](size_t b, size_t begin, size_t end) {
+            for (size_t c = begin; c < end; ++c) {
+                const double flux = m.area[c] * out[c];
+                ps[b] += flux;
+                pa[b] += std::abs(flux);
+                pd[b] += flux * mean[c];
+            }
+        });
+        for (size_t b = 0; b < blocks; ++b) {
+            res.sum_W += ps[b];
+            res.absolute_sum_W += pa[b];
+            res.dissipation_W_K += pd[b];
+        }
+        std::memcpy(source_out, out.data(), n * 8);
+        std::memcpy(mean_out, mean.data(), n * 8);
+        *result = res;
+    });
 }
 
 }  // extern "C"

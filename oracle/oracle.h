@@ -90,6 +90,31 @@ int orc_bicgstab(const void* mesh, const void* mg, const double* diagonal, const
                  const double* b, double* x, double tolerance, int max_iterations,
                  uint64_t workers, int* iterations);
 
+/* solve_newton_system (heat_transport.cpp:328-429): PCG with the V-cycle on
+ * the Newton matrix of slope (held cells: identity rows), on a fresh
+ * hierarchy of the mesh. */
+int orc_newton_system(const void* mesh, double conductance, const double* slope, const double* rhs,
+                      double cg_tolerance, int max_cg_iterations, uint64_t workers, double* solution,
+                      int* iterations);
+
+/* solve_implicit_transport (heat_transport.cpp:595-772) with a C response
+ * callback: response(user, source[n], mean[n] out, slope[n] out, n). */
+typedef void (*orc_response_fn)(void* user, const double* source, double* mean, double* slope, uint64_t n);
+typedef struct {
+    int max_newton_iterations, max_line_search_halvings;
+    double rounding_floor;
+    int max_non_monotone_steps, pad;
+    double max_non_monotone_growth, newton_tolerance, cg_relative_tolerance;
+    int max_cg_iterations, pad2;
+} orc_transport_settings;
+typedef struct {
+    int newton_iterations, cg_iterations;
+    double consistency_residual, sum_W, absolute_sum_W, dissipation_W_K;
+} orc_transport_result;
+int orc_solve_implicit_transport(const void* mesh, double conductance, const double* floor, orc_response_fn response,
+                                 void* user, const orc_transport_settings* settings, uint64_t workers,
+                                 const double* latent, double* source, double* mean, orc_transport_result* result);
+
 /* Synthetic heat-step workload (the project's seeded input generator).
  *   instance_seed = orc_mix_random_key(base_seed ^ instance)
  *   conductance_m2_s[E], capacity_area[n] = C_i A_i, land[n] in {0,1},

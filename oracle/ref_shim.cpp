@@ -377,4 +377,76 @@ int ref_coupled_linear_solve(const void* handle, const double* conductance, cons
     });
 }
 
+// solve_newton_system (heat_transport.cpp:328-429, anonymous namespace) on
+// a fresh hierarchy of the graph: PCG with the Gauss-Seidel V-cycle.
+int ref_newton_system(const void* handle, double conductance, const double* slope, const double* rhs,
+                      double cg_tolerance, int max_cg_iterations, std::uint64_t workers, double* solution,
+                      int* iterations) {
+    return guarded([&] {
+        const auto& graph = static_cast<const RefMesh*>(handle)->graph;
+        const std::size_t n = graph.size();
+        auto levels = planetsim::build_hierarchy(graph, n);
+        planetsim::ImplicitTransportSettings settings;
+        settings.cg_relative_tolerance = cg_tolerance;
+        settings.max_cg_iterations = max_cg_iterations;
+        const planetsim::Field2D<double> s(std::vector<double>(slope, slope + n));
+        const planetsim::Field2D<double> b(std::vector<double>(rhs, rhs + n));
+        planetsim::Field2D<double> x;
+        *iterations = planetsim::solve_newton_system(graph, conductance, s, b, levels, x, settings, workers);
+        std::memcpy(solution, x.values().data(), n * sizeof(double));
+    });
+}
+
+// The reference's public Newton transport solve (heat_transport.hpp:135)
+// with a C response callback.
+typedef void (*ref_response_fn)(void* user, const double* source, double* mean, double* slope, std::uint64_t n);
+struct RefTransportSettings {
+    int max_newton_iterations, max_line_search_halvings;
+    double rounding_floor;
+    int max_non_monotone_steps, pad;
+    double max_non_monotone_growth, newton_tolerance, cg_relative_tolerance;
+    int max_cg_iterations, pad2;
+};
+struct RefTransportResult {
+    int newton_iterations, cg_iterations;
+    double consistency_residual, sum_W, absolute_sum_W, dissipation_W_K;
+};
+
+int ref_solve_implicit_transport(const void* handle, double conductance, const double* floor,
+                                 ref_response_fn response, void* user, const RefTransportSettings* st,
+                                 std::uint64_t workers, const double* latent, double* source, double* mean,
+                                 RefTransportResult* result) {
+    return guarded([&] {
+        const auto& graph = static_cast<const RefMesh*>(handle)->graph;
+        const std::size_t n = graph.size();
+        const planetsim::TransportResponse bridge = [&](const planetsim::Field2D<double>& h,
+                                                        planetsim::Field2D<double>& m,
+                                                        planetsim::Field2D<double>& d) {
+            response(user, h.values().data(), m.values().data(), d.values().data(), n);
+        };
+        planetsim::ImplicitTransportSettings settings;
+        settings.max_newton_iterations = st->max_newton_iterations;
+        settings.max_line_search_halvings = st->max_line_search_halvings;
+        settings.rounding_floor_W_m2 = st->rounding_floor;
+        settings.max_non_monotone_steps = st->max_non_monotone_steps;
+        settings.max_non_monotone_growth = st->max_non_monotone_growth;
+        settings.newton_tolerance_W_m2 = st->newton_tolerance;
+        settings.cg_relative_tolerance = st->cg_relative_tolerance;
+        settings.max_cg_iterations = st->max_cg_iterations;
+        const planetsim::Field2D<double> fl(std::vector<double>(floor, floor + n));
+        std::vector<double> lat;
+        if (latent != nullptr) lat.assign(latent, latent + n);
+        const auto r = planetsim::solve_implicit_transport(graph, conductance, fl, bridge, settings, workers,
+                                                           latent != nullptr ? &lat : nullptr);
+        std::memcpy(source, r.source_W_m2.values().data(), n * sizeof(double));
+        std::memcpy(mean, r.mean_K.values().data(), n * sizeof(double));
+        result->newton_iterations = r.newton_iterations;
+        result->cg_iterations = r.cg_iterations;
+        result->consistency_residual = r.consistency_residual_W_m2;
+        result->sum_W = r.sum_W;
+        result->absolute_sum_W = r.absolute_sum_W;
+        result->dissipation_W_K = r.dissipation_W_K;
+    });
+}
+
 }  // extern "C"

@@ -20,6 +20,7 @@ __all__ = [
     "LIB_PATH", "lib", "PsimError", "Mesh", "Context", "FV", "Graph", "System", "SolveInfo",
     "HOST", "DEVICE", "F64", "F32", "PLAIN_FP64", "MIXED", "instance_seed", "heat_instance",
     "heat_forcing", "Hierarchy", "Batch", "DeviceArray", "Server", "ServeRequest", "ServeRecord",
+    "implicit_transport", "TransportSettings", "TransportResult",
 ]
 
 # PSIM_LIBRARY names another build of the same ABI (e.g. a tuning variant).
@@ -158,6 +159,27 @@ class ServeRecord(C.Structure):
                 ("restarts", C.c_int32), ("v_cycles", C.c_int32), ("relative_residual", C.c_double)]
 
 
+RESPONSE = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                       C.c_uint64)
+
+
+class TransportSettings(C.Structure):
+    _fields_ = [("max_newton_iterations", C.c_int32), ("max_line_search_halvings", C.c_int32),
+                ("rounding_floor_W_m2", C.c_double), ("max_non_monotone_steps", C.c_int32), ("pad", C.c_int32),
+                ("max_non_monotone_growth", C.c_double), ("newton_tolerance_W_m2", C.c_double),
+                ("cg_relative_tolerance", C.c_double), ("max_cg_iterations", C.c_int32), ("pad2", C.c_int32)]
+
+
+class TransportResult(C.Structure):
+    _fields_ = [("newton_iterations", C.c_int32), ("cg_iterations", C.c_int32),
+                ("consistency_residual_W_m2", C.c_double), ("sum_W", C.c_double), ("absolute_sum_W", C.c_double),
+                ("dissipation_W_K", C.c_double)]
+
+
+_sig("psim_transport_settings_default", None, C.POINTER(TransportSettings))
+_sig("psim_implicit_transport", C.c_int, _vp, _dbl, _pd, RESPONSE, _vp, C.POINTER(TransportSettings), _vp, _pd, _pd,
+     C.POINTER(TransportResult))
+_sig("psim_system_pcg", C.c_int, _vp, _vp, _vp, _vp, _dbl, _i32, C.POINTER(SolveInfo), C.c_int)
 _sig("psim_server_create", C.c_int, _vp, _pp, C.POINTER(_dbl), C.POINTER(_i32), _i32, C.c_int, _u64, C.POINTER(_vp))
 _sig("psim_server_destroy", None, _vp)
 _sig("psim_server_plan", C.c_int, _vp, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_u64))
@@ -561,6 +583,20 @@ class System:
         check(lib.psim_system_last_info(self.handle, C.byref(info)))
         return info
 
+    def pcg(self, b, hold=None, tolerance=1e-6, max_iterations=2000):
+        """Newton-system PCG on the matrix set by set_matrix (host arrays);
+        hold: rows kept at 0 (identity rows of the matrix, b = 0 there)."""
+        bb = np.ascontiguousarray(b, np.float64)
+        hh = None if hold is None else np.ascontiguousarray(hold, np.uint8)
+        x = np.empty_like(bb)
+        info = SolveInfo()
+        status = lib.psim_system_pcg(self.handle, bb.ctypes.data_as(_vp),
+                                     None if hh is None else hh.ctypes.data_as(_vp), x.ctypes.data_as(_vp),
+                                     tolerance, max_iterations, C.byref(info), HOST)
+        if status not in (0, 5):
+            check(status)
+        return x, info
+
     def heat_step(self, t_old, forcing, dt=2.63e6, tolerance=1e-10, max_iterations=1000):
         t = np.ascontiguousarray(t_old, np.float64)
         f = np.ascontiguousarray(forcing, np.float64)
@@ -691,3 +727,25 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+def implicit_transport(graph: Graph, conductance, floor, response, settings=None, latent=None):
+    """psim_implicit_transport: response(source ndarray) -> (mean, slope).
+    Returns (source_W_m2, mean_K, TransportResult)."""
+    n = graph.mesh.cells
+
+    def call(_user, source, mean, slope, count):
+        m, s = response(np.ctypeslib.as_array(source, shape=(count,)).copy())
+        np.ctypeslib.as_array(mean, shape=(count,))[:] = m
+        np.ctypeslib.as_array(slope, shape=(count,))[:] = s
+
+    cb = RESPONSE(call)
+    if settings is None:
+        settings = TransportSettings()
+        lib.psim_transport_settings_default(C.byref(settings))
+    fl = np.ascontiguousarray(floor, np.float64)
+    lat = None if latent is None else np.ascontiguousarray(latent, np.float64)
+    src, mean, res = np.zeros(n), np.zeros(n), TransportResult()
+    check(lib.psim_implicit_transport(graph.handle, conductance, fl, cb, None, C.byref(settings),
+                                      None if lat is None else lat.ctypes.data_as(_vp), src, mean, C.byref(res)))
+    return src, mean, res

@@ -286,6 +286,7 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
         auto gr = std::make_unique<psim_graph>();
         gr->ctx = ctx;
         gr->host = psim::build_hierarchy(g);
+        gr->graph = g;
         gr->n = g.n;
         gr->nnz = static_cast<int32_t>(nnz);
         gr->depth = static_cast<int32_t>(gr->host.levels.size());
@@ -645,6 +646,42 @@ int psim_system_solve(psim_system* s, const double* b, double* x, double toleran
             psim::hip_check(hipMemcpyAsync(s->stage_a, b, bytes, hipMemcpyHostToDevice, stream), "copy");
             p.in = s->stage_a;
             p.out = s->stage_b;
+        }
+        launch(s, p);
+        if (where == PSIM_HOST)
+            psim::hip_check(hipMemcpyAsync(x, s->stage_b, bytes, hipMemcpyDeviceToHost, stream), "copy");
+        if (where == PSIM_HOST || info != nullptr) fill_info(fetch_result(s), info, s->graph->ctx->wall_clock_hz);
+    });
+}
+
+int psim_system_pcg(psim_system* s, const double* b, const uint8_t* hold, double* x, double tolerance,
+                    int32_t max_iterations, psim_solve_info* info, int where) {
+    return guarded([&] {
+        require_matrix(s);
+        psim::check_where(where);
+        if (s->precision != PSIM_PLAIN_FP64) throw std::invalid_argument("PCG needs a plain FP64 system");
+        if (b == nullptr || x == nullptr) throw std::invalid_argument("null argument");
+        if (!(tolerance >= 0.0) || max_iterations < 0) throw std::invalid_argument("invalid solver settings");
+        hipStream_t stream = s->graph->ctx->stream;
+        const size_t bytes = static_cast<size_t>(s->graph->n) * 8;
+        Launch p{};
+        p.op = psim::dev::kOpPcg;
+        p.tolerance = tolerance;
+        p.max_iterations = max_iterations;
+        p.in = b;
+        p.out = x;
+        p.hold = hold;
+        if (where == PSIM_HOST) {
+            psim::hip_check(hipMemcpyAsync(s->stage_a, b, bytes, hipMemcpyHostToDevice, stream), "copy");
+            p.in = s->stage_a;
+            p.out = s->stage_b;
+            if (hold != nullptr) {
+                // the mask rides in the third staging vector (n bytes <= 8 n)
+                psim::hip_check(hipMemcpyAsync(s->stage_c, hold, static_cast<size_t>(s->graph->n), hipMemcpyHostToDevice,
+                                               stream),
+                                "copy");
+                p.hold = reinterpret_cast<const uint8_t*>(s->stage_c);
+            }
         }
         launch(s, p);
         if (where == PSIM_HOST)

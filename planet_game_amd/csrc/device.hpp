@@ -35,7 +35,8 @@ enum Op : int32_t {
     kOpPrecondition = 16, // out = V-cycle(in)
     kOpApply = 32,        // out = A in (FP64)
     kOpCsrValues = 64,    // level-0 off-diagonals given in CSR order in `in` (with kOpSetMatrix)
-    kOpChain = 128        // `steps` heat steps, each T feeding the next (with kOpSolve | kOpHeatRhs)
+    kOpChain = 128,       // `steps` heat steps, each T feeding the next (with kOpSolve | kOpHeatRhs)
+    kOpPcg = 256          // PCG on the FP64 matrix: b in `in`, x to `out` (plain systems)
 };
 
 // Structure of one multigrid level (per graph). Off-diagonals are stored
@@ -132,6 +133,7 @@ struct Launch {
     const double* in;   // b for kOpSolve, input of precondition / apply, CSR values
     double* out;        // output of precondition / apply
     SolveResult* step_results;         // kOpChain: one record per step, or null
+    const uint8_t* hold;               // kOpPcg: rows held at 0 (identity rows), or null
     uint64_t ticket;                   // server: echoed in the completion record
     int64_t completion;                // server: completion record index
 };

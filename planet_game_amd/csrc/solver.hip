@@ -52,7 +52,7 @@ __device__ inline double wave_max(double v) {
 struct Lds {
     double red[kPartials][4];        // per-wave partials
     double bcast[kPartials];
-    double tile[kTile];              // restriction staging
+    double tile[2][kTile];           // restriction staging (two tiles in flight)
     double cvec[kMaxCoarse];         // coarse right-hand side, then solution
     double crhs[kMaxCoarse];         // coarse right-hand side (FP64 staging)
     double colk[kMaxCoarse];         // sweep pivot column (set_matrix)
@@ -87,6 +87,53 @@ template <int K> __device__ inline void store_partials(double* part, int tile, c
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < K; ++q) part[tile * kPartials + q] = tot[q];
+    }
+}
+
+// Partials of R tiles at once (one reduction pass, one pair of barriers):
+// tile tl[r] gets exactly what store_partials<K> would give it.
+template <int K, int R>
+__device__ inline void store_partials_n(double* part, const int (&tl)[R], const double (&v)[R][K], Lds& lds) {
+    static_assert(K * R <= kPartials, "reduction slots");
+    if constexpr (R == 1) {
+        store_partials<K>(part, tl[0], v[0], lds);
+    } else {
+        double flat[K * R], tot[K * R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int q = 0; q < K; ++q) flat[r * K + q] = v[r][q];
+        tile_sums<K * R>(flat, tot, lds);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int q = 0; q < K; ++q) part[tl[r] * kPartials + q] = tot[r * K + q];
+        }
+    }
+}
+
+// This workgroup's tiles of a sweep (g, g + G, g + 2G, ...), two at a time
+// while two remain, so that each thread keeps two rows' loads in flight;
+// f is called with the tile list (one or two tiles). Pairs hold two rows'
+// slots in registers, which FP32 sweeps afford and FP64 sweeps do not (the
+// solver would spill), so FP64 sweeps go one tile at a time.
+template <bool Pairs = true, class F> __device__ inline void tile_loop(int tiles, int g, int G, const F& f) {
+    if constexpr (!Pairs) {
+        for (int t = g; t < tiles; t += G) {
+            const int tl[1] = {t};
+            f(tl);
+        }
+        return;
+    }
+    for (int t = g; t < tiles; t += 2 * G) {
+        if (t + G < tiles) {
+            const int tl[2] = {t, t + G};
+            f(tl);
+        } else {
+            const int tl[1] = {t};
+            f(tl);
+        }
     }
 }
 
@@ -264,7 +311,7 @@ template <class VT> struct Source {
 // a valid address, and is discarded), then the folds, so a row costs two
 // dependent memory round trips instead of two per slot. Fold order and
 // every rounding are those of the plain slot loop.
-constexpr int kMaxWidth = 8;
+constexpr int kMaxWidth = 7;   // widest icosphere level (hexagons + aggregation: 6, 7)
 
 template <class T, class Gather, class Fold>
 __device__ inline void ell_row(const int32_t* col, const T* off, int n, int W, int i, const Gather& gather,
@@ -293,6 +340,56 @@ __device__ inline void ell_row(const int32_t* col, const T* off, int n, int W, i
     }
 }
 
+// The same for R rows of a thread at once (rows of different tiles): all
+// rows' column and value loads, then all their gathers, then each row's
+// folds in slot order. fold(r, off, value) and gather(r, slot, column).
+template <int R, class T, class Gather, class Fold>
+__device__ inline void ell_rows(const int32_t* col, const T* off, int n, int W, const int (&i)[R], const Gather& gather,
+                                const Fold& fold) {
+    if (W <= kMaxWidth) {
+        int c[R][kMaxWidth];
+        T o[R][kMaxWidth];
+#pragma unroll
+        for (int s = 0; s < kMaxWidth; ++s)
+            if (s < W) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    c[r][s] = col[s * n + i[r]];
+                    o[r][s] = off[s * n + i[r]];
+                }
+            }
+        decltype(gather(0, 0, 0)) v[R][kMaxWidth];
+#pragma unroll
+        for (int s = 0; s < kMaxWidth; ++s)
+            if (s < W) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r][s] = gather(r, s, c[r][s] >= 0 ? c[r][s] : i[r]);
+            }
+#pragma unroll
+        for (int s = 0; s < kMaxWidth; ++s)
+            if (s < W) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (c[r][s] >= 0) fold(r, o[r][s], v[r][s]);
+            }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            for (int s = 0; s < W; ++s) {
+                const int cs = col[s * n + i[r]];
+                if (cs >= 0) fold(r, off[s * n + i[r]], gather(r, s, cs));
+            }
+    }
+}
+
+// Row of tile tl and this thread; rows past n read row 0 (a valid address)
+// and are discarded.
+__device__ inline int tile_row(int tl, int n, bool& ok) {
+    const int i = tl * kTile + static_cast<int>(threadIdx.x);
+    ok = i < n;
+    return ok ? i : 0;
+}
+
 // ------------------------------------------------------------ V-cycle phases
 
 // Pre-smoothing from zero, residual and restriction of level j in one sweep
@@ -308,32 +405,48 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
     const VT* diag = V.diag(j);
     const VT* dinv = V.dinv(j);
     const VT* off = V.off(j);
-    for (int tile = g; tile < L.down_tiles; tile += G) {
-        const int c0 = L.tile_start[tile], c1 = L.tile_start[tile + 1];
-        const int base = L.agg_start[c0], count = L.agg_start[c1] - base;
-        double sq[1] = {0.0};
-        if (static_cast<int>(threadIdx.x) < count) {
-            const int i = L.agg_rows[base + threadIdx.x];
-            const VT wi = src.at(i);
-            if (keep != nullptr) keep[i] = wi;
-            sq[0] = static_cast<double>(wi) * static_cast<double>(wi);
-            const VT xi = om * dinv[i] * wi;
-            VT acc = wi - diag[i] * xi;
-            ell_row(
-                L.col, off, n, W, i, [&](int, int c) { return om * dinv[c] * src.at(c); },
-                [&](VT o, VT v) { acc -= o * v; });
-            lds.tile[threadIdx.x] = static_cast<double>(acc);
+    tile_loop<sizeof(VT) == 4>(L.down_tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+        int c0[R], c1[R], base[R], i[R];
+        bool ok[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            c0[r] = L.tile_start[tl[r]];
+            c1[r] = L.tile_start[tl[r] + 1];
+            base[r] = L.agg_start[c0[r]];
+            ok[r] = static_cast<int>(threadIdx.x) < L.agg_start[c1[r]] - base[r];
+            i[r] = ok[r] ? L.agg_rows[base[r] + threadIdx.x] : 0;
         }
-        if (part != nullptr) store_partials<1>(part, tile, sq, lds);
-        __syncthreads();
-        if (static_cast<int>(threadIdx.x) < c1 - c0) {
-            const int c = c0 + threadIdx.x;
-            double sum = 0.0;
-            for (int q = L.agg_start[c] - base; q < L.agg_start[c + 1] - base; ++q) sum += lds.tile[q];
-            coarse_rhs[c] = static_cast<VT>(sum);
+        VT wi[R], acc[R];
+        double sq[R][1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) wi[r] = src.at(i[r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            sq[r][0] = ok[r] ? static_cast<double>(wi[r]) * static_cast<double>(wi[r]) : 0.0;
+            const VT xi = om * dinv[i[r]] * wi[r];
+            acc[r] = wi[r] - diag[i[r]] * xi;
         }
+        ell_rows<R>(
+            L.col, off, n, W, i, [&](int, int, int c) { return om * dinv[c] * src.at(c); },
+            [&](int r, VT o, VT v) { acc[r] -= o * v; });
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (ok[r]) {
+                if (keep != nullptr) keep[i[r]] = wi[r];
+                lds.tile[r][threadIdx.x] = static_cast<double>(acc[r]);
+            }
+        if (part != nullptr) store_partials_n<1, R>(part, tl, sq, lds);
         __syncthreads();
-    }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (static_cast<int>(threadIdx.x) < c1[r] - c0[r]) {
+                const int c = c0[r] + threadIdx.x;
+                double sum = 0.0;
+                for (int q = L.agg_start[c] - base[r]; q < L.agg_start[c + 1] - base[r]; ++q) sum += lds.tile[r][q];
+                coarse_rhs[c] = static_cast<VT>(sum);
+            }
+        __syncthreads();
+    });
 }
 
 // Rows [list[0..count)) of the coarsest level's solution A_c^-1 rhs into
@@ -377,7 +490,7 @@ __device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs
 // from the coarse solve each workgroup recomputes into LDS.
 template <class VT>
 __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V, int j, const VT* rhs, const VT* corr, VT* out, int g,
-                         int G, Lds& lds) {
+                         int G, Lds& lds, double* dot_part = nullptr, const VT* dot_with = nullptr) {
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
     const VT om = static_cast<VT>(I.omega[j]);
@@ -386,27 +499,63 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
     const VT* off = V.off(j);
     const bool fused = j + 2 == I.depth;
     const int tiles = (n + kTile - 1) / kTile;
-    for (int tile = g; tile < tiles; tile += G) {
-        if (fused)
+    if (fused) {
+        // the level above the coarsest: each tile first solves the coarse rows
+        // it reads into LDS, so its tiles go one at a time
+        for (int tile = g; tile < tiles; tile += G) {
             coarse_rows_lds<VT>(I, static_cast<const VT*>(I.lvl_b[j + 1]), L.need + L.need_start[tile], 0,
                                 L.need_start[tile + 1] - L.need_start[tile], lds);
-        const int i = tile * kTile + threadIdx.x;
-        if (i < n) {
-            const auto correction = [&](int a) { return fused ? static_cast<VT>(lds.cvec[a]) : corr[a]; };
-            const VT x2 = om * dinv[i] * rhs[i] + correction(L.agg[i]);
-            VT acc = rhs[i] - diag[i] * x2;
-            // the neighbour's aggregate comes from the slot's own map, so the
-            // correction gather does not wait for the column
-            ell_row(
-                L.col, off, n, W, i,
-                [&](int s, int c) {
-                    const int a = L.agg_col[s * n + i];
-                    return om * dinv[c] * rhs[c] + correction(a >= 0 ? a : 0);
-                },
-                [&](VT o, VT v) { acc -= o * v; });
-            out[i] = x2 + om * dinv[i] * acc;
+            const int i = tile * kTile + threadIdx.x;
+            double q[1] = {0.0};
+            if (i < n) {
+                const VT x2 = om * dinv[i] * rhs[i] + static_cast<VT>(lds.cvec[L.agg[i]]);
+                VT acc = rhs[i] - diag[i] * x2;
+                // the neighbour's aggregate comes from the slot's own map, so the
+                // correction read does not wait for the column
+                ell_row(
+                    L.col, off, n, W, i,
+                    [&](int s, int c) {
+                        const int a = L.agg_col[s * n + i];
+                        return om * dinv[c] * rhs[c] + static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
+                    },
+                    [&](VT o, VT v) { acc -= o * v; });
+                out[i] = x2 + om * dinv[i] * acc;
+                q[0] = dot_with != nullptr ? static_cast<double>(out[i]) * static_cast<double>(dot_with[i]) : 0.0;
+            }
+            if (dot_part != nullptr) store_partials<1>(dot_part, tile, q, lds);
         }
+        return;
     }
+    tile_loop<sizeof(VT) == 4>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+        int i[R];
+        bool ok[R];
+        VT x2[R], acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) i[r] = tile_row(tl[r], n, ok[r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            x2[r] = om * dinv[i[r]] * rhs[i[r]] + corr[L.agg[i[r]]];
+            acc[r] = rhs[i[r]] - diag[i[r]] * x2[r];
+        }
+        ell_rows<R>(
+            L.col, off, n, W, i,
+            [&](int r, int s, int c) {
+                const int a = L.agg_col[s * n + i[r]];
+                return om * dinv[c] * rhs[c] + corr[a >= 0 ? a : 0];
+            },
+            [&](int r, VT o, VT v) { acc[r] -= o * v; });
+        double q[R][1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            q[r][0] = 0.0;
+            if (ok[r]) {
+                const VT value = x2[r] + om * dinv[i[r]] * acc[r];
+                out[i[r]] = value;
+                if (dot_with != nullptr) q[r][0] = static_cast<double>(value) * static_cast<double>(dot_with[i[r]]);
+            }
+        }
+        if (dot_part != nullptr) store_partials_n<1, R>(dot_part, tl, q, lds);
+    });
 }
 
 // First sweep of a V-cycle: level 0's down phase (input evaluated from
@@ -439,14 +588,19 @@ __device__ inline int first_sweep_tiles(const Instance& I) { return I.depth == 1
 // input, out0 the level-0 output. Ends with a barrier.
 template <class VT>
 __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>& V, const VT* w0, VT* out0, int g, int G, Lds& lds,
-                            int& timeouts) {
+                            int& timeouts, double* dot_part = nullptr, const VT* dot_with = nullptr) {
     const int J = I.depth;
     if (J == 1) {
         for (int tile = g; tile < I.tiles; tile += G) {
             const int first = tile * kTile, rows = min(kTile, I.n - first);
             coarse_rows_lds<VT>(I, w0, nullptr, first, rows, lds);
             const int i = first + threadIdx.x;
-            if (i < I.n) out0[i] = static_cast<VT>(lds.cvec[i]);
+            double q[1] = {0.0};
+            if (i < I.n) {
+                out0[i] = static_cast<VT>(lds.cvec[i]);
+                if (dot_with != nullptr) q[0] = static_cast<double>(out0[i]) * static_cast<double>(dot_with[i]);
+            }
+            if (dot_part != nullptr) store_partials<1>(dot_part, tile, q, lds);
         }
         group_sync(timeouts);
         return;
@@ -461,7 +615,8 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
     for (int j = J - 2; j >= 0; --j) {
         const VT* rhs = j == 0 ? w0 : static_cast<const VT*>(I.lvl_b[j]);
         VT* out = j == 0 ? out0 : static_cast<VT*>(I.lvl_e[j]);
-        up_phase<VT>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds);
+        up_phase<VT>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds, j == 0 ? dot_part : nullptr,
+                     j == 0 ? dot_with : nullptr);
         group_sync(timeouts);
     }
 }
@@ -595,16 +750,18 @@ __device__ void coarse_inverse_blocked(const Instance& I, Lds& lds) {
         }
         __syncthreads();
         if (active) {
+            // row scales are formed as they are used (rs = colk[r] / pivot), so
+            // only the block and the column values are live across the update
             const double dinv = 1.0 / colk[k];
-            double rs[kSweepBlock], cv[kSweepBlock];
-#pragma unroll
-            for (int u = 0; u < kSweepBlock; ++u) rs[u] = colk[r0 + u] * dinv;
+            double cv[kSweepBlock];
 #pragma unroll
             for (int v = 0; v < kSweepBlock; ++v) cv[v] = colk[c0 + v];
 #pragma unroll
-            for (int u = 0; u < kSweepBlock; ++u)
+            for (int u = 0; u < kSweepBlock; ++u) {
+                const double rs = colk[r0 + u] * dinv;
 #pragma unroll
-                for (int v = 0; v < kSweepBlock; ++v) a[u][v] = fma(-rs[u], cv[v], a[u][v]);
+                for (int v = 0; v < kSweepBlock; ++v) a[u][v] = fma(-rs, cv[v], a[u][v]);
+            }
 #pragma unroll
             for (int KK = 0; KK < kSweepBlock; ++KK) {
                 if (KK != kk) continue;
@@ -614,7 +771,7 @@ __device__ void coarse_inverse_blocked(const Instance& I, Lds& lds) {
                 }
                 if (bj == K) {
 #pragma unroll
-                    for (int u = 0; u < kSweepBlock; ++u) a[u][KK] = rs[u];
+                    for (int u = 0; u < kSweepBlock; ++u) a[u][KK] = colk[r0 + u] * dinv;
                 }
                 if (bi == K && bj == K) a[KK][KK] = -dinv;
             }
@@ -706,19 +863,22 @@ __device__ __forceinline__ void set_matrix(const Instance& I, const Launch& P, i
             const int work_tiles = (nc + slots + kTile - 1) / kTile;
             for (int tile = g; tile < work_tiles; tile += G) {
                 const int e = tile * kTile + threadIdx.x;
-                // sums in list order; operands fetched four at a time
+                // sums in list order; operands fetched sixteen at a time (a
+                // coarse diagonal gathers ~50 fine values), codes first, then
+                // all values, so a chunk costs two round trips
                 if (e < nc) {
                     double s = 0.0;
                     const int q1 = L.gd_start[e + 1];
-                    for (int q = L.gd_start[e]; q < q1; q += 4) {
-                        double v[4];
+                    for (int q = L.gd_start[e]; q < q1; q += 16) {
+                        int code[16];
+                        double v[16];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int code = q + u < q1 ? L.gd[q + u] : 0;
-                            v[u] = q + u < q1 ? (code < 0 ? I.diag64[j][-code - 1] : I.off64[j][code]) : 0.0;
-                        }
+                        for (int u = 0; u < 16; ++u) code[u] = q + u < q1 ? L.gd[q + u] : 0;
 #pragma unroll
-                        for (int u = 0; u < 4; ++u)
+                        for (int u = 0; u < 16; ++u)
+                            v[u] = q + u < q1 ? (code[u] < 0 ? I.diag64[j][-code[u] - 1] : I.off64[j][code[u]]) : 0.0;
+#pragma unroll
+                        for (int u = 0; u < 16; ++u)
                             if (q + u < q1) s += v[u];
                     }
                     I.diag64[j + 1][e] = s;
@@ -752,6 +912,16 @@ __device__ __forceinline__ void set_matrix(const Instance& I, const Launch& P, i
             coarse_inverse(I, I.tri_lds ? dynamic_lds() : I.tri, lds);
     }
     group_sync(timeouts);
+}
+
+// A pointer every lane holds, rebuilt in scalar registers: descriptor reads
+// through it are scalar loads (a function argument arrives in vector
+// registers otherwise).
+template <class T> __device__ inline const T* uniform_ptr(const T* p) {
+    const uint64_t bits = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits >> 32));
+    return reinterpret_cast<const T*>((static_cast<uint64_t>(hi) << 32) | lo);
 }
 
 // set_matrix out of line (its register-blocked inverse would otherwise add
@@ -800,34 +970,44 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
         parity ^= 1;
         return part;
     };
-    const auto spmv_row = [&](const VT* in, int i) {
-        VT acc = diag[i] * in[i];
-        ell_row(
-            L0.col, off, n, W, i, [&](int, int c) { return in[c]; }, [&](VT o, VT v) { acc += o * v; });
-        return acc;
+    // acc[r] = (A in)_i[r] for R rows at once
+    const auto spmv_rows = [&]<int R>(const VT* in, const int (&i)[R], VT (&acc)[R]) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = diag[i[r]] * in[i[r]];
+        ell_rows<R>(
+            L0.col, off, n, W, i, [&](int, int, int c) { return in[c]; },
+            [&](int r, VT o, VT v) { acc[r] += o * v; });
     };
 
     // b (assembled or given), x = 0, r = r0 = b.
     double* part = next_partials();
     {
         double* x = I.x[xi];
-        for (int tile = g; tile < tiles; tile += G) {
-            const int i = tile * kTile + threadIdx.x;
-            double q[2] = {0.0, 0.0};
-            if (i < n) {
-                const double bi = (P.op & kOpHeatRhs)
-                                      ? P.capacity_area[i] * t_old[i] + P.dt * I.area[i] * forcing[i]
-                                      : P.in[i];
-                b[i] = bi;
-                x[i] = 0.0;
-                const VT ri = static_cast<VT>(bi);
-                r[i] = ri;
-                r0[i] = ri;
-                q[0] = bi * bi;
-                q[1] = static_cast<double>(ri) * static_cast<double>(ri);
+        tile_loop(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+            int i[R];
+            bool ok[R];
+            double bi[R], q[R][2];
+#pragma unroll
+            for (int k = 0; k < R; ++k) i[k] = tile_row(tl[k], n, ok[k]);
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                bi[k] = (P.op & kOpHeatRhs) ? P.capacity_area[i[k]] * t_old[i[k]] + P.dt * I.area[i[k]] * forcing[i[k]]
+                                            : P.in[i[k]];
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                q[k][0] = q[k][1] = 0.0;
+                if (ok[k]) {
+                    b[i[k]] = bi[k];
+                    x[i[k]] = 0.0;
+                    const VT ri = static_cast<VT>(bi[k]);
+                    r[i[k]] = ri;
+                    r0[i[k]] = ri;
+                    q[k][0] = bi[k] * bi[k];
+                    q[k][1] = static_cast<double>(ri) * static_cast<double>(ri);
+                }
             }
-            store_partials<2>(part, tile, q, lds);
-        }
+            store_partials_n<2, R>(part, tl, q, lds);
+        });
     }
     group_sync(timeouts);
     double tot2[2];
@@ -894,16 +1074,26 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 first_iteration = false;
                 // v = A ph, (r0, v)
                 part = next_partials();
-                for (int tile = g; tile < tiles; tile += G) {
-                    const int i = tile * kTile + threadIdx.x;
-                    double q[1] = {0.0};
-                    if (i < n) {
-                        const VT acc = spmv_row(ph, i);
-                        v[i] = acc;
-                        q[0] = static_cast<double>(r0[i]) * static_cast<double>(acc);
+                tile_loop<sizeof(VT) == 4>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+                    int i[R];
+                    bool ok[R];
+                    VT acc[R], rz[R];
+                    double q[R][1];
+#pragma unroll
+                    for (int k = 0; k < R; ++k) i[k] = tile_row(tl[k], n, ok[k]);
+#pragma unroll
+                    for (int k = 0; k < R; ++k) rz[k] = r0[i[k]];
+                    spmv_rows(ph, i, acc);
+#pragma unroll
+                    for (int k = 0; k < R; ++k) {
+                        q[k][0] = 0.0;
+                        if (ok[k]) {
+                            v[i[k]] = acc[k];
+                            q[k][0] = static_cast<double>(rz[k]) * static_cast<double>(acc[k]);
+                        }
                     }
-                    store_partials<1>(part, tile, q, lds);
-                }
+                    store_partials_n<1, R>(part, tl, q, lds);
+                });
                 group_sync(timeouts);
                 double r0v[1];
                 group_totals<1>(part, tiles, r0v, lds);
@@ -952,18 +1142,28 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 ++v_cycles;
                 // t = A sh, (t, t), (t, s)
                 part = next_partials();
-                for (int tile = g; tile < tiles; tile += G) {
-                    const int i = tile * kTile + threadIdx.x;
-                    double q[2] = {0.0, 0.0};
-                    if (i < n) {
-                        const VT acc = spmv_row(sh, i);
-                        t[i] = acc;
-                        const double td = static_cast<double>(acc);
-                        q[0] = td * td;
-                        q[1] = td * static_cast<double>(s[i]);
+                tile_loop<sizeof(VT) == 4>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+                    int i[R];
+                    bool ok[R];
+                    VT acc[R], sv[R];
+                    double q[R][2];
+#pragma unroll
+                    for (int k = 0; k < R; ++k) i[k] = tile_row(tl[k], n, ok[k]);
+#pragma unroll
+                    for (int k = 0; k < R; ++k) sv[k] = s[i[k]];
+                    spmv_rows(sh, i, acc);
+#pragma unroll
+                    for (int k = 0; k < R; ++k) {
+                        q[k][0] = q[k][1] = 0.0;
+                        if (ok[k]) {
+                            t[i[k]] = acc[k];
+                            const double td = static_cast<double>(acc[k]);
+                            q[k][0] = td * td;
+                            q[k][1] = td * static_cast<double>(sv[k]);
+                        }
                     }
-                    store_partials<2>(part, tile, q, lds);
-                }
+                    store_partials_n<2, R>(part, tl, q, lds);
+                });
                 group_sync(timeouts);
                 double tt[2];
                 group_totals<2>(part, tiles, tt, lds);
@@ -972,19 +1172,35 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 part = next_partials();
                 {
                     double* x = I.x[xi];
-                    for (int tile = g; tile < tiles; tile += G) {
-                        const int i = tile * kTile + threadIdx.x;
-                        double q[2] = {0.0, 0.0};
-                        if (i < n) {
-                            x[i] += alpha * static_cast<double>(ph[i]) + omega * static_cast<double>(sh[i]);
-                            const double ri = static_cast<double>(s[i]) - omega * static_cast<double>(t[i]);
-                            const VT rv = static_cast<VT>(ri);
-                            r[i] = rv;
-                            q[0] = static_cast<double>(rv) * static_cast<double>(rv);
-                            q[1] = static_cast<double>(r0[i]) * static_cast<double>(rv);
+                    tile_loop(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+                        int i[R];
+                        bool ok[R];
+                        double xv[R], pv[R], shv[R], sv[R], tv[R], rz[R], q[R][2];
+#pragma unroll
+                        for (int k = 0; k < R; ++k) i[k] = tile_row(tl[k], n, ok[k]);
+#pragma unroll
+                        for (int k = 0; k < R; ++k) {
+                            xv[k] = x[i[k]];
+                            pv[k] = static_cast<double>(ph[i[k]]);
+                            shv[k] = static_cast<double>(sh[i[k]]);
+                            sv[k] = static_cast<double>(s[i[k]]);
+                            tv[k] = static_cast<double>(t[i[k]]);
+                            rz[k] = static_cast<double>(r0[i[k]]);
                         }
-                        store_partials<2>(part, tile, q, lds);
-                    }
+#pragma unroll
+                        for (int k = 0; k < R; ++k) {
+                            q[k][0] = q[k][1] = 0.0;
+                            if (ok[k]) {
+                                x[i[k]] = xv[k] + (alpha * pv[k] + omega * shv[k]);
+                                const double ri = sv[k] - omega * tv[k];
+                                const VT rv = static_cast<VT>(ri);
+                                r[i[k]] = rv;
+                                q[k][0] = static_cast<double>(rv) * static_cast<double>(rv);
+                                q[k][1] = rz[k] * static_cast<double>(rv);
+                            }
+                        }
+                        store_partials_n<2, R>(part, tl, q, lds);
+                    });
                 }
                 group_sync(timeouts);
                 double rr[2];
@@ -1011,6 +1227,148 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     out.t_begin = t_begin;
     out.t_end = static_cast<uint64_t>(wall_clock64());
     return out;
+}
+
+// ------------------------------------------------------------ PCG
+
+// Preconditioned conjugate gradients on the system's FP64 matrix, the
+// reference's Newton-system solve (heat_transport.cpp:328-429): x = 0,
+// r = b (held rows already zeroed by the caller; the matrix has identity
+// rows there), z = M r, d = z, rho = (r, z), target = tol ||b||; then while
+// it < max, ||r|| > target and rho > 0:
+//   q = A d, alpha = rho / (d, q), x += alpha d, r -= alpha q,
+//   z = M r, rho' = (r, z), beta = rho' / rho, d = z + beta d.
+// M is the V-cycle (symmetric: the same Jacobi weights before and after the
+// coarse correction), restricted to the free rows when rows are held (held
+// rows are decoupled identity rows with b = 0: their direction is 0, as the
+// reference's Gauss-Seidel sweep leaves it). d = z + beta d is evaluated inside the next SpMV
+// (double-buffered d), and (r, z) inside the V-cycle's last sweep, so an
+// iteration is six barrier-separated phases. x goes to P.out.
+__device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts,
+                                           int& parity) {
+    const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
+    const Values<double> V{I};
+    const LevelView& L0 = I.lv[0];
+    const int n = I.n, W = L0.width, tiles = I.tiles;
+    double* r = static_cast<double*>(I.r);
+    double* z = static_cast<double*>(I.ph);
+    double* q = static_cast<double*>(I.v);
+    double* kept = static_cast<double*>(I.s);
+    double* x = I.x[0];
+    double* d[2] = {static_cast<double*>(I.p[0]), static_cast<double*>(I.p[1])};
+    const double* diag = I.diag64[0];
+    const double* off = I.off64[0];
+    const auto next_partials = [&]() {
+        double* part = I.partial + parity * I.partial_slots * kPartials;
+        parity ^= 1;
+        return part;
+    };
+    // r = b, x = 0, (r, r)
+    double* part = next_partials();
+    for (int tile = g; tile < tiles; tile += G) {
+        const int i = tile * kTile + threadIdx.x;
+        double v[1] = {0.0};
+        if (i < n) {
+            const double bi = P.in[i];
+            r[i] = bi;
+            x[i] = 0.0;
+            v[0] = bi * bi;
+        }
+        store_partials<1>(part, tile, v, lds);
+    }
+    group_sync(timeouts);
+    double tot[1];
+    group_totals<1>(part, tiles, tot, lds);
+    const double b_norm = sqrt(tot[0]);
+    const double target = P.tolerance * b_norm;
+    double r_norm = b_norm;
+    // z = M r with (r, z)
+    const auto precondition = [&]() {
+        const Source<double> src{r, nullptr, nullptr, 0.0, 0.0, 0};
+        vcycle_first<double>(I, V, src, kept, nullptr, g, G, lds, timeouts);
+        double* dot = next_partials();
+        vcycle_rest<double>(I, V, r, z, g, G, lds, timeouts, dot, r);
+        double rz[1];
+        group_totals<1>(dot, tiles, rz, lds);   // one partial per level-0 tile
+        return rz[0];
+    };
+    double rho = b_norm > 0.0 ? precondition() : 0.0;
+    int iterations = 0, di = 0, v_cycles = b_norm > 0.0 ? 1 : 0;
+    double beta = 0.0;
+    bool first = true;
+    while (iterations < P.max_iterations && r_norm > target && rho > 0.0) {
+        // d = z + beta d (d = z at first), q = A d, (d, q)
+        const Source<double> src = first ? Source<double>{z, nullptr, nullptr, 0.0, 0.0, 0}
+                                         : Source<double>{z, d[di], d[di], beta, 0.0, 1};
+        double* dn = d[di ^ 1];
+        part = next_partials();
+        for (int tile = g; tile < tiles; tile += G) {
+            const int i = tile * kTile + threadIdx.x;
+            double v[1] = {0.0};
+            if (i < n) {
+                // held rows keep d = 0, so x, r and q stay exactly 0 there
+                const double di_value = P.hold != nullptr && P.hold[i] != 0 ? 0.0 : src.at(i);
+                double acc = diag[i] * di_value;
+                ell_row(
+                    L0.col, off, n, W, i, [&](int, int c) { return src.at(c); },
+                    [&](double o, double w) { acc += o * w; });
+                dn[i] = di_value;
+                q[i] = acc;
+                v[0] = di_value * acc;
+            }
+            store_partials<1>(part, tile, v, lds);
+        }
+        group_sync(timeouts);
+        group_totals<1>(part, tiles, tot, lds);
+        di ^= 1;
+        first = false;
+        const double alpha = rho / tot[0];
+        // x += alpha d, r -= alpha q, (r, r)
+        part = next_partials();
+        for (int tile = g; tile < tiles; tile += G) {
+            const int i = tile * kTile + threadIdx.x;
+            double v[1] = {0.0};
+            if (i < n) {
+                x[i] += alpha * dn[i];
+                r[i] -= alpha * q[i];
+                v[0] = r[i] * r[i];
+            }
+            store_partials<1>(part, tile, v, lds);
+        }
+        group_sync(timeouts);
+        group_totals<1>(part, tiles, tot, lds);
+        r_norm = sqrt(tot[0]);
+        ++iterations;
+        if (!(iterations < P.max_iterations && r_norm > target)) break;
+        const double next_rho = precondition();
+        ++v_cycles;
+        beta = next_rho / rho;
+        rho = next_rho;
+    }
+    for (int tile = g; tile < tiles; tile += G) {
+        const int i = tile * kTile + threadIdx.x;
+        if (i < n) P.out[i] = x[i];
+    }
+    SolveResult out{};
+    out.iterations = iterations;
+    out.status = r_norm <= target ? 0 : kNotConverged;
+    out.relative_residual = b_norm > 0.0 ? r_norm / b_norm : 0.0;
+    out.b_norm = b_norm;
+    out.v_cycles = v_cycles;
+    out.t_begin = t_begin;
+    out.t_end = static_cast<uint64_t>(wall_clock64());
+    return out;
+}
+
+// PCG out of line (not the hot path; its registers stay out of the solver's).
+__device__ __noinline__ SolveResult pcg_call(const Instance* instance, const Launch& P, int g, int G, Lds& lds,
+                                             SyncState& st) {
+    const Instance& I = *uniform_ptr(instance);
+    int timeouts = st.timeouts, parity = st.parity;
+    const SolveResult res = pcg(I, P, __builtin_amdgcn_readfirstlane(g), __builtin_amdgcn_readfirstlane(G), lds,
+                                timeouts, parity);
+    st = SyncState{timeouts, parity};
+    return res;
 }
 
 // ------------------------------------------------------------ the kernel
@@ -1114,6 +1472,22 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
         else
             run_solve<float>(I, P, g, G, lds, timeouts, parity);
     }
+    if (P.op & kOpPcg) {
+        SyncState st{timeouts, parity};
+        const SolveResult res = pcg_call(&I, P, g, G, lds, st);
+        timeouts = st.timeouts;
+        parity = st.parity;
+        if (g == 0 && threadIdx.x == 0) {
+            grp.result->iterations = res.iterations;
+            grp.result->restarts = 0;
+            grp.result->status = res.status;
+            grp.result->relative_residual = res.relative_residual;
+            grp.result->b_norm = res.b_norm;
+            grp.result->v_cycles = res.v_cycles;
+            grp.result->t_begin = res.t_begin;
+            grp.result->t_end = res.t_end;
+        }
+    }
     if (timeouts != 0 && threadIdx.x == 0) atomicAdd(&grp.result->barrier_timeouts, timeouts);
 }
 
@@ -1149,16 +1523,6 @@ __device__ int64_t claim_request(const ServeParams& S, const ServeQueue& Q, long
         ++idle;
         head = __hip_atomic_load(Q.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-}
-
-// A pointer every lane holds, rebuilt in scalar registers: descriptor reads
-// through it are scalar loads (a function argument arrives in vector
-// registers otherwise).
-template <class T> __device__ inline const T* uniform_ptr(const T* p) {
-    const uint64_t bits = reinterpret_cast<uint64_t>(p);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits));
-    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits >> 32));
-    return reinterpret_cast<const T*>((static_cast<uint64_t>(hi) << 32) | lo);
 }
 
 // One request on the slot's group, out of line so that the server loop's

@@ -7,6 +7,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace psim {
@@ -47,6 +48,7 @@ void check_where(int where);
 
 struct psim_graph {
     psim_ctx* ctx = nullptr;
+    psim::HostGraph graph;         // the caller's graph (area, CSR, weights) on the host
     psim::HostHierarchy host;
     int32_t n = 0, nnz = 0, depth = 0;
     psim::Arena arena;
@@ -54,7 +56,13 @@ struct psim_graph {
     double* weight = nullptr;
     int32_t* csr_col = nullptr;   // level-0 CSR columns (the diffusion sweep reads CSR order)
     psim::dev::LevelView views[psim::dev::kMaxLevels];
-    ~psim_graph() { arena.release(); }
+    // the Newton-system solver of psim_implicit_transport, made on first use
+    psim_system* newton = nullptr;
+    std::mutex newton_mutex;
+    ~psim_graph() {
+        if (newton != nullptr) psim_system_destroy(newton);
+        arena.release();
+    }
 };
 
 struct psim_system {

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <numbers>
 #include <stdexcept>
 #include <string>
@@ -196,6 +197,42 @@ void device_part() {
             gross += graph.area_m2[c] * std::abs(source[c]);
         }
         EXPECT(std::abs(net) <= 1e-12 * gross);
+    }
+    // solve_implicit_transport with a linear response (the reference's
+    // check_linear, test_heat_transport.cpp:73-107).
+    {
+        const auto graph = planetsim::mesh_transport_graph(mesh);
+        std::vector<double> base(n), slopes(n);
+        for (std::size_t c = 0; c < n; ++c) {
+            base[c] = 250.0 + 50.0 * uniform(61 * 1000003ULL + c);
+            slopes[c] = c % 5U == 0U ? 0.0 : 0.01 + 0.2 * uniform(67 * 1000003ULL + c);
+        }
+        const planetsim::TransportResponse response = [&](const planetsim::Field2D<double>& source,
+                                                          planetsim::Field2D<double>& mean,
+                                                          planetsim::Field2D<double>& slope) {
+            for (std::size_t c = 0; c < n; ++c) {
+                mean[c] = base[c] + slopes[c] * source[c];
+                slope[c] = slopes[c];
+            }
+        };
+        const double conductance = 0.5 * 6'371'000.0 * 6'371'000.0;
+        const planetsim::Field2D<double> floor(n, -std::numeric_limits<double>::infinity());
+        const auto result = planetsim::solve_implicit_transport(graph, conductance, floor, response, {}, 4U);
+        EXPECT(result.newton_iterations <= 3);
+        EXPECT(result.consistency_residual_W_m2 <= 1e-6);
+        EXPECT(std::abs(result.sum_W) <= 1e-13 * result.absolute_sum_W);
+        EXPECT(result.dissipation_W_K <= 0.0);
+        EXPECT(result.source_W_m2.size() == n && result.mean_K.size() == n);
+        EXPECT(throws_with<std::invalid_argument>(
+            [&] { (void)planetsim::solve_implicit_transport(graph, 0.0, floor, response); },
+            "transport conductance must be finite and positive"));
+        const planetsim::TransportResponse failing = [](const planetsim::Field2D<double>&,
+                                                        planetsim::Field2D<double>&, planetsim::Field2D<double>&) {
+            throw std::runtime_error("column did not converge");
+        };
+        EXPECT(throws_with<std::runtime_error>(
+            [&] { (void)planetsim::solve_implicit_transport(graph, conductance, floor, failing); },
+            "column did not converge"));
     }
     // HeatStepBatch: solvers of different graphs and precisions in one
     // launch give exactly what each solver's own step() gives.

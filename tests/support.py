@@ -66,6 +66,84 @@ def _declare(lib, prefix):
     fn("mg_precondition", C.c_int, _vp, _dp, _dp, _u64, _u64)
     fn("bicgstab", C.c_int, _vp, _vp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _u64,
        C.POINTER(C.c_int))
+    fn("newton_system", C.c_int, _vp, C.c_double, _dp, _dp, C.c_double, C.c_int, _u64, _dp,
+       C.POINTER(C.c_int))
+    fn("solve_implicit_transport", C.c_int, _vp, C.c_double, _dp, RESPONSE, _vp,
+       C.POINTER(TransportSettings), _u64, _vp, _dp, _dp, C.POINTER(TransportResult))
+
+
+# solve_implicit_transport's C form (both libraries): a response callback
+# response(user, source, mean_out, slope_out, n), settings and result.
+RESPONSE = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                       C.POINTER(C.c_double), C.c_uint64)
+
+
+class TransportSettings(C.Structure):
+    """ImplicitTransportSettings (heat_transport.hpp:82-106) defaults."""
+    _fields_ = [("max_newton_iterations", C.c_int), ("max_line_search_halvings", C.c_int),
+                ("rounding_floor", C.c_double), ("max_non_monotone_steps", C.c_int), ("pad", C.c_int),
+                ("max_non_monotone_growth", C.c_double), ("newton_tolerance", C.c_double),
+                ("cg_relative_tolerance", C.c_double), ("max_cg_iterations", C.c_int), ("pad2", C.c_int)]
+
+    def __init__(self, **kw):
+        defaults = dict(max_newton_iterations=30, max_line_search_halvings=12, rounding_floor=1e-3,
+                        max_non_monotone_steps=4, max_non_monotone_growth=100.0, newton_tolerance=1e-6,
+                        cg_relative_tolerance=1e-6, max_cg_iterations=2000)
+        defaults.update(kw)
+        super().__init__(**defaults)
+
+
+class TransportResult(C.Structure):
+    _fields_ = [("newton_iterations", C.c_int), ("cg_iterations", C.c_int), ("consistency_residual", C.c_double),
+                ("sum_W", C.c_double), ("absolute_sum_W", C.c_double), ("dissipation_W_K", C.c_double)]
+
+
+def response_callback(fn):
+    """Wrap fn(source ndarray) -> (mean, slope) as a C response callback."""
+
+    def call(_user, source, mean, slope, n):
+        h = np.ctypeslib.as_array(source, shape=(n,))
+        m, s = fn(h.copy())
+        np.ctypeslib.as_array(mean, shape=(n,))[:] = m
+        np.ctypeslib.as_array(slope, shape=(n,))[:] = s
+
+    return RESPONSE(call)
+
+
+def implicit_transport(lib, mesh_handle, n, conductance, floor, fn, settings=None, latent=None):
+    """One library's solve_implicit_transport: (source, mean, result)."""
+    cb = response_callback(fn)
+    src, mean = np.zeros(n), np.zeros(n)
+    res = TransportResult()
+    lat = None if latent is None else np.ascontiguousarray(latent, np.float64)
+    lib.check(lib.solve_implicit_transport(mesh_handle, conductance, np.ascontiguousarray(floor, np.float64), cb,
+                                           None, C.byref(settings or TransportSettings()), 1,
+                                           None if lat is None else lat.ctypes.data, src, mean, C.byref(res)))
+    return src, mean, res
+
+
+def quartic_response(mesh_area_n, seed_values, a, radiative=0.75 * 0.97 * 5.670374419e-8):
+    """A tile-like response (test_heat_transport.cpp:30-59 in spirit): each
+    cell solves a x + r x^4 = b + h by Newton from 250 K (fixed iteration
+    count, so every caller gets the same bits); cells with index % 7 == 0
+    are held at 273.15 K (slope 0) whenever they would warm past it."""
+    target = 200.0 + 120.0 * seed_values
+    b = a * target + radiative * target ** 4
+
+    def fn(h):
+        x = np.full_like(h, 250.0)
+        rhs = b + h
+        for _ in range(40):
+            f = a * x + radiative * x ** 4 - rhs
+            x = x - f / (a + 4.0 * radiative * x ** 3)
+        slope = 1.0 / (a + 4.0 * radiative * x ** 3)
+        held = (np.arange(h.size) % 7 == 0) & (x > 273.15)
+        mean = np.where(held, 273.15, x)
+        slope = np.where(held, 0.0, slope)
+        return mean, slope
+
+    floor = a * 100.0 + radiative * 100.0 ** 4 - b
+    return fn, floor
 
 
 class Lib:

@@ -1,6 +1,7 @@
 """The CPU oracle pinned against the reference: its golden vectors and
 published values, fixtures made from the compiled reference sources
 (tests/golden/make_golden.py), closed forms and a dense direct solve."""
+import ctypes as C
 import json
 import math
 import os
@@ -244,3 +245,118 @@ def test_oracle_equals_compiled_reference_directly():
             out.append((e, no, lap))
         for u, v in zip(*out):
             assert np.array_equal(u.view(np.uint64), v.view(np.uint64))
+
+
+# ---------------------------------------------------------------- Newton transport
+# solve_newton_system (heat_transport.cpp:328-429) and solve_implicit_transport
+# (:595-772): the oracle's restatement against the compiled reference, bit for
+# bit, plus the properties the reference's own tests assert
+# (test_heat_transport.cpp:73-107).
+
+NEWTON_K = 0.5 * S.RADIUS ** 2
+
+
+def newton_case(level, seed=3):
+    om = S.mesh("oracle", level)
+    rng = np.random.default_rng(seed + level)
+    slope = np.where(np.arange(om.n) % 5 == 0, 0.0, 0.01 + 0.2 * rng.random(om.n))
+    rhs = rng.standard_normal(om.n) * om.g_area
+    return om, slope, rhs
+
+
+def newton_solve(lib, mesh, slope, rhs, tol, max_it=2000):
+    x = np.zeros(mesh.n)
+    it = C.c_int()
+    lib.check(lib.newton_system(mesh.handle, NEWTON_K, slope, rhs, tol, max_it, 1, x, C.byref(it)))
+    return x, it.value
+
+
+@pytest.mark.skipif(not S.have_ref(), reason="reference shim not built")
+@pytest.mark.parametrize("level", [3, 4])
+def test_newton_system_matches_reference(level):
+    om, slope, rhs = newton_case(level)
+    rm = S.mesh("ref", level)
+    for tol in (1e-6, 1e-12):
+        xo, io = newton_solve(S.oracle(), om, slope, rhs, tol)
+        xr, ir = newton_solve(S.ref(), rm, slope, rhs, tol)
+        assert io == ir and np.array_equal(xo.view(np.uint64), xr.view(np.uint64)), (tol, io, ir)
+
+
+@pytest.mark.parametrize("level", [3, 4])
+def test_newton_system_solves_the_held_system(level):
+    """The PCG answer solves the Newton matrix of heat_transport.cpp:350-363:
+    held rows stay 0 and the true residual is at the CG tolerance."""
+    om, slope, rhs = newton_case(level)
+    x, it = newton_solve(S.oracle(), om, slope, rhs, 1e-12)
+    free = slope > 0.0
+    assert np.all(x[~free] == 0.0)
+    diag = np.where(free, om.g_area / np.where(free, slope, 1.0) + NEWTON_K * om.weight_sum, 1.0)
+    nb = om.g_neighbour.astype(np.int64)
+    rows = np.repeat(np.arange(om.n), np.diff(om.offset.astype(np.int64)))
+    off = np.where(free[rows] & free[nb], -NEWTON_K * om.weight, 0.0)
+    ax = diag * x + np.bincount(rows, weights=off * x[nb], minlength=om.n)
+    b = np.where(free, rhs, 0.0)
+    assert np.linalg.norm(b - ax) <= 1e-10 * np.linalg.norm(b)
+    assert 1 <= it < 100
+
+
+def linear_transport_case(level, seed=9):
+    om = S.mesh("oracle", level)
+    rng = np.random.default_rng(seed + level)
+    base = 250.0 + 50.0 * rng.random(om.n)
+    slopes = np.where(np.arange(om.n) % 5 == 0, 0.0, 0.01 + 0.2 * rng.random(om.n))
+
+    def fn(h):
+        return base + slopes * h, slopes.copy()
+
+    return om, fn, np.full(om.n, -np.inf)
+
+
+def same_transport(a, b):
+    (sa, ma, ra), (sb, mb, rb) = a, b
+    return (np.array_equal(sa.view(np.uint64), sb.view(np.uint64)) and
+            np.array_equal(ma.view(np.uint64), mb.view(np.uint64)) and
+            all(getattr(ra, f) == getattr(rb, f) for f, _ in S.TransportResult._fields_))
+
+
+@pytest.mark.skipif(not S.have_ref(), reason="reference shim not built")
+@pytest.mark.parametrize("level", [3, 4])
+def test_implicit_transport_linear_matches_reference(level):
+    om, fn, floor = linear_transport_case(level)
+    rm = S.mesh("ref", level)
+    got = S.implicit_transport(S.oracle(), om.handle, om.n, NEWTON_K, floor, fn)
+    ref = S.implicit_transport(S.ref(), rm.handle, rm.n, NEWTON_K, floor, fn)
+    assert same_transport(got, ref)
+    src, mean, res = got
+    # test_heat_transport.cpp:99-106
+    assert res.newton_iterations <= 3
+    assert res.consistency_residual <= 1e-6
+    assert abs(res.sum_W) <= 1e-13 * res.absolute_sum_W
+    assert res.dissipation_W_K <= 0.0
+
+
+@pytest.mark.skipif(not S.have_ref(), reason="reference shim not built")
+@pytest.mark.parametrize("a", [0.04 + 0.37, 1.0e5 / 600.0])
+def test_implicit_transport_quartic_matches_reference(a):
+    level = 3
+    om = S.mesh("oracle", level)
+    rm = S.mesh("ref", level)
+    u = np.random.default_rng(21).random(om.n)
+    fn, floor = S.quartic_response(om.n, u, a)
+    latent = 5.0 * np.sin(np.arange(om.n))
+    for lat in (None, latent):
+        got = S.implicit_transport(S.oracle(), om.handle, om.n, NEWTON_K, floor, fn, latent=lat)
+        ref = S.implicit_transport(S.ref(), rm.handle, rm.n, NEWTON_K, floor, fn, latent=lat)
+        assert same_transport(got, ref)
+        src, mean, res = got
+        assert np.all(mean[::7] <= 273.15)
+        assert res.consistency_residual <= 1e-6
+        # the diffusion conserves energy; the returned source is diffusion - latent
+        expected = 0.0 if lat is None else -float(np.sum(om.g_area * lat))
+        assert abs(res.sum_W - expected) <= 1e-12 * res.absolute_sum_W
+
+
+def test_implicit_transport_rejects_bad_conductance():
+    om, fn, floor = linear_transport_case(3)
+    with pytest.raises(RuntimeError, match="transport conductance must be finite and positive"):
+        S.implicit_transport(S.oracle(), om.handle, om.n, 0.0, floor, fn)
