@@ -9,8 +9,12 @@
 #include "planet_mesh.hpp"
 
 #include <cstddef>
+#include <cstdint>
 #include <functional>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <utility>
 #include <span>
 #include <stdexcept>
 #include <exception>
@@ -86,6 +90,120 @@ struct TransportGraph {
     graph.blocks.assign(mesh.blocks().begin(), mesh.blocks().end());
     return graph;
 }
+
+namespace detail {
+// One agglomeration per subdivision and radius for the process, as the
+// reference caches it (heat_transport.cpp:517-530); built by libpsim's
+// psim_agglomeration (build_agglomeration, :467-512, bit-identical).
+struct Agglomeration {
+    TransportGraph graph;
+    std::vector<std::size_t> group_of_cell;
+    std::unique_ptr<PlanetMesh> coarse_mesh;   // null at subdivision 0: the mesh itself
+};
+
+inline const Agglomeration& cached_agglomeration(const PlanetMesh& mesh) {
+    static std::mutex mutex;
+    static std::map<std::pair<std::uint32_t, double>, std::unique_ptr<Agglomeration>> cache;
+    const std::lock_guard lock(mutex);
+    auto& entry = cache[{mesh.subdivision(), mesh.radius_m()}];
+    if (!entry || entry->group_of_cell.size() != mesh.cell_count()) {
+        psim_mesh* raw = nullptr;
+        gpu::check(psim_mesh_create(mesh.subdivision(), mesh.radius_m(), &raw));
+        const std::unique_ptr<psim_mesh, void (*)(psim_mesh*)> fine(raw, psim_mesh_destroy);
+        psim_agglomeration* handle = nullptr;
+        gpu::check(psim_agglomeration_create(raw, &handle));
+        const std::unique_ptr<psim_agglomeration, void (*)(psim_agglomeration*)> owned(handle,
+                                                                                      psim_agglomeration_destroy);
+        std::uint64_t cells = 0, groups = 0;
+        gpu::check(psim_agglomeration_counts(handle, &cells, &groups));
+        std::vector<std::uint32_t> group(cells);
+        std::vector<double> area(groups);
+        gpu::check(psim_agglomeration_groups(handle, group.data(), area.data()));
+        auto next = std::make_unique<Agglomeration>();
+        if (mesh.subdivision() == 0U) {
+            next->graph = mesh_transport_graph(mesh);
+        } else {
+            next->coarse_mesh = std::make_unique<PlanetMesh>(
+                mesh_from_handle(psim_agglomeration_mesh(handle), mesh.subdivision() - 1U, mesh.radius_m()));
+            next->graph = mesh_transport_graph(*next->coarse_mesh);
+        }
+        next->graph.area_m2 = std::move(area);
+        next->group_of_cell.assign(group.begin(), group.end());
+        entry = std::move(next);
+    }
+    return *entry;
+}
+}  // namespace detail
+
+// heat_transport.hpp:45-50: the transport graph of the mesh's groups (each
+// cell joins the nearest cell of the mesh one level coarser; a group's area
+// is its cells'), and that coarser mesh (the mesh itself at subdivision 0).
+// References stay valid for the process, as in the reference.
+[[nodiscard]] inline const TransportGraph& agglomerated_transport_graph(
+    const PlanetMesh& mesh, const std::vector<std::size_t>*& group_of_cell) {
+    const detail::Agglomeration& entry = detail::cached_agglomeration(mesh);
+    group_of_cell = &entry.group_of_cell;
+    return entry.graph;
+}
+
+[[nodiscard]] inline const PlanetMesh& agglomerated_mesh(const PlanetMesh& mesh) {
+    const detail::Agglomeration& entry = detail::cached_agglomeration(mesh);
+    return entry.coarse_mesh ? *entry.coarse_mesh : mesh;
+}
+
+// The cell-to-group maps of a transport on the agglomerated graph, on the
+// device. The reference writes them as loops around its response
+// (surface_energy.cpp:1226-1247,1285-1300,1306-1324); this class has no
+// counterpart there. Several fields may be given one after another.
+class GroupMaps {
+  public:
+    explicit GroupMaps(const PlanetMesh& mesh) {
+        const std::vector<std::size_t>* map = nullptr;
+        const TransportGraph& graph = agglomerated_transport_graph(mesh, map);
+        const std::vector<std::uint32_t> group(map->begin(), map->end());
+        std::vector<double> area(mesh.cell_count());
+        for (std::size_t c = 0; c < area.size(); ++c) area[c] = mesh.cells()[c].area_m2;
+        psim_groups* handle = nullptr;
+        gpu::check(psim_groups_create(gpu::context(), area.size(), graph.size(), group.data(), area.data(),
+                                      graph.area_m2.data(), &handle));
+        handle_.reset(handle);
+        cells_ = area.size();
+        groups_ = graph.size();
+    }
+
+    [[nodiscard]] std::size_t cells() const noexcept { return cells_; }
+    [[nodiscard]] std::size_t groups() const noexcept { return groups_; }
+
+    // A group's area-weighted mean of its cells (surface_energy.cpp:1285-1300).
+    void mean(std::span<const double> cell_values, std::span<double> group_values) const {
+        gpu::check(psim_groups_mean(handle_.get(), fields(cell_values.size(), cells_, group_values.size(), groups_),
+                                    cell_values.data(), group_values.data(), PSIM_HOST));
+    }
+    // A group's highest cell value, from -inf (surface_energy.cpp:1306,1323-1324).
+    void max(std::span<const double> cell_values, std::span<double> group_values) const {
+        gpu::check(psim_groups_max(handle_.get(), fields(cell_values.size(), cells_, group_values.size(), groups_),
+                                   cell_values.data(), group_values.data(), PSIM_HOST));
+    }
+    // Each cell takes its group's value (surface_energy.cpp:1247).
+    void broadcast(std::span<const double> group_values, std::span<double> cell_values) const {
+        gpu::check(psim_groups_broadcast(handle_.get(),
+                                         fields(group_values.size(), groups_, cell_values.size(), cells_),
+                                         group_values.data(), cell_values.data(), PSIM_HOST));
+    }
+
+  private:
+    static std::uint32_t fields(std::size_t in, std::size_t per_in, std::size_t out, std::size_t per_out) {
+        const std::size_t count = in / per_in;
+        if (count == 0U || in != count * per_in || out != count * per_out || count > 65535U)
+            throw std::invalid_argument("field size does not match the groups");
+        return static_cast<std::uint32_t>(count);
+    }
+    struct Free {
+        void operator()(psim_groups* g) const noexcept { psim_groups_destroy(g); }
+    };
+    std::unique_ptr<psim_groups, Free> handle_;
+    std::size_t cells_ = 0, groups_ = 0;
+};
 
 namespace detail {
 struct SystemFree {

@@ -92,6 +92,10 @@ inline constexpr std::size_t deterministic_cell_block_size = 256U;
 
 class PlanetMesh;
 [[nodiscard]] PlanetMesh make_icosphere(std::uint32_t subdivision, double radius_m);
+namespace detail {
+// A PlanetMesh copied out of a libpsim mesh handle.
+[[nodiscard]] PlanetMesh mesh_from_handle(const psim_mesh* raw, std::uint32_t subdivision, double radius_m);
+}  // namespace detail
 
 class PlanetMesh {
   public:
@@ -146,7 +150,7 @@ class PlanetMesh {
     }
 
   private:
-    friend PlanetMesh make_icosphere(std::uint32_t subdivision, double radius_m);
+    friend PlanetMesh detail::mesh_from_handle(const psim_mesh* raw, std::uint32_t subdivision, double radius_m);
     struct DeviceCopy {
         std::once_flag once;
         struct Free {
@@ -167,6 +171,10 @@ inline PlanetMesh make_icosphere(std::uint32_t subdivision, double radius_m) {
     psim_mesh* raw = nullptr;
     gpu::check(psim_mesh_create(subdivision, radius_m, &raw));
     std::unique_ptr<psim_mesh, void (*)(psim_mesh*)> handle(raw, psim_mesh_destroy);
+    return detail::mesh_from_handle(raw, subdivision, radius_m);
+}
+
+inline PlanetMesh detail::mesh_from_handle(const psim_mesh* raw, std::uint32_t subdivision, double radius_m) {
     std::uint64_t n = 0, e = 0, d = 0;
     gpu::check(psim_mesh_counts(raw, &n, &e, &d));
     std::vector<double> center(3 * n), east(3 * n), north(3 * n), area(n), length(e), distance(e);

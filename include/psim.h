@@ -67,6 +67,27 @@ int psim_mesh_edges(const psim_mesh* mesh, uint32_t* first_cell, uint32_t* secon
 int psim_mesh_graph(const psim_mesh* mesh, double* area_m2, uint64_t* offset, uint64_t* neighbour, double* weight,
                     double* weight_sum);
 
+/* ------------------------------------------------------ agglomeration
+ * build_agglomeration (sim/planet/surface/heat_transport.cpp:467-512),
+ * bit-identical: each cell joins the nearest cell of the mesh one level
+ * coarser (greedy walk from the previous cell's group); a group's area is
+ * its cells' areas summed in cell order. The handle replaces the process
+ * cache behind agglomerated_transport_graph and agglomerated_mesh
+ * (heat_transport.hpp:45-50, heat_transport.cpp:520-543):
+ *   psim_agglomeration_mesh      the mesh one level coarser (at subdivision
+ *                                0 the mesh itself), owned by the handle;
+ *   the agglomerated graph       psim_mesh_graph of that mesh with its
+ *                                area replaced by group_area_m2.
+ * PSIM_LOGIC_ERROR when a coarse cell receives no fine cell (:505-509). */
+typedef struct psim_agglomeration psim_agglomeration;
+int psim_agglomeration_create(const psim_mesh* mesh, psim_agglomeration** out);
+void psim_agglomeration_destroy(psim_agglomeration* agglomeration);
+int psim_agglomeration_counts(const psim_agglomeration* agglomeration, uint64_t* cells, uint64_t* groups);
+/* group_of_cell[cells], group_area_m2[groups]; either may be NULL. */
+int psim_agglomeration_groups(const psim_agglomeration* agglomeration, uint32_t* group_of_cell,
+                              double* group_area_m2);
+const psim_mesh* psim_agglomeration_mesh(const psim_agglomeration* agglomeration);
+
 /* --------------------------------------------------- hierarchy (host view)
  * build_hierarchy (heat_transport.cpp:175-200): the aggregation hierarchy's
  * structure and unit-conductance Galerkin values, bit-identical. Exposed for
@@ -162,6 +183,28 @@ uint64_t psim_graph_depth(const psim_graph* graph);
 /* diffusion_source (heat_transport.cpp:579-593), bit-identical. */
 int psim_diffusion_source(psim_graph* graph, double conductance_W_K, const double* temperature_K,
                           double* source_W_m2, int where);
+
+/* ------------------------------------------------- groups on the device
+ * The cell-to-group maps around a transport solve on the agglomerated graph
+ * (sim/planet/surface/surface_energy.cpp:1226-1247,1285-1300,1306-1338; the
+ * reference writes them as loops inline around its response). A group's
+ * cells are visited in cell order. Fields are stored one after another
+ * (fields x cells in, fields x groups out, or the reverse), 1..65535. */
+typedef struct psim_groups psim_groups;
+/* Host arrays: group_of_cell[cells] (each < groups, every group non-empty),
+ * cell_area_m2[cells], group_area_m2[groups] (the agglomeration's). */
+int psim_groups_create(psim_ctx* ctx, uint64_t cells, uint64_t groups, const uint32_t* group_of_cell,
+                       const double* cell_area_m2, const double* group_area_m2, psim_groups** out);
+void psim_groups_destroy(psim_groups* groups);
+/* Area-weighted mean, bit-identical: group[g] = (sum over the group's cells
+ * of A_c v_c, from 0) / A_g (surface_energy.cpp:1285-1300). */
+int psim_groups_mean(psim_groups* groups, uint32_t fields, const double* cell_values, double* group_values, int where);
+/* Highest value, from -inf with std::max in cell order (NaN cells leave the
+ * running value as std::max does) (surface_energy.cpp:1306,1323-1324). */
+int psim_groups_max(psim_groups* groups, uint32_t fields, const double* cell_values, double* group_values, int where);
+/* cell[c] = group[group_of_cell[c]] (surface_energy.cpp:1247,1369). */
+int psim_groups_broadcast(psim_groups* groups, uint32_t fields, const double* group_values, double* cell_values,
+                          int where);
 
 /* ---------------------------------------------- linear system on a graph
  * GraphMultigrid::set_matrix / precondition (heat_transport.hpp:60-76) and

@@ -29,6 +29,7 @@
 #include <memory>
 #include <atomic>
 #include <cmath>
+#include <limits>
 #include <condition_variable>
 #include <cstring>
 #include <functional>
@@ -894,6 +895,48 @@ void heat_rhs(const Mesh& m, const double* ca, const double* temp, const double*
     for (size_t c = 0; c < m.n; ++c) rhs[c] = ca[c] * temp[c] + dt * m.area[c] * forcing[c];
 }
 
+// ------------------------------------------------------- agglomeration
+// heat_transport.cpp:467-512 (build_agglomeration): a fine cell's group is
+// found by walking the coarse mesh uphill in dot(fine centre, coarse
+// centre), starting where the previous fine cell ended; a step scans the
+// current cell then its cell_edges neighbours and takes the first strictly
+// larger dot; a group's area accumulates its cells' areas in cell order.
+// The result is the coarse mesh (one level down; the mesh itself at level 0)
+// with `area` holding the group areas: the agglomerated TransportGraph.
+Mesh* agglomerated(const Mesh& fine, uint32_t* group_of_cell) {
+    if (fine.level == 0) {
+        for (size_t c = 0; c < fine.n; ++c) group_of_cell[c] = static_cast<uint32_t>(c);
+        return new Mesh(fine);
+    }
+    std::unique_ptr<Mesh> coarse(build_mesh(fine.level - 1, fine.radius));
+    std::vector<double> group_area(coarse->n, 0.0);
+    size_t guess = 0;
+    for (size_t c = 0; c < fine.n; ++c) {
+        const V3 p = fine.center[c];
+        while (true) {
+            size_t best = guess;
+            double best_dot = dot3(p, coarse->center[guess]);
+            const size_t first = coarse->edge_offset[guess];
+            for (size_t k = first; k < first + coarse->edge_count[guess]; ++k) {
+                const size_t other = coarse->ce_neighbor[k];
+                const double d = dot3(p, coarse->center[other]);
+                if (d > best_dot) {
+                    best_dot = d;
+                    best = other;
+                }
+            }
+            if (best == guess) break;
+            guess = best;
+        }
+        group_of_cell[c] = static_cast<uint32_t>(guess);
+        group_area[guess] += fine.area[c];
+    }
+    for (const double a : group_area)
+        if (!(a > 0.0)) throw std::logic_error("a coarse transport cell received no fine cell");
+    coarse->area = std::move(group_area);
+    return coarse.release();
+}
+
 const Mesh& M(const void* h) { return *static_cast<const Mesh*>(h); }
 
 }  // namespace
@@ -1302,6 +1345,39 @@ int orc_solve_implicit_transport(const void* h, double k, const double* floor, o
         std::memcpy(mean_out, mean.data(), n * 8);
         *result = res;
     });
+}
+
+// surface_energy.cpp:1285-1300: a group's area-weighted mean over its cells
+// in cell order, divided by the group's area.
+int orc_group_mean(uint64_t cells, uint64_t groups, const uint32_t* group_of_cell, const double* cell_area,
+                   const double* group_area, const double* cell_values, double* group_mean) {
+    return guard([&] {
+        std::vector<std::vector<uint32_t>> members(groups);
+        for (uint64_t c = 0; c < cells; ++c) members.at(group_of_cell[c]).push_back(static_cast<uint32_t>(c));
+        for (uint64_t g = 0; g < groups; ++g) {
+            double air = 0.0;
+            for (const uint32_t c : members[g]) air += cell_area[c] * cell_values[c];
+            group_mean[g] = air / group_area[g];
+        }
+    });
+}
+
+// surface_energy.cpp:1306,1323-1324: each group's highest value, from -inf,
+// std::max over the cells in cell order.
+int orc_group_max(uint64_t cells, uint64_t groups, const uint32_t* group_of_cell, const double* cell_values,
+                  double* group_max) {
+    return guard([&] {
+        std::fill(group_max, group_max + groups, -std::numeric_limits<double>::infinity());
+        for (uint64_t c = 0; c < cells; ++c) {
+            double& g = group_max[group_of_cell[c]];
+            g = std::max(g, cell_values[c]);
+        }
+    });
+}
+
+void* orc_agglomerate(const void* h, uint32_t* group_of_cell) {
+    Mesh* out = nullptr;
+    return guard([&] { out = agglomerated(M(h), group_of_cell); }) == 0 ? out : nullptr;
 }
 
 }  // extern "C"

@@ -14,6 +14,8 @@
  *   multigrid      sim/planet/surface/heat_transport.cpp:54-326,545-577
  *   BiCGSTAB       sim/planet/surface/heat_transport.cpp:820-897
  *   diffusion      sim/planet/surface/heat_transport.cpp:28-36,579-593
+ *   agglomeration  sim/planet/surface/heat_transport.cpp:467-512
+ *   group maps     sim/planet/surface/surface_energy.cpp:1285-1300,1306-1324
  *   RNG            sim/core/random/counter_rng.hpp:19-45
  * and the synthetic heat-step workload (this project's input generator).
  *
@@ -114,6 +116,19 @@ typedef struct {
 int orc_solve_implicit_transport(const void* mesh, double conductance, const double* floor, orc_response_fn response,
                                  void* user, const orc_transport_settings* settings, uint64_t workers,
                                  const double* latent, double* source, double* mean, orc_transport_result* result);
+
+/* build_agglomeration (heat_transport.cpp:467-512): group_of_cell[cells]
+ * and a new mesh handle (free with orc_mesh_free) for the agglomerated
+ * transport graph: the mesh one level coarser (the mesh itself at level 0)
+ * whose cell areas are the group areas. NULL on error. */
+void* orc_agglomerate(const void* mesh, uint32_t* group_of_cell);
+/* The cell-to-group restriction around the agglomerated transport
+ * (surface_energy.cpp:1285-1300): mean[g] = sum_c A_c v_c / A_g over the
+ * group's cells in cell order; and the groups' floor (:1306,1323-1324). */
+int orc_group_mean(uint64_t cells, uint64_t groups, const uint32_t* group_of_cell, const double* cell_area,
+                   const double* group_area, const double* cell_values, double* group_mean);
+int orc_group_max(uint64_t cells, uint64_t groups, const uint32_t* group_of_cell, const double* cell_values,
+                  double* group_max);
 
 /* Synthetic heat-step workload (the project's seeded input generator).
  *   instance_seed = orc_mix_random_key(base_seed ^ instance)

@@ -449,4 +449,32 @@ int ref_solve_implicit_transport(const void* handle, double conductance, const d
     });
 }
 
+// agglomerated_transport_graph / agglomerated_mesh (heat_transport.cpp:
+// 520-543) of the handle's mesh: the group count, then the group map, the
+// graph's arrays and the coarse mesh's subdivision.
+std::uint64_t ref_agglomeration_groups(const void* handle) {
+    const std::vector<std::size_t>* map = nullptr;
+    return planetsim::agglomerated_transport_graph(static_cast<const RefMesh*>(handle)->mesh, map).size();
+}
+
+int ref_agglomeration(const void* handle, std::uint64_t* group_of_cell, double* area, std::uint64_t* offset,
+                      std::uint64_t* neighbour, double* weight, double* weight_sum, std::uint32_t* subdivision) {
+    return guarded([&] {
+        const auto& mesh = static_cast<const RefMesh*>(handle)->mesh;
+        const std::vector<std::size_t>* map = nullptr;
+        const auto& graph = planetsim::agglomerated_transport_graph(mesh, map);
+        for (std::size_t c = 0; c < map->size(); ++c) group_of_cell[c] = (*map)[c];
+        for (std::size_t i = 0; i < graph.size(); ++i) {
+            area[i] = graph.area_m2[i];
+            weight_sum[i] = graph.weight_sum[i];
+        }
+        for (std::size_t i = 0; i < graph.offset.size(); ++i) offset[i] = graph.offset[i];
+        for (std::size_t k = 0; k < graph.neighbour.size(); ++k) {
+            neighbour[k] = graph.neighbour[k];
+            weight[k] = graph.weight[k];
+        }
+        *subdivision = planetsim::agglomerated_mesh(mesh).subdivision();
+    });
+}
+
 }  // extern "C"

@@ -12,6 +12,7 @@ from __future__ import annotations
 import atexit
 import ctypes as C
 import os
+import types
 import weakref
 
 import numpy as np
@@ -20,7 +21,7 @@ __all__ = [
     "LIB_PATH", "lib", "PsimError", "Mesh", "Context", "FV", "Graph", "System", "SolveInfo",
     "HOST", "DEVICE", "F64", "F32", "PLAIN_FP64", "MIXED", "instance_seed", "heat_instance",
     "heat_forcing", "Hierarchy", "Batch", "DeviceArray", "Server", "ServeRequest", "ServeRecord",
-    "implicit_transport", "TransportSettings", "TransportResult",
+    "implicit_transport", "TransportSettings", "TransportResult", "Agglomeration", "Groups",
 ]
 
 # PSIM_LIBRARY names another build of the same ABI (e.g. a tuning variant).
@@ -44,7 +45,8 @@ def _track(obj):
 
 @atexit.register
 def _release_all():
-    for name in ("Server", "Batch", "System", "FV", "Graph", "DeviceArray", "Context", "Mesh"):
+    for name in ("Server", "Batch", "System", "Groups", "FV", "Graph", "DeviceArray", "Context", "Agglomeration",
+                 "Mesh"):
         for obj in list(_LIVE.get(name, ())):
             try:
                 obj.close()
@@ -94,6 +96,11 @@ _sig("psim_mesh_cells", C.c_int, _vp, _pd, _pd, _pd, _pd, _pu32, _pu8)
 _sig("psim_mesh_cell_edges", C.c_int, _vp, _pu32, _pu32, _pf, _pf)
 _sig("psim_mesh_edges", C.c_int, _vp, _pu32, _pu32, _pd, _pd)
 _sig("psim_mesh_graph", C.c_int, _vp, _pd, _pu64, _pu64, _pd, _pd)
+_sig("psim_agglomeration_create", C.c_int, _vp, C.POINTER(_vp))
+_sig("psim_agglomeration_destroy", None, _vp)
+_sig("psim_agglomeration_counts", C.c_int, _vp, C.POINTER(_u64), C.POINTER(_u64))
+_sig("psim_agglomeration_groups", C.c_int, _vp, _pu32, _pd)
+_sig("psim_agglomeration_mesh", _vp, _vp)
 _sig("psim_hierarchy_create", C.c_int, _u64, _pu64, _pu64, _pd, _pd, C.POINTER(_vp))
 _sig("psim_hierarchy_destroy", None, _vp)
 _sig("psim_hierarchy_depth", _u64, _vp)
@@ -129,6 +136,10 @@ _sig("psim_graph_size", _u64, _vp)
 _sig("psim_graph_entries", _u64, _vp)
 _sig("psim_graph_depth", _u64, _vp)
 _sig("psim_diffusion_source", C.c_int, _vp, _dbl, _vp, _vp, C.c_int)
+_sig("psim_groups_create", C.c_int, _vp, _u64, _u64, _pu32, _pd, _pd, C.POINTER(_vp))
+_sig("psim_groups_destroy", None, _vp)
+for _name in ("psim_groups_mean", "psim_groups_max", "psim_groups_broadcast"):
+    _sig(_name, C.c_int, _vp, C.c_uint32, _vp, _vp, C.c_int)
 _sig("psim_system_create", C.c_int, _vp, C.c_int, C.POINTER(_vp))
 _sig("psim_system_destroy", None, _vp)
 _sig("psim_system_set_workgroups", C.c_int, _vp, _i32)
@@ -212,11 +223,17 @@ def _ptr(a):
 class Mesh:
     """make_icosphere(subdivision, radius) on the host, with its arrays."""
 
-    def __init__(self, subdivision: int, radius: float = 6371000.0):
-        h = _vp()
-        check(lib.psim_mesh_create(subdivision, radius, C.byref(h)))
+    def __init__(self, subdivision: int, radius: float = 6371000.0, _view=None):
+        """_view: (handle, owner) of a mesh owned by another native object."""
+        if _view is None:
+            h = _vp()
+            check(lib.psim_mesh_create(subdivision, radius, C.byref(h)))
+            self._owner = None
+        else:
+            h, self._owner = _vp(_view[0]), _view[1]
         self.handle = h
-        _track(self)
+        if self._owner is None:
+            _track(self)
         self.subdivision = subdivision
         self.radius = radius
         n, e, d = _u64(), _u64(), _u64()
@@ -254,8 +271,44 @@ class Mesh:
 
     def close(self):
         """Release the native object now (also done at exit and on collection)."""
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and getattr(self, "_owner", None) is None:
             lib.psim_mesh_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+class Agglomeration:
+    """build_agglomeration of a mesh (agglomerated_transport_graph and
+    agglomerated_mesh): each cell's group, a cell of the mesh one level
+    coarser (`coarse`), the groups' areas, and `graph`, the agglomerated
+    transport graph (the coarse mesh's graph with the groups' areas), which
+    Graph() accepts as it accepts a Mesh."""
+
+    def __init__(self, mesh: Mesh):
+        h = _vp()
+        check(lib.psim_agglomeration_create(mesh.handle, C.byref(h)))
+        self.handle = h
+        _track(self)
+        n, g = _u64(), _u64()
+        check(lib.psim_agglomeration_counts(h, C.byref(n), C.byref(g)))
+        self.cells, self.groups = n.value, g.value
+        self.group_of_cell = np.zeros(self.cells, np.uint32)
+        self.group_area = np.zeros(self.groups)
+        check(lib.psim_agglomeration_groups(h, self.group_of_cell, self.group_area))
+        self.coarse = Mesh(max(mesh.subdivision - 1, 0), mesh.radius,
+                           _view=(lib.psim_agglomeration_mesh(h), self))
+        c = self.coarse
+        self.graph = types.SimpleNamespace(cells=self.groups, area=self.group_area, offset=c.offset,
+                                           graph_neighbour=c.graph_neighbour, weight=c.weight,
+                                           weight_sum=c.weight_sum)
+
+    def close(self):
+        """Release the native object now (also done at exit and on collection)."""
+        if getattr(self, "handle", None):
+            self.coarse.close()
+            lib.psim_agglomeration_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -468,6 +521,62 @@ class Graph:
         """Release the native object now (also done at exit and on collection)."""
         if getattr(self, "handle", None):
             lib.psim_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+class Groups:
+    """psim_groups: a group map's restriction (area-weighted mean, highest
+    value) and broadcast on the device. Host arrays are (fields, n) or (n,);
+    the *_device forms take device pointers."""
+
+    def __init__(self, ctx: Context, group_of_cell, cell_area, group_area):
+        g = np.ascontiguousarray(group_of_cell, np.uint32)
+        ca = np.ascontiguousarray(cell_area, np.float64)
+        ga = np.ascontiguousarray(group_area, np.float64)
+        h = _vp()
+        check(lib.psim_groups_create(ctx.handle, g.size, ga.size, g, ca, ga, C.byref(h)))
+        self.handle, self.ctx = h, ctx
+        _track(self)
+        self.cells, self.groups = g.size, ga.size
+
+    @classmethod
+    def of(cls, ctx: Context, agglomeration: "Agglomeration", mesh: Mesh):
+        return cls(ctx, agglomeration.group_of_cell, mesh.area, agglomeration.group_area)
+
+    def _host(self, fn, values, n_in, n_out):
+        v = np.ascontiguousarray(values, np.float64)
+        if v.shape[-1] != n_in:
+            raise ValueError("field length does not match the map")
+        fields = v.size // n_in
+        out = np.empty(v.shape[:-1] + (n_out,))
+        check(fn(self.handle, fields, _ptr(v), _ptr(out), HOST))
+        return out
+
+    def mean(self, values):
+        return self._host(lib.psim_groups_mean, values, self.cells, self.groups)
+
+    def max(self, values):
+        return self._host(lib.psim_groups_max, values, self.cells, self.groups)
+
+    def broadcast(self, values):
+        return self._host(lib.psim_groups_broadcast, values, self.groups, self.cells)
+
+    def mean_device(self, cell_values, group_values, fields=1):
+        check(lib.psim_groups_mean(self.handle, fields, cell_values, group_values, DEVICE))
+
+    def max_device(self, cell_values, group_values, fields=1):
+        check(lib.psim_groups_max(self.handle, fields, cell_values, group_values, DEVICE))
+
+    def broadcast_device(self, group_values, cell_values, fields=1):
+        check(lib.psim_groups_broadcast(self.handle, fields, group_values, cell_values, DEVICE))
+
+    def close(self):
+        """Release the native object now (also done at exit and on collection)."""
+        if getattr(self, "handle", None):
+            lib.psim_groups_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

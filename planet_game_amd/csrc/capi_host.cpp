@@ -18,6 +18,12 @@ struct psim_hierarchy {
     psim::HostHierarchy h;
 };
 
+struct psim_agglomeration {
+    psim_mesh coarse;
+    std::vector<uint32_t> group;
+    std::vector<double> area;
+};
+
 namespace psim {
 
 namespace {
@@ -109,6 +115,38 @@ int psim_mesh_graph(const psim_mesh* m, double* area, uint64_t* offset, uint64_t
         if (weight_sum) std::memcpy(weight_sum, h.weight_sum.data(), n * 8);
     });
 }
+
+int psim_agglomeration_create(const psim_mesh* mesh, psim_agglomeration** out) {
+    return guarded([&] {
+        if (mesh == nullptr || out == nullptr) throw std::invalid_argument("null argument");
+        psim::HostAgglomeration a = psim::agglomerate(mesh->mesh);
+        auto result = std::make_unique<psim_agglomeration>();
+        result->coarse.mesh = std::move(a.coarse);
+        result->group = std::move(a.group);
+        result->area = std::move(a.area);
+        *out = result.release();
+    });
+}
+
+void psim_agglomeration_destroy(psim_agglomeration* a) { delete a; }
+
+int psim_agglomeration_counts(const psim_agglomeration* a, uint64_t* cells, uint64_t* groups) {
+    return guarded([&] {
+        if (a == nullptr) throw std::invalid_argument("null agglomeration");
+        if (cells) *cells = a->group.size();
+        if (groups) *groups = a->area.size();
+    });
+}
+
+int psim_agglomeration_groups(const psim_agglomeration* a, uint32_t* group_of_cell, double* group_area_m2) {
+    return guarded([&] {
+        if (a == nullptr) throw std::invalid_argument("null agglomeration");
+        if (group_of_cell) std::memcpy(group_of_cell, a->group.data(), a->group.size() * 4);
+        if (group_area_m2) std::memcpy(group_area_m2, a->area.data(), a->area.size() * 8);
+    });
+}
+
+const psim_mesh* psim_agglomeration_mesh(const psim_agglomeration* a) { return a ? &a->coarse : nullptr; }
 
 int psim_hierarchy_create(uint64_t n, const uint64_t* offset, const uint64_t* neighbour, const double* weight,
                           const double* weight_sum, psim_hierarchy** out) {

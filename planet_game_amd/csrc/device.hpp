@@ -246,4 +246,27 @@ namespace psim {
 // Throws DeviceFailure (PSIM_DEVICE_ERROR) or DeviceOutOfMemory with the
 // HIP error text when `status` is not hipSuccess.
 void hip_check(hipError_t status, const char* what);
+
+// Host-pointer calls stage through temporary device buffers, freed when the
+// call returns (after its stream has been synchronized).
+struct Staging {
+    std::vector<void*> blocks;
+    Staging() = default;
+    Staging(const Staging&) = delete;
+    Staging& operator=(const Staging&) = delete;
+    ~Staging() {
+        for (void* p : blocks) (void)hipFree(p);
+    }
+    void* in(const void* host, size_t bytes) {
+        void* d = out(bytes);
+        hip_check(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice), "staging copy");
+        return d;
+    }
+    void* out(size_t bytes) {
+        void* d = nullptr;
+        hip_check(hipMalloc(&d, bytes), "staging allocation");
+        blocks.push_back(d);
+        return d;
+    }
+};
 }  // namespace psim

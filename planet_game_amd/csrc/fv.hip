@@ -202,27 +202,6 @@ psim_fv* upload_fv(psim_ctx* ctx, uint64_t cells, uint64_t edges, const double* 
     return fv.release();
 }
 
-// Host-pointer calls stage through temporary device buffers.
-struct Staging {
-    std::vector<void*> blocks;
-    ~Staging() {
-        for (void* p : blocks) (void)hipFree(p);
-    }
-    void* in(const void* host, size_t bytes) {
-        void* d = nullptr;
-        psim::hip_check(hipMalloc(&d, bytes), "staging allocation");
-        blocks.push_back(d);
-        psim::hip_check(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice), "staging copy");
-        return d;
-    }
-    void* out(size_t bytes) {
-        void* d = nullptr;
-        psim::hip_check(hipMalloc(&d, bytes), "staging allocation");
-        blocks.push_back(d);
-        return d;
-    }
-};
-
 void check_fields(const psim_fv* fv, int dtype, uint64_t fields) {
     if (fv == nullptr) throw std::invalid_argument("null mesh");
     if (dtype != PSIM_F64 && dtype != PSIM_F32) throw std::invalid_argument("unknown dtype");
@@ -281,7 +260,7 @@ int psim_gradient(psim_fv* fv, int dtype, uint64_t fields, const void* scalar, v
     return guarded([&] {
         check_fields(fv, dtype, fields);
         const size_t es = dtype == PSIM_F64 ? 8 : 4, bytes = es * fields * static_cast<size_t>(fv->n);
-        Staging st;
+        psim::Staging st;
         const void* s = where == PSIM_HOST ? st.in(scalar, bytes) : scalar;
         void* e = where == PSIM_HOST ? st.out(bytes) : east;
         void* no = where == PSIM_HOST ? st.out(bytes) : north;
@@ -309,7 +288,7 @@ int psim_divergence(psim_fv* fv, int dtype, uint64_t fields, const void* flux, v
         const size_t es = dtype == PSIM_F64 ? 8 : 4;
         const size_t in_bytes = es * fields * static_cast<size_t>(fv->edges);
         const size_t out_bytes = es * fields * static_cast<size_t>(fv->n);
-        Staging st;
+        psim::Staging st;
         const void* f = where == PSIM_HOST ? st.in(flux, in_bytes) : flux;
         void* o = where == PSIM_HOST ? st.out(out_bytes) : result;
         hipStream_t stream = fv->ctx->stream;
@@ -333,7 +312,7 @@ int psim_laplacian(psim_fv* fv, int dtype, uint64_t fields, const void* scalar, 
     return guarded([&] {
         check_fields(fv, dtype, fields);
         const size_t es = dtype == PSIM_F64 ? 8 : 4, bytes = es * fields * static_cast<size_t>(fv->n);
-        Staging st;
+        psim::Staging st;
         const void* s = where == PSIM_HOST ? st.in(scalar, bytes) : scalar;
         void* o = where == PSIM_HOST ? st.out(bytes) : result;
         hipStream_t stream = fv->ctx->stream;

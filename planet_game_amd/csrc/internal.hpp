@@ -140,6 +140,17 @@ HostHierarchy build_hierarchy(const HostGraph& graph);
 
 HostGraph graph_of(const HostMesh& mesh);
 
+// The agglomerated transport graph's groups (heat_transport.cpp:467-512):
+// the mesh one level coarser, each fine cell's group (a coarse cell) and
+// each group's area, the sum of its fine cells' areas in cell order. At
+// subdivision 0 the groups are the cells themselves.
+struct HostAgglomeration {
+    HostMesh coarse;
+    std::vector<uint32_t> group;   // per fine cell
+    std::vector<double> area;      // per group
+};
+HostAgglomeration agglomerate(const HostMesh& fine);
+
 constexpr int32_t tile_rows = 256;     // the reference's cell block size
 constexpr int32_t coarsest_rows = 400;  // heat_transport.cpp:188
 

@@ -256,4 +256,57 @@ HostGraph graph_of(const HostMesh& m) {
     return g;
 }
 
+// Each fine cell belongs to the coarse cell whose centre lies nearest on the
+// sphere (largest dot product of unit centres). The search hill-climbs the
+// coarse mesh from the group of the previous fine cell: fine cells are
+// numbered along a space-filling order, so the climb is a step or two. One
+// climbing step scans the current cell, then its neighbours in cell-edge
+// order, and moves to the first strictly better one than all before it; the
+// climb ends at a cell no neighbour improves on. Where two centres tie, the
+// start cell and that scan order decide, so both follow the reference
+// exactly (heat_transport.cpp:485-503); the dot product sums x, y, z in that
+// order (vec3d.hpp:30-32).
+HostAgglomeration agglomerate(const HostMesh& fine) {
+    HostAgglomeration a;
+    const size_t n = static_cast<size_t>(fine.cells);
+    a.group.resize(n);
+    if (fine.level == 0) {
+        a.coarse = fine;
+        a.area = fine.area;
+        for (size_t c = 0; c < n; ++c) a.group[c] = static_cast<uint32_t>(c);
+        return a;
+    }
+    a.coarse = build_icosphere(fine.level - 1, fine.radius);
+    const HostMesh& cm = a.coarse;
+    a.area.assign(static_cast<size_t>(cm.cells), 0.0);
+    const auto closeness = [&](const double* p, uint32_t q) {
+        const double* c = &cm.center[3 * static_cast<size_t>(q)];
+        return p[0] * c[0] + p[1] * c[1] + p[2] * c[2];
+    };
+    uint32_t at = 0;
+    for (size_t c = 0; c < n; ++c) {
+        const double* p = &fine.center[3 * c];
+        for (;;) {
+            uint32_t best = at;
+            double best_value = closeness(p, at);
+            const uint32_t begin = cm.edge_offset[at], end = begin + cm.edge_count[at];
+            for (uint32_t k = begin; k < end; ++k) {
+                const uint32_t q = cm.neighbor[k];
+                const double value = closeness(p, q);
+                if (value > best_value) {
+                    best_value = value;
+                    best = q;
+                }
+            }
+            if (best == at) break;
+            at = best;
+        }
+        a.group[c] = at;
+        a.area[at] += fine.area[c];
+    }
+    for (const double area : a.area)
+        if (!(area > 0.0)) throw std::logic_error("a coarse transport cell received no fine cell");
+    return a;
+}
+
 }  // namespace psim

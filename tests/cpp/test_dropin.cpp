@@ -74,6 +74,36 @@ void host_part() {
         const auto graph = planetsim::mesh_transport_graph(mesh);
         EXPECT(graph.size() == mesh.cell_count() && graph.offset.back() == mesh.directed_edge_count());
     }
+    // test_heat_transport.cpp:173-212: the agglomerated graph covers every
+    // cell once, its areas sum to the sphere's, its weights are symmetric,
+    // and groups hold about four cells; the cache hands out one graph.
+    for (std::uint32_t level = 0; level <= 5; ++level) {
+        const auto fine = planetsim::make_icosphere(level, 6'371'000.0);
+        const std::vector<std::size_t>* map = nullptr;
+        const auto& graph = planetsim::agglomerated_transport_graph(fine, map);
+        const auto& coarse = planetsim::agglomerated_mesh(fine);
+        EXPECT(level == 0 ? &coarse == &fine : coarse.subdivision() == level - 1);
+        EXPECT(graph.size() == coarse.cell_count() && map->size() == fine.cell_count());
+        const std::vector<std::size_t>* again = nullptr;
+        EXPECT(&planetsim::agglomerated_transport_graph(fine, again) == &graph && again == map);
+        double fine_area = 0.0, graph_area = 0.0;
+        for (const auto& cell : fine.cells()) fine_area += cell.area_m2;
+        for (const double a : graph.area_m2) graph_area += a;
+        EXPECT(std::abs(graph_area - fine_area) <= 1e-12 * fine_area);
+        bool covered = true, symmetric = true;
+        for (const std::size_t g : *map) covered = covered && g < graph.size();
+        for (std::size_t g = 0; g < graph.size(); ++g)
+            for (std::size_t k = graph.offset[g]; k < graph.offset[g + 1]; ++k) {
+                const std::size_t other = graph.neighbour[k];
+                bool found = false;
+                for (std::size_t j = graph.offset[other]; j < graph.offset[other + 1]; ++j)
+                    found = found || (graph.neighbour[j] == g && graph.weight[j] == graph.weight[k]);
+                symmetric = symmetric && found && other != g && graph.weight[k] > 0.0;
+            }
+        EXPECT(covered && symmetric);
+        const double per_group = static_cast<double>(fine.cell_count()) / static_cast<double>(graph.size());
+        EXPECT(level < 3 || (per_group > 3.9 && per_group < 4.1));
+    }
     const auto mesh = planetsim::make_icosphere(2, 6'371'000.0);
     EXPECT(throws_with<std::invalid_argument>([&] { (void)planetsim::make_icosphere(2, -5.0); },
                                               "icosphere radius must be finite and positive"));
@@ -233,6 +263,46 @@ void device_part() {
         EXPECT(throws_with<std::runtime_error>(
             [&] { (void)planetsim::solve_implicit_transport(graph, conductance, floor, failing); },
             "column did not converge"));
+    }
+    // The agglomerated transport as the reference's surface step runs it:
+    // group sources broadcast to cells, a per-cell response, the groups'
+    // area-weighted means back (surface_energy.cpp:1222-1300), on the
+    // device maps.
+    {
+        const std::vector<std::size_t>* map = nullptr;
+        const auto& graph = planetsim::agglomerated_transport_graph(mesh, map);
+        const planetsim::GroupMaps maps(mesh);
+        EXPECT(maps.cells() == n && maps.groups() == graph.size());
+        std::vector<double> ones(n, 1.0), mean(graph.size());
+        maps.mean(ones, mean);
+        bool exact = true;
+        for (const double m : mean) exact = exact && m == 1.0;
+        EXPECT(exact);
+        EXPECT(throws_with<std::invalid_argument>([&] { maps.mean(std::span(ones).first(n - 1), mean); },
+                                                  "field size does not match the groups"));
+        std::vector<double> base(n), cell_h(n), cell_t(2 * n), both(2 * graph.size());
+        for (std::size_t c = 0; c < n; ++c) base[c] = 250.0 + 40.0 * uniform(900 + c);
+        const planetsim::TransportResponse response = [&](const planetsim::Field2D<double>& source,
+                                                          planetsim::Field2D<double>& out_mean,
+                                                          planetsim::Field2D<double>& out_slope) {
+            maps.broadcast(source.values(), cell_h);
+            for (std::size_t c = 0; c < n; ++c) {
+                const double slope = 0.02 + 0.01 * uniform(1900 + c);
+                cell_t[c] = base[c] + slope * cell_h[c];
+                cell_t[n + c] = slope;
+            }
+            maps.mean(cell_t, both);
+            for (std::size_t g = 0; g < graph.size(); ++g) {
+                out_mean[g] = both[g];
+                out_slope[g] = both[graph.size() + g];
+            }
+        };
+        planetsim::Field2D<double> floor(graph.size(), -std::numeric_limits<double>::infinity());
+        const double conductance = 0.5 * 6'371'000.0 * 6'371'000.0;
+        const auto result = planetsim::solve_implicit_transport(graph, conductance, floor, response);
+        EXPECT(result.consistency_residual_W_m2 <= 1e-6);
+        EXPECT(std::abs(result.sum_W) <= 1e-12 * result.absolute_sum_W);
+        EXPECT(result.dissipation_W_K <= 0.0);
     }
     // HeatStepBatch: solvers of different graphs and precisions in one
     // launch give exactly what each solver's own step() gives.

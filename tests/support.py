@@ -171,11 +171,21 @@ class Lib:
             f.orc_mg_coarsest_factor.argtypes = [_vp, _dp]
             f.orc_mg_level_values.restype = None
             f.orc_mg_level_values.argtypes = [_vp, _u64, _dp, _dp]
+            f.orc_agglomerate.restype = _vp
+            f.orc_agglomerate.argtypes = [_vp, _u32p]
+            f.orc_group_mean.restype = C.c_int
+            f.orc_group_mean.argtypes = [_u64, _u64, _u32p, _dp, _dp, _dp, _dp]
+            f.orc_group_max.restype = C.c_int
+            f.orc_group_max.argtypes = [_u64, _u64, _u32p, _dp, _dp]
         else:
             f = self.lib
             f.ref_coupled_linear_solve.restype = C.c_int
             f.ref_coupled_linear_solve.argtypes = [_vp, _dp, _dp, _dp, C.c_double, C.c_double, _dp,
                                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]
+            f.ref_agglomeration_groups.restype = _u64
+            f.ref_agglomeration_groups.argtypes = [_vp]
+            f.ref_agglomeration.restype = C.c_int
+            f.ref_agglomeration.argtypes = [_vp, _u64p, _dp, _u64p, _u64p, _dp, _dp, C.POINTER(C.c_uint32)]
 
     def __getattr__(self, name):
         return getattr(self.lib, self.p + name)
@@ -204,10 +214,10 @@ def ref() -> Lib:
 class MeshArrays:
     """A mesh built by one library, with its arrays copied out."""
 
-    def __init__(self, lib: Lib, level: int, radius: float = RADIUS):
+    def __init__(self, lib: Lib, level: int, radius: float = RADIUS, handle=None):
         self.lib = lib
         self.level = level
-        self.handle = lib.mesh_build(level, radius)
+        self.handle = lib.mesh_build(level, radius) if handle is None else handle
         if not self.handle:
             raise RuntimeError(lib.last_error().decode())
         n, e, d = _u64(), _u64(), _u64()
@@ -258,6 +268,48 @@ class MeshArrays:
 
 
 _mesh_cache: dict = {}
+
+
+def agglomerate(level: int):
+    """The oracle's agglomeration of an L`level` mesh: (group_of_cell, the
+    agglomerated graph as MeshArrays of the coarse mesh with group areas)."""
+    fine = mesh("oracle", level)
+    group = np.zeros(fine.n, np.uint32)
+    h = oracle().agglomerate(fine.handle, group)
+    if not h:
+        raise RuntimeError(oracle().last_error().decode())
+    return group, MeshArrays(oracle(), max(level - 1, 0), handle=h)
+
+
+def ref_agglomeration(level: int):
+    """The compiled reference's agglomerated_transport_graph of L`level`."""
+    fine = mesh("ref", level)
+    groups = ref().agglomeration_groups(fine.handle)
+    nnz_guess = 6 * groups
+    out = {"group": np.zeros(fine.n, np.uint64), "area": np.zeros(groups),
+           "offset": np.zeros(groups + 1, np.uint64), "neighbour": np.zeros(nnz_guess, np.uint64),
+           "weight": np.zeros(nnz_guess), "weight_sum": np.zeros(groups)}
+    sub = C.c_uint32()
+    ref().check(ref().agglomeration(fine.handle, out["group"], out["area"], out["offset"], out["neighbour"],
+                                    out["weight"], out["weight_sum"], C.byref(sub)))
+    nnz = int(out["offset"][-1])
+    assert nnz <= nnz_guess
+    out["neighbour"], out["weight"] = out["neighbour"][:nnz], out["weight"][:nnz]
+    out["subdivision"] = sub.value
+    return out
+
+
+def group_mean(group, cell_area, group_area, values):
+    out = np.zeros(group_area.size)
+    oracle().check(oracle().group_mean(group.size, group_area.size, group, cell_area, group_area,
+                                       np.ascontiguousarray(values, np.float64), out))
+    return out
+
+
+def group_max(group, groups, values):
+    out = np.zeros(groups)
+    oracle().check(oracle().group_max(group.size, groups, group, np.ascontiguousarray(values, np.float64), out))
+    return out
 
 
 def mesh(lib_name: str, level: int) -> MeshArrays:
