@@ -385,6 +385,92 @@ def served_section(P, ctx, meshes, graphs, plain_value):
     }
 
 
+def response_table(n, points, seed):
+    """A concave increasing piecewise-linear response per node (the
+    device-resident response form), a flat (held) segment on every sixth
+    node, the floor at the table's first source."""
+    rng = np.random.default_rng(seed)
+    dh = rng.uniform(20.0, 80.0, (points - 1, n))
+    h = np.empty((points, n))
+    h[0] = -150.0 + rng.uniform(-20.0, 20.0, n)
+    h[1:] = h[0] + np.cumsum(dh, axis=0)
+    slopes = -np.sort(-rng.uniform(0.01, 0.3, (points - 1, n)), axis=0)
+    slopes[points // 2, ::6] = 0.0
+    t = np.empty((points, n))
+    t[0] = 250.0 + 30.0 * rng.random(n)
+    t[1:] = t[0] + np.cumsum(slopes * dh, axis=0)
+    return h, t, h[0].copy()
+
+
+def host_table_response(h_tab, t_tab):
+    """The same table evaluated by a host callback (the caller's response in
+    the reference's form), with the device kernel's operations."""
+    points, n = h_tab.shape
+    cols = np.arange(n)
+
+    def fn(h):
+        k = np.zeros(n, np.int64)
+        for j in range(1, points - 1):
+            k += (h >= h_tab[j]).astype(np.int64)
+        h0, h1, t0, t1 = h_tab[k, cols], h_tab[k + 1, cols], t_tab[k, cols], t_tab[k + 1, cols]
+        s = (t1 - t0) / (h1 - h0)
+        return t0 + s * (h - h0), s
+
+    return fn
+
+
+def transport_section(P, ctx, meshes, with_oracle, repeats=5):
+    """solve_implicit_transport (Newton + line search, PCG with the V-cycle)
+    on the L6 mesh's agglomerated graph (10,242 groups, the production
+    configuration) and on the L6 mesh itself, with a 6-point table response:
+    device-resident (psim_implicit_transport_table, inputs in HBM) and as a
+    host callback (psim_implicit_transport); the oracle, on one host thread,
+    with the same callback."""
+    K = 0.5 * 6371000.0 ** 2
+    agg = P.Agglomeration(meshes[6])
+    cases = {"l6_groups": (P.Graph(ctx, agg.graph), agg.groups), "l6_cells": (P.Graph(ctx, meshes[6]), meshes[6].cells)}
+    out = {"config": {"workload": "implicit transport, 6-point concave table response per node, every sixth node "
+                                  "held on one segment, conductance 0.5 R^2, default Newton settings",
+                      "repeats": repeats}}
+    for name, (graph, n) in cases.items():
+        h, t, floor = response_table(n, 6, 4242 + n)
+        d_h, d_t, d_f = (P.DeviceArray.of(ctx, np.ascontiguousarray(a.ravel())) for a in (h, t, floor))
+        d_s, d_m = P.DeviceArray(ctx, n), P.DeviceArray(ctx, n)
+        res = P.implicit_transport_table_device(graph, K, d_f, d_h, d_t, 6, d_s, d_m)
+        ctx.synchronize()
+        start = time.perf_counter()
+        with MARKS.range(f"transport_{name}"):
+            for _ in range(repeats):
+                res = P.implicit_transport_table_device(graph, K, d_f, d_h, d_t, 6, d_s, d_m)
+            ctx.synchronize()
+        device_ms = (time.perf_counter() - start) / repeats * 1e3
+        fn = host_table_response(h, t)
+        start = time.perf_counter()
+        for _ in range(repeats):
+            src, _, cres = P.implicit_transport(graph, K, floor, fn)
+        callback_ms = (time.perf_counter() - start) / repeats * 1e3
+        entry = {"nodes": n, "newton_iterations": res.newton_iterations, "cg_iterations": res.cg_iterations,
+                 "consistency_residual_W_m2": res.consistency_residual_W_m2,
+                 "device_table_ms": round(device_ms, 3), "host_callback_ms": round(callback_ms, 3),
+                 "table_equals_callback": bool(np.array_equal(d_s.download(), src))}
+        if with_oracle:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import support as S
+
+            if name == "l6_groups":
+                _, om = S.agglomerate(6)
+            else:
+                om = S.mesh("oracle", 6)
+            start = time.perf_counter()
+            osrc, _, ores = S.implicit_transport(S.oracle(), om.handle, om.n, K, floor, fn)
+            oracle_ms = (time.perf_counter() - start) * 1e3
+            entry.update({"oracle_ms_1_thread": round(oracle_ms, 1), "oracle_newton_iterations": ores.newton_iterations,
+                          "max_abs_dH_vs_oracle_W_m2": float(np.max(np.abs(src - osrc))),
+                          "device_table_speedup_vs_oracle": round(oracle_ms / device_ms, 1)})
+        out[name] = entry
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -577,6 +663,7 @@ def main():
         line["mixed_precision"] = mixed_section(P, ctx, mesh, graph, levels, not args.no_cpu)
         line["batched"] = batched_section(P, ctx, meshes, graphs, hierarchy_levels)
         line["served"] = served_section(P, ctx, meshes, graphs, value)
+        line["implicit_transport"] = transport_section(P, ctx, meshes, not args.no_cpu)
     print(json.dumps(line))
 
 

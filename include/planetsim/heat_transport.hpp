@@ -322,6 +322,28 @@ inline void response_trampoline(void* user, const double* source, double* mean, 
         std::fill(slope, slope + n, 0.0);
     }
 }
+
+inline psim_transport_settings transport_settings(const ImplicitTransportSettings& settings) {
+    psim_transport_settings st{};
+    st.max_newton_iterations = settings.max_newton_iterations;
+    st.max_line_search_halvings = settings.max_line_search_halvings;
+    st.rounding_floor_W_m2 = settings.rounding_floor_W_m2;
+    st.max_non_monotone_steps = settings.max_non_monotone_steps;
+    st.max_non_monotone_growth = settings.max_non_monotone_growth;
+    st.newton_tolerance_W_m2 = settings.newton_tolerance_W_m2;
+    st.cg_relative_tolerance = settings.cg_relative_tolerance;
+    st.max_cg_iterations = settings.max_cg_iterations;
+    return st;
+}
+
+inline void fill_result(const psim_transport_result& r, ImplicitTransportResult& result) {
+    result.consistency_residual_W_m2 = r.consistency_residual_W_m2;
+    result.newton_iterations = r.newton_iterations;
+    result.cg_iterations = r.cg_iterations;
+    result.sum_W = r.sum_W;
+    result.absolute_sum_W = r.absolute_sum_W;
+    result.dissipation_W_K = r.dissipation_W_K;
+}
 }  // namespace detail
 
 [[nodiscard]] inline ImplicitTransportResult solve_implicit_transport(
@@ -333,15 +355,7 @@ inline void response_trampoline(void* user, const double* source, double* mean, 
     if (latent_W_m2 != nullptr && latent_W_m2->size() != n)
         throw std::invalid_argument("transport latent heat does not match the mesh");
     gpu::check_workers(worker_count);
-    psim_transport_settings st{};
-    st.max_newton_iterations = settings.max_newton_iterations;
-    st.max_line_search_halvings = settings.max_line_search_halvings;
-    st.rounding_floor_W_m2 = settings.rounding_floor_W_m2;
-    st.max_non_monotone_steps = settings.max_non_monotone_steps;
-    st.max_non_monotone_growth = settings.max_non_monotone_growth;
-    st.newton_tolerance_W_m2 = settings.newton_tolerance_W_m2;
-    st.cg_relative_tolerance = settings.cg_relative_tolerance;
-    st.max_cg_iterations = settings.max_cg_iterations;
+    const psim_transport_settings st = detail::transport_settings(settings);
     detail::ResponseCall call{&response, n, nullptr};
     ImplicitTransportResult result;
     result.source_W_m2 = Field2D<double>(n, 0.0);
@@ -353,12 +367,46 @@ inline void response_trampoline(void* user, const double* source, double* mean, 
                                                result.source_W_m2.values().data(), result.mean_K.values().data(), &r);
     if (call.error) std::rethrow_exception(call.error);
     gpu::check(status);
-    result.consistency_residual_W_m2 = r.consistency_residual_W_m2;
-    result.newton_iterations = r.newton_iterations;
-    result.cg_iterations = r.cg_iterations;
-    result.sum_W = r.sum_W;
-    result.absolute_sum_W = r.absolute_sum_W;
-    result.dissipation_W_K = r.dissipation_W_K;
+    detail::fill_result(r, result);
+    return result;
+}
+
+// A response held on the device, so that no field crosses PCIe inside the
+// Newton loop (no reference counterpart: the reference's response is a
+// callback). Per node a piecewise-linear table of `points` (source,
+// temperature) pairs, point-major (pair k of node i at k size() + i),
+// sources increasing: T(h) follows the segment whose left end is the last
+// one <= h (the end segments extend past the table), the slope is that
+// segment's, and a flat segment holds the node.
+struct TransportResponseTable {
+    std::size_t points = 0;
+    std::vector<double> source_W_m2;
+    std::vector<double> mean_K;
+};
+
+[[nodiscard]] inline ImplicitTransportResult solve_implicit_transport(
+    const TransportGraph& graph, double conductance_W_K, const Field2D<double>& source_floor_W_m2,
+    const TransportResponseTable& table, const ImplicitTransportSettings& settings = {},
+    std::size_t worker_count = 1U, const std::vector<double>* latent_W_m2 = nullptr) {
+    const std::size_t n = graph.size();
+    if (source_floor_W_m2.size() != n) throw std::invalid_argument("transport floor does not match the mesh");
+    if (latent_W_m2 != nullptr && latent_W_m2->size() != n)
+        throw std::invalid_argument("transport latent heat does not match the mesh");
+    if (table.source_W_m2.size() != table.points * n || table.mean_K.size() != table.points * n)
+        throw std::invalid_argument("transport response table does not match the mesh");
+    gpu::check_workers(worker_count);
+    const psim_transport_settings st = detail::transport_settings(settings);
+    const psim_response_table t{static_cast<std::uint32_t>(table.points), 0U, table.source_W_m2.data(),
+                                table.mean_K.data()};
+    ImplicitTransportResult result;
+    result.source_W_m2 = Field2D<double>(n, 0.0);
+    result.mean_K = Field2D<double>(n, 0.0);
+    psim_transport_result r{};
+    gpu::check(psim_implicit_transport_table(graph.device(), conductance_W_K, source_floor_W_m2.values().data(), &t,
+                                             &st, latent_W_m2 != nullptr ? latent_W_m2->data() : nullptr,
+                                             result.source_W_m2.values().data(), result.mean_K.values().data(), &r,
+                                             PSIM_HOST));
+    detail::fill_result(r, result);
     return result;
 }
 

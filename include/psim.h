@@ -326,6 +326,26 @@ int psim_implicit_transport(psim_graph* graph, double conductance_W_K, const dou
                             const double* latent_W_m2, double* source_W_m2, double* mean_K,
                             psim_transport_result* result);
 
+/* The same driver with a device-resident response, so that no field crosses
+ * PCIe inside the Newton loop (the reference's response is always a
+ * callback; this form has no counterpart there). Per node, a piecewise-
+ * linear table of `points` (source, temperature) pairs, stored point-major
+ * (pair k of node i at k n + i), sources increasing: T(h) follows the
+ * segment whose left end is the last one <= h (the first segment below the
+ * table, the last above it), and the slope is that segment's
+ * (T1 - T0) / (h1 - h0); a flat segment holds the node (slope 0). A linear
+ * response T0 + s h is a 2-point table. where = PSIM_DEVICE: the table,
+ * floor, latent and outputs are device arrays; PSIM_HOST: host arrays. */
+typedef struct {
+    uint32_t points, reserved;
+    const double* source_W_m2;   /* points x n */
+    const double* mean_K;        /* points x n */
+} psim_response_table;
+int psim_implicit_transport_table(psim_graph* graph, double conductance_W_K, const double* source_floor_W_m2,
+                                  const psim_response_table* table, const psim_transport_settings* settings,
+                                  const double* latent_W_m2, double* source_W_m2, double* mean_K,
+                                  psim_transport_result* result, int where);
+
 /* ---- Batches (no counterpart in the reference, whose solves run one at a
  * time): many systems, of any graphs and precisions on one context, solved
  * in ONE launch of the persistent solver. Each system owns a group of

@@ -22,6 +22,7 @@ __all__ = [
     "HOST", "DEVICE", "F64", "F32", "PLAIN_FP64", "MIXED", "instance_seed", "heat_instance",
     "heat_forcing", "Hierarchy", "Batch", "DeviceArray", "Server", "ServeRequest", "ServeRecord",
     "implicit_transport", "TransportSettings", "TransportResult", "Agglomeration", "Groups",
+    "ResponseTable", "implicit_transport_table", "implicit_transport_table_device",
 ]
 
 # PSIM_LIBRARY names another build of the same ABI (e.g. a tuning variant).
@@ -191,6 +192,14 @@ _sig("psim_transport_settings_default", None, C.POINTER(TransportSettings))
 _sig("psim_implicit_transport", C.c_int, _vp, _dbl, _pd, RESPONSE, _vp, C.POINTER(TransportSettings), _vp, _pd, _pd,
      C.POINTER(TransportResult))
 _sig("psim_system_pcg", C.c_int, _vp, _vp, _vp, _vp, _dbl, _i32, C.POINTER(SolveInfo), C.c_int)
+
+
+class ResponseTable(C.Structure):
+    _fields_ = [("points", C.c_uint32), ("reserved", C.c_uint32), ("source_W_m2", _vp), ("mean_K", _vp)]
+
+
+_sig("psim_implicit_transport_table", C.c_int, _vp, _dbl, _vp, C.POINTER(ResponseTable), C.POINTER(TransportSettings),
+     _vp, _vp, _vp, C.POINTER(TransportResult), C.c_int)
 _sig("psim_server_create", C.c_int, _vp, _pp, C.POINTER(_dbl), C.POINTER(_i32), _i32, C.c_int, _u64, C.POINTER(_vp))
 _sig("psim_server_destroy", None, _vp)
 _sig("psim_server_plan", C.c_int, _vp, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_u64))
@@ -858,3 +867,41 @@ def implicit_transport(graph: Graph, conductance, floor, response, settings=None
     check(lib.psim_implicit_transport(graph.handle, conductance, fl, cb, None, C.byref(settings),
                                       None if lat is None else lat.ctypes.data_as(_vp), src, mean, C.byref(res)))
     return src, mean, res
+
+
+def _settings(settings):
+    if settings is None:
+        settings = TransportSettings()
+        lib.psim_transport_settings_default(C.byref(settings))
+    return settings
+
+
+def implicit_transport_table(graph: Graph, conductance, floor, table_source, table_mean, settings=None, latent=None):
+    """psim_implicit_transport_table with host arrays: a device-resident
+    piecewise-linear response, table_* of shape (points, n), sources
+    increasing per node. Returns (source_W_m2, mean_K, TransportResult)."""
+    n = graph.mesh.cells
+    h = np.ascontiguousarray(table_source, np.float64)
+    t = np.ascontiguousarray(table_mean, np.float64)
+    if h.shape != t.shape or h.ndim != 2 or h.shape[1] != n:
+        raise ValueError("response table must be (points, n) for both arrays")
+    table = ResponseTable(h.shape[0], 0, h.ctypes.data_as(_vp), t.ctypes.data_as(_vp))
+    fl = np.ascontiguousarray(floor, np.float64)
+    lat = None if latent is None else np.ascontiguousarray(latent, np.float64)
+    src, mean, res = np.zeros(n), np.zeros(n), TransportResult()
+    check(lib.psim_implicit_transport_table(graph.handle, conductance, _ptr(fl), C.byref(table),
+                                            C.byref(_settings(settings)), None if lat is None else _ptr(lat),
+                                            _ptr(src), _ptr(mean), C.byref(res), HOST))
+    return src, mean, res
+
+
+def implicit_transport_table_device(graph: Graph, conductance, floor: "DeviceArray", table_source: "DeviceArray",
+                                    table_mean: "DeviceArray", points, source_out: "DeviceArray",
+                                    mean_out: "DeviceArray", settings=None, latent: "DeviceArray" = None):
+    """The same with device arrays (table arrays hold points x n values)."""
+    table = ResponseTable(points, 0, table_source.ptr, table_mean.ptr)
+    res = TransportResult()
+    check(lib.psim_implicit_transport_table(graph.handle, conductance, floor.ptr, C.byref(table),
+                                            C.byref(_settings(settings)), None if latent is None else latent.ptr,
+                                            source_out.ptr, mean_out.ptr, C.byref(res), DEVICE))
+    return res

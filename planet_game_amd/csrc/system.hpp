@@ -7,6 +7,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -56,8 +57,10 @@ struct psim_graph {
     double* weight = nullptr;
     int32_t* csr_col = nullptr;   // level-0 CSR columns (the diffusion sweep reads CSR order)
     psim::dev::LevelView views[psim::dev::kMaxLevels];
-    // the Newton-system solver of psim_implicit_transport, made on first use
+    // the Newton-system solver of psim_implicit_transport and its device
+    // vectors, made on first use; calls on the graph hold newton_mutex
     psim_system* newton = nullptr;
+    std::shared_ptr<void> transport_work;
     std::mutex newton_mutex;
     ~psim_graph() {
         if (newton != nullptr) psim_system_destroy(newton);

@@ -118,9 +118,13 @@ void host_part() {
                                               "edge field size does not match the mesh"));
 }
 
+// Names the section that runs next, so that an exception names its place.
+void stage(const char* name) { std::fprintf(stderr, "[device part] %s\n", name); }
+
 void device_part() {
     const auto mesh = planetsim::make_icosphere(5, 6'371'000.0);
     const std::size_t n = mesh.cell_count();
+    stage("operator conservation");
     // test_operator_conservation.cpp:36-57: constant fields give exactly 0.
     {
         const planetsim::Field2D<double> constant(n, 287.5);
@@ -132,6 +136,7 @@ void device_part() {
         for (std::size_t c = 0; c < n; ++c) zero = zero && east[c] == 0.0 && north[c] == 0.0 && lap[c] == 0.0 && div[c] == 0.0;
         EXPECT(zero);
     }
+    stage("operator determinism");
     // test_operator_determinism.cpp: bit-identical for any worker count;
     // the Laplacian is conservative and dissipative.
     {
@@ -162,6 +167,7 @@ void device_part() {
         EXPECT(std::abs(sum) <= 1e-13 * abs_sum);
         EXPECT(dissipation <= 0.0);
     }
+    stage("multigrid and heat step");
     // GraphMultigrid and the heat step on the transport graph.
     {
         const auto graph = planetsim::mesh_transport_graph(mesh);
@@ -228,6 +234,7 @@ void device_part() {
         }
         EXPECT(std::abs(net) <= 1e-12 * gross);
     }
+    stage("implicit transport");
     // solve_implicit_transport with a linear response (the reference's
     // check_linear, test_heat_transport.cpp:73-107).
     {
@@ -263,7 +270,29 @@ void device_part() {
         EXPECT(throws_with<std::runtime_error>(
             [&] { (void)planetsim::solve_implicit_transport(graph, conductance, floor, failing); },
             "column did not converge"));
+        // the same linear response as a device-resident 2-point table:
+        // T(h) = base + s h through (0, base) and (100, base + 100 s)
+        planetsim::TransportResponseTable table;
+        table.points = 2;
+        table.source_W_m2.assign(2 * n, 0.0);
+        table.mean_K.assign(2 * n, 0.0);
+        for (std::size_t c = 0; c < n; ++c) {
+            table.source_W_m2[n + c] = 100.0;
+            table.mean_K[c] = base[c];
+            table.mean_K[n + c] = base[c] + 100.0 * slopes[c];
+        }
+        const auto tabled = planetsim::solve_implicit_transport(graph, conductance, floor, table);
+        EXPECT(tabled.consistency_residual_W_m2 <= 1e-6 && tabled.newton_iterations <= 3);
+        double largest = 0.0;
+        for (std::size_t c = 0; c < n; ++c)
+            largest = std::max(largest, std::abs(tabled.source_W_m2[c] - result.source_W_m2[c]));
+        EXPECT(largest <= 1e-5);
+        table.points = 3;
+        EXPECT(throws_with<std::invalid_argument>(
+            [&] { (void)planetsim::solve_implicit_transport(graph, conductance, floor, table); },
+            "transport response table does not match the mesh"));
     }
+    stage("agglomerated transport");
     // The agglomerated transport as the reference's surface step runs it:
     // group sources broadcast to cells, a per-cell response, the groups'
     // area-weighted means back (surface_energy.cpp:1222-1300), on the
@@ -304,6 +333,7 @@ void device_part() {
         EXPECT(std::abs(result.sum_W) <= 1e-12 * result.absolute_sum_W);
         EXPECT(result.dissipation_W_K <= 0.0);
     }
+    stage("heat step batch");
     // HeatStepBatch: solvers of different graphs and precisions in one
     // launch give exactly what each solver's own step() gives.
     {
@@ -351,8 +381,11 @@ void device_part() {
 
 }  // namespace
 
-int main(int argc, char** argv) {
-    const std::string part = argc > 1 ? argv[1] : "cpu";
+// Runs one part: 0 when every expectation held, 1 on failed expectations,
+// 2 on an unexpected exception.
+extern "C" int dropin_run(const char* part_name) {
+    const std::string part = part_name != nullptr ? part_name : "cpu";
+    failures = 0;
     try {
         if (part == "cpu") host_part();
         if (part == "gpu") device_part();
@@ -361,5 +394,12 @@ int main(int argc, char** argv) {
         return 2;
     }
     std::printf("%s part: %d failure(s)\n", part.c_str(), failures);
+    std::fflush(stdout);
     return failures == 0 ? 0 : 1;
 }
+
+// The program form; the shared-object form (-DDROPIN_LIBRARY) lets a test
+// runner call dropin_run inside its own process.
+#ifndef DROPIN_LIBRARY
+int main(int argc, char** argv) { return dropin_run(argc > 1 ? argv[1] : "cpu"); }
+#endif

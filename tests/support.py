@@ -146,6 +146,42 @@ def quartic_response(mesh_area_n, seed_values, a, radiative=0.75 * 0.97 * 5.6703
     return fn, floor
 
 
+def table_response(table_h, table_t):
+    """A piecewise-linear response per node, evaluated on the host exactly as
+    psim_implicit_transport_table evaluates it on the device: the segment is
+    the last one whose left end is <= h (first below the table, last above),
+    s = (t1 - t0) / (h1 - h0), T = t0 + s (h - h0)."""
+    points, n = table_h.shape
+    cols = np.arange(n)
+
+    def fn(h):
+        k = np.zeros(n, np.int64)
+        for j in range(1, points - 1):
+            k += (h >= table_h[j]).astype(np.int64)
+        h0, h1 = table_h[k, cols], table_h[k + 1, cols]
+        t0, t1 = table_t[k, cols], table_t[k + 1, cols]
+        s = (t1 - t0) / (h1 - h0)
+        return t0 + s * (h - h0), s
+
+    return fn
+
+
+def response_table(n, points, seed):
+    """A concave increasing table per node (slopes falling along h), with a
+    flat (held) segment on every sixth node, and the floor at the table's
+    first source."""
+    rng = np.random.default_rng(seed)
+    dh = rng.uniform(20.0, 80.0, (points - 1, n))
+    table_h = np.vstack([np.full((1, n), -150.0) + rng.uniform(-20.0, 20.0, (1, n)),
+                         np.zeros((points - 1, n))])
+    table_h[1:] = table_h[0] + np.cumsum(dh, axis=0)
+    slopes = -np.sort(-rng.uniform(0.01, 0.3, (points - 1, n)), axis=0)
+    slopes[points // 2, ::6] = 0.0
+    table_t = np.vstack([250.0 + 30.0 * rng.random((1, n)), np.zeros((points - 1, n))])
+    table_t[1:] = table_t[0] + np.cumsum(slopes * dh, axis=0)
+    return np.ascontiguousarray(table_h), np.ascontiguousarray(table_t), table_h[0].copy()
+
+
 class Lib:
     """One of the two CPU libraries (oracle or reference shim)."""
 

@@ -20,7 +20,7 @@ import pytest
 
 import planet_game_amd as P
 import support as S
-from test_gpu_parity import announce, product_mesh
+from test_gpu_parity import announce, bits_equal, product_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -116,3 +116,61 @@ def test_transport_rejects_bad_conductance(ctx):
     for bad in (0.0, -1.0, float("nan"), float("inf")):
         with pytest.raises(P.PsimError, match="transport conductance must be finite and positive"):
             P.implicit_transport(g, bad, floor, fn)
+
+
+def same_result(a, b):
+    return all(getattr(a, f) == getattr(b, f) for f, _ in P.TransportResult._fields_)
+
+
+@pytest.mark.parametrize("level", [3, 5])
+def test_table_response_equals_host_callback(ctx, level):
+    """The device-resident table and the same table evaluated by a host
+    callback drive the same Newton iterates: bit-identical outputs."""
+    announce("implicit transport table vs callback", level)
+    m = product_mesh(level)
+    h, t, floor = S.response_table(m.cells, 6, 70 + level)
+    g = P.Graph(ctx, m)
+    st, mt, rt = P.implicit_transport_table(g, K, floor, h, t)
+    sc, mc, rc = P.implicit_transport(g, K, floor, S.table_response(h, t))
+    print("level", level, "newton", rt.newton_iterations, "cg", rt.cg_iterations, "residual",
+          rt.consistency_residual_W_m2, flush=True)
+    assert bits_equal(st, sc) and bits_equal(mt, mc) and same_result(rt, rc)
+    latent = 3.0 * np.cos(np.arange(m.cells))
+    st, mt, rt = P.implicit_transport_table(g, K, floor, h, t, latent=latent)
+    sc, mc, rc = P.implicit_transport(g, K, floor, S.table_response(h, t), latent=latent)
+    assert bits_equal(st, sc) and bits_equal(mt, mc) and same_result(rt, rc)
+
+
+@pytest.mark.parametrize("level", [4, 6])
+def test_table_transport_matches_oracle(ctx, level):
+    announce("implicit transport table vs oracle", level)
+    om = S.mesh("oracle", level)
+    h, t, floor = S.response_table(om.n, 6, 70 + level)
+    dev = P.implicit_transport_table(P.Graph(ctx, product_mesh(level)), K, floor, h, t)
+    ref = S.implicit_transport(S.oracle(), om.handle, om.n, K, floor, S.table_response(h, t))
+    (sd, md, rd), (sr, mr, rr) = dev, ref
+    print("level", level, "newton", rd.newton_iterations, rr.newton_iterations, "cg", rd.cg_iterations,
+          rr.cg_iterations, "residual", rd.consistency_residual_W_m2, rr.consistency_residual,
+          "dH", float(np.max(np.abs(sd - sr))), "dT", float(np.max(np.abs(md - mr))), flush=True)
+    assert rd.consistency_residual_W_m2 <= 1e-6
+    assert abs(rd.sum_W) <= 1e-12 * rd.absolute_sum_W
+    assert rd.dissipation_W_K <= 0.0
+    assert np.max(np.abs(sd - sr)) <= 1e-5
+    assert np.max(np.abs(md - mr)) <= 1e-6
+
+
+def test_table_transport_device_pointers(ctx):
+    announce("implicit transport table device pointers")
+    level = 4
+    m = product_mesh(level)
+    h, t, floor = S.response_table(m.cells, 4, 90)
+    g = P.Graph(ctx, m)
+    expect = P.implicit_transport_table(g, K, floor, h, t)
+    d_h, d_t, d_f = P.DeviceArray.of(ctx, h.ravel()), P.DeviceArray.of(ctx, t.ravel()), P.DeviceArray.of(ctx, floor)
+    d_s, d_m = P.DeviceArray(ctx, m.cells), P.DeviceArray(ctx, m.cells)
+    res = P.implicit_transport_table_device(g, K, d_f, d_h, d_t, 4, d_s, d_m)
+    ctx.synchronize()
+    assert bits_equal(d_s.download(), expect[0]) and bits_equal(d_m.download(), expect[1])
+    assert same_result(res, expect[2])
+    with pytest.raises(P.PsimError, match="response table needs 2 to 4096 points"):
+        P.implicit_transport_table_device(g, K, d_f, d_h, d_t, 1, d_s, d_m)
