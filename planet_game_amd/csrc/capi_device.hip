@@ -407,7 +407,7 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         s->tri_bytes = nc_rows * (nc_rows + 1) / 2 * sizeof(double);
         int per_cu = ctx->blocks_per_cu;
         if (nc_rows <= static_cast<size_t>(psim::dev::kBlockedCoarse)) {
-            s->tri_bytes = 0;   // inverted in registers
+            s->tri_bytes = 0;   // swept by blocks in global scratch (I.tri)
         } else if (s->tri_bytes <= ctx->lds_per_block) {
             // set_matrix launches carry the packed coarsest matrix in dynamic
             // LDS; co-residency then follows from the CU's LDS
@@ -462,7 +462,10 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         A.plan(&I.partial, 2 * static_cast<size_t>(I.partial_slots) * kPartials);
         A.plan(&s->csr_stage, static_cast<size_t>(g->nnz));
         A.plan(&I.barrier, psim::dev::kBarrierWordCount);
-        if (!I.tri_lds && s->tri_bytes > 0) A.plan(&I.tri, nc_rows * (nc_rows + 1) / 2);
+        if (!I.tri_lds && s->tri_bytes > 0)
+            A.plan(&I.tri, nc_rows * (nc_rows + 1) / 2);
+        else if (nc_rows <= static_cast<size_t>(psim::dev::kBlockedCoarse))
+            A.plan(&I.tri, static_cast<size_t>(psim::dev::sweep_words(static_cast<int>(nc_rows))));
         A.plan(&I.result, 1);
         A.plan(&s->d_inst, 1);
         s->map_capacity = cap;

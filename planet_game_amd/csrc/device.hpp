@@ -23,8 +23,15 @@ constexpr int kTile = 256;        // rows per tile = threads per workgroup = the
 constexpr int kMaxLevels = 8;
 constexpr int kPartials = 4;      // reduction slots per tile
 constexpr int kMaxCoarse = 512;   // dense coarsest level (the hierarchy stops at <= 400 rows)
-// Coarsest levels up to this size are inverted in registers (no scratch).
+// Coarsest levels up to this size are inverted by the block sweep (8 pivots
+// per step, working matrix in sweep_words(n) words of I.tri).
 constexpr int kBlockedCoarse = 168;
+constexpr int kSweepBlock = 8;       // pivots per block step of the coarsest inverse
+// Words of I.tri the block sweep uses for a coarsest level of n <= kBlockedCoarse rows.
+constexpr int sweep_words(int n) {
+    return (n + kSweepBlock - 1) / kSweepBlock * ((n + kSweepBlock - 1) / kSweepBlock + 1) / 2 * kSweepBlock *
+           kSweepBlock;
+}
 constexpr int32_t kNotConverged = 5;   // PSIM_NOT_CONVERGED
 
 enum Op : int32_t {
@@ -105,7 +112,7 @@ struct Instance {
     void* t = nullptr;
     void* lvl_b[kMaxLevels] = {};   // V-cycle right-hand sides, levels >= 1
     void* lvl_e[kMaxLevels] = {};   // V-cycle corrections, levels >= 1
-    double* tri = nullptr;          // packed coarsest matrix when it does not fit in LDS
+    double* tri = nullptr;          // coarsest working matrix: block sweep, or packed when not in LDS
     int32_t tri_lds = 0;            // 1: set_matrix launches carry it in dynamic LDS
     double* partial = nullptr;      // 2 x partial_slots x kPartials (double-buffered)
     uint32_t* barrier = nullptr;    // group barrier words (kBarrierWordCount, zeroed)

@@ -49,13 +49,24 @@ __device__ inline double wave_max(double v) {
     return v;
 }
 
+constexpr int kPanelStride = kSweepBlock + 1;
+
 struct Lds {
     double red[kPartials][4];        // per-wave partials
     double bcast[kPartials];
-    double tile[2][kTile];           // restriction staging (two tiles in flight)
-    double cvec[kMaxCoarse];         // coarse right-hand side, then solution
-    double crhs[kMaxCoarse];         // coarse right-hand side (FP64 staging)
-    double colk[kMaxCoarse];         // sweep pivot column (set_matrix)
+    union {
+        struct {
+            double tile[2][kTile];       // restriction staging (two tiles in flight)
+            double cvec[kMaxCoarse];     // coarse right-hand side, then solution
+            double crhs[kMaxCoarse];     // coarse right-hand side (FP64 staging)
+            double colk[kMaxCoarse];     // sweep pivot column (set_matrix, large coarsest)
+        };
+        // set_matrix, coarsest of at most kBlockedCoarse rows: the block
+        // column under sweep, double-buffered, and its value before the
+        // sweep (row r, column w at r * kPanelStride + w; the odd stride
+        // spreads rows eight apart over different banks)
+        double panel[3][kBlockedCoarse * kPanelStride];
+    };
 };
 
 // Dynamic LDS: the packed coarsest matrix during set_matrix launches.
@@ -630,6 +641,16 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
 // (no pivoting needed). Only the packed lower triangle is stored, in
 // `tri` (LDS when it fits, else global scratch); the result is written
 // negated and fully expanded, column k contiguous, to I.cinv.
+// A pointer every lane holds, rebuilt in scalar registers: descriptor reads
+// through it are scalar loads (a function argument arrives in vector
+// registers otherwise).
+template <class T> __device__ inline const T* uniform_ptr(const T* p) {
+    const uint64_t bits = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits >> 32));
+    return reinterpret_cast<const T*>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 __device__ inline int tri_index(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 
 __device__ void coarse_inverse(const Instance& I, double* tri, Lds& lds) {
@@ -679,115 +700,170 @@ __device__ void coarse_inverse(const Instance& I, double* tri, Lds& lds) {
     }
 }
 
-// The same sweep for a coarsest level of at most kBlockedCoarse rows, held
-// in registers: the lower triangle, padded to 8-row blocks, is cut into
-// 8 x 8 blocks (bi >= bj), one per thread (21 block rows give 231 blocks).
-// Pivot k broadcasts column k through LDS, double-buffered so that one
-// barrier per pivot suffices, and each thread updates its 64 entries:
-//   a_rc <- a_rc - (a_rk / d) a_kc,  a_rk <- a_rk / d,  a_kk <- -1 / d.
-// Diagonal blocks keep both halves but only their lower entries are ever
-// read, so the result is exactly symmetric. Written negated to I.cinv.
-constexpr int kSweepBlock = 8;
+// The same sweep for a coarsest level of at most kBlockedCoarse rows, by
+// blocks of eight pivots (one workgroup). The working matrix M is the lower
+// triangle cut into 8 x 8 blocks, block (bi, bj) (bi >= bj) numbered
+// b = bi (bi + 1) / 2 + bj, stored interleaved in I.tri (entry (u, v) of
+// block b at (8 u + v) blocks + b) so that threads owning consecutive
+// blocks touch consecutive words; inside a diagonal block only u >= v is
+// used. Block step K:
+//   1. block column K of M goes to an LDS panel A0[r][w] = M(r, 8K + w) for
+//      every row r (by symmetry above the diagonal), and a working copy;
+//   2. its eight pivots sweep the working panel in LDS (double-buffered,
+//      one barrier per pivot): entries of a swept panel depend only on the
+//      panel, so afterwards it holds M(r,K) M(K,K)^-1 outside block K and
+//      -M(K,K)^-1 inside it;
+//   3. every block outside the panel takes the block form of the eight
+//      pivots at once, M(r,c) -= sum_w P[r][w] A0[c][w], one thread per
+//      block;
+//   4. the swept panel goes back into M.
+// At the end I.cinv = -M = A^-1, written in full (column-major).
 constexpr int kSweepBlockRows = kBlockedCoarse / kSweepBlock;
 static_assert(kSweepBlockRows * kSweepBlock == kBlockedCoarse, "whole sweep blocks");
-static_assert(kSweepBlockRows * (kSweepBlockRows + 1) / 2 <= kTile, "one sweep block per thread");
-static_assert(2 * 256 <= kMaxCoarse && kBlockedCoarse <= 256, "two pivot-column buffers in Lds::colk");
 
-__device__ void coarse_inverse_blocked(const Instance& I, Lds& lds) {
+__device__ __noinline__ void coarse_inverse_blocked(const Instance* instance, Lds& lds) {
+    const Instance& I = *uniform_ptr(instance);
+    constexpr int B = kSweepBlock;
     const int j = I.depth - 1;
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
-    const int nb = (n + kSweepBlock - 1) / kSweepBlock;
-    const int q = static_cast<int>(threadIdx.x);
-    int bi = static_cast<int>((sqrtf(8.0f * static_cast<float>(q) + 1.0f) - 1.0f) * 0.5f);
-    if (bi * (bi + 1) / 2 > q) --bi;
-    if ((bi + 1) * (bi + 2) / 2 <= q) ++bi;
-    const int bj = q - bi * (bi + 1) / 2;
-    const bool active = bi < nb;
-    const int r0 = bi * kSweepBlock, c0 = bj * kSweepBlock;
-    double a[kSweepBlock][kSweepBlock];
-#pragma unroll
-    for (int u = 0; u < kSweepBlock; ++u)
-#pragma unroll
-        for (int v = 0; v < kSweepBlock; ++v) a[u][v] = 0.0;
-    if (active) {
-        // lower-triangle entries (c <= r) from row r of the ELL level
-#pragma unroll
-        for (int u = 0; u < kSweepBlock; ++u) {
-            const int r = r0 + u;
-            if (r < n) {
-                const double d = I.diag64[j][r];
-#pragma unroll
-                for (int v = 0; v < kSweepBlock; ++v)
-                    if (c0 + v == r) a[u][v] = d;
-                for (int s = 0; s < W; ++s) {
-                    const int c = L.col[s * n + r];
-                    const double o = I.off64[j][s * n + r];
-#pragma unroll
-                    for (int v = 0; v < kSweepBlock; ++v)
-                        if (c == c0 + v && c < r) a[u][v] = o;
-                }
-            }
+    const int nb = (n + B - 1) / B;
+    const int blocks = nb * (nb + 1) / 2;
+    const int t = static_cast<int>(threadIdx.x);
+    double* M = I.tri;
+    // position of entry (r, c) of the symmetric matrix
+    const auto at = [blocks](int r, int c) {
+        if (r < c) {
+            const int x = r;
+            r = c;
+            c = x;
+        }
+        const int bi = r / B, bj = c / B;
+        return ((r - bi * B) * B + (c - bj * B)) * blocks + bi * (bi + 1) / 2 + bj;
+    };
+    for (int e = t; e < blocks * B * B; e += kTile) M[e] = 0.0;
+    __syncthreads();
+    for (int r = t; r < n; r += kTile) {
+        M[at(r, r)] = I.diag64[j][r];
+        for (int s = 0; s < W; ++s) {
+            const int c = L.col[s * n + r];
+            if (c >= 0 && c < r) M[at(r, c)] = I.off64[j][s * n + r];
         }
     }
-    for (int k = 0; k < n; ++k) {
-        double* colk = lds.colk + (k & 1) * 256;
-        const int K = k / kSweepBlock, kk = k % kSweepBlock;
-        if (active) {
-#pragma unroll
-            for (int KK = 0; KK < kSweepBlock; ++KK) {
-                if (KK != kk) continue;
-                if (bj == K) {
-                    // rows >= 8K: column KK of the block (lower half on the diagonal)
-#pragma unroll
-                    for (int u = 0; u < kSweepBlock; ++u) colk[r0 + u] = (bi == K && u < KK) ? a[KK][u] : a[u][KK];
-                } else if (bi == K) {
-                    // rows < 8K: row KK of the block, by symmetry
-#pragma unroll
-                    for (int v = 0; v < kSweepBlock; ++v) colk[c0 + v] = a[KK][v];
-                }
-            }
+    __syncthreads();
+    constexpr int S = kPanelStride;
+    constexpr int kEntries = (kBlockedCoarse * kSweepBlock + kTile - 1) / kTile;   // panel entries per thread
+    double* A0 = lds.panel[2];
+    for (int K = 0; K < nb; ++K) {
+        const int k0 = K * B, width = min(B, n - k0);
+        // 1. the panel, original and working copy
+        for (int e = t; e < n * B; e += kTile) {
+            const int r = e / B, w = e - r * B;
+            const double value = w < width ? M[at(r, k0 + w)] : 0.0;
+            A0[r * S + w] = value;
+            lds.panel[0][r * S + w] = value;
         }
         __syncthreads();
-        if (active) {
-            // row scales are formed as they are used (rs = colk[r] / pivot), so
-            // only the block and the column values are live across the update
-            const double dinv = 1.0 / colk[k];
-            double cv[kSweepBlock];
+        // 2. the pivots of block K on the panel; each thread's entries are
+        // read first, then computed and written
+        int cur = 0;
+        for (int kk = 0; kk < width; ++kk) {
+            const int k = k0 + kk;
+            const double* Pc = lds.panel[cur];
+            double* Pn = lds.panel[cur ^ 1];
+            const double pivot = Pc[k * S + kk];
+            double rk[kEntries], ck[kEntries], rc[kEntries];
 #pragma unroll
-            for (int v = 0; v < kSweepBlock; ++v) cv[v] = colk[c0 + v];
-#pragma unroll
-            for (int u = 0; u < kSweepBlock; ++u) {
-                const double rs = colk[r0 + u] * dinv;
-#pragma unroll
-                for (int v = 0; v < kSweepBlock; ++v) a[u][v] = fma(-rs, cv[v], a[u][v]);
+            for (int m = 0; m < kEntries; ++m) {
+                const int e = t + m * kTile;
+                const int r = min(e / B, n - 1), w = e - (e / B) * B;
+                rk[m] = Pc[r * S + kk];
+                ck[m] = Pc[(k0 + min(w, width - 1)) * S + kk];
+                rc[m] = Pc[r * S + w];
             }
+            const double dinv = 1.0 / pivot;
 #pragma unroll
-            for (int KK = 0; KK < kSweepBlock; ++KK) {
-                if (KK != kk) continue;
-                if (bi == K) {
+            for (int m = 0; m < kEntries; ++m) {
+                const int e = t + m * kTile;
+                const int r = e / B, w = e - r * B;
+                if (r >= n || w >= width) continue;
+                const bool diagonal_block = r >= k0 && r < k0 + B;
+                if (diagonal_block && r - k0 < w) continue;   // upper entry: its mirror writes it
+                double value;
+                if (r == k && w == kk)
+                    value = -dinv;
+                else if (w == kk)
+                    value = rk[m] * dinv;   // column k
+                else if (r == k)
+                    value = ck[m] * dinv;   // row k (by symmetry)
+                else
+                    value = fma(-(rk[m] * dinv), ck[m], rc[m]);
+                Pn[r * S + w] = value;
+                if (diagonal_block) Pn[(k0 + w) * S + (r - k0)] = value;
+            }
+            __syncthreads();
+            cur ^= 1;
+        }
+        const double* P = lds.panel[cur];
+        // 3. every block outside the panel: M(r,c) -= sum_w P[r][w] A0[c][w]
+        // (rows and columns past n carry values no real entry reads, and the
+        // upper half of a diagonal block is computed but not stored)
+        for (int b = t; b < blocks; b += kTile) {
+            int bi = static_cast<int>((sqrtf(8.0f * static_cast<float>(b) + 1.0f) - 1.0f) * 0.5f);
+            if (bi * (bi + 1) / 2 > b) --bi;
+            if ((bi + 1) * (bi + 2) / 2 <= b) ++bi;
+            const int bj = b - bi * (bi + 1) / 2;
+            if (bi == K || bj == K) continue;
+            const int c0 = bj * B;
+#pragma unroll 1
+            for (int half = 0; half < 2; ++half) {
+                const int uh = half * (B / 2), u0 = bi * B + uh;
+                double a[B / 2][B];
 #pragma unroll
-                    for (int v = 0; v < kSweepBlock; ++v) a[KK][v] = cv[v] * dinv;
+                for (int u = 0; u < B / 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < B; ++v) a[u][v] = M[((uh + u) * B + v) * blocks + b];
+#pragma unroll 1
+                for (int w = 0; w < width; ++w) {
+                    double cw[B];
+#pragma unroll
+                    for (int v = 0; v < B; ++v) cw[v] = A0[(c0 + v) * S + w];
+#pragma unroll
+                    for (int u = 0; u < B / 2; ++u) {
+                        const double pr = P[(u0 + u) * S + w];
+#pragma unroll
+                        for (int v = 0; v < B; ++v) a[u][v] = fma(-pr, cw[v], a[u][v]);
+                    }
                 }
-                if (bj == K) {
 #pragma unroll
-                    for (int u = 0; u < kSweepBlock; ++u) a[u][KK] = colk[r0 + u] * dinv;
-                }
-                if (bi == K && bj == K) a[KK][KK] = -dinv;
+                for (int u = 0; u < B / 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < B; ++v)
+                        if (bi > bj || uh + u >= v) M[((uh + u) * B + v) * blocks + b] = a[u][v];
             }
         }
+        // 4. the swept panel back into M (a diagonal-block entry and its
+        // mirror write the same word with the same value)
+        for (int e = t; e < n * B; e += kTile) {
+            const int r = e / B, w = e - r * B;
+            if (w < width) M[at(r, k0 + w)] = P[r * S + w];
+        }
+        __syncthreads();
     }
-    if (active) {
-#pragma unroll
-        for (int u = 0; u < kSweepBlock; ++u)
-#pragma unroll
-            for (int v = 0; v < kSweepBlock; ++v) {
-                const int r = r0 + u, c = c0 + v;
-                if (r < n && c < n && (bi > bj || u >= v)) {
-                    I.cinv[c * n + r] = -a[u][v];
-                    I.cinv[r * n + c] = -a[u][v];
-                }
-            }
+    // I.cinv = -M in full: each stored entry (read in storage order) goes to
+    // both of its places
+    for (int e = t; e < blocks * B * B; e += kTile) {
+        const int uv = e / blocks, b = e - uv * blocks, u = uv / B, v = uv - u * B;
+        int bi = static_cast<int>((sqrtf(8.0f * static_cast<float>(b) + 1.0f) - 1.0f) * 0.5f);
+        if (bi * (bi + 1) / 2 > b) --bi;
+        if ((bi + 1) * (bi + 2) / 2 <= b) ++bi;
+        const int bj = b - bi * (bi + 1) / 2;
+        const int r = bi * B + u, c = bj * B + v;
+        if (r < n && c < n && r >= c) {
+            const double value = -M[e];
+            I.cinv[c * n + r] = value;
+            I.cinv[r * n + c] = value;
+        }
     }
 }
 
@@ -907,21 +983,11 @@ __device__ __forceinline__ void set_matrix(const Instance& I, const Launch& P, i
     }
     if (g == 0) {
         if (I.lv[J - 1].n <= kBlockedCoarse)
-            coarse_inverse_blocked(I, lds);
+            coarse_inverse_blocked(&I, lds);
         else
             coarse_inverse(I, I.tri_lds ? dynamic_lds() : I.tri, lds);
     }
     group_sync(timeouts);
-}
-
-// A pointer every lane holds, rebuilt in scalar registers: descriptor reads
-// through it are scalar loads (a function argument arrives in vector
-// registers otherwise).
-template <class T> __device__ inline const T* uniform_ptr(const T* p) {
-    const uint64_t bits = reinterpret_cast<uint64_t>(p);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits));
-    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits >> 32));
-    return reinterpret_cast<const T*>((static_cast<uint64_t>(hi) << 32) | lo);
 }
 
 // set_matrix out of line (its register-blocked inverse would otherwise add
