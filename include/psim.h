@@ -346,6 +346,40 @@ int psim_implicit_transport_table(psim_graph* graph, double conductance_W_K, con
                                   const double* latent_W_m2, double* source_W_m2, double* mean_K,
                                   psim_transport_result* result, int where);
 
+/* ------------------------------------------------------- block ILU(0)
+ * IncompleteLU (sim/core/math/incomplete_lu.hpp:14-43, .cpp:40-146), block
+ * form, on the device: rows [block_start[b], block_start[b+1]) keep their
+ * diagonal and their in-block off-diagonals of the CSR pattern
+ * (offset[n+1], neighbour[offset[n]]) and are factored on their own.
+ * Factors and solves are bit-identical to the reference's. One workgroup
+ * per block walks its rows' level schedule. */
+typedef struct psim_ilu psim_ilu;
+int psim_ilu_create(psim_ctx* ctx, uint64_t n, const uint64_t* offset, const uint64_t* neighbour,
+                    const uint64_t* block_start, uint64_t blocks, psim_ilu** out);
+void psim_ilu_destroy(psim_ilu* ilu);
+/* Factor diag x diagonal_scale + off (CSR order). PSIM_RUNTIME_ERROR
+ * "incomplete LU: zero pivot" as the reference throws. */
+int psim_ilu_factor(psim_ilu* ilu, const double* diagonal, const double* off, double diagonal_scale, int where);
+/* x <- (LU)^-1 x. */
+int psim_ilu_solve(psim_ilu* ilu, double* x, int where);
+
+/* solve_implicit_tracer (heat_transport.hpp:210-239, heat_transport.cpp:
+ * 901-1081): a tracer's backward-Euler step under layer mass fluxes and
+ * eddy exchange on the graph, the stacked system solved on the device by
+ * BiCGSTAB with block ILU(0) (blocks of 256 whole columns). Host arrays:
+ * conductance_W_K[E], outflow_kg_s[layers x E] (per CSR entry, layer-
+ * major), layer_mass_kg[n], tracer[layers x n] in, tracer_out[layers x n]. */
+typedef struct {
+    int32_t iterations, reserved;
+    double relative_residual;        /* of the linear solve */
+    double clipped_kg;               /* mass added where rounding left c' < 0 */
+    double column_divergence_kg_s;   /* max over nodes of |sum_l sum_k F| */
+} psim_tracer_result;
+int psim_tracer_step(psim_graph* graph, uint32_t layers, const double* conductance_W_K, const double* outflow_kg_s,
+                     double heat_capacity_J_kg_K, const double* layer_mass_kg, const double* tracer, double dt_s,
+                     double relative_tolerance, int32_t max_iterations, double* tracer_out,
+                     psim_tracer_result* result);
+
 /* ---- Batches (no counterpart in the reference, whose solves run one at a
  * time): many systems, of any graphs and precisions on one context, solved
  * in ONE launch of the persistent solver. Each system owns a group of

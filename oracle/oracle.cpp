@@ -180,12 +180,12 @@ Pool& pool() {
 
 constexpr size_t block_cells = 256;
 
-template <class Body> void each_block(size_t n, size_t workers, const Body& body) {
+template <class Body> void each_block(size_t n, size_t workers, const Body& body, size_t block = block_cells) {
     if (workers == 0) throw std::invalid_argument("worker count must be positive");
-    const size_t blocks = (n + block_cells - 1) / block_cells;
+    const size_t blocks = (n + block - 1) / block;
     const size_t active = std::min(workers, blocks);
     const auto run_block = [&](size_t b) {
-        body(b, b * block_cells, std::min(n, (b + 1) * block_cells));
+        body(b, b * block, std::min(n, (b + 1) * block));
     };
     if (active <= 1) {
         for (size_t b = 0; b < blocks; ++b) run_block(b);
@@ -196,12 +196,11 @@ template <class Body> void each_block(size_t n, size_t workers, const Body& body
     });
 }
 
-template <class Body> double block_sum(size_t n, size_t workers, const Body& partial) {
-    const size_t blocks = (n + block_cells - 1) / block_cells;
+template <class Body> double block_sum(size_t n, size_t workers, const Body& partial, size_t block = block_cells) {
+    const size_t blocks = (n + block - 1) / block;
     std::vector<double> partials(blocks, 0.0);
-    each_block(n, workers, [&](size_t b, size_t begin, size_t end) {
-        partials[b] = partial(begin, end);
-    });
+    each_block(
+        n, workers, [&](size_t b, size_t begin, size_t end) { partials[b] = partial(begin, end); }, block);
     double total = 0.0;
     for (const double p : partials) total += p;
     return total;
@@ -722,13 +721,17 @@ struct Multigrid {
 // block-ordered dot products, half-step and full-step stopping tests.
 template <class Apply, class Prec>
 int bicgstab(size_t n, size_t workers, const Apply& apply, const Prec& prec,
-             const std::vector<double>& b, std::vector<double>& x, double tol, int max_it) {
+             const std::vector<double>& b, std::vector<double>& x, double tol, int max_it,
+             size_t block = block_cells) {
     const auto dot = [&](const std::vector<double>& u, const std::vector<double>& v) {
-        return block_sum(n, workers, [&](size_t begin, size_t end) {
-            double s = 0.0;
-            for (size_t i = begin; i < end; ++i) s += u[i] * v[i];
-            return s;
-        });
+        return block_sum(
+            n, workers,
+            [&](size_t begin, size_t end) {
+                double s = 0.0;
+                for (size_t i = begin; i < end; ++i) s += u[i] * v[i];
+                return s;
+            },
+            block);
     };
     x.assign(n, 0.0);
     std::vector<double> r = b;
@@ -766,6 +769,208 @@ int bicgstab(size_t n, size_t workers, const Apply& apply, const Prec& prec,
         if (std::sqrt(dot(r, r)) <= tol * bnorm) break;
     }
     return it;
+}
+
+// ------------------------------------------------------------ block ILU(0)
+// incomplete_lu.cpp:40-146. Each row keeps its (scaled) diagonal and the
+// off-diagonals whose column lies in its own block, sorted by column; the
+// blocks are factored on their own by IKJ elimination restricted to that
+// pattern, in row order; the solve is forward (unit lower) then backward
+// substitution, both in the stored order of a row's entries.
+struct Ilu {
+    std::vector<size_t> block_start, start, column, diag;
+    std::vector<double> value;
+};
+
+Ilu ilu_build(size_t n, const uint64_t* offset, const uint64_t* neighbour, const double* diagonal, const double* off,
+              const std::vector<size_t>& block_start, double scale) {
+    if (block_start.size() < 2 || block_start.front() != 0 || block_start.back() != n)
+        throw std::invalid_argument("incomplete LU: the pattern does not match");
+    Ilu f;
+    f.block_start = block_start;
+    std::vector<size_t> block_of(n);
+    for (size_t b = 0; b + 1 < block_start.size(); ++b)
+        for (size_t i = block_start[b]; i < block_start[b + 1]; ++i) block_of[i] = b;
+    f.start.assign(n + 1, 0);
+    f.diag.assign(n, 0);
+    std::vector<std::pair<size_t, double>> row;
+    for (size_t i = 0; i < n; ++i) {
+        row.assign(1, {i, diagonal[i] * scale});
+        for (size_t k = offset[i]; k < offset[i + 1]; ++k)
+            if (block_of[neighbour[k]] == block_of[i]) row.push_back({neighbour[k], off[k]});
+        std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        for (const auto& [c, v] : row) {
+            if (c == i) f.diag[i] = f.column.size();
+            f.column.push_back(c);
+            f.value.push_back(v);
+        }
+        f.start[i + 1] = f.column.size();
+    }
+    std::vector<size_t> where(n, SIZE_MAX);
+    for (size_t b = 0; b + 1 < block_start.size(); ++b) {
+        for (size_t i = block_start[b]; i < block_start[b + 1]; ++i) {
+            for (size_t q = f.start[i]; q < f.start[i + 1]; ++q) where[f.column[q]] = q;
+            for (size_t q = f.start[i]; q < f.diag[i]; ++q) {
+                const size_t j = f.column[q];
+                const double pivot = f.value[f.diag[j]];
+                if (pivot == 0.0) throw std::runtime_error("incomplete LU: zero pivot");
+                f.value[q] /= pivot;
+                for (size_t t = f.diag[j] + 1; t < f.start[j + 1]; ++t)
+                    if (where[f.column[t]] != SIZE_MAX) f.value[where[f.column[t]]] -= f.value[q] * f.value[t];
+            }
+            for (size_t q = f.start[i]; q < f.start[i + 1]; ++q) where[f.column[q]] = SIZE_MAX;
+        }
+    }
+    return f;
+}
+
+void ilu_solve(const Ilu& f, double* x) {
+    for (size_t b = 0; b + 1 < f.block_start.size(); ++b) {
+        const size_t first = f.block_start[b], last = f.block_start[b + 1];
+        for (size_t i = first; i < last; ++i) {
+            double sum = x[i];
+            for (size_t q = f.start[i]; q < f.diag[i]; ++q) sum -= f.value[q] * x[f.column[q]];
+            x[i] = sum;
+        }
+        for (size_t i = last; i-- > first;) {
+            double sum = x[i];
+            for (size_t q = f.diag[i] + 1; q < f.start[i + 1]; ++q) sum -= f.value[q] * x[f.column[q]];
+            x[i] = sum / f.value[f.diag[i]];
+        }
+    }
+}
+
+// ------------------------------------------------------------ tracer step
+// heat_transport.cpp:901-1081 (solve_implicit_tracer): vertical fluxes that
+// close each layer's horizontal divergence, the stacked node-major backward
+// Euler system (horizontal entries, then below, then above), block ILU(0)
+// with blocks of whole columns, BiCGSTAB on the change from the start, and
+// the flux form of the solution with clipping.
+struct TracerOut {
+    int iterations = 0;
+    double relative_residual = 0.0, clipped_kg = 0.0, column_divergence_kg_s = 0.0;
+};
+
+TracerOut tracer_step(const Mesh& m, size_t layers, const double* conductance, const double* outflow,
+                      double heat_capacity, const double* layer_mass, const double* tracer, double dt, double tol,
+                      int max_it, size_t workers, double* out) {
+    const size_t nodes = m.n, entries = m.ce_neighbor.size(), L = layers;
+    if (L == 0 || !(dt > 0.0)) throw std::invalid_argument("tracer transport inputs do not match the graph");
+    const double exchange = 1.0 / (heat_capacity * static_cast<double>(L));
+    TracerOut res;
+    std::vector<double> vertical(nodes * L, 0.0);
+    for (size_t a = 0; a < nodes; ++a) {
+        double w = 0.0;
+        for (size_t l = 0; l < L; ++l) {
+            for (size_t k = m.g_offset[a]; k < m.g_offset[a + 1]; ++k) w -= outflow[l * entries + k];
+            vertical[a * L + l] = l + 1 < L ? w : 0.0;
+        }
+        res.column_divergence_kg_s = std::max(res.column_divergence_kg_s, std::abs(w));
+    }
+    const size_t rows = nodes * L;
+    std::vector<uint64_t> soff(rows + 1, 0), snb;
+    for (size_t a = 0; a < nodes; ++a)
+        for (size_t l = 0; l < L; ++l) {
+            const size_t row = a * L + l;
+            for (size_t k = m.g_offset[a]; k < m.g_offset[a + 1]; ++k) snb.push_back(m.ce_neighbor[k] * L + l);
+            if (l > 0) snb.push_back(row - 1);
+            if (l + 1 < L) snb.push_back(row + 1);
+            soff[row + 1] = snb.size();
+        }
+    std::vector<double> diag(rows, 0.0), off(snb.size(), 0.0), rhs(rows, 0.0);
+    for (size_t a = 0; a < nodes; ++a) {
+        const double rate = layer_mass[a] / dt;
+        for (size_t l = 0; l < L; ++l) {
+            const size_t row = a * L + l;
+            double own = rate;
+            size_t at = soff[row];
+            for (size_t k = m.g_offset[a]; k < m.g_offset[a + 1]; ++k, ++at) {
+                const double flow = outflow[l * entries + k];
+                const double eddy = conductance[k] * exchange;
+                own += std::max(flow, 0.0) + eddy;
+                off[at] = -std::max(-flow, 0.0) - eddy;
+            }
+            const double below = l > 0 ? vertical[a * L + l - 1] : 0.0;
+            const double above = vertical[row];
+            if (l > 0) {
+                own += std::max(-below, 0.0);
+                off[at++] = -std::max(below, 0.0);
+            }
+            if (l + 1 < L) {
+                own += std::max(above, 0.0);
+                off[at++] = -std::max(-above, 0.0);
+            }
+            diag[row] = own;
+            rhs[row] = rate * tracer[l * nodes + a];
+        }
+    }
+    std::vector<size_t> block_start{0};
+    for (size_t b = 0; b * block_cells < nodes; ++b) block_start.push_back(std::min(nodes, (b + 1) * block_cells) * L);
+    const Ilu f = ilu_build(rows, soff.data(), snb.data(), diag.data(), off.data(), block_start, 1.0);
+    const size_t block = block_cells * L;
+    const auto apply = [&](const std::vector<double>& x, std::vector<double>& y) {
+        each_block(
+            rows, workers,
+            [&](size_t, size_t begin, size_t end) {
+                for (size_t row = begin; row < end; ++row) {
+                    double sum = diag[row] * x[row];
+                    for (size_t k = soff[row]; k < soff[row + 1]; ++k) sum += off[k] * x[snb[k]];
+                    y[row] = sum;
+                }
+            },
+            block);
+    };
+    const auto prec = [&](const std::vector<double>& x, std::vector<double>& y) {
+        y = x;
+        ilu_solve(f, y.data());
+    };
+    std::vector<double> start(rows, 0.0), applied(rows, 0.0), residual(rows, 0.0), delta;
+    for (size_t a = 0; a < nodes; ++a)
+        for (size_t l = 0; l < L; ++l) start[a * L + l] = tracer[l * nodes + a];
+    apply(start, applied);
+    for (size_t row = 0; row < rows; ++row) residual[row] = rhs[row] - applied[row];
+    res.iterations = bicgstab(rows, workers, apply, prec, residual, delta, tol, max_it, block);
+    std::vector<double> solved(rows, 0.0);
+    for (size_t row = 0; row < rows; ++row) solved[row] = std::max(0.0, start[row] + (delta.empty() ? 0.0 : delta[row]));
+    apply(solved, applied);
+    double rn = 0.0, bn = 0.0;
+    for (size_t row = 0; row < rows; ++row) {
+        rn += (rhs[row] - applied[row]) * (rhs[row] - applied[row]);
+        bn += rhs[row] * rhs[row];
+    }
+    res.relative_residual = bn > 0.0 ? std::sqrt(rn / bn) : 0.0;
+    std::vector<double> change(rows, 0.0);
+    for (size_t a = 0; a < nodes; ++a)
+        for (size_t l = 0; l < L; ++l) {
+            const size_t row = a * L + l;
+            for (size_t k = m.g_offset[a]; k < m.g_offset[a + 1]; ++k) {
+                const size_t b = m.ce_neighbor[k];
+                if (b < a) continue;
+                const size_t other = b * L + l;
+                const double flow = outflow[l * entries + k];
+                const double eddy = conductance[k] * exchange;
+                const double flux =
+                    (flow > 0.0 ? flow * solved[row] : flow * solved[other]) + eddy * (solved[row] - solved[other]);
+                change[row] -= flux;
+                change[other] += flux;
+            }
+            if (l + 1 < L) {
+                const double w = vertical[row];
+                const double flux = w > 0.0 ? w * solved[row] : w * solved[row + 1];
+                change[row] -= flux;
+                change[row + 1] += flux;
+            }
+        }
+    for (size_t a = 0; a < nodes; ++a)
+        for (size_t l = 0; l < L; ++l) {
+            double value = tracer[l * nodes + a] + dt * change[a * L + l] / layer_mass[a];
+            if (value < 0.0) {
+                res.clipped_kg -= value * layer_mass[a];
+                value = 0.0;
+            }
+            out[l * nodes + a] = value;
+        }
+    return res;
 }
 
 // CSR apply as heat_transport.cpp:1204-1214: diag·x, then entries in order.
@@ -1378,6 +1583,34 @@ int orc_group_max(uint64_t cells, uint64_t groups, const uint32_t* group_of_cell
 void* orc_agglomerate(const void* h, uint32_t* group_of_cell) {
     Mesh* out = nullptr;
     return guard([&] { out = agglomerated(M(h), group_of_cell); }) == 0 ? out : nullptr;
+}
+
+void* orc_ilu_create(uint64_t n, const uint64_t* offset, const uint64_t* neighbour, const double* diagonal,
+                     const double* off, const uint64_t* block_start, uint64_t blocks, double scale) {
+    Ilu* out = nullptr;
+    return guard([&] {
+               std::vector<size_t> bs(block_start, block_start + blocks + 1);
+               out = new Ilu(ilu_build(n, offset, neighbour, diagonal, off, bs, scale));
+           }) == 0
+               ? out
+               : nullptr;
+}
+void orc_ilu_free(void* f) { delete static_cast<Ilu*>(f); }
+int orc_ilu_solve(const void* f, double* x) {
+    return guard([&] { ilu_solve(*static_cast<const Ilu*>(f), x); });
+}
+
+int orc_tracer_step(const void* h, uint32_t layers, const double* conductance, const double* outflow,
+                    double heat_capacity, const double* layer_mass, const double* tracer, double dt, double tol,
+                    int max_it, uint64_t workers, double* out, orc_tracer_result* result) {
+    return guard([&] {
+        const TracerOut r = tracer_step(M(h), layers, conductance, outflow, heat_capacity, layer_mass, tracer, dt, tol,
+                                        max_it, workers, out);
+        result->iterations = r.iterations;
+        result->relative_residual = r.relative_residual;
+        result->clipped_kg = r.clipped_kg;
+        result->column_divergence_kg_s = r.column_divergence_kg_s;
+    });
 }
 
 }  // extern "C"

@@ -16,6 +16,8 @@
  *   diffusion      sim/planet/surface/heat_transport.cpp:28-36,579-593
  *   agglomeration  sim/planet/surface/heat_transport.cpp:467-512
  *   group maps     sim/planet/surface/surface_energy.cpp:1285-1300,1306-1324
+ *   block ILU(0)   sim/core/math/incomplete_lu.cpp:40-146
+ *   tracer step    sim/planet/surface/heat_transport.cpp:901-1081
  *   RNG            sim/core/random/counter_rng.hpp:19-45
  * and the synthetic heat-step workload (this project's input generator).
  *
@@ -129,6 +131,25 @@ int orc_group_mean(uint64_t cells, uint64_t groups, const uint32_t* group_of_cel
                    const double* group_area, const double* cell_values, double* group_mean);
 int orc_group_max(uint64_t cells, uint64_t groups, const uint32_t* group_of_cell, const double* cell_values,
                   double* group_max);
+
+/* Block ILU(0) (incomplete_lu.cpp:40-146): rows [block_start[b],
+ * block_start[b+1]) factored on their own, diagonal scaled by `scale`;
+ * orc_ilu_solve: x <- (LU)^-1 x. NULL / non-zero on error (zero pivot). */
+void* orc_ilu_create(uint64_t n, const uint64_t* offset, const uint64_t* neighbour, const double* diagonal,
+                     const double* off, const uint64_t* block_start, uint64_t blocks, double scale);
+void orc_ilu_free(void* ilu);
+int orc_ilu_solve(const void* ilu, double* x);
+
+/* solve_implicit_tracer (heat_transport.cpp:901-1081) on the mesh's graph:
+ * conductance[E] (W/K), outflow[layers x E] (kg/s), layer_mass[n] (kg),
+ * tracer[layers x n] (layer-major in and out). */
+typedef struct {
+    int iterations, pad;
+    double relative_residual, clipped_kg, column_divergence_kg_s;
+} orc_tracer_result;
+int orc_tracer_step(const void* mesh, uint32_t layers, const double* conductance, const double* outflow,
+                    double heat_capacity, const double* layer_mass, const double* tracer, double dt, double tol,
+                    int max_iterations, uint64_t workers, double* out, orc_tracer_result* result);
 
 /* Synthetic heat-step workload (the project's seeded input generator).
  *   instance_seed = orc_mix_random_key(base_seed ^ instance)

@@ -12,6 +12,7 @@
 
 #include "sim/planet/surface/heat_transport.cpp"  // NOLINT: see header comment
 
+#include "sim/core/math/incomplete_lu.hpp"
 #include "sim/core/random/counter_rng.hpp"
 #include "sim/planet/coordinates/local_tangent_basis.hpp"
 #include "sim/planet/mesh/icosphere.hpp"
@@ -474,6 +475,54 @@ int ref_agglomeration(const void* handle, std::uint64_t* group_of_cell, double* 
             weight[k] = graph.weight[k];
         }
         *subdivision = planetsim::agglomerated_mesh(mesh).subdivision();
+    });
+}
+
+// IncompleteLU (incomplete_lu.hpp:14-43), block form.
+void* ref_ilu_create(std::uint64_t n, const std::uint64_t* offset, const std::uint64_t* neighbour,
+                     const double* diagonal, const double* off, const std::uint64_t* block_start,
+                     std::uint64_t blocks, double scale) {
+    planetsim::IncompleteLU* out = nullptr;
+    const int status = guarded([&] {
+        const std::vector<std::size_t> o(offset, offset + n + 1), nb(neighbour, neighbour + offset[n]),
+            bs(block_start, block_start + blocks + 1);
+        out = new planetsim::IncompleteLU(o, nb, std::span<const double>(diagonal, n),
+                                          std::span<const double>(off, offset[n]), bs, scale, 1U);
+    });
+    return status == 0 ? out : nullptr;
+}
+
+void ref_ilu_free(void* f) { delete static_cast<planetsim::IncompleteLU*>(f); }
+
+int ref_ilu_solve(const void* f, double* x, std::uint64_t n) {
+    return guarded([&] {
+        std::vector<double> v(x, x + n);
+        static_cast<const planetsim::IncompleteLU*>(f)->solve(v);
+        std::copy(v.begin(), v.end(), x);
+    });
+}
+
+// solve_implicit_tracer (heat_transport.hpp:227-239) on the mesh's graph.
+int ref_tracer_step(const void* handle, std::uint32_t layers, const double* conductance, const double* outflow,
+                    double heat_capacity, const double* layer_mass, const double* tracer, double dt, double tol,
+                    int max_iterations, std::uint64_t workers, double* out, int* iterations,
+                    double* relative_residual, double* clipped_kg, double* column_divergence_kg_s) {
+    return guarded([&] {
+        const auto& graph = static_cast<const RefMesh*>(handle)->graph;
+        const std::size_t n = graph.size(), e = graph.neighbour.size();
+        planetsim::AdvectionDiffusion transport;
+        transport.layers = layers;
+        transport.conductance_W_K.assign(conductance, conductance + e);
+        transport.outflow_kg_s.assign(outflow, outflow + layers * e);
+        transport.heat_capacity_J_kg_K = heat_capacity;
+        const std::vector<double> c(tracer, tracer + layers * n);
+        const auto r = planetsim::solve_implicit_tracer(graph, transport, std::span<const double>(layer_mass, n), c,
+                                                        dt, tol, max_iterations, workers);
+        std::copy(r.tracer.begin(), r.tracer.end(), out);
+        *iterations = r.iterations;
+        *relative_residual = r.relative_residual;
+        *clipped_kg = r.clipped_kg;
+        *column_divergence_kg_s = r.column_divergence_kg_s;
     });
 }
 

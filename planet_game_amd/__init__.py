@@ -22,7 +22,8 @@ __all__ = [
     "HOST", "DEVICE", "F64", "F32", "PLAIN_FP64", "MIXED", "instance_seed", "heat_instance",
     "heat_forcing", "Hierarchy", "Batch", "DeviceArray", "Server", "ServeRequest", "ServeRecord",
     "implicit_transport", "TransportSettings", "TransportResult", "Agglomeration", "Groups",
-    "ResponseTable", "implicit_transport_table", "implicit_transport_table_device",
+    "ResponseTable", "implicit_transport_table", "implicit_transport_table_device", "Ilu", "TracerResult",
+    "tracer_step",
 ]
 
 # PSIM_LIBRARY names another build of the same ABI (e.g. a tuning variant).
@@ -46,7 +47,7 @@ def _track(obj):
 
 @atexit.register
 def _release_all():
-    for name in ("Server", "Batch", "System", "Groups", "FV", "Graph", "DeviceArray", "Context", "Agglomeration",
+    for name in ("Server", "Batch", "System", "Groups", "Ilu", "FV", "Graph", "DeviceArray", "Context", "Agglomeration",
                  "Mesh"):
         for obj in list(_LIVE.get(name, ())):
             try:
@@ -137,6 +138,10 @@ _sig("psim_graph_size", _u64, _vp)
 _sig("psim_graph_entries", _u64, _vp)
 _sig("psim_graph_depth", _u64, _vp)
 _sig("psim_diffusion_source", C.c_int, _vp, _dbl, _vp, _vp, C.c_int)
+_sig("psim_ilu_create", C.c_int, _vp, _u64, _pu64, _pu64, _pu64, _u64, C.POINTER(_vp))
+_sig("psim_ilu_destroy", None, _vp)
+_sig("psim_ilu_factor", C.c_int, _vp, _vp, _vp, _dbl, C.c_int)
+_sig("psim_ilu_solve", C.c_int, _vp, _vp, C.c_int)
 _sig("psim_groups_create", C.c_int, _vp, _u64, _u64, _pu32, _pd, _pd, C.POINTER(_vp))
 _sig("psim_groups_destroy", None, _vp)
 for _name in ("psim_groups_mean", "psim_groups_max", "psim_groups_broadcast"):
@@ -905,3 +910,57 @@ def implicit_transport_table_device(graph: Graph, conductance, floor: "DeviceArr
                                             C.byref(_settings(settings)), None if latent is None else latent.ptr,
                                             source_out.ptr, mean_out.ptr, C.byref(res), DEVICE))
     return res
+
+
+class Ilu:
+    """psim_ilu: block ILU(0) of a CSR pattern on the device."""
+
+    def __init__(self, ctx: Context, offset, neighbour, block_start):
+        o = np.ascontiguousarray(offset, np.uint64)
+        nb = np.ascontiguousarray(neighbour, np.uint64)
+        bs = np.ascontiguousarray(block_start, np.uint64)
+        h = _vp()
+        check(lib.psim_ilu_create(ctx.handle, o.size - 1, o, nb, bs, bs.size - 1, C.byref(h)))
+        self.handle, self.ctx, self.n = h, ctx, o.size - 1
+        _track(self)
+
+    def factor(self, diagonal, off, scale=1.0):
+        d = np.ascontiguousarray(diagonal, np.float64)
+        o = np.ascontiguousarray(off, np.float64)
+        check(lib.psim_ilu_factor(self.handle, _ptr(d), _ptr(o), scale, HOST))
+
+    def solve(self, x):
+        y = np.ascontiguousarray(x, np.float64).copy()
+        check(lib.psim_ilu_solve(self.handle, _ptr(y), HOST))
+        return y
+
+    def close(self):
+        """Release the native object now (also done at exit and on collection)."""
+        if getattr(self, "handle", None):
+            lib.psim_ilu_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+class TracerResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("reserved", C.c_int32), ("relative_residual", C.c_double),
+                ("clipped_kg", C.c_double), ("column_divergence_kg_s", C.c_double)]
+
+
+_sig("psim_tracer_step", C.c_int, _vp, C.c_uint32, _pd, _pd, _dbl, _pd, _pd, _dbl, _dbl, _i32, _pd,
+     C.POINTER(TracerResult))
+
+
+def tracer_step(graph: Graph, layers, conductance, outflow, layer_mass, tracer, dt, heat_capacity=1004.64,
+                tolerance=1e-12, max_iterations=2000):
+    """psim_tracer_step (solve_implicit_tracer): returns (tracer_out, TracerResult)."""
+    t = np.ascontiguousarray(tracer, np.float64)
+    out = np.zeros_like(t)
+    res = TracerResult()
+    check(lib.psim_tracer_step(graph.handle, layers, np.ascontiguousarray(conductance, np.float64),
+                               np.ascontiguousarray(outflow, np.float64), heat_capacity,
+                               np.ascontiguousarray(layer_mass, np.float64), t, dt, tolerance, max_iterations, out,
+                               C.byref(res)))
+    return out, res

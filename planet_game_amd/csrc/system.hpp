@@ -61,6 +61,7 @@ struct psim_graph {
     // vectors, made on first use; calls on the graph hold newton_mutex
     psim_system* newton = nullptr;
     std::shared_ptr<void> transport_work;
+    std::shared_ptr<void> tracer_work;   // stacked tracer systems by layer count
     std::mutex newton_mutex;
     ~psim_graph() {
         if (newton != nullptr) psim_system_destroy(newton);

@@ -182,6 +182,36 @@ def response_table(n, points, seed):
     return np.ascontiguousarray(table_h), np.ascontiguousarray(table_t), table_h[0].copy()
 
 
+class TracerResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("pad", C.c_int), ("relative_residual", C.c_double),
+                ("clipped_kg", C.c_double), ("column_divergence_kg_s", C.c_double)]
+
+
+def tracer_case(m, layers, seed, heat_capacity=1004.64):
+    """Synthetic tracer-step inputs on a mesh's graph: per-edge layer mass
+    fluxes (antisymmetric per entry), symmetric eddy conductances, layer
+    masses from the cell areas, positive mixing ratios."""
+    rng = np.random.default_rng(seed)
+    edges = m.n_edges
+    sign = np.where(m.first[m.edge] == np.repeat(np.arange(m.n), np.diff(m.offset.astype(np.int64))), 1.0, -1.0)
+    flux_e = rng.standard_normal((layers, edges)) * 2.0e7
+    outflow = np.ascontiguousarray((flux_e[:, m.edge] * sign[None, :]).ravel())
+    conductance = np.ascontiguousarray(rng.uniform(0.5, 1.5, edges)[m.edge] * 3.0e9)
+    layer_mass = np.ascontiguousarray(m.area * 1.0e4)
+    tracer = np.ascontiguousarray(rng.uniform(0.0, 2.0e-3, layers * m.n))
+    return dict(layers=layers, conductance=conductance, outflow=outflow, heat_capacity=heat_capacity,
+                layer_mass=layer_mass, tracer=tracer, dt=2.63e6)
+
+
+def oracle_tracer(m, case, tol=1e-12, max_it=2000):
+    out = np.zeros_like(case["tracer"])
+    res = TracerResult()
+    oracle().check(oracle().tracer_step(m.handle, case["layers"], case["conductance"], case["outflow"],
+                                        case["heat_capacity"], case["layer_mass"], case["tracer"], case["dt"], tol,
+                                        max_it, 1, out, C.byref(res)))
+    return out, res
+
+
 class Lib:
     """One of the two CPU libraries (oracle or reference shim)."""
 
@@ -213,11 +243,30 @@ class Lib:
             f.orc_group_mean.argtypes = [_u64, _u64, _u32p, _dp, _dp, _dp, _dp]
             f.orc_group_max.restype = C.c_int
             f.orc_group_max.argtypes = [_u64, _u64, _u32p, _dp, _dp]
+            f.orc_ilu_create.restype = _vp
+            f.orc_ilu_create.argtypes = [_u64, _u64p, _u64p, _dp, _dp, _u64p, _u64, C.c_double]
+            f.orc_ilu_free.restype = None
+            f.orc_ilu_free.argtypes = [_vp]
+            f.orc_ilu_solve.restype = C.c_int
+            f.orc_ilu_solve.argtypes = [_vp, _dp]
+            f.orc_tracer_step.restype = C.c_int
+            f.orc_tracer_step.argtypes = [_vp, C.c_uint32, _dp, _dp, C.c_double, _dp, _dp, C.c_double, C.c_double,
+                                          C.c_int, _u64, _dp, C.POINTER(TracerResult)]
         else:
             f = self.lib
             f.ref_coupled_linear_solve.restype = C.c_int
             f.ref_coupled_linear_solve.argtypes = [_vp, _dp, _dp, _dp, C.c_double, C.c_double, _dp,
                                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]
+            f.ref_ilu_create.restype = _vp
+            f.ref_ilu_create.argtypes = [_u64, _u64p, _u64p, _dp, _dp, _u64p, _u64, C.c_double]
+            f.ref_ilu_free.restype = None
+            f.ref_ilu_free.argtypes = [_vp]
+            f.ref_ilu_solve.restype = C.c_int
+            f.ref_ilu_solve.argtypes = [_vp, _dp, _u64]
+            _i, _d = C.POINTER(C.c_int), C.POINTER(C.c_double)
+            f.ref_tracer_step.restype = C.c_int
+            f.ref_tracer_step.argtypes = [_vp, C.c_uint32, _dp, _dp, C.c_double, _dp, _dp, C.c_double, C.c_double,
+                                          C.c_int, _u64, _dp, _i, _d, _d, _d]
             f.ref_agglomeration_groups.restype = _u64
             f.ref_agglomeration_groups.argtypes = [_vp]
             f.ref_agglomeration.restype = C.c_int
