@@ -261,6 +261,13 @@ int psim_system_solve(psim_system* system, const double* b, double* x, double to
  * and x stays exactly 0 on them. info->iterations counts CG iterations. */
 int psim_system_pcg(psim_system* system, const double* b, const uint8_t* hold, double* x, double tolerance,
                     int32_t max_iterations, psim_solve_info* info, int where);
+/* The same PCG where the operator's diagonal differs from the V-cycle's
+ * matrix (off-diagonals shared): pressure_redistribution.cpp:48-58,150-178
+ * solves the two-point Laplacian with the V-cycle of its regularised form
+ * (diagonal x (1 + 1e-10)). operator_diagonal NULL: the system's own. */
+int psim_system_pcg_operator(psim_system* system, const double* operator_diagonal, const double* b,
+                             const uint8_t* hold, double* x, double tolerance, int32_t max_iterations,
+                             psim_solve_info* info, int where);
 
 /* ------------------------------------------------ implicit heat step
  * The benchmark's unit of work: (diag(C_i A_i) + dt L_K) T = C_i A_i T_old
@@ -379,6 +386,38 @@ int psim_tracer_step(psim_graph* graph, uint32_t layers, const double* conductan
                      double heat_capacity_J_kg_K, const double* layer_mass_kg, const double* tracer, double dt_s,
                      double relative_tolerance, int32_t max_iterations, double* tracer_out,
                      psim_tracer_result* result);
+
+/* ----------------------------------------- operator solves (other consumers)
+ * BiCGSTAB on an operator that differs from the multigrid's matrix: a
+ * diagonal and off-diagonals in the graph's CSR order (host or device
+ * arrays by `where`). Preconditioner (right): the V-cycle of `multigrid`
+ * (a plain FP64 system of the same graph whose matrix was set, typically
+ * the operator's symmetric part), its input multiplied by pre_scale and
+ * its output divided by post_divide when given, then the block ILU(0) of
+ * the operator (`ilu`, factored) applied to what the V-cycle leaves:
+ *   out = M in;  out += ILU^-1 (in - A out).
+ * Without a multigrid the preconditioner is the ILU(0) alone.
+ *   PSIM_FORM_BALANCE    balanced_circulation.cpp:316-444: iteration cap
+ *                        -> status PSIM_NOT_CONVERGED, breakdown ->
+ *                        PSIM_RUNTIME_ERROR (the reference throws both);
+ *   PSIM_FORM_TRANSPORT  heat_transport.cpp:820-897 (the advection-
+ *                        diffusion Newton system, :1176-1226): stops at the
+ *                        cap or a breakdown.
+ * info->relative_residual is the true ||b - A x|| / ||b||. */
+enum psim_solve_form { PSIM_FORM_TRANSPORT = 0, PSIM_FORM_BALANCE = 1 };
+typedef struct {
+    psim_system* multigrid;      /* or NULL */
+    const double* pre_scale;     /* n, or NULL */
+    const double* post_divide;   /* n, or NULL */
+    psim_ilu* ilu;               /* or NULL */
+} psim_preconditioner;
+typedef struct {
+    int32_t iterations, status;
+    double relative_residual;
+} psim_operator_info;
+int psim_operator_solve(psim_graph* graph, const double* diagonal, const double* off,
+                        const psim_preconditioner* preconditioner, int form, const double* b, double* x,
+                        double relative_tolerance, int32_t max_iterations, psim_operator_info* info, int where);
 
 /* ---- Batches (no counterpart in the reference, whose solves run one at a
  * time): many systems, of any graphs and precisions on one context, solved

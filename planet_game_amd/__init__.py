@@ -23,7 +23,8 @@ __all__ = [
     "heat_forcing", "Hierarchy", "Batch", "DeviceArray", "Server", "ServeRequest", "ServeRecord",
     "implicit_transport", "TransportSettings", "TransportResult", "Agglomeration", "Groups",
     "ResponseTable", "implicit_transport_table", "implicit_transport_table_device", "Ilu", "TracerResult",
-    "tracer_step",
+    "tracer_step", "pcg_operator", "operator_solve", "Preconditioner", "OperatorInfo", "FORM_TRANSPORT",
+    "FORM_BALANCE",
 ]
 
 # PSIM_LIBRARY names another build of the same ABI (e.g. a tuning variant).
@@ -964,3 +965,47 @@ def tracer_step(graph: Graph, layers, conductance, outflow, layer_mass, tracer, 
                                np.ascontiguousarray(layer_mass, np.float64), t, dt, tolerance, max_iterations, out,
                                C.byref(res)))
     return out, res
+
+
+_sig("psim_system_pcg_operator", C.c_int, _vp, _vp, _vp, _vp, _vp, _dbl, _i32, C.POINTER(SolveInfo), C.c_int)
+
+FORM_TRANSPORT, FORM_BALANCE = 0, 1
+
+
+class Preconditioner(C.Structure):
+    _fields_ = [("multigrid", _vp), ("pre_scale", _vp), ("post_divide", _vp), ("ilu", _vp)]
+
+
+class OperatorInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("status", C.c_int32), ("relative_residual", C.c_double)]
+
+
+_sig("psim_operator_solve", C.c_int, _vp, _vp, _vp, C.POINTER(Preconditioner), C.c_int, _vp, _vp, _dbl, _i32,
+     C.POINTER(OperatorInfo), C.c_int)
+
+
+def pcg_operator(system: System, operator_diagonal, b, tolerance=1e-6, max_iterations=500, hold=None):
+    """psim_system_pcg_operator with host arrays: (x, SolveInfo)."""
+    d = np.ascontiguousarray(operator_diagonal, np.float64)
+    bb = np.ascontiguousarray(b, np.float64)
+    x = np.zeros_like(bb)
+    h = None if hold is None else np.ascontiguousarray(hold, np.uint8)
+    info = SolveInfo()
+    check(lib.psim_system_pcg_operator(system.handle, _ptr(d), _ptr(bb), None if h is None else _ptr(h), _ptr(x),
+                                       tolerance, max_iterations, C.byref(info), HOST))
+    return x, info
+
+
+def operator_solve(graph: Graph, diagonal, off, b, multigrid: System = None, ilu: "Ilu" = None, pre_scale=None,
+                   post_divide=None, form=FORM_BALANCE, tolerance=1e-10, max_iterations=500):
+    """psim_operator_solve with host arrays: (x, OperatorInfo)."""
+    keep = [np.ascontiguousarray(a, np.float64) if a is not None else None
+            for a in (diagonal, off, b, pre_scale, post_divide)]
+    d, o, bb, ps, pd = keep
+    pre = Preconditioner(multigrid.handle if multigrid is not None else None, _ptr(ps) if ps is not None else None,
+                         _ptr(pd) if pd is not None else None, ilu.handle if ilu is not None else None)
+    x = np.zeros_like(bb)
+    info = OperatorInfo()
+    check(lib.psim_operator_solve(graph.handle, _ptr(d), _ptr(o), C.byref(pre), form, _ptr(bb), _ptr(x), tolerance,
+                                  max_iterations, C.byref(info), HOST))
+    return x, info

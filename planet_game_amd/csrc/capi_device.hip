@@ -659,6 +659,11 @@ int psim_system_solve(psim_system* s, const double* b, double* x, double toleran
 
 int psim_system_pcg(psim_system* s, const double* b, const uint8_t* hold, double* x, double tolerance,
                     int32_t max_iterations, psim_solve_info* info, int where) {
+    return psim_system_pcg_operator(s, nullptr, b, hold, x, tolerance, max_iterations, info, where);
+}
+
+int psim_system_pcg_operator(psim_system* s, const double* operator_diagonal, const double* b, const uint8_t* hold,
+                             double* x, double tolerance, int32_t max_iterations, psim_solve_info* info, int where) {
     return guarded([&] {
         require_matrix(s);
         psim::check_where(where);
@@ -674,6 +679,10 @@ int psim_system_pcg(psim_system* s, const double* b, const uint8_t* hold, double
         p.in = b;
         p.out = x;
         p.hold = hold;
+        p.operator_diag = operator_diagonal;
+        psim::Staging staging;
+        if (where == PSIM_HOST && operator_diagonal != nullptr)
+            p.operator_diag = static_cast<const double*>(staging.in(operator_diagonal, bytes));
         if (where == PSIM_HOST) {
             psim::hip_check(hipMemcpyAsync(s->stage_a, b, bytes, hipMemcpyHostToDevice, stream), "copy");
             p.in = s->stage_a;
@@ -689,7 +698,8 @@ int psim_system_pcg(psim_system* s, const double* b, const uint8_t* hold, double
         launch(s, p);
         if (where == PSIM_HOST)
             psim::hip_check(hipMemcpyAsync(x, s->stage_b, bytes, hipMemcpyDeviceToHost, stream), "copy");
-        if (where == PSIM_HOST || info != nullptr) fill_info(fetch_result(s), info, s->graph->ctx->wall_clock_hz);
+        if (where == PSIM_HOST || info != nullptr || operator_diagonal != nullptr)
+            fill_info(fetch_result(s), info, s->graph->ctx->wall_clock_hz);
     });
 }
 

@@ -141,6 +141,7 @@ struct Launch {
     double* out;        // output of precondition / apply
     SolveResult* step_results;         // kOpChain: one record per step, or null
     const uint8_t* hold;               // kOpPcg: rows held at 0 (identity rows), or null
+    const double* operator_diag;       // kOpPcg: the operator's diagonal where it differs from the V-cycle's, or null
     uint64_t ticket;                   // server: echoed in the completion record
     int64_t completion;                // server: completion record index
 };

@@ -1304,7 +1304,9 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
 // it < max, ||r|| > target and rho > 0:
 //   q = A d, alpha = rho / (d, q), x += alpha d, r -= alpha q,
 //   z = M r, rho' = (r, z), beta = rho' / rho, d = z + beta d.
-// M is the V-cycle (symmetric: the same Jacobi weights before and after the
+// The operator's diagonal may differ from the V-cycle's matrix (a
+// preconditioner regularised against a null mode, pressure_redistribution.
+// cpp:48-58,150-178). M is the V-cycle (symmetric: the same Jacobi weights before and after the
 // coarse correction), restricted to the free rows when rows are held (held
 // rows are decoupled identity rows with b = 0: their direction is 0, as the
 // reference's Gauss-Seidel sweep leaves it). d = z + beta d is evaluated inside the next SpMV
@@ -1322,7 +1324,7 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
     double* kept = static_cast<double*>(I.s);
     double* x = I.x[0];
     double* d[2] = {static_cast<double*>(I.p[0]), static_cast<double*>(I.p[1])};
-    const double* diag = I.diag64[0];
+    const double* diag = P.operator_diag != nullptr ? P.operator_diag : I.diag64[0];
     const double* off = I.off64[0];
     const auto next_partials = [&]() {
         double* part = I.partial + parity * I.partial_slots * kPartials;

@@ -62,6 +62,7 @@ struct psim_graph {
     psim_system* newton = nullptr;
     std::shared_ptr<void> transport_work;
     std::shared_ptr<void> tracer_work;   // stacked tracer systems by layer count
+    std::shared_ptr<void> operator_work; // device vectors of psim_operator_solve
     std::mutex newton_mutex;
     ~psim_graph() {
         if (newton != nullptr) psim_system_destroy(newton);

@@ -1,0 +1,163 @@
+"""The other GraphMultigrid consumers' solves on the device, reusing the
+graph's hierarchy:
+
+- pressure_redistribution.cpp:48-58,150-178: PCG on the two-point Laplacian
+  of the agglomerated groups with the V-cycle of its regularised form
+  (diagonal x (1 + 1e-10)), a zero-mean right-hand side (psim_system_pcg_
+  operator);
+- balanced_circulation.cpp:316-444: BiCGSTAB on a non-symmetric flux
+  operator (eddy diffusion plus upwind advection), preconditioned by the
+  V-cycle of its symmetric part then the block ILU(0) of the operator
+  (psim_operator_solve, balance form), tolerance 1e-10, true residual;
+- heat_transport.cpp:1176-1226: the same with the V-cycle's input scaled
+  by the areas and its output divided by the slopes (transport form).
+
+The reference runs these inside classes that need the whole climate state,
+so the checks are against what the mathematics fixes: a sparse direct solve
+of the same system (scipy), to the tolerance's reach.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import planet_game_amd as P
+import support as S
+from test_gpu_parity import announce, product_mesh
+from test_ilu_tracer import blocks_of
+
+pytestmark = pytest.mark.gpu
+
+
+def csr(m, diag, off):
+    rows = np.repeat(np.arange(m.cells), np.diff(m.offset.astype(np.int64)))
+    A = sp.csr_matrix((off, (rows, m.graph_neighbour.astype(np.int64))), shape=(m.cells, m.cells))
+    return (A + sp.diags(diag)).tocsr()
+
+
+def graph_of(ctx, level, agglomerated):
+    if agglomerated:
+        a = P.Agglomeration(product_mesh(level))
+        return P.Graph(ctx, a.graph), a.graph
+    m = product_mesh(level)
+    return P.Graph(ctx, m), m
+
+
+@pytest.mark.parametrize("level,agglomerated", [(6, True), (4, False)])
+def test_pressure_poisson_pcg(ctx, level, agglomerated):
+    announce("pressure poisson", level, agglomerated)
+    g, m = graph_of(ctx, level, agglomerated)
+    s = P.System(g)
+    s.set_matrix(m.weight_sum * (1.0 + 1.0e-10), -m.weight)
+    rng = np.random.default_rng(level)
+    rhs = rng.standard_normal(m.cells) * m.area
+    rhs -= rhs.mean()
+    x, info = P.pcg_operator(s, m.weight_sum, rhs, tolerance=1e-6, max_iterations=500)
+    L = csr(m, m.weight_sum, -m.weight)
+    true = np.linalg.norm(rhs - L @ x) / np.linalg.norm(rhs)
+    print("level", level, "groups" if agglomerated else "cells", m.cells, "iterations", info.iterations,
+          "recursive", info.relative_residual, "true", true, flush=True)
+    assert info.status == 0 and info.relative_residual <= 1e-6 and true <= 2e-6
+    # to 1e-12 against a direct solve with node 0 pinned (the Laplacian's
+    # null mode is the constant)
+    x12, info12 = P.pcg_operator(s, m.weight_sum, rhs, tolerance=1e-12, max_iterations=500)
+    exact = np.zeros(m.cells)
+    exact[1:] = spla.spsolve(L[1:, 1:].tocsc(), rhs[1:])
+    dx = (x12 - x12.mean()) - (exact - exact.mean())
+    assert np.linalg.norm(dx) <= 1e-9 * np.linalg.norm(exact - exact.mean())
+
+
+def flux_operator(m, seed, epsilon=1e-9):
+    """Eddy diffusion plus upwind advection in flux form, made diagonally
+    dominant where the flow converges (rows sum to at least epsilon A), and
+    its symmetric part."""
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(m.cells), np.diff(m.offset.astype(np.int64)))
+    nb = m.graph_neighbour.astype(np.int64)
+    key, back = rows * m.cells + nb, nb * m.cells + rows
+    order = np.argsort(key)
+    reverse = order[np.searchsorted(key[order], back)]   # entry b -> a of entry a -> b
+    g = m.weight * rng.uniform(0.5, 1.5, m.weight.size)
+    g = 0.5 * (g + g[reverse])   # symmetric per face
+    face = rng.standard_normal(m.cells)
+    flow = 0.8 * (face[nb] - face[rows]) * m.weight   # antisymmetric
+    off = -g - np.maximum(-flow, 0.0)
+    divergence = np.bincount(rows, weights=flow, minlength=m.cells)
+    diag = (np.bincount(rows, weights=g + np.maximum(flow, 0.0), minlength=m.cells) + np.maximum(-divergence, 0.0)
+            + epsilon * m.area)
+    sym_diag = np.bincount(rows, weights=g, minlength=m.cells) + epsilon * m.area
+    return np.ascontiguousarray(diag), np.ascontiguousarray(off), sym_diag, -g
+
+
+@pytest.mark.parametrize("level,agglomerated", [(6, True), (5, False)])
+def test_balance_operator_solve(ctx, level, agglomerated):
+    announce("balance operator solve", level, agglomerated)
+    g, m = graph_of(ctx, level, agglomerated)
+    diag, off, sym_diag, sym_off = flux_operator(m, 7 + level)
+    mg = P.System(g)
+    mg.set_matrix(sym_diag * (1.0 + 1.0e-8), sym_off)
+    ilu = P.Ilu(ctx, m.offset, m.graph_neighbour, blocks_of(m.cells))
+    ilu.factor(diag, off, 1.0 + 1.0e-8)
+    rng = np.random.default_rng(level)
+    b = rng.standard_normal(m.cells) * m.area
+    x, info = P.operator_solve(g, diag, off, b, multigrid=mg, ilu=ilu, form=P.FORM_BALANCE, tolerance=1e-10)
+    A = csr(m, diag, off)
+    exact = spla.spsolve(A.tocsc(), b)
+    # without the ILU stage the same solve takes more iterations
+    x2, info2 = P.operator_solve(g, diag, off, b, multigrid=mg, form=P.FORM_BALANCE, tolerance=1e-10)
+    print("level", level, "iterations", info.iterations, "without ilu", info2.iterations, "true residual",
+          info.relative_residual, "error", np.linalg.norm(x - exact) / np.linalg.norm(exact), flush=True)
+    assert info.status == 0 and info.relative_residual <= 1e-10 * 1.5
+    assert np.linalg.norm(x - exact) <= 1e-7 * np.linalg.norm(exact)
+    assert info2.status == 0 and info2.iterations >= info.iterations
+    # the iteration cap is reported as the reference throws it
+    _, capped = P.operator_solve(g, diag, off, b, multigrid=mg, ilu=ilu, form=P.FORM_BALANCE, tolerance=1e-14,
+                                 max_iterations=1)
+    assert capped.status == 5 and capped.iterations == 1
+
+
+K = 0.5 * S.RADIUS ** 2   # W/K per unit weight, as the transport tests use
+
+
+def test_transport_form_with_scaling(ctx):
+    """The advection-diffusion Newton system's preconditioner shape: the
+    V-cycle of A/s + K on (A x), divided by s, then ILU(0) of J."""
+    level = 4
+    announce("transport form operator solve", level)
+    g, m = graph_of(ctx, level, False)
+    rng = np.random.default_rng(3)
+    slope = 0.05 + 0.1 * rng.random(m.cells)
+    diag_k, off_k, sym_diag, sym_off = (K * v for v in flux_operator(m, 2, epsilon=0.0))
+    # J = I + (1/A) (K-part) Theta, in the form of heat_transport.cpp:1154-1174
+    rows = np.repeat(np.arange(m.cells), np.diff(m.offset.astype(np.int64)))
+    nb = m.graph_neighbour.astype(np.int64)
+    off = off_k * slope[nb] / m.area[rows]
+    diag = 1.0 + diag_k * slope / m.area
+    mg = P.System(g)
+    mg.set_matrix(m.area / slope + sym_diag, sym_off)
+    ilu = P.Ilu(ctx, m.offset, m.graph_neighbour, blocks_of(m.cells))
+    ilu.factor(diag, off, 1.0)
+    b = rng.standard_normal(m.cells)
+    x, info = P.operator_solve(g, diag, off, b, multigrid=mg, ilu=ilu, pre_scale=m.area, post_divide=slope,
+                               form=P.FORM_TRANSPORT, tolerance=1e-10)
+    exact = spla.spsolve(csr(m, diag, off).tocsc(), b)
+    A = csr(m, diag, off)
+    print("iterations", info.iterations, "true residual", info.relative_residual, "off/diag",
+          float(np.max(np.abs(off)) / np.min(diag)), flush=True)
+    assert np.max(np.abs(off)) >= 0.1 * np.min(diag)   # far from the identity
+    assert info.status == 0 and info.relative_residual <= 2e-10
+    assert abs(info.relative_residual - np.linalg.norm(b - A @ x) / np.linalg.norm(b)) <= 1e-12
+    assert np.linalg.norm(x - exact) <= 1e-8 * np.linalg.norm(exact)
+
+
+def test_operator_solve_errors(ctx):
+    announce("operator solve errors")
+    g, m = graph_of(ctx, 3, False)
+    diag, off, _, _ = flux_operator(m, 1)
+    other = P.System(P.Graph(ctx, product_mesh(4)))
+    with pytest.raises(P.PsimError, match="another graph"):
+        P.operator_solve(g, diag, off, np.ones(m.cells), multigrid=other)
+    with pytest.raises(P.PsimError, match="scaling needs a multigrid"):
+        P.operator_solve(g, diag, off, np.ones(m.cells), pre_scale=m.area)
+    with pytest.raises(P.PsimError, match="unknown solve form"):
+        P.operator_solve(g, diag, off, np.ones(m.cells), form=7)
