@@ -419,6 +419,36 @@ int psim_operator_solve(psim_graph* graph, const double* diagonal, const double*
                         const psim_preconditioner* preconditioner, int form, const double* b, double* x,
                         double relative_tolerance, int32_t max_iterations, psim_operator_info* info, int where);
 
+/* ------------------------------------------- coupled advection-diffusion
+ * AdvectionDiffusion (heat_transport.hpp:140-170): conductance_W_K[E]
+ * (symmetric, >= 0), outflow_kg_s[layers x E] (antisymmetric, per CSR
+ * entry, layer-major), eddy_latent_W_m2[n] or NULL.
+ * advection_diffusion_source (:172-178, heat_transport.cpp:774-816),
+ * bit-identical: eddy and advective sources (W/m^2) of mean_K[n] and
+ * energy_J_kg[layers x n]. */
+int psim_advection_source(psim_graph* graph, uint32_t layers, const double* conductance_W_K,
+                          const double* outflow_kg_s, const double* eddy_latent_W_m2, const double* mean_K,
+                          const double* energy_J_kg, double* eddy_W_m2, double* advective_W_m2, int where);
+/* solve_implicit_advection_diffusion (heat_transport.hpp:241-250,
+ * heat_transport.cpp:1083-1318) with a host callback for the columns'
+ * response (AdvectionResponse, hpp:162-166): source[n] in; mean_K[n],
+ * slope[n], energy[layers x n], energy_slope[layers x n] out. Host arrays;
+ * initial_source_W_m2 NULL: start at max(0, floor). The Jacobian's block
+ * ILU(0), the V-cycle of A / theta' + K and the BiCGSTAB run on the device. */
+typedef void (*psim_advection_response_fn)(void* user, const double* source_W_m2, double* mean_K,
+                                           double* slope_K_m2_W, double* energy_J_kg, double* energy_slope_J_kg_m2_W,
+                                           uint64_t n, uint32_t layers);
+typedef struct {
+    int32_t newton_iterations, linear_iterations;
+    double consistency_residual_W_m2, sum_W, absolute_sum_W, eddy_absolute_sum_W, advective_absolute_sum_W;
+} psim_advection_result;
+int psim_implicit_advection_diffusion(psim_graph* graph, uint32_t layers, const double* conductance_W_K,
+                                      const double* outflow_kg_s, const double* eddy_latent_W_m2,
+                                      const double* source_floor_W_m2, psim_advection_response_fn response,
+                                      void* user, const psim_transport_settings* settings,
+                                      const double* initial_source_W_m2, double* source_W_m2, double* mean_K,
+                                      double* energy_J_kg, psim_advection_result* result);
+
 /* ---- Batches (no counterpart in the reference, whose solves run one at a
  * time): many systems, of any graphs and precisions on one context, solved
  * in ONE launch of the persistent solver. Each system owns a group of

@@ -1613,4 +1613,180 @@ int orc_tracer_step(const void* h, uint32_t layers, const double* conductance, c
     });
 }
 
+// heat_transport.cpp:774-816 (advection_diffusion_source): per node, over
+// its entries in order, the eddies' G (theta_b - theta_a) and each layer's
+// upwind -F s_up; divided by the area; the eddies' latent heat taken off.
+void advection_source(const Mesh& m, size_t L, const double* conductance, const double* outflow,
+                      const double* eddy_latent, const double* mean, const double* energy, double* eddy_out,
+                      double* advective_out) {
+    const size_t n = m.n, E = m.ce_neighbor.size();
+    for (size_t a = 0; a < n; ++a) {
+        double eddy = 0.0, advective = 0.0;
+        for (size_t k = m.g_offset[a]; k < m.g_offset[a + 1]; ++k) {
+            const size_t b = m.ce_neighbor[k];
+            eddy += conductance[k] * (mean[b] - mean[a]);
+            for (size_t l = 0; l < L; ++l) {
+                const double out = outflow[l * E + k];
+                advective -= out * energy[l * n + (out > 0.0 ? a : b)];
+            }
+        }
+        eddy_out[a] = eddy / m.area[a];
+        if (eddy_latent != nullptr) eddy_out[a] -= eddy_latent[a];
+        advective_out[a] = advective / m.area[a];
+    }
+}
+
+int orc_advection_source(const void* h, uint32_t layers, const double* conductance, const double* outflow,
+                         const double* eddy_latent, const double* mean, const double* energy, double* eddy_out,
+                         double* advective_out) {
+    return guard([&] {
+        advection_source(M(h), layers, conductance, outflow, eddy_latent, mean, energy, eddy_out, advective_out);
+    });
+}
+
+// heat_transport.cpp:1083-1318 (solve_implicit_advection_diffusion):
+// Newton on F(h) = h - H(theta(h), s(h)) with J = I - dH/dh on the graph's
+// pattern, BiCGSTAB preconditioned by the V-cycle of A / theta' + K on
+// (A x) divided by theta', then block ILU(0) of J on what is left; the
+// backtracking line search of the implicit transport; the final source of
+// the last iterate and its sums (plain node order).
+int orc_solve_advection_diffusion(const void* h, uint32_t layers, const double* conductance, const double* outflow,
+                                  const double* eddy_latent, const double* floor, orc_advection_response_fn response,
+                                  void* user, const orc_transport_settings* settings, const double* initial,
+                                  uint64_t workers, double* source_out, double* mean_out, double* energy_out,
+                                  orc_advection_result* result) {
+    return guard([&] {
+        const Mesh& m = M(h);
+        const orc_transport_settings& st = *settings;
+        const size_t n = m.n, E = m.ce_neighbor.size(), L = layers;
+        for (size_t k = 0; k < E; ++k)
+            if (!(conductance[k] >= 0.0) || !std::isfinite(conductance[k]))
+                throw std::invalid_argument("eddy conductances must be finite and non-negative");
+        struct Iterate {
+            std::vector<double> source, mean, slope, energy, energy_slope, residual;
+            double merit = 0.0;
+        };
+        std::vector<double> eddy(n, 0.0), advective(n, 0.0);
+        const auto evaluate = [&](Iterate& at) {
+            at.mean.assign(n, 0.0);
+            at.slope.assign(n, 0.0);
+            at.energy.assign(L * n, 0.0);
+            at.energy_slope.assign(L * n, 0.0);
+            response(user, at.source.data(), at.mean.data(), at.slope.data(), at.energy.data(),
+                     at.energy_slope.data(), n, layers);
+            advection_source(m, L, conductance, outflow, eddy_latent, at.mean.data(), at.energy.data(), eddy.data(),
+                             advective.data());
+            at.residual.assign(n, 0.0);
+            at.merit = block_sum(n, workers, [&](size_t begin, size_t end) {
+                double merit = 0.0;
+                for (size_t a = begin; a < end; ++a) {
+                    at.residual[a] = at.source[a] - (eddy[a] + advective[a]);
+                    merit += m.area[a] * at.residual[a] * at.residual[a];
+                }
+                return merit;
+            });
+        };
+        orc_advection_result res{};
+        Iterate current, trial;
+        current.source.assign(n, 0.0);
+        for (size_t a = 0; a < n; ++a)
+            current.source[a] = initial != nullptr ? std::max(initial[a], floor[a]) : std::max(0.0, floor[a]);
+        evaluate(current);
+        std::vector<size_t> block_start{0};
+        for (size_t b = 0; b * block_cells < n; ++b) block_start.push_back(std::min(n, (b + 1) * block_cells));
+        auto levels = hierarchy(m);
+        const std::vector<uint64_t> neighbour64(m.ce_neighbor.begin(), m.ce_neighbor.end());
+        std::vector<double> diagonal(n, 0.0), off(E, 0.0), rhs(n, 0.0), delta, scaled(n, 0.0), leftover(n, 0.0);
+        constexpr double minimum_slope = 1.0e-12;
+        int non_monotone = 0;
+        for (int iteration = 0; iteration < st.max_newton_iterations; ++iteration) {
+            if (block_max_abs(n, workers, current.residual) <= st.newton_tolerance) break;
+            for (size_t a = 0; a < n; ++a) {
+                const double inverse_area = 1.0 / m.area[a];
+                double own = 0.0;
+                for (size_t k = m.g_offset[a]; k < m.g_offset[a + 1]; ++k) {
+                    const size_t b = m.ce_neighbor[k];
+                    const double g = conductance[k];
+                    own -= g * current.slope[a];
+                    double neighbour = g * current.slope[b];
+                    for (size_t l = 0; l < L; ++l) {
+                        const double out = outflow[l * E + k];
+                        if (out > 0.0)
+                            own -= out * current.energy_slope[l * n + a];
+                        else
+                            neighbour -= out * current.energy_slope[l * n + b];
+                    }
+                    off[k] = -neighbour * inverse_area;
+                }
+                diagonal[a] = 1.0 - own * inverse_area;
+                rhs[a] = -current.residual[a];
+            }
+            const Ilu factor = ilu_build(n, m.g_offset.data(), neighbour64.data(), diagonal.data(), off.data(), block_start, 1.0);
+            const auto apply = [&](const std::vector<double>& x, std::vector<double>& y) {
+                csr_apply(m, diagonal.data(), off.data(), x, y, workers);
+            };
+            MgLevel& fine = levels[0];
+            for (size_t a = 0; a < n; ++a) {
+                double sum = 0.0;
+                for (size_t k = m.g_offset[a]; k < m.g_offset[a + 1]; ++k) {
+                    sum += conductance[k];
+                    fine.val[k] = -conductance[k];
+                }
+                fine.diag[a] = m.area[a] / std::max(current.slope[a], minimum_slope) + sum;
+            }
+            for (size_t j = 0; j + 1 < levels.size(); ++j) galerkin(levels[j], levels[j + 1]);
+            cholesky(levels.back());
+            const auto prec = [&](const std::vector<double>& x, std::vector<double>& y) {
+                for (size_t a = 0; a < n; ++a) scaled[a] = m.area[a] * x[a];
+                vcycle(levels, 0, scaled, y, workers);
+                for (size_t a = 0; a < n; ++a) y[a] /= std::max(current.slope[a], minimum_slope);
+                apply(y, leftover);
+                for (size_t a = 0; a < n; ++a) leftover[a] = x[a] - leftover[a];
+                ilu_solve(factor, leftover.data());
+                for (size_t a = 0; a < n; ++a) y[a] += leftover[a];
+            };
+            res.linear_iterations +=
+                bicgstab(n, workers, apply, prec, rhs, delta, st.cg_relative_tolerance, st.max_cg_iterations);
+            double step = 1.0;
+            bool accepted = false;
+            for (int halving = 0;; ++halving) {
+                trial.source.assign(n, 0.0);
+                for (size_t a = 0; a < n; ++a) trial.source[a] = std::max(current.source[a] + step * delta[a], floor[a]);
+                evaluate(trial);
+                if (trial.merit < current.merit) {
+                    accepted = true;
+                    break;
+                }
+                if (halving == st.max_line_search_halvings) {
+                    if (block_max_abs(n, workers, current.residual) > st.rounding_floor &&
+                        non_monotone < st.max_non_monotone_steps) {
+                        ++non_monotone;
+                        for (size_t a = 0; a < n; ++a) trial.source[a] = std::max(current.source[a] + delta[a], floor[a]);
+                        evaluate(trial);
+                        accepted = std::isfinite(trial.merit) && trial.merit <= st.max_non_monotone_growth * current.merit;
+                    }
+                    break;
+                }
+                step *= 0.5;
+            }
+            if (!accepted) break;
+            std::swap(current, trial);
+            ++res.newton_iterations;
+        }
+        advection_source(m, L, conductance, outflow, eddy_latent, current.mean.data(), current.energy.data(),
+                         eddy.data(), advective.data());
+        for (size_t a = 0; a < n; ++a) {
+            source_out[a] = eddy[a] + advective[a];
+            res.sum_W += m.area[a] * source_out[a];
+            res.absolute_sum_W += m.area[a] * std::abs(source_out[a]);
+            res.eddy_absolute_sum_W += m.area[a] * std::abs(eddy[a]);
+            res.advective_absolute_sum_W += m.area[a] * std::abs(advective[a]);
+        }
+        std::copy(current.mean.begin(), current.mean.end(), mean_out);
+        std::copy(current.energy.begin(), current.energy.end(), energy_out);
+        res.consistency_residual = block_max_abs(n, workers, current.residual);
+        *result = res;
+    });
+}
+
 }  // extern "C"

@@ -18,6 +18,7 @@
  *   group maps     sim/planet/surface/surface_energy.cpp:1285-1300,1306-1324
  *   block ILU(0)   sim/core/math/incomplete_lu.cpp:40-146
  *   tracer step    sim/planet/surface/heat_transport.cpp:901-1081
+ *   advection-diffusion  sim/planet/surface/heat_transport.cpp:774-816,1083-1318
  *   RNG            sim/core/random/counter_rng.hpp:19-45
  * and the synthetic heat-step workload (this project's input generator).
  *
@@ -150,6 +151,27 @@ typedef struct {
 int orc_tracer_step(const void* mesh, uint32_t layers, const double* conductance, const double* outflow,
                     double heat_capacity, const double* layer_mass, const double* tracer, double dt, double tol,
                     int max_iterations, uint64_t workers, double* out, orc_tracer_result* result);
+
+/* advection_diffusion_source (heat_transport.cpp:774-816): eddy_out[n] and
+ * advective_out[n] (W/m^2) from mean[n] and energy[layers x n];
+ * eddy_latent NULL: none. */
+int orc_advection_source(const void* mesh, uint32_t layers, const double* conductance, const double* outflow,
+                         const double* eddy_latent, const double* mean, const double* energy, double* eddy_out,
+                         double* advective_out);
+/* solve_implicit_advection_diffusion (heat_transport.cpp:1083-1318) with a
+ * C response: (user, source[n], mean[n], slope[n], energy[L n],
+ * energy_slope[L n], n, L). initial NULL: start at max(0, floor). */
+typedef void (*orc_advection_response_fn)(void* user, const double* source, double* mean, double* slope,
+                                          double* energy, double* energy_slope, uint64_t n, uint32_t layers);
+typedef struct {
+    int newton_iterations, linear_iterations;
+    double consistency_residual, sum_W, absolute_sum_W, eddy_absolute_sum_W, advective_absolute_sum_W;
+} orc_advection_result;
+int orc_solve_advection_diffusion(const void* mesh, uint32_t layers, const double* conductance, const double* outflow,
+                                  const double* eddy_latent, const double* floor, orc_advection_response_fn response,
+                                  void* user, const orc_transport_settings* settings, const double* initial,
+                                  uint64_t workers, double* source_out, double* mean_out, double* energy_out,
+                                  orc_advection_result* result);
 
 /* Synthetic heat-step workload (the project's seeded input generator).
  *   instance_seed = orc_mix_random_key(base_seed ^ instance)

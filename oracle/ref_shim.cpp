@@ -526,4 +526,86 @@ int ref_tracer_step(const void* handle, std::uint32_t layers, const double* cond
     });
 }
 
+// advection_diffusion_source and solve_implicit_advection_diffusion
+// (heat_transport.hpp:140-200) on the mesh's graph.
+int ref_advection_source(const void* handle, std::uint32_t layers, const double* conductance, const double* outflow,
+                         const double* eddy_latent, const double* mean, const double* energy, double* eddy_out,
+                         double* advective_out) {
+    return guarded([&] {
+        const auto& graph = static_cast<const RefMesh*>(handle)->graph;
+        const std::size_t n = graph.size(), e = graph.neighbour.size();
+        planetsim::AdvectionDiffusion transport;
+        transport.layers = layers;
+        transport.conductance_W_K.assign(conductance, conductance + e);
+        transport.outflow_kg_s.assign(outflow, outflow + layers * e);
+        if (eddy_latent != nullptr) transport.eddy_latent_W_m2.assign(eddy_latent, eddy_latent + n);
+        const planetsim::Field2D<double> m(std::vector<double>(mean, mean + n));
+        const std::vector<double> s(energy, energy + layers * n);
+        planetsim::Field2D<double> eddy, advective;
+        planetsim::advection_diffusion_source(graph, transport, m, s, eddy, advective, 1U);
+        std::memcpy(eddy_out, eddy.values().data(), n * sizeof(double));
+        std::memcpy(advective_out, advective.values().data(), n * sizeof(double));
+    });
+}
+
+typedef void (*ref_advection_response_fn)(void* user, const double* source, double* mean, double* slope,
+                                          double* energy, double* energy_slope, std::uint64_t n,
+                                          std::uint32_t layers);
+struct RefAdvectionResult {
+    int newton_iterations, linear_iterations;
+    double consistency_residual, sum_W, absolute_sum_W, eddy_absolute_sum_W, advective_absolute_sum_W;
+};
+
+int ref_solve_advection_diffusion(const void* handle, std::uint32_t layers, const double* conductance,
+                                  const double* outflow, const double* eddy_latent, const double* floor,
+                                  ref_advection_response_fn response, void* user, const RefTransportSettings* st,
+                                  const double* initial, std::uint64_t workers, double* source_out, double* mean_out,
+                                  double* energy_out, RefAdvectionResult* result) {
+    return guarded([&] {
+        const auto& graph = static_cast<const RefMesh*>(handle)->graph;
+        const std::size_t n = graph.size(), e = graph.neighbour.size();
+        planetsim::AdvectionDiffusion transport;
+        transport.layers = layers;
+        transport.conductance_W_K.assign(conductance, conductance + e);
+        transport.outflow_kg_s.assign(outflow, outflow + layers * e);
+        if (eddy_latent != nullptr) transport.eddy_latent_W_m2.assign(eddy_latent, eddy_latent + n);
+        const planetsim::AdvectionResponse bridge =
+            [&](const planetsim::Field2D<double>& h, planetsim::Field2D<double>& m, planetsim::Field2D<double>& d,
+                std::vector<double>& s, std::vector<double>& ds) {
+                m = planetsim::Field2D<double>(n, 0.0);
+                d = planetsim::Field2D<double>(n, 0.0);
+                s.assign(layers * n, 0.0);
+                ds.assign(layers * n, 0.0);
+                response(user, h.values().data(), m.values().data(), d.values().data(), s.data(), ds.data(), n,
+                         layers);
+            };
+        planetsim::ImplicitTransportSettings settings;
+        settings.max_newton_iterations = st->max_newton_iterations;
+        settings.max_line_search_halvings = st->max_line_search_halvings;
+        settings.rounding_floor_W_m2 = st->rounding_floor;
+        settings.max_non_monotone_steps = st->max_non_monotone_steps;
+        settings.max_non_monotone_growth = st->max_non_monotone_growth;
+        settings.newton_tolerance_W_m2 = st->newton_tolerance;
+        settings.cg_relative_tolerance = st->cg_relative_tolerance;
+        settings.max_cg_iterations = st->max_cg_iterations;
+        planetsim::Field2D<double> start;
+        if (initial != nullptr) {
+            start = planetsim::Field2D<double>(std::vector<double>(initial, initial + n));
+            settings.initial_source_W_m2 = &start;
+        }
+        const planetsim::Field2D<double> fl(std::vector<double>(floor, floor + n));
+        const auto r = planetsim::solve_implicit_advection_diffusion(graph, transport, fl, bridge, settings, workers);
+        std::memcpy(source_out, r.source_W_m2.values().data(), n * sizeof(double));
+        std::memcpy(mean_out, r.mean_K.values().data(), n * sizeof(double));
+        std::memcpy(energy_out, r.energy_J_kg.data(), layers * n * sizeof(double));
+        result->newton_iterations = r.newton_iterations;
+        result->linear_iterations = r.linear_iterations;
+        result->consistency_residual = r.consistency_residual_W_m2;
+        result->sum_W = r.sum_W;
+        result->absolute_sum_W = r.absolute_sum_W;
+        result->eddy_absolute_sum_W = r.eddy_absolute_sum_W;
+        result->advective_absolute_sum_W = r.advective_absolute_sum_W;
+    });
+}
+
 }  // extern "C"

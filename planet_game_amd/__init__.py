@@ -24,7 +24,7 @@ __all__ = [
     "implicit_transport", "TransportSettings", "TransportResult", "Agglomeration", "Groups",
     "ResponseTable", "implicit_transport_table", "implicit_transport_table_device", "Ilu", "TracerResult",
     "tracer_step", "pcg_operator", "operator_solve", "Preconditioner", "OperatorInfo", "FORM_TRANSPORT",
-    "FORM_BALANCE",
+    "FORM_BALANCE", "advection_source", "implicit_advection_diffusion", "AdvectionResult",
 ]
 
 # PSIM_LIBRARY names another build of the same ABI (e.g. a tuning variant).
@@ -1009,3 +1009,57 @@ def operator_solve(graph: Graph, diagonal, off, b, multigrid: System = None, ilu
     check(lib.psim_operator_solve(graph.handle, _ptr(d), _ptr(o), C.byref(pre), form, _ptr(bb), _ptr(x), tolerance,
                                   max_iterations, C.byref(info), HOST))
     return x, info
+
+
+ADVECTION_RESPONSE = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64,
+                                 C.c_uint32)
+
+
+class AdvectionResult(C.Structure):
+    _fields_ = [("newton_iterations", C.c_int32), ("linear_iterations", C.c_int32),
+                ("consistency_residual_W_m2", C.c_double), ("sum_W", C.c_double), ("absolute_sum_W", C.c_double),
+                ("eddy_absolute_sum_W", C.c_double), ("advective_absolute_sum_W", C.c_double)]
+
+
+_sig("psim_advection_source", C.c_int, _vp, C.c_uint32, _pd, _pd, _vp, _pd, _pd, _pd, _pd, C.c_int)
+_sig("psim_implicit_advection_diffusion", C.c_int, _vp, C.c_uint32, _pd, _pd, _vp, _pd, ADVECTION_RESPONSE, _vp,
+     C.POINTER(TransportSettings), _vp, _pd, _pd, _pd, C.POINTER(AdvectionResult))
+
+
+def advection_source(graph: Graph, layers, conductance, outflow, mean, energy, latent=None):
+    """psim_advection_source with host arrays: (eddy, advective)."""
+    n = graph.mesh.cells
+    lat = None if latent is None else np.ascontiguousarray(latent, np.float64)
+    eddy, adv = np.zeros(n), np.zeros(n)
+    check(lib.psim_advection_source(graph.handle, layers, np.ascontiguousarray(conductance, np.float64),
+                                    np.ascontiguousarray(outflow, np.float64), None if lat is None else _ptr(lat),
+                                    np.ascontiguousarray(mean, np.float64), np.ascontiguousarray(energy, np.float64),
+                                    eddy, adv, HOST))
+    return eddy, adv
+
+
+def implicit_advection_diffusion(graph: Graph, layers, conductance, outflow, floor, response, settings=None,
+                                 latent=None, initial=None):
+    """psim_implicit_advection_diffusion: response(source) -> (mean, slope,
+    energy, energy_slope); returns (source, mean, energy, AdvectionResult)."""
+    n = graph.mesh.cells
+
+    def call(_user, source, mean, slope, energy, energy_slope, count, nl):
+        th, sl, e, es = response(np.ctypeslib.as_array(source, shape=(count,)).copy())
+        np.ctypeslib.as_array(mean, shape=(count,))[:] = th
+        np.ctypeslib.as_array(slope, shape=(count,))[:] = sl
+        np.ctypeslib.as_array(energy, shape=(nl * count,))[:] = e
+        np.ctypeslib.as_array(energy_slope, shape=(nl * count,))[:] = es
+
+    cb = ADVECTION_RESPONSE(call)
+    lat = None if latent is None else np.ascontiguousarray(latent, np.float64)
+    ini = None if initial is None else np.ascontiguousarray(initial, np.float64)
+    src, mean, energy, res = np.zeros(n), np.zeros(n), np.zeros(layers * n), AdvectionResult()
+    check(lib.psim_implicit_advection_diffusion(graph.handle, layers, np.ascontiguousarray(conductance, np.float64),
+                                                np.ascontiguousarray(outflow, np.float64),
+                                                None if lat is None else _ptr(lat),
+                                                np.ascontiguousarray(floor, np.float64), cb, None,
+                                                C.byref(_settings(settings)), None if ini is None else _ptr(ini),
+                                                src, mean, energy, C.byref(res)))
+    return src, mean, energy, res

@@ -63,6 +63,8 @@ struct psim_graph {
     std::shared_ptr<void> transport_work;
     std::shared_ptr<void> tracer_work;   // stacked tracer systems by layer count
     std::shared_ptr<void> operator_work; // device vectors of psim_operator_solve
+    std::shared_ptr<void> advection_work;   // psim_implicit_advection_diffusion's systems, under advection_mutex
+    std::mutex advection_mutex;
     std::mutex newton_mutex;
     ~psim_graph() {
         if (newton != nullptr) psim_system_destroy(newton);

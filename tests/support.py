@@ -70,6 +70,9 @@ def _declare(lib, prefix):
        C.POINTER(C.c_int))
     fn("solve_implicit_transport", C.c_int, _vp, C.c_double, _dp, RESPONSE, _vp,
        C.POINTER(TransportSettings), _u64, _vp, _dp, _dp, C.POINTER(TransportResult))
+    fn("advection_source", C.c_int, _vp, C.c_uint32, _dp, _dp, _vp, _dp, _dp, _dp, _dp)
+    fn("solve_advection_diffusion", C.c_int, _vp, C.c_uint32, _dp, _dp, _vp, _dp, ADVECTION_RESPONSE, _vp,
+       C.POINTER(TransportSettings), _vp, _u64, _dp, _dp, _dp, C.POINTER(AdvectionResult))
 
 
 # solve_implicit_transport's C form (both libraries): a response callback
@@ -201,6 +204,73 @@ def tracer_case(m, layers, seed, heat_capacity=1004.64):
     tracer = np.ascontiguousarray(rng.uniform(0.0, 2.0e-3, layers * m.n))
     return dict(layers=layers, conductance=conductance, outflow=outflow, heat_capacity=heat_capacity,
                 layer_mass=layer_mass, tracer=tracer, dt=2.63e6)
+
+
+ADVECTION_RESPONSE = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64,
+                                 C.c_uint32)
+
+
+class AdvectionResult(C.Structure):
+    _fields_ = [("newton_iterations", C.c_int), ("linear_iterations", C.c_int), ("consistency_residual", C.c_double),
+                ("sum_W", C.c_double), ("absolute_sum_W", C.c_double), ("eddy_absolute_sum_W", C.c_double),
+                ("advective_absolute_sum_W", C.c_double)]
+
+
+def advection_case(m, layers, seed, quartic=False):
+    """Synthetic coupled-transport inputs: symmetric eddy conductances (W/K),
+    antisymmetric layer mass fluxes (kg/s), and a column response per node:
+    theta(h) (linear, or the quartic tile-like response), each layer's s_l =
+    c_p (T_l + w_l (theta - 250)) with its slope."""
+    rng = np.random.default_rng(seed)
+    sign = np.where(m.first[m.edge] == np.repeat(np.arange(m.n), np.diff(m.offset.astype(np.int64))), 1.0, -1.0)
+    conductance = np.ascontiguousarray(rng.uniform(0.5, 1.5, m.n_edges)[m.edge] * m.weight * 0.5 * RADIUS ** 2)
+    outflow = np.ascontiguousarray((rng.standard_normal((layers, m.n_edges))[:, m.edge] * sign[None, :]
+                                    * 3.0e8).ravel())
+    cp = 1004.64
+    w = rng.uniform(0.2, 1.0, layers)
+    t0 = 250.0 + 10.0 * np.arange(layers)
+    if quartic:
+        theta_fn, floor = quartic_response(m.n, rng.random(m.n), 0.41)
+    else:
+        base = 250.0 + 40.0 * rng.random(m.n)
+        slopes = 0.02 + 0.1 * rng.random(m.n)
+        theta_fn = lambda h: (base + slopes * h, slopes.copy())   # noqa: E731
+        floor = np.full(m.n, -np.inf)
+
+    def fn(h):
+        theta, slope = theta_fn(h)
+        energy = cp * (t0[:, None] + w[:, None] * (theta[None, :] - 250.0))
+        energy_slope = cp * w[:, None] * slope[None, :]
+        return theta, slope, np.ascontiguousarray(energy.ravel()), np.ascontiguousarray(energy_slope.ravel())
+
+    return dict(layers=layers, conductance=conductance, outflow=outflow, floor=floor, response=fn)
+
+
+def advection_callback(fn):
+    def call(_user, source, mean, slope, energy, energy_slope, n, layers):
+        th, sl, e, es = fn(np.ctypeslib.as_array(source, shape=(n,)).copy())
+        np.ctypeslib.as_array(mean, shape=(n,))[:] = th
+        np.ctypeslib.as_array(slope, shape=(n,))[:] = sl
+        np.ctypeslib.as_array(energy, shape=(layers * n,))[:] = e
+        np.ctypeslib.as_array(energy_slope, shape=(layers * n,))[:] = es
+
+    return ADVECTION_RESPONSE(call)
+
+
+def solve_advection(lib, handle, n, case, settings=None, latent=None, initial=None):
+    layers = case["layers"]
+    src, mean, energy = np.zeros(n), np.zeros(n), np.zeros(layers * n)
+    res = AdvectionResult()
+    cb = advection_callback(case["response"])
+    st = settings or TransportSettings()
+    lat = None if latent is None else np.ascontiguousarray(latent, np.float64)
+    ini = None if initial is None else np.ascontiguousarray(initial, np.float64)
+    lib.check(lib.solve_advection_diffusion(handle, layers, case["conductance"], case["outflow"],
+                                            None if lat is None else lat.ctypes.data_as(_vp), case["floor"], cb,
+                                            None, C.byref(st), None if ini is None else ini.ctypes.data_as(_vp), 1,
+                                            src, mean, energy, C.byref(res)))
+    return src, mean, energy, res
 
 
 def oracle_tracer(m, case, tol=1e-12, max_it=2000):
