@@ -411,6 +411,169 @@ struct TransportResponseTable {
 }
 
 // ---------------------------------------------------------------------------
+// The coupled climate transport (heat_transport.hpp:140-250 of the
+// reference): eddy diffusion of the columns' theta plus the upwind advection
+// of each layer's dry static energy, its source, its implicit Newton solve
+// (block ILU(0), V-cycle and BiCGSTAB on the device) and a tracer's implicit
+// step under the same fluxes.
+struct AdvectionDiffusion {
+    std::size_t layers = 0;
+    std::vector<double> conductance_W_K;
+    std::vector<double> outflow_kg_s;
+    double heat_capacity_J_kg_K = 1004.64;
+    std::vector<double> eddy_latent_W_m2;
+};
+
+using AdvectionResponse =
+    std::function<void(const Field2D<double>& source_W_m2, Field2D<double>& mean_K, Field2D<double>& slope_K_m2_W,
+                       std::vector<double>& energy_J_kg, std::vector<double>& energy_slope_J_kg_m2_W)>;
+
+struct AdvectionDiffusionResult {
+    Field2D<double> source_W_m2;
+    Field2D<double> mean_K;
+    std::vector<double> energy_J_kg;
+    double consistency_residual_W_m2 = 0.0;
+    int newton_iterations = 0;
+    int linear_iterations = 0;
+    double sum_W = 0.0;
+    double absolute_sum_W = 0.0;
+    double eddy_absolute_sum_W = 0.0;
+    double advective_absolute_sum_W = 0.0;
+};
+
+namespace detail {
+inline void check_advection(const TransportGraph& graph, const AdvectionDiffusion& transport) {
+    const std::size_t e = graph.neighbour.size();
+    if (transport.layers == 0U || transport.conductance_W_K.size() != e ||
+        transport.outflow_kg_s.size() != transport.layers * e ||
+        (!transport.eddy_latent_W_m2.empty() && transport.eddy_latent_W_m2.size() != graph.size()))
+        throw std::invalid_argument("advection-diffusion inputs do not match the graph");
+}
+
+struct AdvectionCall {
+    const AdvectionResponse* response;
+    std::exception_ptr error;
+};
+// C callback into the std::function (sizes checked as the reference
+// checks them); an exception is held and rethrown after the library call.
+inline void advection_trampoline(void* user, const double* source, double* mean, double* slope, double* energy,
+                                 double* energy_slope, std::uint64_t n, std::uint32_t layers) {
+    auto* call = static_cast<AdvectionCall*>(user);
+    const std::size_t cells = static_cast<std::size_t>(n), values = cells * layers;
+    if (!call->error) {
+        try {
+            const Field2D<double> h(std::vector<double>(source, source + cells));
+            Field2D<double> m(cells, 0.0), s(cells, 0.0);
+            std::vector<double> e(values, 0.0), es(values, 0.0);
+            (*call->response)(h, m, s, e, es);
+            if (m.size() != cells || s.size() != cells || e.size() != values || es.size() != values)
+                throw std::logic_error("advection-diffusion response has the wrong size");
+            std::copy(m.values().begin(), m.values().end(), mean);
+            std::copy(s.values().begin(), s.values().end(), slope);
+            std::copy(e.begin(), e.end(), energy);
+            std::copy(es.begin(), es.end(), energy_slope);
+            return;
+        } catch (...) {
+            call->error = std::current_exception();
+        }
+    }
+    std::fill(mean, mean + cells, 0.0);
+    std::fill(slope, slope + cells, 0.0);
+    std::fill(energy, energy + values, 0.0);
+    std::fill(energy_slope, energy_slope + values, 0.0);
+}
+}  // namespace detail
+
+// heat_transport.cpp:774-816, bit-identical.
+inline void advection_diffusion_source(const TransportGraph& graph, const AdvectionDiffusion& transport,
+                                       const Field2D<double>& mean_K, const std::vector<double>& energy_J_kg,
+                                       Field2D<double>& eddy_W_m2, Field2D<double>& advective_W_m2,
+                                       std::size_t worker_count = 1U) {
+    detail::check_advection(graph, transport);
+    const std::size_t n = graph.size();
+    if (mean_K.size() != n || energy_J_kg.size() != transport.layers * n)
+        throw std::invalid_argument("advection-diffusion inputs do not match the graph");
+    gpu::check_workers(worker_count);
+    if (eddy_W_m2.size() != n) eddy_W_m2 = Field2D<double>(n, 0.0);
+    if (advective_W_m2.size() != n) advective_W_m2 = Field2D<double>(n, 0.0);
+    gpu::check(psim_advection_source(graph.device(), static_cast<std::uint32_t>(transport.layers),
+                                     transport.conductance_W_K.data(), transport.outflow_kg_s.data(),
+                                     transport.eddy_latent_W_m2.empty() ? nullptr : transport.eddy_latent_W_m2.data(),
+                                     mean_K.values().data(), energy_J_kg.data(), eddy_W_m2.values().data(),
+                                     advective_W_m2.values().data(), PSIM_HOST));
+}
+
+// heat_transport.cpp:1083-1318.
+[[nodiscard]] inline AdvectionDiffusionResult solve_implicit_advection_diffusion(
+    const TransportGraph& graph, const AdvectionDiffusion& transport, const Field2D<double>& source_floor_W_m2,
+    const AdvectionResponse& response, const ImplicitTransportSettings& settings = {},
+    std::size_t worker_count = 1U) {
+    detail::check_advection(graph, transport);
+    const std::size_t n = graph.size();
+    if (source_floor_W_m2.size() != n) throw std::invalid_argument("advection-diffusion inputs do not match the graph");
+    if (settings.initial_source_W_m2 != nullptr && settings.initial_source_W_m2->size() != n)
+        throw std::invalid_argument("the initial sources do not match the graph");
+    gpu::check_workers(worker_count);
+    const psim_transport_settings st = detail::transport_settings(settings);
+    detail::AdvectionCall call{&response, nullptr};
+    AdvectionDiffusionResult result;
+    result.source_W_m2 = Field2D<double>(n, 0.0);
+    result.mean_K = Field2D<double>(n, 0.0);
+    result.energy_J_kg.assign(transport.layers * n, 0.0);
+    psim_advection_result r{};
+    const int status = psim_implicit_advection_diffusion(
+        graph.device(), static_cast<std::uint32_t>(transport.layers), transport.conductance_W_K.data(),
+        transport.outflow_kg_s.data(), transport.eddy_latent_W_m2.empty() ? nullptr : transport.eddy_latent_W_m2.data(),
+        source_floor_W_m2.values().data(), &detail::advection_trampoline, &call, &st,
+        settings.initial_source_W_m2 != nullptr ? settings.initial_source_W_m2->values().data() : nullptr,
+        result.source_W_m2.values().data(), result.mean_K.values().data(), result.energy_J_kg.data(), &r);
+    if (call.error) std::rethrow_exception(call.error);
+    gpu::check(status);
+    result.consistency_residual_W_m2 = r.consistency_residual_W_m2;
+    result.newton_iterations = r.newton_iterations;
+    result.linear_iterations = r.linear_iterations;
+    result.sum_W = r.sum_W;
+    result.absolute_sum_W = r.absolute_sum_W;
+    result.eddy_absolute_sum_W = r.eddy_absolute_sum_W;
+    result.advective_absolute_sum_W = r.advective_absolute_sum_W;
+    return result;
+}
+
+struct TracerTransportResult {
+    std::vector<double> tracer;
+    int iterations = 0;
+    double relative_residual = 0.0;
+    double clipped_kg = 0.0;
+    double column_divergence_kg_s = 0.0;
+};
+
+// heat_transport.cpp:901-1081.
+[[nodiscard]] inline TracerTransportResult solve_implicit_tracer(const TransportGraph& graph,
+                                                                 const AdvectionDiffusion& transport,
+                                                                 std::span<const double> layer_mass_kg,
+                                                                 const std::vector<double>& tracer, double dt_s,
+                                                                 double relative_tolerance = 1e-12,
+                                                                 int max_iterations = 2'000,
+                                                                 std::size_t worker_count = 1U) {
+    const std::size_t n = graph.size(), e = graph.neighbour.size(), layers = transport.layers;
+    if (layers == 0U || layer_mass_kg.size() != n || tracer.size() != layers * n ||
+        transport.conductance_W_K.size() != e || transport.outflow_kg_s.size() != layers * e || !(dt_s > 0.0))
+        throw std::invalid_argument("tracer transport inputs do not match the graph");
+    gpu::check_workers(worker_count);
+    TracerTransportResult result;
+    result.tracer.assign(layers * n, 0.0);
+    psim_tracer_result r{};
+    gpu::check(psim_tracer_step(graph.device(), static_cast<std::uint32_t>(layers), transport.conductance_W_K.data(),
+                                transport.outflow_kg_s.data(), transport.heat_capacity_J_kg_K, layer_mass_kg.data(),
+                                tracer.data(), dt_s, relative_tolerance, max_iterations, result.tracer.data(), &r));
+    result.iterations = r.iterations;
+    result.relative_residual = r.relative_residual;
+    result.clipped_kg = r.clipped_kg;
+    result.column_divergence_kg_s = r.column_divergence_kg_s;
+    return result;
+}
+
+// ---------------------------------------------------------------------------
 // The implicit heat step (new; no reference counterpart beyond its parts):
 //   (diag(C_i A_i) + dt L_K) T = C_i A_i T_old + dt A_i S_i,
 // L_K the two-point Laplacian with entry weights K_e l_e / d_e, solved by

@@ -333,6 +333,88 @@ void device_part() {
         EXPECT(std::abs(result.sum_W) <= 1e-12 * result.absolute_sum_W);
         EXPECT(result.dissipation_W_K <= 0.0);
     }
+    stage("incomplete lu, tracer, advection-diffusion");
+    {
+        // IncompleteLU with one-row blocks drops every coupling: the solve
+        // divides by the scaled diagonal, exactly
+        const auto graph = planetsim::mesh_transport_graph(mesh);
+        std::vector<double> diag(n), off(graph.neighbour.size());
+        for (std::size_t c = 0; c < n; ++c) diag[c] = graph.weight_sum[c] * (1.0 + uniform(3000 + c));
+        for (std::size_t k = 0; k < off.size(); ++k) off[k] = -graph.weight[k];
+        std::vector<std::size_t> singles(n + 1);
+        for (std::size_t c = 0; c <= n; ++c) singles[c] = c;
+        const planetsim::IncompleteLU isolated(graph.offset, graph.neighbour, diag, off, singles, 2.0, 1U);
+        std::vector<double> x(n, 3.0);
+        isolated.solve(x);
+        bool exact = true;
+        for (std::size_t c = 0; c < n; ++c) exact = exact && x[c] == 3.0 / (diag[c] * 2.0);
+        EXPECT(exact);
+        EXPECT(throws_with<std::invalid_argument>(
+            [&] { (void)planetsim::IncompleteLU(graph.offset, graph.neighbour, std::span(diag).first(n - 1), off); },
+            "incomplete LU: the pattern does not match"));
+        // a tracer step conserves mass and keeps c >= 0
+        planetsim::AdvectionDiffusion transport;
+        transport.layers = 2;
+        transport.conductance_W_K.resize(graph.neighbour.size());
+        transport.outflow_kg_s.assign(2 * graph.neighbour.size(), 0.0);
+        for (std::size_t c = 0; c < n; ++c)
+            for (std::size_t k = graph.offset[c]; k < graph.offset[c + 1]; ++k) {
+                const std::size_t b = graph.neighbour[k];
+                const double face = uniform(7000 + std::min(c, b) * 7 + std::max(c, b));
+                transport.conductance_W_K[k] = 1.0e9 * (0.5 + face);
+                const double sign = c < b ? 1.0 : -1.0;
+                transport.outflow_kg_s[k] = sign * 2.0e7 * (face - 0.5);
+                transport.outflow_kg_s[graph.neighbour.size() + k] = -sign * 1.0e7 * (face - 0.3);
+            }
+        std::vector<double> mass(n), tracer(2 * n);
+        for (std::size_t c = 0; c < n; ++c) {
+            mass[c] = graph.area_m2[c] * 1.0e4;
+            tracer[c] = 1.0e-3 * (1.0 + uniform(8000 + c));
+            tracer[n + c] = 2.0e-3 * uniform(9000 + c);
+        }
+        const auto stepped = planetsim::solve_implicit_tracer(graph, transport, mass, tracer, 2.63e6);
+        double before = 0.0, after = 0.0;
+        bool positive = true;
+        for (std::size_t i = 0; i < 2 * n; ++i) {
+            before += tracer[i] * mass[i % n];
+            after += stepped.tracer[i] * mass[i % n];
+            positive = positive && stepped.tracer[i] >= 0.0;
+        }
+        EXPECT(positive && stepped.relative_residual <= 1e-11);
+        EXPECT(std::abs(after - before - stepped.clipped_kg) <= 1e-12 * before);
+        EXPECT(throws_with<std::invalid_argument>(
+            [&] { (void)planetsim::solve_implicit_tracer(graph, transport, mass, tracer, -1.0); },
+            "tracer transport inputs do not match the graph"));
+        // the coupled transport converges and conserves energy
+        for (auto& g : transport.conductance_W_K) g *= 1.0e4;
+        const planetsim::AdvectionResponse response = [&](const planetsim::Field2D<double>& h,
+                                                          planetsim::Field2D<double>& theta,
+                                                          planetsim::Field2D<double>& slope, std::vector<double>& s,
+                                                          std::vector<double>& ds) {
+            for (std::size_t c = 0; c < n; ++c) {
+                const double k = 0.02 + 0.05 * uniform(10000 + c);
+                theta[c] = 260.0 + 20.0 * uniform(11000 + c) + k * h[c];
+                slope[c] = k;
+                for (std::size_t l = 0; l < 2; ++l) {
+                    s[l * n + c] = 1004.64 * (250.0 + 10.0 * l + 0.5 * (theta[c] - 250.0));
+                    ds[l * n + c] = 1004.64 * 0.5 * k;
+                }
+            }
+        };
+        const planetsim::Field2D<double> floor(n, -std::numeric_limits<double>::infinity());
+        const auto coupled = planetsim::solve_implicit_advection_diffusion(graph, transport, floor, response);
+        EXPECT(coupled.consistency_residual_W_m2 <= 1e-6 && coupled.newton_iterations >= 1);
+        EXPECT(std::abs(coupled.sum_W) <= 1e-12 * coupled.absolute_sum_W);
+        EXPECT(coupled.energy_J_kg.size() == 2 * n);
+        const planetsim::AdvectionResponse short_response = [&](const planetsim::Field2D<double>&,
+                                                                planetsim::Field2D<double>&, planetsim::Field2D<double>&,
+                                                                std::vector<double>& s, std::vector<double>&) {
+            s.resize(1);
+        };
+        EXPECT(throws_with<std::logic_error>(
+            [&] { (void)planetsim::solve_implicit_advection_diffusion(graph, transport, floor, short_response); },
+            "advection-diffusion response has the wrong size"));
+    }
     stage("heat step batch");
     // HeatStepBatch: solvers of different graphs and precisions in one
     // launch give exactly what each solver's own step() gives.
