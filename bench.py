@@ -471,6 +471,184 @@ def transport_section(P, ctx, meshes, with_oracle, repeats=5):
     return out
 
 
+def _entry_sign(mesh):
+    """+1 where a CSR entry's cell is its edge's first cell, else -1."""
+    rows = np.repeat(np.arange(mesh.cells), np.diff(mesh.offset.astype(np.int64)))
+    return np.where(mesh.first[mesh.edge] == rows, 1.0, -1.0), rows
+
+
+def _tracer_inputs(mesh, layers, seed):
+    """Tracer-step inputs: antisymmetric layer mass fluxes per entry (kg/s),
+    symmetric eddy conductances, layer masses from the areas, mixing ratios."""
+    rng = np.random.default_rng(seed)
+    sign, _ = _entry_sign(mesh)
+    flux = rng.standard_normal((layers, mesh.edges)) * 2.0e7
+    return dict(layers=layers, outflow=np.ascontiguousarray((flux[:, mesh.edge] * sign[None, :]).ravel()),
+                conductance=np.ascontiguousarray(rng.uniform(0.5, 1.5, mesh.edges)[mesh.edge] * 3.0e9),
+                layer_mass=np.ascontiguousarray(mesh.area * 1.0e4), heat_capacity=1004.64, dt=2.63e6,
+                tracer=np.ascontiguousarray(rng.uniform(0.0, 2.0e-3, layers * mesh.cells)))
+
+
+def _advection_inputs(mesh, layers, seed):
+    """Coupled-transport inputs with a linear column response per node."""
+    rng = np.random.default_rng(seed)
+    sign, _ = _entry_sign(mesh)
+    conductance = np.ascontiguousarray(rng.uniform(0.5, 1.5, mesh.edges)[mesh.edge] * mesh.weight * 0.5
+                                       * mesh.radius ** 2)
+    outflow = np.ascontiguousarray((rng.standard_normal((layers, mesh.edges))[:, mesh.edge] * sign[None, :]
+                                    * 3.0e8).ravel())
+    cp, w, t0 = 1004.64, rng.uniform(0.2, 1.0, layers), 250.0 + 10.0 * np.arange(layers)
+    base, slopes = 250.0 + 40.0 * rng.random(mesh.cells), 0.02 + 0.1 * rng.random(mesh.cells)
+
+    def response(h):
+        theta = base + slopes * h
+        energy = cp * (t0[:, None] + w[:, None] * (theta[None, :] - 250.0))
+        return (theta, slopes.copy(), np.ascontiguousarray(energy.ravel()),
+                np.ascontiguousarray((cp * w[:, None] * slopes[None, :]).ravel()))
+
+    return dict(layers=layers, conductance=conductance, outflow=outflow, floor=np.full(mesh.cells, -np.inf),
+                response=response)
+
+
+def _flux_operator(mesh, seed, epsilon=1e-9):
+    """Eddy diffusion plus upwind advection in flux form (the balance
+    operator's shape) and its symmetric part, on a graph's CSR pattern."""
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(mesh.cells), np.diff(mesh.offset.astype(np.int64)))
+    nb = mesh.graph_neighbour.astype(np.int64)
+    key, back = rows * mesh.cells + nb, nb * mesh.cells + rows
+    order = np.argsort(key)
+    reverse = order[np.searchsorted(key[order], back)]
+    g = mesh.weight * rng.uniform(0.5, 1.5, mesh.weight.size)
+    g = 0.5 * (g + g[reverse])
+    face = rng.standard_normal(mesh.cells)
+    flow = 0.8 * (face[nb] - face[rows]) * mesh.weight
+    off = -g - np.maximum(-flow, 0.0)
+    div = np.bincount(rows, weights=flow, minlength=mesh.cells)
+    diag = (np.bincount(rows, weights=g + np.maximum(flow, 0.0), minlength=mesh.cells) + np.maximum(-div, 0.0)
+            + epsilon * mesh.area)
+    sym_diag = np.bincount(rows, weights=g, minlength=mesh.cells) + epsilon * mesh.area
+    return np.ascontiguousarray(diag), np.ascontiguousarray(off), sym_diag, -g
+
+
+def _timed(fn, repeats):
+    out = fn()
+    start = time.perf_counter()
+    for _ in range(repeats):
+        out = fn()
+    return out, (time.perf_counter() - start) / repeats * 1e3
+
+
+def transport_rows_section(P, ctx, meshes, with_oracle, repeats=3):
+    """The other consumers of the path, each on the device against the
+    oracle on one host thread where the oracle restates it: the implicit
+    tracer step (BiCGSTAB + block ILU(0) on the stacked layers), block
+    ILU(0) factor and solve, the coupled advection-diffusion Newton solve,
+    the pressure-Poisson PCG on the agglomerated groups and the balance
+    BiCGSTAB (V-cycle of the symmetric part, then block ILU(0)). Host
+    arrays in and out (PCIe included): these entry points take the
+    reference's host fields."""
+    S = None
+    if with_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import support as S
+    out = {"config": {"workload": "F3/F4 consumers on the L6 mesh (40,962 cells) and its agglomerated groups "
+                                  "(10,242); host arrays, timings per call", "repeats": repeats}}
+    m6 = meshes[6]
+    g6 = P.Graph(ctx, m6)
+    blocks = np.array(list(range(0, m6.cells, 256)) + [m6.cells], np.uint64)
+
+    # implicit tracer step, 2 layers (81,924 unknowns)
+    case = _tracer_inputs(m6, 2, 16)
+    (tr, tres), dev_ms = _timed(lambda: P.tracer_step(g6, case["layers"], case["conductance"], case["outflow"],
+                                                      case["layer_mass"], case["tracer"], case["dt"],
+                                                      case["heat_capacity"]), repeats)
+    entry = {"unknowns": 2 * m6.cells, "iterations": tres.iterations, "relative_residual": tres.relative_residual,
+             "device_ms": round(dev_ms, 3)}
+    if S is not None:
+        om = S.mesh("oracle", 6)
+        start = time.perf_counter()
+        expect, eres = S.oracle_tracer(om, case)
+        oracle_ms = (time.perf_counter() - start) * 1e3
+        entry.update({"oracle_ms_1_thread": round(oracle_ms, 1), "oracle_iterations": eres.iterations,
+                      "max_rel_diff_vs_oracle": float(np.max(np.abs(tr - expect)) / np.max(np.abs(expect))),
+                      "speedup_vs_oracle": round(oracle_ms / dev_ms, 1)})
+    out["tracer_step_l6_2_layers"] = entry
+
+    # block ILU(0): factor and one solve of a non-symmetric L6 operator
+    rng = np.random.default_rng(6)
+    off = -m6.weight * rng.uniform(0.5, 1.5, m6.weight.size)
+    _, rows = _entry_sign(m6)
+    diag = -np.bincount(rows, weights=off, minlength=m6.cells) + m6.area * 1e-9
+    ilu = P.Ilu(ctx, m6.offset, m6.graph_neighbour, blocks)
+    _, factor_ms = _timed(lambda: ilu.factor(diag, off), repeats)
+    x = rng.standard_normal(m6.cells)
+    y, solve_ms = _timed(lambda: ilu.solve(x), repeats)
+    entry = {"blocks": int(blocks.size - 1), "factor_ms": round(factor_ms, 3), "solve_ms": round(solve_ms, 3)}
+    if S is not None:
+        lib = S.oracle()
+        start = time.perf_counter()
+        h = lib.ilu_create(m6.cells, m6.offset, m6.graph_neighbour, np.ascontiguousarray(diag),
+                           np.ascontiguousarray(off), blocks, blocks.size - 1, 1.0)
+        mid = time.perf_counter()
+        z = x.copy()
+        lib.check(lib.ilu_solve(h, z))
+        end = time.perf_counter()
+        lib.ilu_free(h)
+        entry.update({"oracle_factor_ms_1_thread": round((mid - start) * 1e3, 3),
+                      "oracle_solve_ms_1_thread": round((end - mid) * 1e3, 3),
+                      "solve_bit_identical_to_oracle": bool(np.array_equal(y.view(np.uint64), z.view(np.uint64)))})
+    out["block_ilu_l6"] = entry
+
+    # coupled advection-diffusion (Newton with a host response), L5 and L6
+    for level in (5, 6):
+        m = meshes[level]
+        g = g6 if level == 6 else P.Graph(ctx, m)
+        case = _advection_inputs(m, 2, 9 + level)
+        (sd, md, _, rd), dev_ms = _timed(lambda: P.implicit_advection_diffusion(
+            g, case["layers"], case["conductance"], case["outflow"], case["floor"], case["response"]), repeats)
+        entry = {"nodes": m.cells, "layers": 2, "newton_iterations": rd.newton_iterations,
+                 "linear_iterations": rd.linear_iterations,
+                 "consistency_residual_W_m2": rd.consistency_residual_W_m2, "device_ms": round(dev_ms, 3)}
+        if S is not None:
+            om = S.mesh("oracle", level)
+            start = time.perf_counter()
+            sr, mr, _, rr = S.solve_advection(S.oracle(), om.handle, om.n, case)
+            oracle_ms = (time.perf_counter() - start) * 1e3
+            entry.update({"oracle_ms_1_thread": round(oracle_ms, 1), "oracle_newton_iterations": rr.newton_iterations,
+                          "max_abs_dH_vs_oracle_W_m2": float(np.max(np.abs(sd - sr))),
+                          "max_abs_dT_vs_oracle_K": float(np.max(np.abs(md - mr))),
+                          "speedup_vs_oracle": round(oracle_ms / dev_ms, 1)})
+        out[f"advection_diffusion_l{level}"] = entry
+
+    # pressure-Poisson PCG on the agglomerated L6 groups (10,242 nodes)
+    agg = P.Agglomeration(m6)
+    gg, am = P.Graph(ctx, agg.graph), agg.graph
+    s = P.System(gg)
+    s.set_matrix(am.weight_sum * (1.0 + 1.0e-10), -am.weight)
+    rhs = np.random.default_rng(6).standard_normal(am.cells) * am.area
+    rhs -= rhs.mean()
+    (_, pinfo), pcg_ms = _timed(lambda: P.pcg_operator(s, am.weight_sum, rhs, tolerance=1e-6, max_iterations=500),
+                                repeats)
+    out["pressure_poisson_pcg_l6_groups"] = {"nodes": am.cells, "iterations": pinfo.iterations,
+                                             "relative_residual": pinfo.relative_residual,
+                                             "device_ms": round(pcg_ms, 3)}
+
+    # balance BiCGSTAB on L6 cells: V-cycle of the symmetric part, then ILU(0)
+    diag, off, sym_diag, sym_off = _flux_operator(m6, 13)
+    mg = P.System(g6)
+    mg.set_matrix(sym_diag * (1.0 + 1.0e-8), sym_off)
+    bilu = P.Ilu(ctx, m6.offset, m6.graph_neighbour, blocks)
+    bilu.factor(diag, off, 1.0 + 1.0e-8)
+    b = np.random.default_rng(13).standard_normal(m6.cells) * m6.area
+    (_, binfo), bal_ms = _timed(lambda: P.operator_solve(g6, diag, off, b, multigrid=mg, ilu=bilu,
+                                                         form=P.FORM_BALANCE, tolerance=1e-10), repeats)
+    out["balance_bicgstab_l6"] = {"nodes": m6.cells, "iterations": binfo.iterations,
+                                  "true_relative_residual": binfo.relative_residual, "status": binfo.status,
+                                  "device_ms": round(bal_ms, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -479,6 +657,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--workgroups", type=int, default=0, help="override the solver's workgroup count")
+    ap.add_argument("--only-consumers", action="store_true",
+                    help="run only the F3/F4 consumer measurements (tracer, ILU, advection, operator solves)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed-precision, batched and served measurements")
     args = ap.parse_args()
@@ -490,6 +670,11 @@ def main():
     import planet_game_amd as P
 
     level = 6
+    if args.only_consumers:
+        ctx = P.Context(0)
+        meshes = {L: P.Mesh(L) for L in (5, 6)}
+        print(json.dumps({"transport_consumers": transport_rows_section(P, ctx, meshes, not args.no_cpu)}))
+        return
     mesh = P.Mesh(level)
     seed = P.instance_seed(0)
     cond, ca, land, t0 = P.heat_instance(mesh, seed, 1.0e6, 0.3)
@@ -664,6 +849,7 @@ def main():
         line["batched"] = batched_section(P, ctx, meshes, graphs, hierarchy_levels)
         line["served"] = served_section(P, ctx, meshes, graphs, value)
         line["implicit_transport"] = transport_section(P, ctx, meshes, not args.no_cpu)
+        line["transport_consumers"] = transport_rows_section(P, ctx, meshes, not args.no_cpu)
     print(json.dumps(line))
 
 

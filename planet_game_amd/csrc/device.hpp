@@ -159,7 +159,7 @@ int program_blocks_per_cu(size_t dynamic_lds);
 // The program kernel's static LDS bytes.
 size_t program_static_lds();
 
-constexpr int kBarrierWordCount = 64 + 32 * 8;   // generation, root, 8 leaves (128-byte apart)
+constexpr int kBarrierWordCount = 96 + 64 * 8;   // generation, 2 roots, 8 leaves x 2 (128-byte apart)
 
 // ---- server: a persistent launch whose workgroups are partitioned into
 // slots; each slot takes requests from one queue, one at a time.

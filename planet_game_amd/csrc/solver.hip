@@ -38,6 +38,15 @@ namespace {
 
 // ------------------------------------------------------------ reductions
 
+// Values that a later phase reads from other workgroups are written with
+// agent-scope (coherent) stores: they go through to memory instead of
+// staying dirty in this XCD's L2, so the release fence of the next group
+// barrier has (almost) nothing left to write back. The barrier's fences
+// still order them; only where the bytes wait changes.
+template <class T> __device__ __forceinline__ void publish(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ inline double wave_sum(double v) {
     // xor butterfly: every lane ends with the same bits
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -97,7 +106,7 @@ template <int K> __device__ inline void store_partials(double* part, int tile, c
     tile_sums<K>(v, tot, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < K; ++q) part[tile * kPartials + q] = tot[q];
+        for (int q = 0; q < K; ++q) publish(&part[tile * kPartials + q], tot[q]);
     }
 }
 
@@ -119,7 +128,7 @@ __device__ inline void store_partials_n(double* part, const int (&tl)[R], const 
 #pragma unroll
             for (int r = 0; r < R; ++r)
 #pragma unroll
-                for (int q = 0; q < K; ++q) part[tl[r] * kPartials + q] = tot[r * K + q];
+                for (int q = 0; q < K; ++q) publish(&part[tl[r] * kPartials + q], tot[r * K + q]);
         }
     }
 }
@@ -225,19 +234,34 @@ __device__ inline void trace_stamp() {
 // arrives on leaf counter g % kLeaves (separate 128-byte lines, so the
 // arrivals do not serialise on one address); the last arrival of a leaf
 // arrives at the root; the last root arrival bumps the generation word that
-// every thread 0 polls. Writes are published with one agent-scope release
-// fence per workgroup and read after one acquire fence; the poll itself
-// uses relaxed (coherent, non-invalidating) loads. The wait is bounded by
-// the 100 MHz wall clock (about one second); a timeout is recorded and is
-// sticky, so a broken group drains within seconds and the host reports it.
+// every thread 0 polls. The wait is bounded by the 100 MHz wall clock
+// (about one second); a timeout is recorded and is sticky, so a broken
+// group drains within seconds and the host reports it.
 //
-// Barrier words (uint32, zeroed at creation, left zeroed by each episode):
-//   [0]         generation
-//   [32]        root arrivals
-//   [64 + 32 l] arrivals of leaf l
+// Ordering, with the fewest cache-maintenance fences: every workgroup
+// issues one agent-scope release fence before its (relaxed) leaf arrival
+// and one acquire fence after its (relaxed) poll; only the last arrival of
+// a leaf and the last root arrival add an acq_rel fence between the RMW
+// they observed and the one they perform. Fence-fence synchronisation
+// through each counter's RMW chain then orders every workgroup's writes
+// before every workgroup's reads after the barrier. (An acq_rel RMW would
+// add a second L2 write-back and invalidate per workgroup per barrier.)
+//
+// Counters come in two sets used by alternate episodes (the parity of the
+// generation observed on entry). The last root arrival clears the whole
+// other set (root and all eight leaves) before its acq_rel fence, which
+// orders those stores before the generation bump: that set is idle (every
+// workgroup has passed the episode that used it) and is next used only
+// after this episode completes. Between launches the set of the current
+// parity is therefore all zero, whatever the group size was or will be.
+//
+// Barrier words (uint32, zeroed at creation):
+//   [0]                  generation
+//   [32 + 32 p]          root arrivals, parity p
+//   [96 + 64 l + 32 p]   arrivals of leaf l, parity p
 constexpr long long kBarrierTicks = 100000000;
 constexpr int kLeaves = 8;
-static_assert(64 + 32 * kLeaves == kBarrierWordCount, "barrier layout");
+static_assert(96 + 64 * kLeaves == kBarrierWordCount, "barrier layout");
 
 __device__ __forceinline__ void group_sync(int& timeouts) {
     __syncthreads();
@@ -248,21 +272,28 @@ __device__ __forceinline__ void group_sync(int& timeouts) {
     }
     if (threadIdx.x == 0 && timeouts == 0) {
         uint32_t* gen = grp.barrier;
-        uint32_t* root = grp.barrier + 32;
         const int g = static_cast<int>(blockIdx.x) - grp.first;
         const int leaf = g % kLeaves;
         const int leaves = G < kLeaves ? G : kLeaves;
         const uint32_t leaf_size = static_cast<uint32_t>((G - leaf + kLeaves - 1) / kLeaves);
-        uint32_t* mine = grp.barrier + 64 + 32 * leaf;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const uint32_t seen = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t p = seen & 1u;
+        uint32_t* root = grp.barrier + 32 + 32 * p;
+        uint32_t* mine = grp.barrier + 96 + 64 * leaf + 32 * p;
         bool last = false;
-        if (__hip_atomic_fetch_add(mine, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1 == leaf_size) {
-            __hip_atomic_store(mine, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (__hip_atomic_fetch_add(root, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1 ==
+        if (__hip_atomic_fetch_add(mine, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == leaf_size) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            if (__hip_atomic_fetch_add(root, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 ==
                 static_cast<uint32_t>(leaves)) {
-                __hip_atomic_store(root, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(gen, seen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                // clear the whole idle set, every leaf included: a leaf this
+                // group does not use now may hold counts from a larger group
+                __hip_atomic_store(grp.barrier + 32 + 32 * (p ^ 1u), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int l = 0; l < kLeaves; ++l)
+                    __hip_atomic_store(grp.barrier + 96 + 64 * l + 32 * (p ^ 1u), 0u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+                __hip_atomic_store(gen, seen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 last = true;
             }
         }
@@ -275,8 +306,8 @@ __device__ __forceinline__ void group_sync(int& timeouts) {
                     break;
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     trace_stamp();
     __syncthreads();
@@ -443,7 +474,7 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (ok[r]) {
-                if (keep != nullptr) keep[i[r]] = wi[r];
+                if (keep != nullptr) publish(&keep[i[r]], wi[r]);
                 lds.tile[r][threadIdx.x] = static_cast<double>(acc[r]);
             }
         if (part != nullptr) store_partials_n<1, R>(part, tl, sq, lds);
@@ -454,7 +485,7 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
                 const int c = c0[r] + threadIdx.x;
                 double sum = 0.0;
                 for (int q = L.agg_start[c] - base[r]; q < L.agg_start[c + 1] - base[r]; ++q) sum += lds.tile[r][q];
-                coarse_rhs[c] = static_cast<VT>(sum);
+                publish(&coarse_rhs[c], static_cast<VT>(sum));
             }
         __syncthreads();
     });
@@ -530,8 +561,9 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
                         return om * dinv[c] * rhs[c] + static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
                     },
                     [&](VT o, VT v) { acc -= o * v; });
-                out[i] = x2 + om * dinv[i] * acc;
-                q[0] = dot_with != nullptr ? static_cast<double>(out[i]) * static_cast<double>(dot_with[i]) : 0.0;
+                const VT value = x2 + om * dinv[i] * acc;
+                publish(&out[i], value);
+                q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(dot_with[i]) : 0.0;
             }
             if (dot_part != nullptr) store_partials<1>(dot_part, tile, q, lds);
         }
@@ -561,7 +593,7 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
             q[r][0] = 0.0;
             if (ok[r]) {
                 const VT value = x2[r] + om * dinv[i[r]] * acc[r];
-                out[i[r]] = value;
+                publish(&out[i[r]], value);
                 if (dot_with != nullptr) q[r][0] = static_cast<double>(value) * static_cast<double>(dot_with[i[r]]);
             }
         }
@@ -581,7 +613,7 @@ __device__ __forceinline__ void vcycle_first(const Instance& I, const Values<VT>
             double sq[1] = {0.0};
             if (i < I.n) {
                 const VT wi = src.at(i);
-                keep[i] = wi;
+                publish(&keep[i], wi);
                 sq[0] = static_cast<double>(wi) * static_cast<double>(wi);
             }
             if (part != nullptr) store_partials<1>(part, tile, sq, lds);
@@ -608,8 +640,9 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
             const int i = first + threadIdx.x;
             double q[1] = {0.0};
             if (i < I.n) {
-                out0[i] = static_cast<VT>(lds.cvec[i]);
-                if (dot_with != nullptr) q[0] = static_cast<double>(out0[i]) * static_cast<double>(dot_with[i]);
+                const VT value = static_cast<VT>(lds.cvec[i]);
+                publish(&out0[i], value);
+                if (dot_with != nullptr) q[0] = static_cast<double>(value) * static_cast<double>(dot_with[i]);
             }
             if (dot_part != nullptr) store_partials<1>(dot_part, tile, q, lds);
         }
@@ -1063,11 +1096,11 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
             for (int k = 0; k < R; ++k) {
                 q[k][0] = q[k][1] = 0.0;
                 if (ok[k]) {
-                    b[i[k]] = bi[k];
-                    x[i[k]] = 0.0;
+                    publish(&b[i[k]], bi[k]);
+                    publish(&x[i[k]], 0.0);
                     const VT ri = static_cast<VT>(bi[k]);
-                    r[i[k]] = ri;
-                    r0[i[k]] = ri;
+                    publish(&r[i[k]], ri);
+                    publish(&r0[i[k]], ri);
                     q[k][0] = bi[k] * bi[k];
                     q[k][1] = static_cast<double>(ri) * static_cast<double>(ri);
                 }
@@ -1101,8 +1134,8 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         if (i < n) {
                             const double ri = residual_row(I, x, i);
                             const VT rv = static_cast<VT>(ri);
-                            r[i] = rv;
-                            r0[i] = rv;
+                            publish(&r[i], rv);
+                            publish(&r0[i], rv);
                             q[0] = ri * ri;
                             q[1] = static_cast<double>(rv) * static_cast<double>(rv);
                         }
@@ -1154,7 +1187,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     for (int k = 0; k < R; ++k) {
                         q[k][0] = 0.0;
                         if (ok[k]) {
-                            v[i[k]] = acc[k];
+                            publish(&v[i[k]], acc[k]);
                             q[k][0] = static_cast<double>(rz[k]) * static_cast<double>(acc[k]);
                         }
                     }
@@ -1181,7 +1214,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         double q[2] = {0.0, 0.0};
                         if (i < n) {
                             const double x_new = x[i] + alpha * static_cast<double>(ph[i]);
-                            xn[i] = x_new;
+                            publish(&xn[i], x_new);
                             double ax = I.diag64[0][i] * x_new;
                             ell_row(
                                 L0.col, I.off64[0], n, W, i,
@@ -1189,8 +1222,8 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                                 [&](double o, double v) { ax += o * v; });
                             const double ri = b[i] - ax;
                             const VT rv = static_cast<VT>(ri);
-                            r[i] = rv;
-                            r0[i] = rv;
+                            publish(&r[i], rv);
+                            publish(&r0[i], rv);
                             q[0] = ri * ri;
                             q[1] = static_cast<double>(rv) * static_cast<double>(rv);
                         }
@@ -1222,7 +1255,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     for (int k = 0; k < R; ++k) {
                         q[k][0] = q[k][1] = 0.0;
                         if (ok[k]) {
-                            t[i[k]] = acc[k];
+                            publish(&t[i[k]], acc[k]);
                             const double td = static_cast<double>(acc[k]);
                             q[k][0] = td * td;
                             q[k][1] = td * static_cast<double>(sv[k]);
@@ -1257,10 +1290,10 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         for (int k = 0; k < R; ++k) {
                             q[k][0] = q[k][1] = 0.0;
                             if (ok[k]) {
-                                x[i[k]] = xv[k] + (alpha * pv[k] + omega * shv[k]);
+                                publish(&x[i[k]], xv[k] + (alpha * pv[k] + omega * shv[k]));
                                 const double ri = sv[k] - omega * tv[k];
                                 const VT rv = static_cast<VT>(ri);
-                                r[i[k]] = rv;
+                                publish(&r[i[k]], rv);
                                 q[k][0] = static_cast<double>(rv) * static_cast<double>(rv);
                                 q[k][1] = rz[k] * static_cast<double>(rv);
                             }
@@ -1280,7 +1313,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
         const double* x = I.x[xi];
         for (int tile = g; tile < tiles; tile += G) {
             const int i = tile * kTile + threadIdx.x;
-            if (i < n) t_new[i] = x[i];
+            if (i < n) publish(&t_new[i], x[i]);
         }
     }
     SolveResult out{};
@@ -1338,8 +1371,8 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
         double v[1] = {0.0};
         if (i < n) {
             const double bi = P.in[i];
-            r[i] = bi;
-            x[i] = 0.0;
+            publish(&r[i], bi);
+            publish(&x[i], 0.0);
             v[0] = bi * bi;
         }
         store_partials<1>(part, tile, v, lds);
@@ -1380,8 +1413,8 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
                 ell_row(
                     L0.col, off, n, W, i, [&](int, int c) { return src.at(c); },
                     [&](double o, double w) { acc += o * w; });
-                dn[i] = di_value;
-                q[i] = acc;
+                publish(&dn[i], di_value);
+                publish(&q[i], acc);
                 v[0] = di_value * acc;
             }
             store_partials<1>(part, tile, v, lds);
@@ -1397,9 +1430,10 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
             const int i = tile * kTile + threadIdx.x;
             double v[1] = {0.0};
             if (i < n) {
-                x[i] += alpha * dn[i];
-                r[i] -= alpha * q[i];
-                v[0] = r[i] * r[i];
+                const double ri = r[i] - alpha * q[i];
+                publish(&x[i], x[i] + alpha * dn[i]);
+                publish(&r[i], ri);
+                v[0] = ri * ri;
             }
             store_partials<1>(part, tile, v, lds);
         }
@@ -1415,7 +1449,7 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
     }
     for (int tile = g; tile < tiles; tile += G) {
         const int i = tile * kTile + threadIdx.x;
-        if (i < n) P.out[i] = x[i];
+        if (i < n) publish(&P.out[i], x[i]);
     }
     SolveResult out{};
     out.iterations = iterations;
