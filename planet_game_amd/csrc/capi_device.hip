@@ -315,11 +315,9 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
                 A.plan(&ptrs[j][8], L.d_off_start.size());
                 A.plan(&ptrs[j][9], L.d_off.size());
                 A.plan(&ptrs[j][12], L.agg_ell.size());
+                A.plan(&ptrs[j][11], L.down_rows.size());
             }
-            if (!L.need_start.empty()) {
-                A.plan(&ptrs[j][10], L.need_start.size());
-                A.plan(&ptrs[j][11], L.need.size());
-            }
+            if (!L.need_pad.empty()) A.plan(&ptrs[j][10], L.need_pad.size());
         }
         A.allocate();
         psim::put(gr->area, area, n * 8);
@@ -341,6 +339,8 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
                                                       &L.g_diag_start, &L.d_diag, &L.d_off_start, &L.d_off};
                 for (int q = 0; q < 8; ++q) psim::put(ptrs[j][2 + q], src[q]->data(), src[q]->size() * 4);
                 psim::put(ptrs[j][12], L.agg_ell.data(), L.agg_ell.size() * 4);
+                psim::put(ptrs[j][11], L.down_rows.data(), L.down_rows.size() * 4);
+                v.down_rows = ptrs[j][11];
                 v.agg = ptrs[j][2];
                 v.agg_col = ptrs[j][12];
                 v.agg_start = ptrs[j][3];
@@ -353,11 +353,10 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
                 v.go = ptrs[j][9];
                 v.n_next = lv[j + 1].n;
             }
-            if (!L.need_start.empty()) {
-                psim::put(ptrs[j][10], L.need_start.data(), L.need_start.size() * 4);
-                psim::put(ptrs[j][11], L.need.data(), L.need.size() * 4);
-                v.need_start = ptrs[j][10];
-                v.need = ptrs[j][11];
+            if (!L.need_pad.empty()) {
+                psim::put(ptrs[j][10], L.need_pad.data(), L.need_pad.size() * 4);
+                v.need = ptrs[j][10];
+                v.need_stride = L.need_stride;
             }
         }
         *out = gr.release();

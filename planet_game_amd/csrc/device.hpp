@@ -62,8 +62,9 @@ struct LevelView {
     const int32_t* gd = nullptr;          //   (ELL positions, or -(row+1) for a fine diagonal)
     const int32_t* go_start = nullptr;    // ... and of level j+1's ELL positions
     const int32_t* go = nullptr;
-    const int32_t* need_start = nullptr;  // level above the coarsest: coarse rows each tile reads
-    const int32_t* need = nullptr;
+    const int32_t* down_rows = nullptr;   // down_tiles * 256: each restriction tile's rows (-1 padding)
+    const int32_t* need = nullptr;        // level above the coarsest: coarse rows each 256-row tile
+    int32_t need_stride = 0;              //   reads, tile t's at need[t * need_stride ..) (-1 padding)
 };
 
 struct SolveResult {

@@ -146,6 +146,13 @@ void restriction_tiles(HostLevel& fine, int32_t coarse_n) {
         }
     }
     fine.tile_start.push_back(coarse_n);
+    const size_t down_tiles = fine.tile_start.size() - 1;
+    fine.down_rows.assign(down_tiles * static_cast<size_t>(tile_rows), -1);
+    for (size_t t = 0; t < down_tiles; ++t) {
+        const int32_t first = fine.agg_start[fine.tile_start[t]], last = fine.agg_start[fine.tile_start[t + 1]];
+        for (int32_t q = first; q < last; ++q)
+            fine.down_rows[t * static_cast<size_t>(tile_rows) + static_cast<size_t>(q - first)] = fine.agg_rows[q];
+    }
 }
 
 // Slot-major ELL columns of a level (padding -1).
@@ -239,6 +246,13 @@ HostHierarchy build_hierarchy(const HostGraph& g) {
             lv.need.insert(lv.need.end(), rows.begin(), rows.end());
             lv.need_start[t + 1] = static_cast<int32_t>(lv.need.size());
         }
+        lv.need_stride = 0;
+        for (int32_t t = 0; t < tiles; ++t) lv.need_stride = std::max(lv.need_stride, lv.need_start[t + 1] - lv.need_start[t]);
+        lv.need_pad.assign(static_cast<size_t>(tiles) * static_cast<size_t>(lv.need_stride), -1);
+        for (int32_t t = 0; t < tiles; ++t)
+            for (int32_t q = lv.need_start[t]; q < lv.need_start[t + 1]; ++q)
+                lv.need_pad[static_cast<size_t>(t) * static_cast<size_t>(lv.need_stride) +
+                            static_cast<size_t>(q - lv.need_start[t])] = lv.need[q];
     }
     return h;
 }

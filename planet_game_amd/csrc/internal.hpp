@@ -126,6 +126,13 @@ struct HostLevel {
     // coarse rows its prolongation reads (aggregates of its rows and of
     // their neighbours), sorted.
     std::vector<int32_t> need_start, need;
+    // The same lists padded to a common stride (-1 padding), so that a
+    // tile's row ids are one load away: tile t's list at need_pad[t * need_stride ..).
+    int32_t need_stride = 0;
+    std::vector<int32_t> need_pad;
+    // Rows of each restriction tile in aggregate-major order, padded to the
+    // tile size (-1 padding): down tile t's rows at down_rows[t * tile_rows ..).
+    std::vector<int32_t> down_rows;
 };
 
 // ELL position of CSR entry k of row i.

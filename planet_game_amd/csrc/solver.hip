@@ -448,15 +448,21 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
     const VT* dinv = V.dinv(j);
     const VT* off = V.off(j);
     tile_loop<sizeof(VT) == 4>(L.down_tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+        // the tile's rows come from its padded row list (one load); the
+        // aggregate bounds the restriction needs load alongside
         int c0[R], c1[R], base[R], i[R];
         bool ok[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
+            i[r] = L.down_rows[tl[r] * kTile + threadIdx.x];
             c0[r] = L.tile_start[tl[r]];
             c1[r] = L.tile_start[tl[r] + 1];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            ok[r] = i[r] >= 0;
+            i[r] = ok[r] ? i[r] : 0;
             base[r] = L.agg_start[c0[r]];
-            ok[r] = static_cast<int>(threadIdx.x) < L.agg_start[c1[r]] - base[r];
-            i[r] = ok[r] ? L.agg_rows[base[r] + threadIdx.x] : 0;
         }
         VT wi[R], acc[R];
         double sq[R][1];
@@ -491,37 +497,47 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
     });
 }
 
-// Rows [list[0..count)) of the coarsest level's solution A_c^-1 rhs into
-// lds.cvec (indexed by coarse row), in FP64. One wave per row: lanes split
-// the columns (row-major reads of the symmetric inverse, coalesced) and a
-// butterfly sums them, so every workgroup computing a row gets the same
-// bits. `list` null means rows first .. first + count - 1.
+// Rows of the coarsest level's solution A_c^-1 rhs into lds.cvec (indexed
+// by coarse row), in FP64: the `count` rows list[0..count) (entries < 0 are
+// skipped), or rows first .. first + count - 1 when list is null. One wave
+// per row: lanes split the columns (row-major reads of the symmetric
+// inverse, coalesced), each lane adding its columns in ascending order from
+// 0, and a butterfly sums the lanes, so every workgroup computing a row
+// gets the same bits. A wave takes kRows rows at a time; the first round's
+// row ids are loaded before the right-hand side is staged.
 template <class VT>
 __device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs, const int32_t* list, int first, int count,
                                 Lds& lds) {
+    constexpr int kRows = 4;   // rows in flight per wave
     const int n = I.lv[I.depth - 1].n;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int a[kRows];
+    const auto row_ids = [&](int q0) {
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const int q = q0 + u;
+            a[u] = q < count ? (list != nullptr ? list[q] : first + q) : -1;
+        }
+    };
+    row_ids(wave * kRows);
     for (int k = threadIdx.x; k < n; k += blockDim.x) lds.crhs[k] = static_cast<double>(rhs[k]);
     __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const double* inv = I.cinv;
-    constexpr int kRows = 4;   // rows in flight per wave
     for (int q0 = wave * kRows; q0 < count; q0 += 4 * kRows) {
+        if (q0 != wave * kRows) row_ids(q0);
         double acc[kRows];
 #pragma unroll
         for (int u = 0; u < kRows; ++u) {
             acc[u] = 0.0;
-            const int q = q0 + u;
-            if (q < count) {
-                const int a = list != nullptr ? list[q] : first + q;
-                const double* row = inv + static_cast<size_t>(a) * n;
+            if (a[u] >= 0) {
+                const double* row = inv + static_cast<size_t>(a[u]) * n;
                 for (int k = lane; k < n; k += 64) acc[u] += row[k] * lds.crhs[k];
             }
         }
 #pragma unroll
         for (int u = 0; u < kRows; ++u) {
             const double total = wave_sum(acc[u]);
-            const int q = q0 + u;
-            if (lane == 0 && q < count) lds.cvec[list != nullptr ? list[q] : first + q] = total;
+            if (lane == 0 && a[u] >= 0) lds.cvec[a[u]] = total;
         }
     }
     __syncthreads();
@@ -545,10 +561,10 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
         // the level above the coarsest: each tile first solves the coarse rows
         // it reads into LDS, so its tiles go one at a time
         for (int tile = g; tile < tiles; tile += G) {
-            coarse_rows_lds<VT>(I, static_cast<const VT*>(I.lvl_b[j + 1]), L.need + L.need_start[tile], 0,
-                                L.need_start[tile + 1] - L.need_start[tile], lds);
             const int i = tile * kTile + threadIdx.x;
             double q[1] = {0.0};
+            coarse_rows_lds<VT>(I, static_cast<const VT*>(I.lvl_b[j + 1]), L.need + tile * L.need_stride, 0,
+                                L.need_stride, lds);
             if (i < n) {
                 const VT x2 = om * dinv[i] * rhs[i] + static_cast<VT>(lds.cvec[L.agg[i]]);
                 VT acc = rhs[i] - diag[i] * x2;
