@@ -299,7 +299,7 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
         A.plan(&gr->area, n);
         A.plan(&gr->weight, nnz);
         A.plan(&gr->csr_col, nnz);
-        int32_t* ptrs[kMaxLevels][13] = {};
+        int32_t* ptrs[kMaxLevels][17] = {};
         for (int j = 0; j < gr->depth; ++j) {
             const psim::HostLevel& L = lv[j];
             const bool last = j + 1 == gr->depth;
@@ -318,6 +318,12 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
                 A.plan(&ptrs[j][11], L.down_rows.size());
             }
             if (!L.need_pad.empty()) A.plan(&ptrs[j][10], L.need_pad.size());
+            if (!L.mid_pad.empty()) {
+                A.plan(&ptrs[j][13], L.mid_pad.size());
+                A.plan(&ptrs[j][14], L.mid_coarse_pad.size());
+                A.plan(&ptrs[j][15], L.agg_loc.size());
+                A.plan(&ptrs[j][16], L.agg_col_loc.size());
+            }
         }
         A.allocate();
         psim::put(gr->area, area, n * 8);
@@ -357,6 +363,18 @@ int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint6
                 psim::put(ptrs[j][10], L.need_pad.data(), L.need_pad.size() * 4);
                 v.need = ptrs[j][10];
                 v.need_stride = L.need_stride;
+            }
+            if (!L.mid_pad.empty()) {
+                psim::put(ptrs[j][13], L.mid_pad.data(), L.mid_pad.size() * 4);
+                psim::put(ptrs[j][14], L.mid_coarse_pad.data(), L.mid_coarse_pad.size() * 4);
+                psim::put(ptrs[j][15], L.agg_loc.data(), L.agg_loc.size() * 4);
+                psim::put(ptrs[j][16], L.agg_col_loc.data(), L.agg_col_loc.size() * 4);
+                v.mid = ptrs[j][13];
+                v.mid_coarse = ptrs[j][14];
+                v.agg_loc = ptrs[j][15];
+                v.agg_col_loc = ptrs[j][16];
+                v.mid_stride = L.mid_stride;
+                v.mid_coarse_stride = L.mid_coarse_stride;
             }
         }
         *out = gr.release();

@@ -65,6 +65,14 @@ struct LevelView {
     const int32_t* down_rows = nullptr;   // down_tiles * 256: each restriction tile's rows (-1 padding)
     const int32_t* need = nullptr;        // level above the coarsest: coarse rows each 256-row tile
     int32_t need_stride = 0;              //   reads, tile t's at need[t * need_stride ..) (-1 padding)
+    // two levels above the coarsest (merged prolongation): per tile, the
+    // next level's rows and the coarsest rows it reads (-1 padding), and each
+    // row's / ELL slot's aggregate as a position in its tile's mid list
+    const int32_t* mid = nullptr;
+    const int32_t* mid_coarse = nullptr;
+    const int32_t* agg_loc = nullptr;
+    const int32_t* agg_col_loc = nullptr;
+    int32_t mid_stride = 0, mid_coarse_stride = 0;
 };
 
 struct SolveResult {

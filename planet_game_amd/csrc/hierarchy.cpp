@@ -193,6 +193,55 @@ void ell_galerkin(HostLevel& fine, const HostLevel& coarse) {
     }
 }
 
+// Sorted unique aggregates of the given rows and of their neighbours.
+std::vector<int32_t> read_aggregates(const HostLevel& lv, const std::vector<int32_t>& rows) {
+    std::vector<int32_t> out;
+    for (const int32_t i : rows) {
+        out.push_back(lv.aggregate[i]);
+        for (int32_t k = lv.row[i]; k < lv.row[i + 1]; ++k) out.push_back(lv.aggregate[lv.col[k]]);
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+}
+
+// The lists of the merged prolongation phase on `fine` (two levels above
+// the coarsest), whose aggregates are the rows of `mid`. Left empty when a
+// tile would read more mid rows than a tile holds.
+void mid_lists(HostLevel& fine, const HostLevel& mid) {
+    const int32_t tiles = (fine.n + tile_rows - 1) / tile_rows;
+    std::vector<std::vector<int32_t>> mids(static_cast<size_t>(tiles)), coarse(static_cast<size_t>(tiles));
+    int32_t s1 = 0, s2 = 0;
+    for (int32_t t = 0; t < tiles; ++t) {
+        std::vector<int32_t> rows;
+        for (int32_t i = t * tile_rows; i < std::min(fine.n, (t + 1) * tile_rows); ++i) rows.push_back(i);
+        mids[t] = read_aggregates(fine, rows);
+        coarse[t] = read_aggregates(mid, mids[t]);
+        s1 = std::max(s1, static_cast<int32_t>(mids[t].size()));
+        s2 = std::max(s2, static_cast<int32_t>(coarse[t].size()));
+    }
+    if (s1 > tile_rows) return;
+    fine.mid_stride = s1;
+    fine.mid_coarse_stride = s2;
+    fine.mid_pad.assign(static_cast<size_t>(tiles) * static_cast<size_t>(s1), -1);
+    fine.mid_coarse_pad.assign(static_cast<size_t>(tiles) * static_cast<size_t>(s2), -1);
+    fine.agg_loc.assign(static_cast<size_t>(fine.n), -1);
+    fine.agg_col_loc.assign(fine.col_ell.size(), -1);
+    for (int32_t t = 0; t < tiles; ++t) {
+        const auto& m = mids[t];
+        std::copy(m.begin(), m.end(), fine.mid_pad.begin() + static_cast<std::ptrdiff_t>(t) * s1);
+        std::copy(coarse[t].begin(), coarse[t].end(), fine.mid_coarse_pad.begin() + static_cast<std::ptrdiff_t>(t) * s2);
+        const auto local = [&](int32_t a) {
+            return static_cast<int32_t>(std::lower_bound(m.begin(), m.end(), a) - m.begin());
+        };
+        for (int32_t i = t * tile_rows; i < std::min(fine.n, (t + 1) * tile_rows); ++i) {
+            fine.agg_loc[i] = local(fine.aggregate[i]);
+            for (int32_t k = fine.row[i]; k < fine.row[i + 1]; ++k)
+                fine.agg_col_loc[ell_position(fine, i, k)] = local(fine.aggregate[fine.col[k]]);
+        }
+    }
+}
+
 }  // namespace
 
 HostHierarchy build_hierarchy(const HostGraph& g) {
@@ -254,6 +303,7 @@ HostHierarchy build_hierarchy(const HostGraph& g) {
                 lv.need_pad[static_cast<size_t>(t) * static_cast<size_t>(lv.need_stride) +
                             static_cast<size_t>(q - lv.need_start[t])] = lv.need[q];
     }
+    if (h.levels.size() >= 3) mid_lists(h.levels[h.levels.size() - 3], h.levels[h.levels.size() - 2]);
     return h;
 }
 

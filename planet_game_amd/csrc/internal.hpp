@@ -133,6 +133,15 @@ struct HostLevel {
     // Rows of each restriction tile in aggregate-major order, padded to the
     // tile size (-1 padding): down tile t's rows at down_rows[t * tile_rows ..).
     std::vector<int32_t> down_rows;
+    // On the level two above the coarsest, for the phase that prolongs from
+    // the coarsest level through the next one in one sweep: per 256-row tile
+    // the next level's rows its rows read (aggregates of its rows and of
+    // their neighbours, sorted; mid_pad, stride mid_stride) and the coarsest
+    // rows those read (mid_coarse_pad, stride mid_coarse_stride), both
+    // padded with -1; and for every row (agg_loc) and ELL slot
+    // (agg_col_loc) the position of its aggregate in its tile's list.
+    int32_t mid_stride = 0, mid_coarse_stride = 0;
+    std::vector<int32_t> mid_pad, mid_coarse_pad, agg_loc, agg_col_loc;
 };
 
 // ELL position of CSR entry k of row i.

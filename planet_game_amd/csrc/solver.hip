@@ -36,6 +36,19 @@
 namespace psim::dev {
 namespace {
 
+// A pointer every lane holds, rebuilt in scalar registers: descriptor reads
+// through it are scalar loads (a function argument arrives in vector
+// registers otherwise).
+template <class T> __device__ inline const T* uniform_ptr(const T* p) {
+    const uint64_t bits = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits >> 32));
+    return reinterpret_cast<const T*>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+template <class T> __device__ inline T* uniform_ptr(T* p) {
+    return const_cast<T*>(uniform_ptr(static_cast<const T*>(p)));
+}
+
 // ------------------------------------------------------------ reductions
 
 // Values that a later phase reads from other workgroups are written with
@@ -617,6 +630,79 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
     });
 }
 
+// Prolongation from the coarsest level through level j + 1 and on to level
+// j (two levels above the coarsest) in one sweep, for groups with about a
+// workgroup per tile of level j: each tile solves the coarsest rows its
+// level-(j+1) rows read (coarse_rows_lds), post-smooths those level-(j+1)
+// rows into LDS with exactly up_phase's operations, then prolongs and
+// post-smooths its own rows reading them from LDS. Every value is the one
+// the two separate phases compute; the barrier between them is gone.
+//
+// Called out of line: inlined, its live ranges would push the solve's
+// registers into scratch.
+template <class VT>
+__device__ __noinline__ void merged_up_phase(const Instance* instance, int j, const VT* rhs, VT* out, int g, int G,
+                                             Lds& lds, double* dot_part, const VT* dot_with) {
+    const Instance& I = *uniform_ptr(instance);
+    const Values<VT> V{I};
+    j = __builtin_amdgcn_readfirstlane(j);
+    g = __builtin_amdgcn_readfirstlane(g);
+    G = __builtin_amdgcn_readfirstlane(G);
+    rhs = uniform_ptr(rhs);
+    out = uniform_ptr(out);
+    const LevelView& L = I.lv[j];
+    const LevelView& M = I.lv[j + 1];
+    const int n = L.n, W = L.width, nm = M.n, Wm = M.width;
+    const VT om = static_cast<VT>(I.omega[j]), omm = static_cast<VT>(I.omega[j + 1]);
+    const VT* diag = V.diag(j);
+    const VT* dinv = V.dinv(j);
+    const VT* off = V.off(j);
+    const VT* diag_m = V.diag(j + 1);
+    const VT* dinv_m = V.dinv(j + 1);
+    const VT* off_m = V.off(j + 1);
+    const VT* rhs_m = static_cast<const VT*>(I.lvl_b[j + 1]);
+    double* mid = lds.tile[0];
+    const int tiles = (n + kTile - 1) / kTile;
+    for (int tile = g; tile < tiles; tile += G) {
+        coarse_rows_lds<VT>(I, static_cast<const VT*>(I.lvl_b[j + 2]), L.mid_coarse + tile * L.mid_coarse_stride, 0,
+                            L.mid_coarse_stride, lds);
+        for (int q = threadIdx.x; q < L.mid_stride; q += kTile) {
+            const int m = L.mid[tile * L.mid_stride + q];
+            if (m >= 0) {
+                const VT x2 = omm * dinv_m[m] * rhs_m[m] + static_cast<VT>(lds.cvec[M.agg[m]]);
+                VT acc = rhs_m[m] - diag_m[m] * x2;
+                ell_row(
+                    M.col, off_m, nm, Wm, m,
+                    [&](int s, int c) {
+                        const int a = M.agg_col[s * nm + m];
+                        return omm * dinv_m[c] * rhs_m[c] + static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
+                    },
+                    [&](VT o, VT v) { acc -= o * v; });
+                mid[q] = static_cast<double>(static_cast<VT>(x2 + omm * dinv_m[m] * acc));
+            }
+        }
+        __syncthreads();
+        const int i = tile * kTile + threadIdx.x;
+        double q[1] = {0.0};
+        if (i < n) {
+            const VT x2 = om * dinv[i] * rhs[i] + static_cast<VT>(mid[L.agg_loc[i]]);
+            VT acc = rhs[i] - diag[i] * x2;
+            ell_row(
+                L.col, off, n, W, i,
+                [&](int s, int c) {
+                    const int a = L.agg_col_loc[s * n + i];
+                    return om * dinv[c] * rhs[c] + static_cast<VT>(mid[a >= 0 ? a : 0]);
+                },
+                [&](VT o, VT v) { acc -= o * v; });
+            const VT value = x2 + om * dinv[i] * acc;
+            publish(&out[i], value);
+            q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(dot_with[i]) : 0.0;
+        }
+        if (dot_part != nullptr) store_partials<1>(dot_part, tile, q, lds);
+        __syncthreads();
+    }
+}
+
 // First sweep of a V-cycle: level 0's down phase (input evaluated from
 // `src`, stored into `keep`, squares into `part`); on a one-level hierarchy
 // the input is only materialised. Ends with a barrier.
@@ -672,11 +758,20 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
         down_phase<VT>(I, V, j, src, nullptr, static_cast<VT*>(I.lvl_b[j + 1]), nullptr, g, G, lds);
         group_sync(timeouts);
     }
+    // with about a workgroup per tile, the last two prolongations are one
+    // sweep (merged_up_phase); with fewer workgroups each tile would redo
+    // the coarse work, so they stay two
+    const int jm = J - 3;
+    const bool merge = jm >= 0 && I.lv[jm].mid != nullptr && 2 * G >= (I.lv[jm].n + kTile - 1) / kTile;
     for (int j = J - 2; j >= 0; --j) {
         const VT* rhs = j == 0 ? w0 : static_cast<const VT*>(I.lvl_b[j]);
         VT* out = j == 0 ? out0 : static_cast<VT*>(I.lvl_e[j]);
-        up_phase<VT>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds, j == 0 ? dot_part : nullptr,
-                     j == 0 ? dot_with : nullptr);
+        if (merge && j == jm + 1) continue;
+        if (merge && j == jm)
+            merged_up_phase<VT>(&I, j, rhs, out, g, G, lds, j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr);
+        else
+            up_phase<VT>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds, j == 0 ? dot_part : nullptr,
+                         j == 0 ? dot_with : nullptr);
         group_sync(timeouts);
     }
 }
@@ -690,16 +785,6 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
 // (no pivoting needed). Only the packed lower triangle is stored, in
 // `tri` (LDS when it fits, else global scratch); the result is written
 // negated and fully expanded, column k contiguous, to I.cinv.
-// A pointer every lane holds, rebuilt in scalar registers: descriptor reads
-// through it are scalar loads (a function argument arrives in vector
-// registers otherwise).
-template <class T> __device__ inline const T* uniform_ptr(const T* p) {
-    const uint64_t bits = reinterpret_cast<uint64_t>(p);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits));
-    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(bits >> 32));
-    return reinterpret_cast<const T*>((static_cast<uint64_t>(hi) << 32) | lo);
-}
-
 __device__ inline int tri_index(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 
 __device__ void coarse_inverse(const Instance& I, double* tri, Lds& lds) {
