@@ -471,11 +471,15 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
             c0[r] = L.tile_start[tl[r]];
             c1[r] = L.tile_start[tl[r] + 1];
         }
+        int q0[R], q1[R];   // this thread's coarse row's fine-row range, for the restriction
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             ok[r] = i[r] >= 0;
             i[r] = ok[r] ? i[r] : 0;
             base[r] = L.agg_start[c0[r]];
+            const int c = c0[r] + static_cast<int>(threadIdx.x);
+            q0[r] = c < L.n_next ? L.agg_start[c] : 0;
+            q1[r] = c < L.n_next ? L.agg_start[c + 1] : 0;
         }
         VT wi[R], acc[R];
         double sq[R][1];
@@ -501,10 +505,9 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (static_cast<int>(threadIdx.x) < c1[r] - c0[r]) {
-                const int c = c0[r] + threadIdx.x;
                 double sum = 0.0;
-                for (int q = L.agg_start[c] - base[r]; q < L.agg_start[c + 1] - base[r]; ++q) sum += lds.tile[r][q];
-                publish(&coarse_rhs[c], static_cast<VT>(sum));
+                for (int q = q0[r] - base[r]; q < q1[r] - base[r]; ++q) sum += lds.tile[r][q];
+                publish(&coarse_rhs[c0[r] + threadIdx.x], static_cast<VT>(sum));
             }
         __syncthreads();
     });
@@ -664,10 +667,13 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
     double* mid = lds.tile[0];
     const int tiles = (n + kTile - 1) / kTile;
     for (int tile = g; tile < tiles; tile += G) {
+        // this thread's mid row id (a tile reads at most kTile of them) goes
+        // out before the coarse solve
+        const int mq = threadIdx.x;
+        const int m = mq < L.mid_stride ? L.mid[tile * L.mid_stride + mq] : -1;
         coarse_rows_lds<VT>(I, static_cast<const VT*>(I.lvl_b[j + 2]), L.mid_coarse + tile * L.mid_coarse_stride, 0,
                             L.mid_coarse_stride, lds);
-        for (int q = threadIdx.x; q < L.mid_stride; q += kTile) {
-            const int m = L.mid[tile * L.mid_stride + q];
+        {
             if (m >= 0) {
                 const VT x2 = omm * dinv_m[m] * rhs_m[m] + static_cast<VT>(lds.cvec[M.agg[m]]);
                 VT acc = rhs_m[m] - diag_m[m] * x2;
@@ -678,7 +684,7 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
                         return omm * dinv_m[c] * rhs_m[c] + static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
                     },
                     [&](VT o, VT v) { acc -= o * v; });
-                mid[q] = static_cast<double>(static_cast<VT>(x2 + omm * dinv_m[m] * acc));
+                mid[mq] = static_cast<double>(static_cast<VT>(x2 + omm * dinv_m[m] * acc));
             }
         }
         __syncthreads();
