@@ -33,7 +33,7 @@ namespace psim::dev {
 namespace {
 
 constexpr int kIluThreads = 256;
-constexpr int kGridWords = 64;   // [0] generation, [16] arrivals, [32] abort (separate lines)
+constexpr int kGridWords = 64;   // [0] generation, [16] and [24] arrivals by parity, [32] abort
 
 struct IluDev {
     int32_t n, blocks;
@@ -118,25 +118,34 @@ __global__ __launch_bounds__(kIluThreads) void ilu_solve_kernel(IluDev F, double
     ilu_solve_block(F, static_cast<int>(blockIdx.x), x);
 }
 
-// Grid barrier of a cooperative launch: arrivals on one line, the last
-// arrival resets them and bumps the generation the others poll. Waits are
-// bounded by the wall clock; a timeout sets the sticky abort word, which
-// every waiter also polls, so a broken launch drains at once.
+// Grid barrier of a cooperative launch. Each workgroup issues one
+// agent-scope release fence and a relaxed arrival; the last arrival adds an
+// acq_rel fence (ordering every arrival's writes before its own generation
+// bump), clears the other episode's counter and bumps the generation the
+// others poll with relaxed loads, each then issuing one acquire fence.
+// Arrival counters alternate between two words by the parity of the
+// generation, so the clear never races with an arrival. Waits are bounded
+// by the wall clock; a timeout sets the sticky abort word, which every
+// waiter also polls, so a broken launch drains at once.
+//
+// Words: [0] generation, [16 + 8 p] arrivals of parity p, [32] abort.
 __device__ bool grid_sync(uint32_t* words, uint32_t blocks, long long limit) {
     __shared__ int ok;
     __syncthreads();
     if (threadIdx.x == 0) {
         ok = 1;
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const uint32_t gen = __hip_atomic_load(&words[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t p = gen & 1u;
         const uint32_t arrived =
-            __hip_atomic_fetch_add(&words[16], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+            __hip_atomic_fetch_add(&words[16 + 8 * p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
         if (arrived == blocks) {
-            __hip_atomic_store(&words[16], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&words[0], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&words[16 + 8 * (p ^ 1u)], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            __hip_atomic_store(&words[0], gen + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const long long t0 = static_cast<long long>(wall_clock64());
-            while (__hip_atomic_load(&words[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(&words[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 if (__hip_atomic_load(&words[32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
                     static_cast<long long>(wall_clock64()) - t0 > limit) {
                     __hip_atomic_store(&words[32], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -145,8 +154,8 @@ __device__ bool grid_sync(uint32_t* words, uint32_t blocks, long long limit) {
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        __threadfence();
     }
     __syncthreads();
     return ok != 0;
