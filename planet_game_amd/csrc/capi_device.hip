@@ -58,8 +58,11 @@ void launch(psim_system* s, const Launch& p) {
     const bool cooperative = s->workgroups > 1;
     if (cooperative && !ctx->cooperative) throw psim::DeviceFailure("cooperative launch unsupported on this device");
     const size_t lds = (p.op & psim::dev::kOpSetMatrix) && s->host.tri_lds ? s->tri_bytes : 0;
+    // a group of more than one workgroup that fits one per CU runs the
+    // full-register-file instance of the program
+    const bool wide = cooperative && s->workgroups <= ctx->compute_units;
     psim::hip_check(psim::dev::launch_program(s->d_inst, s->d_map, nullptr, p, s->workgroups, lds, ctx->stream,
-                                              cooperative),
+                                              cooperative, wide),
                     "solver launch");
 }
 

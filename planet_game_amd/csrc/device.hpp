@@ -157,14 +157,16 @@ struct Launch {
 
 // Launch the persistent program. Workgroup w serves instance
 // block_instance[w]; its parameters are launches[that] when `launches` is
-// non-null, else `single`.
+// non-null, else `single`. `wide`: the one-wave-per-SIMD instance of the
+// program (one workgroup per CU, the full register file), for a single
+// instance whose group fits one workgroup per CU.
 hipError_t launch_program(const Instance* instances, const int32_t* block_instance, const Launch* launches,
                           const Launch& single, int32_t blocks, size_t dynamic_lds, hipStream_t stream,
-                          bool cooperative);
+                          bool cooperative, bool wide = false);
 
 // Co-resident program workgroups per compute unit with `dynamic_lds`
 // bytes of dynamic LDS (0 if the kernel cannot launch with that much).
-int program_blocks_per_cu(size_t dynamic_lds);
+int program_blocks_per_cu(size_t dynamic_lds, bool wide = false);
 // The program kernel's static LDS bytes.
 size_t program_static_lds();
 

@@ -643,7 +643,7 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
 //
 // Called out of line: inlined, its live ranges would push the solve's
 // registers into scratch.
-template <class VT>
+template <class VT, int kW>
 __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, const VT* rhs, VT* out, int g, int G,
                                              Lds& lds, double* dot_part, const VT* dot_with) {
     const Instance& I = *uniform_ptr(instance);
@@ -737,7 +737,7 @@ __device__ inline int first_sweep_tiles(const Instance& I) { return I.depth == 1
 
 // The rest of the V-cycle after vcycle_first; w0 is the materialised level-0
 // input, out0 the level-0 output. Ends with a barrier.
-template <class VT>
+template <class VT, int kW>
 __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>& V, const VT* w0, VT* out0, int g, int G, Lds& lds,
                             int& timeouts, double* dot_part = nullptr, const VT* dot_with = nullptr) {
     const int J = I.depth;
@@ -774,7 +774,7 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
         VT* out = j == 0 ? out0 : static_cast<VT*>(I.lvl_e[j]);
         if (merge && j == jm + 1) continue;
         if (merge && j == jm)
-            merged_up_phase<VT>(&I, j, rhs, out, g, G, lds, j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr);
+            merged_up_phase<VT, kW>(&I, j, rhs, out, g, G, lds, j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr);
         else
             up_phase<VT>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds, j == 0 ? dot_part : nullptr,
                          j == 0 ? dot_with : nullptr);
@@ -793,7 +793,7 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
 // negated and fully expanded, column k contiguous, to I.cinv.
 __device__ inline int tri_index(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 
-__device__ void coarse_inverse(const Instance& I, double* tri, Lds& lds) {
+template <int kW> __device__ void coarse_inverse(const Instance& I, double* tri, Lds& lds) {
     const int j = I.depth - 1;
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
@@ -861,7 +861,7 @@ __device__ void coarse_inverse(const Instance& I, double* tri, Lds& lds) {
 constexpr int kSweepBlockRows = kBlockedCoarse / kSweepBlock;
 static_assert(kSweepBlockRows * kSweepBlock == kBlockedCoarse, "whole sweep blocks");
 
-__device__ __noinline__ void coarse_inverse_blocked(const Instance* instance, Lds& lds) {
+template <int kW> __device__ __noinline__ void coarse_inverse_blocked(const Instance* instance, Lds& lds) {
     const Instance& I = *uniform_ptr(instance);
     constexpr int B = kSweepBlock;
     const int j = I.depth - 1;
@@ -1007,6 +1007,7 @@ __device__ __noinline__ void coarse_inverse_blocked(const Instance* instance, Ld
     }
 }
 
+template <int kW>
 __device__ __forceinline__ void set_matrix(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts,
                                            int& parity) {
     const int J = I.depth;
@@ -1123,9 +1124,9 @@ __device__ __forceinline__ void set_matrix(const Instance& I, const Launch& P, i
     }
     if (g == 0) {
         if (I.lv[J - 1].n <= kBlockedCoarse)
-            coarse_inverse_blocked(&I, lds);
+            coarse_inverse_blocked<kW>(&I, lds);
         else
-            coarse_inverse(I, I.tri_lds ? dynamic_lds() : I.tri, lds);
+            coarse_inverse<kW>(I, I.tri_lds ? dynamic_lds() : I.tri, lds);
     }
     group_sync(timeouts);
 }
@@ -1136,9 +1137,10 @@ __device__ __forceinline__ void set_matrix(const Instance& I, const Launch& P, i
 struct SyncState {
     int timeouts, parity;
 };
+template <int kW>
 __device__ __noinline__ SyncState set_matrix_call(const Instance& I, const Launch& P, int g, int G, Lds& lds,
                                                   SyncState st) {
-    set_matrix(I, P, g, G, lds, st.timeouts, st.parity);
+    set_matrix<kW>(I, P, g, G, lds, st.timeouts, st.parity);
     return st;
 }
 
@@ -1154,7 +1156,7 @@ __device__ inline double residual_row(const Instance& I, const double* x, int i)
     return I.b[i] - ax;
 }
 
-template <class VT>
+template <class VT, int kW>
 __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P, const double* t_old, const double* forcing,
                              double* t_new, int g, int G, Lds& lds, int& timeouts, int& parity, int& xi) {
     const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
@@ -1275,7 +1277,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 VT* p_new = static_cast<VT*>(I.p[pi ^ 1]);
                 pi ^= 1;
                 vcycle_first<VT>(I, V, src, p_new, nullptr, g, G, lds, timeouts);
-                vcycle_rest<VT>(I, V, p_new, ph, g, G, lds, timeouts);
+                vcycle_rest<VT, kW>(I, V, p_new, ph, g, G, lds, timeouts);
                 ++v_cycles;
                 first_iteration = false;
                 // v = A ph, (r0, v)
@@ -1344,7 +1346,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     have_residual = true;
                     break;
                 }
-                vcycle_rest<VT>(I, V, s, sh, g, G, lds, timeouts);
+                vcycle_rest<VT, kW>(I, V, s, sh, g, G, lds, timeouts);
                 ++v_cycles;
                 // t = A sh, (t, t), (t, s)
                 part = next_partials();
@@ -1452,6 +1454,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
 // reference's Gauss-Seidel sweep leaves it). d = z + beta d is evaluated inside the next SpMV
 // (double-buffered d), and (r, z) inside the V-cycle's last sweep, so an
 // iteration is six barrier-separated phases. x goes to P.out.
+template <int kW>
 __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts,
                                            int& parity) {
     const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
@@ -1495,7 +1498,7 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
         const Source<double> src{r, nullptr, nullptr, 0.0, 0.0, 0};
         vcycle_first<double>(I, V, src, kept, nullptr, g, G, lds, timeouts);
         double* dot = next_partials();
-        vcycle_rest<double>(I, V, r, z, g, G, lds, timeouts, dot, r);
+        vcycle_rest<double, kW>(I, V, r, z, g, G, lds, timeouts, dot, r);
         double rz[1];
         group_totals<1>(dot, tiles, rz, lds);   // one partial per level-0 tile
         return rz[0];
@@ -1570,11 +1573,12 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
 }
 
 // PCG out of line (not the hot path; its registers stay out of the solver's).
+template <int kW>
 __device__ __noinline__ SolveResult pcg_call(const Instance* instance, const Launch& P, int g, int G, Lds& lds,
                                              SyncState& st) {
     const Instance& I = *uniform_ptr(instance);
     int timeouts = st.timeouts, parity = st.parity;
-    const SolveResult res = pcg(I, P, __builtin_amdgcn_readfirstlane(g), __builtin_amdgcn_readfirstlane(G), lds,
+    const SolveResult res = pcg<kW>(I, P, __builtin_amdgcn_readfirstlane(g), __builtin_amdgcn_readfirstlane(G), lds,
                                 timeouts, parity);
     st = SyncState{timeouts, parity};
     return res;
@@ -1582,7 +1586,7 @@ __device__ __noinline__ SolveResult pcg_call(const Instance* instance, const Lau
 
 // ------------------------------------------------------------ the kernel
 
-template <class VT>
+template <class VT, int kW>
 __device__ __forceinline__ void run_solve(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts, int& parity) {
     int xi = 0;
     const int steps = (P.op & kOpChain) ? P.steps : 1;
@@ -1598,7 +1602,7 @@ __device__ __forceinline__ void run_solve(const Instance& I, const Launch& P, in
             forcing = P.forcing + static_cast<size_t>(step) * I.n;
             t_new = step + 1 == steps ? P.t_new : nullptr;
         }
-        last = solve<VT>(I, P, t_old, forcing, t_new, g, G, lds, timeouts, parity, xi);
+        last = solve<VT, kW>(I, P, t_old, forcing, t_new, g, G, lds, timeouts, parity, xi);
         if (g == 0 && threadIdx.x == 0 && P.step_results != nullptr) P.step_results[step] = last;
     }
     if (g == 0 && threadIdx.x == 0) {
@@ -1616,8 +1620,15 @@ __device__ __forceinline__ void run_solve(const Instance& I, const Launch& P, in
 // Three workgroups per CU (12 waves): the register budget that keeps the
 // solver's live state out of scratch (a tighter cap spills it) while
 // leaving batches 768 co-resident workgroups.
-__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void program_kernel(const Instance* instances, const int32_t* block_instance,
-                                                        const Launch* launches, Launch single) {
+//
+// Two instances of the program: program_kernel_t<3> for batches and
+// servers (three workgroups per CU), and program_kernel_t<1> for one instance whose
+// group has at most one workgroup per CU, where occupancy buys nothing and
+// the register file is worth more: one wave per SIMD lets the compiler keep
+// the solve's state in registers (512 VGPRs) instead of scratch.
+template <int kWaves>
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves))) void program_kernel_t(
+    const Instance* instances, const int32_t* block_instance, const Launch* launches, Launch single) {
     __shared__ Lds lds;
     __shared__ Launch params;
     const int index = block_instance[blockIdx.x];
@@ -1634,7 +1645,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
         I.trace[0] = static_cast<uint64_t>(wall_clock64());
     }
     if (P.op & kOpSetMatrix) {
-        const SyncState st = set_matrix_call(I, P, g, G, lds, SyncState{timeouts, parity});
+        const SyncState st = set_matrix_call<kWaves>(I, P, g, G, lds, SyncState{timeouts, parity});
         timeouts = st.timeouts;
         parity = st.parity;
     }
@@ -1643,7 +1654,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
             const Source<double> src{P.in, nullptr, nullptr, 0.0, 0.0, 0};
             double* keep = static_cast<double*>(I.p[0]);
             vcycle_first<double>(I, Values<double>{I}, src, keep, nullptr, g, G, lds, timeouts);
-            vcycle_rest<double>(I, Values<double>{I}, keep, P.out, g, G, lds, timeouts);
+            vcycle_rest<double, kWaves>(I, Values<double>{I}, keep, P.out, g, G, lds, timeouts);
         } else {
             float* pin = static_cast<float*>(I.p[0]);
             float* pout = static_cast<float*>(I.ph);
@@ -1654,7 +1665,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
             group_sync(timeouts);
             const Source<float> src{pin, nullptr, nullptr, 0.0, 0.0, 0};
             vcycle_first<float>(I, Values<float>{I}, src, static_cast<float*>(I.p[1]), nullptr, g, G, lds, timeouts);
-            vcycle_rest<float>(I, Values<float>{I}, static_cast<float*>(I.p[1]), pout, g, G, lds, timeouts);
+            vcycle_rest<float, kWaves>(I, Values<float>{I}, static_cast<float*>(I.p[1]), pout, g, G, lds, timeouts);
             for (int tile = g; tile < I.tiles; tile += G) {
                 const int i = tile * kTile + threadIdx.x;
                 if (i < I.n) P.out[i] = static_cast<double>(pout[i]);
@@ -1677,13 +1688,13 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
     }
     if (P.op & kOpSolve) {
         if (I.precision == 0)
-            run_solve<double>(I, P, g, G, lds, timeouts, parity);
+            run_solve<double, kWaves>(I, P, g, G, lds, timeouts, parity);
         else
-            run_solve<float>(I, P, g, G, lds, timeouts, parity);
+            run_solve<float, kWaves>(I, P, g, G, lds, timeouts, parity);
     }
     if (P.op & kOpPcg) {
         SyncState st{timeouts, parity};
-        const SolveResult res = pcg_call(&I, P, g, G, lds, st);
+        const SolveResult res = pcg_call<kWaves>(&I, P, g, G, lds, st);
         timeouts = st.timeouts;
         parity = st.parity;
         if (g == 0 && threadIdx.x == 0) {
@@ -1699,6 +1710,9 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
     }
     if (timeouts != 0 && threadIdx.x == 0) atomicAdd(&grp.result->barrier_timeouts, timeouts);
 }
+
+template __global__ void program_kernel_t<3>(const Instance*, const int32_t*, const Launch*, Launch);
+template __global__ void program_kernel_t<1>(const Instance*, const int32_t*, const Launch*, Launch);
 
 // ------------------------------------------------------------ the server
 
@@ -1743,14 +1757,14 @@ __device__ __noinline__ int serve_one(const Instance* instance, const Launch& P,
     G = __builtin_amdgcn_readfirstlane(G);
     int parity = 0;
     if (P.op & kOpSetMatrix) {
-        const SyncState st = set_matrix_call(I, P, g, G, lds, SyncState{timeouts, parity});
+        const SyncState st = set_matrix_call<3>(I, P, g, G, lds, SyncState{timeouts, parity});
         timeouts = st.timeouts;
         parity = st.parity;
     }
     if (I.precision == 0)
-        run_solve<double>(I, P, g, G, lds, timeouts, parity);
+        run_solve<double, 3>(I, P, g, G, lds, timeouts, parity);
     else
-        run_solve<float>(I, P, g, G, lds, timeouts, parity);
+        run_solve<float, 3>(I, P, g, G, lds, timeouts, parity);
     // every workgroup's rows of T are written before the completion
     group_sync(timeouts);
     return timeouts;
@@ -1880,41 +1894,43 @@ int server_blocks_per_cu() {
 
 hipError_t launch_program(const Instance* instances, const int32_t* block_instance, const Launch* launches,
                           const Launch& single, int32_t blocks, size_t dynamic_lds, hipStream_t stream,
-                          bool cooperative) {
+                          bool cooperative, bool wide) {
+    const void* kernel = wide ? reinterpret_cast<const void*>(&program_kernel_t<1>)
+                              : reinterpret_cast<const void*>(&program_kernel_t<3>);
     if (dynamic_lds > 0) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&program_kernel),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 static_cast<int>(dynamic_lds));
+        const hipError_t e =
+            hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamic_lds));
         if (e != hipSuccess) return e;
     }
-    if (cooperative) {
-        Launch copy = single;
-        void* args[] = {const_cast<Instance**>(&instances), const_cast<int32_t**>(&block_instance),
-                        const_cast<Launch**>(&launches), &copy};
-        return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&program_kernel), dim3(blocks), dim3(kTile),
-                                          args, static_cast<unsigned>(dynamic_lds), stream);
-    }
-    hipLaunchKernelGGL(program_kernel, dim3(blocks), dim3(kTile), dynamic_lds, stream, instances, block_instance,
-                       launches, single);
-    return hipGetLastError();
+    Launch copy = single;
+    void* args[] = {const_cast<Instance**>(&instances), const_cast<int32_t**>(&block_instance),
+                    const_cast<Launch**>(&launches), &copy};
+    if (cooperative)
+        return hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(kTile), args, static_cast<unsigned>(dynamic_lds),
+                                          stream);
+    return hipLaunchKernel(kernel, dim3(blocks), dim3(kTile), args, dynamic_lds, stream);
 }
 
 size_t program_static_lds() {
     hipFuncAttributes attributes{};
-    if (hipFuncGetAttributes(&attributes, reinterpret_cast<const void*>(&program_kernel)) != hipSuccess)
+    if (hipFuncGetAttributes(&attributes, reinterpret_cast<const void*>(&program_kernel_t<3>)) != hipSuccess)
         return sizeof(Lds) + sizeof(Launch) + 256;
     return attributes.sharedSizeBytes;
 }
 
-int program_blocks_per_cu(size_t dynamic_lds) {
+int program_blocks_per_cu(size_t dynamic_lds, bool wide) {
+    const void* kernel = wide ? reinterpret_cast<const void*>(&program_kernel_t<1>)
+                              : reinterpret_cast<const void*>(&program_kernel_t<3>);
     int blocks = 0;
     if (dynamic_lds > 0 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&program_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            static_cast<int>(dynamic_lds)) != hipSuccess)
+        hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamic_lds)) !=
+            hipSuccess)
         return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, program_kernel, kTile, dynamic_lds) != hipSuccess)
-        return 0;
-    return blocks;
+    const hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, program_kernel_t<1>, kTile,
+                                                                              dynamic_lds)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, program_kernel_t<3>, kTile,
+                                                                              dynamic_lds);
+    return e == hipSuccess ? blocks : 0;
 }
 
 }  // namespace psim::dev
