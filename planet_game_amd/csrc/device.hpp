@@ -23,10 +23,11 @@ constexpr int kTile = 256;        // rows per tile = threads per workgroup = the
 constexpr int kMaxLevels = 8;
 constexpr int kPartials = 4;      // reduction slots per tile
 constexpr int kMaxCoarse = 512;   // dense coarsest level (the hierarchy stops at <= 400 rows)
-// Coarsest levels up to this size are inverted by the block sweep (8 pivots
-// per step, working matrix in sweep_words(n) words of I.tri).
+// Coarsest levels up to this size are inverted by the block sweep on the
+// matrix cores (16 pivots per step, working matrix in sweep_words(n) words
+// of I.tri: the lower triangle of 16 x 16 blocks).
 constexpr int kBlockedCoarse = 168;
-constexpr int kSweepBlock = 8;       // pivots per block step of the coarsest inverse
+constexpr int kSweepBlock = 16;      // pivots per block step of the coarsest inverse
 // Words of I.tri the block sweep uses for a coarsest level of n <= kBlockedCoarse rows.
 constexpr int sweep_words(int n) {
     return (n + kSweepBlock - 1) / kSweepBlock * ((n + kSweepBlock - 1) / kSweepBlock + 1) / 2 * kSweepBlock *

@@ -71,7 +71,11 @@ __device__ inline double wave_max(double v) {
     return v;
 }
 
-constexpr int kPanelStride = kSweepBlock + 1;
+// The coarsest inverse's MFMA blocks: 16 x 16, kMaxBlocks of them along a
+// side for up to kBlockedCoarse rows.
+constexpr int kMB = kSweepBlock;
+constexpr int kMaxBlocks = (kBlockedCoarse + kMB - 1) / kMB;
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 struct Lds {
     double red[kPartials][4];        // per-wave partials
@@ -83,11 +87,17 @@ struct Lds {
             double crhs[kMaxCoarse];     // coarse right-hand side (FP64 staging)
             double colk[kMaxCoarse];     // sweep pivot column (set_matrix, large coarsest)
         };
-        // set_matrix, coarsest of at most kBlockedCoarse rows: the block
-        // column under sweep, double-buffered, and its value before the
-        // sweep (row r, column w at r * kPanelStride + w; the odd stride
-        // spreads rows eight apart over different banks)
-        double panel[3][kBlockedCoarse * kPanelStride];
+        // set_matrix, coarsest of at most kBlockedCoarse rows (block sweep
+        // with MFMA): the block column K before the step and its product with
+        // the swept pivot block, both as column-major 16 x 16 blocks (entry
+        // (r, c) of block i at [i][c * 16 + r], so MFMA operand reads are
+        // consecutive across lanes), and the pivot block under its scalar
+        // sweeps, double-buffered, row-major
+        struct {
+            double colT[kMaxBlocks][kMB * kMB];
+            double dT[kMaxBlocks][kMB * kMB];
+            double piv[2][kMB * kMB];
+        };
     };
 };
 
@@ -841,49 +851,43 @@ template <int kW> __device__ void coarse_inverse(const Instance& I, double* tri,
 }
 
 // The same sweep for a coarsest level of at most kBlockedCoarse rows, by
-// blocks of eight pivots (one workgroup). The working matrix M is the lower
-// triangle cut into 8 x 8 blocks, block (bi, bj) (bi >= bj) numbered
-// b = bi (bi + 1) / 2 + bj, stored interleaved in I.tri (entry (u, v) of
-// block b at (8 u + v) blocks + b) so that threads owning consecutive
-// blocks touch consecutive words; inside a diagonal block only u >= v is
-// used. Block step K:
-//   1. block column K of M goes to an LDS panel A0[r][w] = M(r, 8K + w) for
-//      every row r (by symmetry above the diagonal), and a working copy;
-//   2. its eight pivots sweep the working panel in LDS (double-buffered,
-//      one barrier per pivot): entries of a swept panel depend only on the
-//      panel, so afterwards it holds M(r,K) M(K,K)^-1 outside block K and
-//      -M(K,K)^-1 inside it;
-//   3. every block outside the panel takes the block form of the eight
-//      pivots at once, M(r,c) -= sum_w P[r][w] A0[c][w], one thread per
-//      block;
-//   4. the swept panel goes back into M.
-// At the end I.cinv = -M = A^-1, written in full (column-major).
-constexpr int kSweepBlockRows = kBlockedCoarse / kSweepBlock;
-static_assert(kSweepBlockRows * kSweepBlock == kBlockedCoarse, "whole sweep blocks");
+// 16 x 16 blocks on the matrix cores (one workgroup). The matrix is padded
+// with identity rows to N = 16 nb and its lower block triangle kept in
+// I.tri, each block in the MFMA accumulator order (lane l holds rows
+// l / 16 + 4 v, v = 0..3, of column l % 16, at [l * 4 + v]). Sweeping the
+// 16 pivots of block K is one block step: with S = SWEEP(A_KK) = -A_KK^-1
+// (its 16 scalar sweeps on the LDS copy, each entry computed from its
+// lower-triangle formula so S is exactly symmetric),
+//   D_i = A_iK S,   A_ij += D_i A_Kj  (i, j != K),   A_iK = -D_i,   A_KK = S,
+// the products by v_mfma_f64_16x16x4 (four per 16 x 16 x 16 product, FP64
+// throughout). After all nb steps I.tri holds -A^-1; the result is written
+// negated to I.cinv from the lower triangle, mirrored, so it is exactly
+// symmetric.
+__device__ inline int block_index(int bi, int bj) { return bi * (bi + 1) / 2 + bj; }
+// Position of entry (r, c) of a 16 x 16 block in accumulator order.
+__device__ inline int acc_pos(int r, int c) { return ((r & 3) * kMB + c) * 4 + (r >> 2); }
 
 template <int kW> __device__ __noinline__ void coarse_inverse_blocked(const Instance* instance, Lds& lds) {
     const Instance& I = *uniform_ptr(instance);
-    constexpr int B = kSweepBlock;
     const int j = I.depth - 1;
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
-    const int nb = (n + B - 1) / B;
+    const int nb = (n + kMB - 1) / kMB;
     const int blocks = nb * (nb + 1) / 2;
     const int t = static_cast<int>(threadIdx.x);
+    const int lane = t & 63, wave = t >> 6;
     double* M = I.tri;
-    // position of entry (r, c) of the symmetric matrix
-    const auto at = [blocks](int r, int c) {
-        if (r < c) {
-            const int x = r;
-            r = c;
-            c = x;
-        }
-        const int bi = r / B, bj = c / B;
-        return ((r - bi * B) * B + (c - bj * B)) * blocks + bi * (bi + 1) / 2 + bj;
+    // entry (r, c), r >= c, of the stored lower triangle
+    const auto at = [](int r, int c) {
+        return block_index(r / kMB, c / kMB) * (kMB * kMB) + acc_pos(r % kMB, c % kMB);
     };
-    for (int e = t; e < blocks * B * B; e += kTile) M[e] = 0.0;
+    for (int e = t; e < blocks * kMB * kMB; e += kTile) M[e] = 0.0;
     __syncthreads();
-    for (int r = t; r < n; r += kTile) {
+    for (int r = t; r < nb * kMB; r += kTile) {
+        if (r >= n) {
+            M[at(r, r)] = 1.0;   // identity padding
+            continue;
+        }
         M[at(r, r)] = I.diag64[j][r];
         for (int s = 0; s < W; ++s) {
             const int c = L.col[s * n + r];
@@ -891,114 +895,97 @@ template <int kW> __device__ __noinline__ void coarse_inverse_blocked(const Inst
         }
     }
     __syncthreads();
-    constexpr int S = kPanelStride;
-    constexpr int kEntries = (kBlockedCoarse * kSweepBlock + kTile - 1) / kTile;   // panel entries per thread
-    double* A0 = lds.panel[2];
+    const int er = t % kMB, ec = t / kMB;   // this thread's entry of a 16 x 16 block
     for (int K = 0; K < nb; ++K) {
-        const int k0 = K * B, width = min(B, n - k0);
-        // 1. the panel, original and working copy
-        for (int e = t; e < n * B; e += kTile) {
-            const int r = e / B, w = e - r * B;
-            const double value = w < width ? M[at(r, k0 + w)] : 0.0;
-            A0[r * S + w] = value;
-            lds.panel[0][r * S + w] = value;
+        // 1. block column K as it is before the step (column-major blocks),
+        // and the pivot block, symmetric from its lower half, row-major
+        for (int i = 0; i < nb; ++i) {
+            const int r = i * kMB + er, c = K * kMB + ec;   // entry (r, c) of A
+            lds.colT[i][ec * kMB + er] = r >= c ? M[at(r, c)] : M[at(c, r)];
+        }
+        {
+            const int hi = er > ec ? er : ec, lo = er > ec ? ec : er;
+            lds.piv[0][er * kMB + ec] = M[at(K * kMB + hi, K * kMB + lo)];
         }
         __syncthreads();
-        // 2. the pivots of block K on the panel; each thread's entries are
-        // read first, then computed and written
+        // 2. S = SWEEP(A_KK): 16 scalar sweeps; entries (r, c) and (c, r)
+        // evaluate the same lower-triangle expression
         int cur = 0;
-        for (int kk = 0; kk < width; ++kk) {
-            const int k = k0 + kk;
-            const double* Pc = lds.panel[cur];
-            double* Pn = lds.panel[cur ^ 1];
-            const double pivot = Pc[k * S + kk];
-            double rk[kEntries], ck[kEntries], rc[kEntries];
-#pragma unroll
-            for (int m = 0; m < kEntries; ++m) {
-                const int e = t + m * kTile;
-                const int r = min(e / B, n - 1), w = e - (e / B) * B;
-                rk[m] = Pc[r * S + kk];
-                ck[m] = Pc[(k0 + min(w, width - 1)) * S + kk];
-                rc[m] = Pc[r * S + w];
-            }
-            const double dinv = 1.0 / pivot;
-#pragma unroll
-            for (int m = 0; m < kEntries; ++m) {
-                const int e = t + m * kTile;
-                const int r = e / B, w = e - r * B;
-                if (r >= n || w >= width) continue;
-                const bool diagonal_block = r >= k0 && r < k0 + B;
-                if (diagonal_block && r - k0 < w) continue;   // upper entry: its mirror writes it
+        {
+            const int r = er > ec ? er : ec, c = er > ec ? ec : er;
+            for (int k = 0; k < kMB; ++k) {
+                const double* Pc = lds.piv[cur];
+                const double pivot = Pc[k * kMB + k];
+                const double rk = Pc[r * kMB + k], ck = Pc[k * kMB + c], rc = Pc[r * kMB + c];
+                const double dinv = 1.0 / pivot;
                 double value;
-                if (r == k && w == kk)
+                if (r == k && c == k)
                     value = -dinv;
-                else if (w == kk)
-                    value = rk[m] * dinv;   // column k
+                else if (c == k)
+                    value = rk * dinv;
                 else if (r == k)
-                    value = ck[m] * dinv;   // row k (by symmetry)
+                    value = ck * dinv;
                 else
-                    value = fma(-(rk[m] * dinv), ck[m], rc[m]);
-                Pn[r * S + w] = value;
-                if (diagonal_block) Pn[(k0 + w) * S + (r - k0)] = value;
+                    value = fma(-(rk * dinv), ck, rc);
+                lds.piv[cur ^ 1][er * kMB + ec] = value;
+                __syncthreads();
+                cur ^= 1;
             }
-            __syncthreads();
-            cur ^= 1;
         }
-        const double* P = lds.panel[cur];
-        // 3. every block outside the panel: M(r,c) -= sum_w P[r][w] A0[c][w]
-        // (rows and columns past n carry values no real entry reads, and the
-        // upper half of a diagonal block is computed but not stored)
-        for (int b = t; b < blocks; b += kTile) {
+        const double* S = lds.piv[cur];
+        // 3. D_i = A_iK S for i != K (wave w takes i = w, w + 4, ...): kept in
+        // LDS (column-major) for the update, -D_i stored as the new A_iK
+        for (int i = wave; i < nb; i += 4) {
+            if (i == K) continue;
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) {
+                const int k = 4 * kc + lane / 16;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(lds.colT[i][k * kMB + lane % 16], S[k * kMB + lane % 16],
+                                                            acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = lane / 16 + 4 * v, c = lane % 16;   // entry (r, c) of D_i
+                lds.dT[i][c * kMB + r] = acc[v];
+                if (i > K)
+                    M[block_index(i, K) * (kMB * kMB) + lane * 4 + v] = -acc[v];
+                else
+                    M[block_index(K, i) * (kMB * kMB) + acc_pos(c, r)] = -acc[v];
+            }
+        }
+        // the new pivot block: S
+        M[block_index(K, K) * (kMB * kMB) + t] = S[((t / 4) / 16 + 4 * (t % 4)) * kMB + (t / 4) % 16];
+        __syncthreads();
+        // 4. A_ij += D_i A_Kj over the stored blocks off row and column K;
+        // A_Kj[k][c] = A_jK[c][k] = colT[j][k * 16 + c]
+        for (int b = wave; b < blocks; b += 4) {
             int bi = static_cast<int>((sqrtf(8.0f * static_cast<float>(b) + 1.0f) - 1.0f) * 0.5f);
             if (bi * (bi + 1) / 2 > b) --bi;
             if ((bi + 1) * (bi + 2) / 2 <= b) ++bi;
             const int bj = b - bi * (bi + 1) / 2;
             if (bi == K || bj == K) continue;
-            const int c0 = bj * B;
-#pragma unroll 1
-            for (int half = 0; half < 2; ++half) {
-                const int uh = half * (B / 2), u0 = bi * B + uh;
-                double a[B / 2][B];
+            double* blk = M + b * (kMB * kMB) + lane * 4;
+            v4d acc = {blk[0], blk[1], blk[2], blk[3]};
 #pragma unroll
-                for (int u = 0; u < B / 2; ++u)
-#pragma unroll
-                    for (int v = 0; v < B; ++v) a[u][v] = M[((uh + u) * B + v) * blocks + b];
-#pragma unroll 1
-                for (int w = 0; w < width; ++w) {
-                    double cw[B];
-#pragma unroll
-                    for (int v = 0; v < B; ++v) cw[v] = A0[(c0 + v) * S + w];
-#pragma unroll
-                    for (int u = 0; u < B / 2; ++u) {
-                        const double pr = P[(u0 + u) * S + w];
-#pragma unroll
-                        for (int v = 0; v < B; ++v) a[u][v] = fma(-pr, cw[v], a[u][v]);
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < B / 2; ++u)
-#pragma unroll
-                    for (int v = 0; v < B; ++v)
-                        if (bi > bj || uh + u >= v) M[((uh + u) * B + v) * blocks + b] = a[u][v];
+            for (int kc = 0; kc < 4; ++kc) {
+                const int k = 4 * kc + lane / 16;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(lds.dT[bi][k * kMB + lane % 16],
+                                                            lds.colT[bj][k * kMB + lane % 16], acc, 0, 0, 0);
             }
-        }
-        // 4. the swept panel back into M (a diagonal-block entry and its
-        // mirror write the same word with the same value)
-        for (int e = t; e < n * B; e += kTile) {
-            const int r = e / B, w = e - r * B;
-            if (w < width) M[at(r, k0 + w)] = P[r * S + w];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) blk[v] = acc[v];
         }
         __syncthreads();
     }
-    // I.cinv = -M in full: each stored entry (read in storage order) goes to
-    // both of its places
-    for (int e = t; e < blocks * B * B; e += kTile) {
-        const int uv = e / blocks, b = e - uv * blocks, u = uv / B, v = uv - u * B;
+    // I.cinv = -M, each stored lower entry to both of its places
+    for (int e = t; e < blocks * kMB * kMB; e += kTile) {
+        const int b = e / (kMB * kMB), q = e % (kMB * kMB), l = q / 4, v = q % 4;
         int bi = static_cast<int>((sqrtf(8.0f * static_cast<float>(b) + 1.0f) - 1.0f) * 0.5f);
         if (bi * (bi + 1) / 2 > b) --bi;
         if ((bi + 1) * (bi + 2) / 2 <= b) ++bi;
         const int bj = b - bi * (bi + 1) / 2;
-        const int r = bi * B + u, c = bj * B + v;
+        const int r = bi * kMB + l / 16 + 4 * v, c = bj * kMB + l % 16;
         if (r < n && c < n && r >= c) {
             const double value = -M[e];
             I.cinv[c * n + r] = value;

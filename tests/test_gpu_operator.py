@@ -43,8 +43,8 @@ def graph_of(ctx, level, agglomerated):
     return P.Graph(ctx, m), m
 
 
-@pytest.mark.parametrize("level,agglomerated", [(6, True), (4, False)])
-def test_pressure_poisson_pcg(ctx, level, agglomerated):
+@pytest.mark.parametrize("level,agglomerated,tight", [(6, True, 1e-10), (4, False, 1e-12)])
+def test_pressure_poisson_pcg(ctx, level, agglomerated, tight):
     announce("pressure poisson", level, agglomerated)
     g, m = graph_of(ctx, level, agglomerated)
     s = P.System(g)
@@ -58,9 +58,13 @@ def test_pressure_poisson_pcg(ctx, level, agglomerated):
     print("level", level, "groups" if agglomerated else "cells", m.cells, "iterations", info.iterations,
           "recursive", info.relative_residual, "true", true, flush=True)
     assert info.status == 0 and info.relative_residual <= 1e-6 and true <= 2e-6
-    # to 1e-12 against a direct solve with node 0 pinned (the Laplacian's
-    # null mode is the constant)
-    x12, info12 = P.pcg_operator(s, m.weight_sum, rhs, tolerance=1e-12, max_iterations=500)
+    # converged tightly, against a direct solve with node 0 pinned (the
+    # Laplacian's null mode is the constant). The regularised operator of the
+    # L6 groups has condition ~1e10 (its coarsest level ~1.7e9): CG's
+    # attainable residual there is ~1e-11 whatever the coarse solve, so it is
+    # converged to 1e-10 and checked to converge; the L4 cells to 1e-12.
+    x12, info12 = P.pcg_operator(s, m.weight_sum, rhs, tolerance=tight, max_iterations=500)
+    assert info12.status == 0 and info12.relative_residual <= tight
     exact = np.zeros(m.cells)
     exact[1:] = spla.spsolve(L[1:, 1:].tocsc(), rhs[1:])
     dx = (x12 - x12.mean()) - (exact - exact.mean())

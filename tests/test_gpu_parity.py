@@ -171,6 +171,32 @@ def test_heat_matrix_apply_and_galerkin_bit_exact(ctx, level):
         assert bits_equal(oj, ro[: oj.size])
 
 
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_coarse_inverse_is_the_dense_inverse(ctx, level):
+    """Up to 400 cells the hierarchy is one dense level, so the V-cycle is the
+    coarsest solve itself: the matrix-core block sweep's inverse (12, 42 and
+    162 rows: one, three and eleven 16 x 16 blocks, with identity padding)
+    against a dense LU solve, on the stiff variant, and its symmetry."""
+    announce("coarse inverse", level)
+    m, seed, cond, ca, land, t_old, forcing = heat_case(level, k_base=0.5 * 6371000.0 ** 2)
+    om = S.mesh("oracle", level)
+    diag, off = S.heat_matrix(om, cond, ca)
+    g = P.Graph(ctx, m)
+    assert g.depth == 1
+    s = P.System(g)
+    s.heat_set_matrix(m.entry_values(cond), ca)
+    A = S.dense_matrix(om, diag, off)
+    rng = np.random.default_rng(level)
+    for _ in range(2):
+        rhs = rng.standard_normal(m.cells)
+        got = s.precondition(rhs)
+        want = np.linalg.solve(A, rhs)
+        assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.cond(A) * np.linalg.norm(want)
+    u, v = rng.standard_normal(m.cells), rng.standard_normal(m.cells)
+    assert abs(u @ s.precondition(v) - s.precondition(u) @ v) <= 1e-14 * np.linalg.norm(u) * np.linalg.norm(v) * \
+        np.linalg.norm(np.linalg.inv(A), 2)
+
+
 @pytest.mark.parametrize("level", [3, 4, 6])
 def test_precondition_is_symmetric_positive(ctx, level):
     """The V(1,1) cycle with the same Jacobi smoother before and after the
