@@ -474,9 +474,10 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         A.plan(&I.omega, kMaxLevels);
         A.plan(&I.cinv, nc * nc);
         A.plan(&I.b, n);
+        A.plan(&I.b_alt, n);
         A.plan(&I.x[0], n);
         A.plan(&I.x[1], n);
-        void** vecs[9] = {&I.r, &I.r0, &I.p[0], &I.p[1], &I.v, &I.ph, &I.s, &I.sh, &I.t};
+        void** vecs[11] = {&I.r, &I.r0, &I.r_alt, &I.r0_alt, &I.p[0], &I.p[1], &I.v, &I.ph, &I.s, &I.sh, &I.t};
         for (void** v : vecs) A.plan(reinterpret_cast<char**>(v), static_cast<size_t>(n) * vt);
         I.partial_slots = std::max(tiles, g->depth > 1 ? g->views[0].down_tiles : 0);
         A.plan(&I.partial, 2 * static_cast<size_t>(I.partial_slots) * kPartials);

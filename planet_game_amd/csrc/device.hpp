@@ -114,6 +114,12 @@ struct Instance {
     double* x[2] = {};           // FP64 iterate, double-buffered for fused updates
     void* r = nullptr;
     void* r0 = nullptr;
+    // b, r, r0 of the other parity: in the full-register-file program a
+    // chained step's last phase assembles the next step's right-hand side
+    // there (the solve's `rp` selects the pair in use)
+    double* b_alt = nullptr;
+    void* r_alt = nullptr;
+    void* r0_alt = nullptr;
     void* p[2] = {};             // search direction, double-buffered
     void* v = nullptr;
     void* ph = nullptr;

@@ -1134,30 +1134,62 @@ __device__ __noinline__ SyncState set_matrix_call(const Instance& I, const Launc
 // ------------------------------------------------------------ BiCGSTAB
 
 // FP64 true residual row of x over level 0 (ELL): b_i - (diag_i x_i + sum).
-__device__ inline double residual_row(const Instance& I, const double* x, int i) {
+__device__ inline double residual_row(const Instance& I, const double* b, const double* x, int i) {
     const LevelView& L = I.lv[0];
     const int n = L.n, W = L.width;
     double ax = I.diag64[0][i] * x[i];
     ell_row(
         L.col, I.off64[0], n, W, i, [&](int, int c) { return x[c]; }, [&](double o, double v) { ax += o * v; });
-    return I.b[i] - ax;
+    return b[i] - ax;
 }
 
 template <class VT, int kW>
+//
+// Chained steps in the full-register-file program (kW == 1): with
+// next_forcing set, every phase that leaves x final (the half-step exit and
+// the FP64 true residual) also assembles the next step's right-hand side
+// b' = C A x + dt A S' into the other (b, r, r0) pair (rp selects the pair in
+// use), with its tile partials beside the phase's own; `prepared` then points
+// at that partial buffer and the next solve starts from it instead of an
+// assembly phase (the same values and partial sums, so the same bits),
+// zeroing its x in its first SpMV sweep.
 __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P, const double* t_old, const double* forcing,
-                             double* t_new, int g, int G, Lds& lds, int& timeouts, int& parity, int& xi) {
+                             const double* next_forcing, double* t_new, int g, int G, Lds& lds, int& timeouts,
+                             int& parity, int& xi, int& rp, double*& prepared) {
+    constexpr bool kPrep = kW == 1;
     const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
     const Values<VT> V{I};
     const LevelView& L0 = I.lv[0];
     const int n = I.n, W = L0.width, tiles = I.tiles;
-    VT* r = static_cast<VT*>(I.r);
-    VT* r0 = static_cast<VT*>(I.r0);
+    const bool swap = kPrep && prepared != nullptr;
+    if (swap) rp ^= 1;
+    const bool alt = kPrep && rp != 0;
+    VT* r = static_cast<VT*>(alt ? I.r_alt : I.r);
+    VT* r0 = static_cast<VT*>(alt ? I.r0_alt : I.r0);
+    double* b = alt ? I.b_alt : I.b;
     VT* v = static_cast<VT*>(I.v);
     VT* ph = static_cast<VT*>(I.ph);
     VT* s = static_cast<VT*>(I.s);
     VT* sh = static_cast<VT*>(I.sh);
     VT* t = static_cast<VT*>(I.t);
-    double* b = I.b;
+    constexpr int KQ = kPrep ? 4 : 2;   // partials of a phase that leaves x final
+    // row i's share of the next step's right-hand side from its final x:
+    // q[2], q[3] get (b', b') and (r', r') as the assembly phase sums them
+    const auto prepare_row = [&](int i, double x_final, double (&q)[KQ]) {
+        if constexpr (kPrep) {
+            double* bn = alt ? I.b : I.b_alt;
+            VT* rn = static_cast<VT*>(alt ? I.r : I.r_alt);
+            VT* rn0 = static_cast<VT*>(alt ? I.r0 : I.r0_alt);
+            const double bi = P.capacity_area[i] * x_final + P.dt * I.area[i] * next_forcing[i];
+            const VT rv = static_cast<VT>(bi);
+            publish(&bn[i], bi);
+            publish(&rn[i], rv);
+            publish(&rn0[i], rv);
+            q[2] = bi * bi;
+            q[3] = static_cast<double>(rv) * static_cast<double>(rv);
+        }
+    };
+    double* prep = nullptr;   // partial buffer of the last phase that prepared the next step
     const VT* diag = V.diag(0);
     const VT* off = V.off(0);
     const auto next_partials = [&]() {
@@ -1174,39 +1206,47 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
             [&](int r, VT o, VT v) { acc[r] += o * v; });
     };
 
-    // b (assembled or given), x = 0, r = r0 = b.
-    double* part = next_partials();
-    {
-        double* x = I.x[xi];
-        tile_loop(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
-            int i[R];
-            bool ok[R];
-            double bi[R], q[R][2];
-#pragma unroll
-            for (int k = 0; k < R; ++k) i[k] = tile_row(tl[k], n, ok[k]);
-#pragma unroll
-            for (int k = 0; k < R; ++k)
-                bi[k] = (P.op & kOpHeatRhs) ? P.capacity_area[i[k]] * t_old[i[k]] + P.dt * I.area[i[k]] * forcing[i[k]]
-                                            : P.in[i[k]];
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                q[k][0] = q[k][1] = 0.0;
-                if (ok[k]) {
-                    publish(&b[i[k]], bi[k]);
-                    publish(&x[i[k]], 0.0);
-                    const VT ri = static_cast<VT>(bi[k]);
-                    publish(&r[i[k]], ri);
-                    publish(&r0[i[k]], ri);
-                    q[k][0] = bi[k] * bi[k];
-                    q[k][1] = static_cast<double>(ri) * static_cast<double>(ri);
-                }
-            }
-            store_partials_n<2, R>(part, tl, q, lds);
-        });
-    }
-    group_sync(timeouts);
+    // b (assembled or given), x = 0, r = r0 = b; or, after a prepared
+    // step, b, r and r0 are in place and x is zeroed in the first SpMV sweep.
+    double* part = nullptr;
     double tot2[2];
-    group_totals<2>(part, tiles, tot2, lds);
+    bool zero_x = swap;
+    if (swap) {
+        group_totals<2>(prepared + 2, tiles, tot2, lds);
+    } else {
+        part = next_partials();
+        {
+            double* x = I.x[xi];
+            tile_loop(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+                int i[R];
+                bool ok[R];
+                double bi[R], q[R][2];
+#pragma unroll
+                for (int k = 0; k < R; ++k) i[k] = tile_row(tl[k], n, ok[k]);
+#pragma unroll
+                for (int k = 0; k < R; ++k)
+                    bi[k] = (P.op & kOpHeatRhs) ? P.capacity_area[i[k]] * t_old[i[k]] + P.dt * I.area[i[k]] * forcing[i[k]]
+                                                : P.in[i[k]];
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    q[k][0] = q[k][1] = 0.0;
+                    if (ok[k]) {
+                        publish(&b[i[k]], bi[k]);
+                        publish(&x[i[k]], 0.0);
+                        const VT ri = static_cast<VT>(bi[k]);
+                        publish(&r[i[k]], ri);
+                        publish(&r0[i[k]], ri);
+                        q[k][0] = bi[k] * bi[k];
+                        q[k][1] = static_cast<double>(ri) * static_cast<double>(ri);
+                    }
+                }
+                store_partials_n<2, R>(part, tl, q, lds);
+            });
+        }
+        group_sync(timeouts);
+        group_totals<2>(part, tiles, tot2, lds);
+    }
+    prepared = nullptr;
     const double b_norm = sqrt(tot2[0]);
     double rho1 = tot2[1];
     double r_norm = b_norm;
@@ -1216,6 +1256,16 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     bool first = true;            // first pass: r = b exactly
     bool have_residual = false;   // the true residual of x is in r, r_norm, rho1
     int pi = 0;                   // current search-direction buffer
+    if (zero_x && !(b_norm > 0.0 && rho1 != 0.0 && P.max_iterations > 0)) {
+        // no SpMV sweep will run before x is read: zero it here
+        double* x = I.x[xi];
+        for (int tile = g; tile < tiles; tile += G) {
+            const int i = tile * kTile + threadIdx.x;
+            if (i < n) publish(&x[i], 0.0);
+        }
+        group_sync(timeouts);
+        zero_x = false;
+    }
 
     if (b_norm > 0.0) {
         for (;;) {
@@ -1226,17 +1276,19 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     const double* x = I.x[xi];
                     for (int tile = g; tile < tiles; tile += G) {
                         const int i = tile * kTile + threadIdx.x;
-                        double q[2] = {0.0, 0.0};
+                        double q[KQ] = {};
                         if (i < n) {
-                            const double ri = residual_row(I, x, i);
+                            const double ri = residual_row(I, b, x, i);
                             const VT rv = static_cast<VT>(ri);
                             publish(&r[i], rv);
                             publish(&r0[i], rv);
                             q[0] = ri * ri;
                             q[1] = static_cast<double>(rv) * static_cast<double>(rv);
+                            if (kPrep && next_forcing != nullptr) prepare_row(i, x[i], q);
                         }
-                        store_partials<2>(part, tile, q, lds);
+                        store_partials<KQ>(part, tile, q, lds);
                     }
+                    if (kPrep) prep = next_forcing != nullptr ? part : nullptr;
                     group_sync(timeouts);
                     group_totals<2>(part, tiles, tot2, lds);
                     r_norm = sqrt(tot2[0]);
@@ -1285,10 +1337,12 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         if (ok[k]) {
                             publish(&v[i[k]], acc[k]);
                             q[k][0] = static_cast<double>(rz[k]) * static_cast<double>(acc[k]);
+                            if (kPrep && zero_x) publish(&I.x[xi][i[k]], 0.0);
                         }
                     }
                     store_partials_n<1, R>(part, tl, q, lds);
                 });
+                zero_x = false;
                 group_sync(timeouts);
                 double r0v[1];
                 group_totals<1>(part, tiles, r0v, lds);
@@ -1307,10 +1361,11 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     double* xn = I.x[xi ^ 1];
                     for (int tile = g; tile < tiles; tile += G) {
                         const int i = tile * kTile + threadIdx.x;
-                        double q[2] = {0.0, 0.0};
+                        double q[KQ] = {};
                         if (i < n) {
                             const double x_new = x[i] + alpha * static_cast<double>(ph[i]);
                             publish(&xn[i], x_new);
+                            if (kPrep && next_forcing != nullptr) prepare_row(i, x_new, q);
                             double ax = I.diag64[0][i] * x_new;
                             ell_row(
                                 L0.col, I.off64[0], n, W, i,
@@ -1323,8 +1378,9 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                             q[0] = ri * ri;
                             q[1] = static_cast<double>(rv) * static_cast<double>(rv);
                         }
-                        store_partials<2>(part, tile, q, lds);
+                        store_partials<KQ>(part, tile, q, lds);
                     }
+                    if (kPrep) prep = next_forcing != nullptr ? part : nullptr;
                     group_sync(timeouts);
                     xi ^= 1;
                     group_totals<2>(part, tiles, tot2, lds);
@@ -1365,6 +1421,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 omega = tt[0] > 0.0 ? tt[1] / tt[0] : 0.0;
                 // x += alpha ph + omega sh, r = s - omega t, (r, r), (r0, r)
                 part = next_partials();
+                prep = nullptr;
                 {
                     double* x = I.x[xi];
                     tile_loop(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
@@ -1412,6 +1469,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
             if (i < n) publish(&t_new[i], x[i]);
         }
     }
+    prepared = prep;
     SolveResult out{};
     out.iterations = iterations;
     out.restarts = restarts;
@@ -1575,21 +1633,27 @@ __device__ __noinline__ SolveResult pcg_call(const Instance* instance, const Lau
 
 template <class VT, int kW>
 __device__ __forceinline__ void run_solve(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts, int& parity) {
-    int xi = 0;
+    int xi = 0, rp = 0;
+    double* prepared = nullptr;
     const int steps = (P.op & kOpChain) ? P.steps : 1;
     SolveResult last{};
     for (int step = 0; step < steps; ++step) {
         const double* t_old = P.t_old;
         const double* forcing = P.forcing;
+        const double* next_forcing = nullptr;
         double* t_new = (P.op & kOpHeatRhs) ? P.t_new : P.out;
         if (P.op & kOpChain) {
             // step k reads the previous step's x (row by row, before this
-            // step zeroes it) and writes the final T only at the end
+            // step zeroes it) and writes the final T only at the end; in the
+            // full-register-file program each step but the last prepares the
+            // next one's right-hand side
             t_old = step == 0 ? P.t_old : I.x[xi];
             forcing = P.forcing + static_cast<size_t>(step) * I.n;
+            if (kW == 1 && step + 1 < steps) next_forcing = forcing + I.n;
             t_new = step + 1 == steps ? P.t_new : nullptr;
         }
-        last = solve<VT, kW>(I, P, t_old, forcing, t_new, g, G, lds, timeouts, parity, xi);
+        last = solve<VT, kW>(I, P, t_old, forcing, next_forcing, t_new, g, G, lds, timeouts, parity, xi, rp,
+                             prepared);
         if (g == 0 && threadIdx.x == 0 && P.step_results != nullptr) P.step_results[step] = last;
     }
     if (g == 0 && threadIdx.x == 0) {
