@@ -60,6 +60,18 @@ template <class T> __device__ __forceinline__ void publish(T* p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A value another workgroup wrote before the last group barrier. kFresh:
+// an agent-scope (coherent) load, which reads past this XCD's L2, for the
+// phases after a lean barrier (group_sync<true>), which does not invalidate
+// the L2; otherwise a plain load (the barrier's acquire fence invalidated
+// the caches).
+template <bool kFresh, class T> __device__ __forceinline__ T fetch(const T* p) {
+    if constexpr (kFresh)
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return *p;
+}
+
 __device__ inline double wave_sum(double v) {
     // xor butterfly: every lane ends with the same bits
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -192,12 +204,13 @@ __device__ inline double tile_max(double v, Lds& lds) {
 }
 
 // Totals of slots 0..K-1 over `tiles` tiles (fixed order), to every thread.
-template <int K> __device__ inline void group_totals(const double* partial, int tiles, double (&out)[K], Lds& lds) {
+template <int K, bool kF = false>
+__device__ inline void group_totals(const double* partial, int tiles, double (&out)[K], Lds& lds) {
     if (threadIdx.x < 64) {
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             double s = 0.0;
-            for (int t = threadIdx.x; t < tiles; t += 64) s += partial[t * kPartials + q];
+            for (int t = threadIdx.x; t < tiles; t += 64) s += fetch<kF>(&partial[t * kPartials + q]);
             s = wave_sum(s);
             if (threadIdx.x == 0) lds.bcast[q] = s;
         }
@@ -286,7 +299,16 @@ constexpr long long kBarrierTicks = 100000000;
 constexpr int kLeaves = 8;
 static_assert(96 + 64 * kLeaves == kBarrierWordCount, "barrier layout");
 
+//
+// Lean form (kLean, the full-register-file program's solve and PCG phases,
+// whose values for other workgroups are all written with coherent stores and
+// read with coherent loads): each thread waits for its own stores instead of
+// a release fence per workgroup, and no acquire fence follows the wait, so
+// this XCD's L2 keeps the read-only operator data across phases. The last
+// arrivals of a leaf and of the root keep their acq_rel fences.
+template <bool kLean = false>
 __device__ __forceinline__ void group_sync(int& timeouts) {
+    if constexpr (kLean) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     const int G = grp.size;
     if (G == 1) {
@@ -299,7 +321,7 @@ __device__ __forceinline__ void group_sync(int& timeouts) {
         const int leaf = g % kLeaves;
         const int leaves = G < kLeaves ? G : kLeaves;
         const uint32_t leaf_size = static_cast<uint32_t>((G - leaf + kLeaves - 1) / kLeaves);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if constexpr (!kLean) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const uint32_t seen = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t p = seen & 1u;
         uint32_t* root = grp.barrier + 32 + 32 * p;
@@ -329,7 +351,7 @@ __device__ __forceinline__ void group_sync(int& timeouts) {
                     break;
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if constexpr (!kLean) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
     }
     trace_stamp();
@@ -361,12 +383,12 @@ template <class VT> struct Source {
     const VT* c;
     double c1, c2;
     int kind;
-    __device__ VT at(int k) const {
-        if (kind == 0) return a[k];
+    template <bool kF = false> __device__ VT at(int k) const {
+        if (kind == 0) return fetch<kF>(a + k);
         if (kind == 1)
-            return static_cast<VT>(static_cast<double>(a[k]) +
-                                   c1 * (static_cast<double>(b[k]) - c2 * static_cast<double>(c[k])));
-        return static_cast<VT>(static_cast<double>(a[k]) - c1 * static_cast<double>(b[k]));
+            return static_cast<VT>(static_cast<double>(fetch<kF>(a + k)) +
+                                   c1 * (static_cast<double>(fetch<kF>(b + k)) - c2 * static_cast<double>(fetch<kF>(c + k))));
+        return static_cast<VT>(static_cast<double>(fetch<kF>(a + k)) - c1 * static_cast<double>(fetch<kF>(b + k)));
     }
 };
 
@@ -461,7 +483,7 @@ __device__ inline int tile_row(int tl, int n, bool& ok) {
 // over the level's restriction tiles. On level 0 the input comes from a
 // Source; each row stores its value in `keep` (if set) and adds its square
 // to the tile partial (if `part` is set).
-template <class VT>
+template <class VT, bool kF = false>
 __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& V, int j, const Source<VT>& src, VT* keep,
                            VT* coarse_rhs, double* part, int g, int G, Lds& lds) {
     const LevelView& L = I.lv[j];
@@ -494,7 +516,7 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
         VT wi[R], acc[R];
         double sq[R][1];
 #pragma unroll
-        for (int r = 0; r < R; ++r) wi[r] = src.at(i[r]);
+        for (int r = 0; r < R; ++r) wi[r] = src.template at<kF>(i[r]);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             sq[r][0] = ok[r] ? static_cast<double>(wi[r]) * static_cast<double>(wi[r]) : 0.0;
@@ -502,7 +524,7 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
             acc[r] = wi[r] - diag[i[r]] * xi;
         }
         ell_rows<R>(
-            L.col, off, n, W, i, [&](int, int, int c) { return om * dinv[c] * src.at(c); },
+            L.col, off, n, W, i, [&](int, int, int c) { return om * dinv[c] * src.template at<kF>(c); },
             [&](int r, VT o, VT v) { acc[r] -= o * v; });
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -531,7 +553,7 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
 // 0, and a butterfly sums the lanes, so every workgroup computing a row
 // gets the same bits. A wave takes kRows rows at a time; the first round's
 // row ids are loaded before the right-hand side is staged.
-template <class VT>
+template <class VT, bool kF = false>
 __device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs, const int32_t* list, int first, int count,
                                 Lds& lds) {
     constexpr int kRows = 4;   // rows in flight per wave
@@ -546,7 +568,7 @@ __device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs
         }
     };
     row_ids(wave * kRows);
-    for (int k = threadIdx.x; k < n; k += blockDim.x) lds.crhs[k] = static_cast<double>(rhs[k]);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) lds.crhs[k] = static_cast<double>(fetch<kF>(&rhs[k]));
     __syncthreads();
     const double* inv = I.cinv;
     for (int q0 = wave * kRows; q0 < count; q0 += 4 * kRows) {
@@ -572,7 +594,7 @@ __device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs
 // Prolongation of the coarse correction fused with post-smoothing on level
 // j. The correction comes from `corr`, or, on the level above the coarsest,
 // from the coarse solve each workgroup recomputes into LDS.
-template <class VT>
+template <class VT, bool kF = false>
 __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V, int j, const VT* rhs, const VT* corr, VT* out, int g,
                          int G, Lds& lds, double* dot_part = nullptr, const VT* dot_with = nullptr) {
     const LevelView& L = I.lv[j];
@@ -589,23 +611,24 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
         for (int tile = g; tile < tiles; tile += G) {
             const int i = tile * kTile + threadIdx.x;
             double q[1] = {0.0};
-            coarse_rows_lds<VT>(I, static_cast<const VT*>(I.lvl_b[j + 1]), L.need + tile * L.need_stride, 0,
-                                L.need_stride, lds);
+            coarse_rows_lds<VT, kF>(I, static_cast<const VT*>(I.lvl_b[j + 1]), L.need + tile * L.need_stride, 0,
+                                    L.need_stride, lds);
             if (i < n) {
-                const VT x2 = om * dinv[i] * rhs[i] + static_cast<VT>(lds.cvec[L.agg[i]]);
-                VT acc = rhs[i] - diag[i] * x2;
+                const VT rhs_i = fetch<kF>(&rhs[i]);
+                const VT x2 = om * dinv[i] * rhs_i + static_cast<VT>(lds.cvec[L.agg[i]]);
+                VT acc = rhs_i - diag[i] * x2;
                 // the neighbour's aggregate comes from the slot's own map, so the
                 // correction read does not wait for the column
                 ell_row(
                     L.col, off, n, W, i,
                     [&](int s, int c) {
                         const int a = L.agg_col[s * n + i];
-                        return om * dinv[c] * rhs[c] + static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
+                        return om * dinv[c] * fetch<kF>(&rhs[c]) + static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
                     },
                     [&](VT o, VT v) { acc -= o * v; });
                 const VT value = x2 + om * dinv[i] * acc;
                 publish(&out[i], value);
-                q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(dot_with[i]) : 0.0;
+                q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i])) : 0.0;
             }
             if (dot_part != nullptr) store_partials<1>(dot_part, tile, q, lds);
         }
@@ -619,14 +642,15 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
         for (int r = 0; r < R; ++r) i[r] = tile_row(tl[r], n, ok[r]);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            x2[r] = om * dinv[i[r]] * rhs[i[r]] + corr[L.agg[i[r]]];
-            acc[r] = rhs[i[r]] - diag[i[r]] * x2[r];
+            const VT rhs_i = fetch<kF>(&rhs[i[r]]);
+            x2[r] = om * dinv[i[r]] * rhs_i + fetch<kF>(&corr[L.agg[i[r]]]);
+            acc[r] = rhs_i - diag[i[r]] * x2[r];
         }
         ell_rows<R>(
             L.col, off, n, W, i,
             [&](int r, int s, int c) {
                 const int a = L.agg_col[s * n + i[r]];
-                return om * dinv[c] * rhs[c] + corr[a >= 0 ? a : 0];
+                return om * dinv[c] * fetch<kF>(&rhs[c]) + fetch<kF>(&corr[a >= 0 ? a : 0]);
             },
             [&](int r, VT o, VT v) { acc[r] -= o * v; });
         double q[R][1];
@@ -636,7 +660,7 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
             if (ok[r]) {
                 const VT value = x2[r] + om * dinv[i[r]] * acc[r];
                 publish(&out[i[r]], value);
-                if (dot_with != nullptr) q[r][0] = static_cast<double>(value) * static_cast<double>(dot_with[i[r]]);
+                if (dot_with != nullptr) q[r][0] = static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i[r]]));
             }
         }
         if (dot_part != nullptr) store_partials_n<1, R>(dot_part, tl, q, lds);
@@ -674,6 +698,7 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
     const VT* dinv_m = V.dinv(j + 1);
     const VT* off_m = V.off(j + 1);
     const VT* rhs_m = static_cast<const VT*>(I.lvl_b[j + 1]);
+    constexpr bool kF = kW == 1;
     double* mid = lds.tile[0];
     const int tiles = (n + kTile - 1) / kTile;
     for (int tile = g; tile < tiles; tile += G) {
@@ -681,17 +706,18 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
         // out before the coarse solve
         const int mq = threadIdx.x;
         const int m = mq < L.mid_stride ? L.mid[tile * L.mid_stride + mq] : -1;
-        coarse_rows_lds<VT>(I, static_cast<const VT*>(I.lvl_b[j + 2]), L.mid_coarse + tile * L.mid_coarse_stride, 0,
-                            L.mid_coarse_stride, lds);
+        coarse_rows_lds<VT, kF>(I, static_cast<const VT*>(I.lvl_b[j + 2]), L.mid_coarse + tile * L.mid_coarse_stride, 0,
+                                L.mid_coarse_stride, lds);
         {
             if (m >= 0) {
-                const VT x2 = omm * dinv_m[m] * rhs_m[m] + static_cast<VT>(lds.cvec[M.agg[m]]);
-                VT acc = rhs_m[m] - diag_m[m] * x2;
+                const VT rhs_mm = fetch<kF>(&rhs_m[m]);
+                const VT x2 = omm * dinv_m[m] * rhs_mm + static_cast<VT>(lds.cvec[M.agg[m]]);
+                VT acc = rhs_mm - diag_m[m] * x2;
                 ell_row(
                     M.col, off_m, nm, Wm, m,
                     [&](int s, int c) {
                         const int a = M.agg_col[s * nm + m];
-                        return omm * dinv_m[c] * rhs_m[c] + static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
+                        return omm * dinv_m[c] * fetch<kF>(&rhs_m[c]) + static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
                     },
                     [&](VT o, VT v) { acc -= o * v; });
                 mid[mq] = static_cast<double>(static_cast<VT>(x2 + omm * dinv_m[m] * acc));
@@ -701,18 +727,19 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
         const int i = tile * kTile + threadIdx.x;
         double q[1] = {0.0};
         if (i < n) {
-            const VT x2 = om * dinv[i] * rhs[i] + static_cast<VT>(mid[L.agg_loc[i]]);
-            VT acc = rhs[i] - diag[i] * x2;
+            const VT rhs_i = fetch<kF>(&rhs[i]);
+            const VT x2 = om * dinv[i] * rhs_i + static_cast<VT>(mid[L.agg_loc[i]]);
+            VT acc = rhs_i - diag[i] * x2;
             ell_row(
                 L.col, off, n, W, i,
                 [&](int s, int c) {
                     const int a = L.agg_col_loc[s * n + i];
-                    return om * dinv[c] * rhs[c] + static_cast<VT>(mid[a >= 0 ? a : 0]);
+                    return om * dinv[c] * fetch<kF>(&rhs[c]) + static_cast<VT>(mid[a >= 0 ? a : 0]);
                 },
                 [&](VT o, VT v) { acc -= o * v; });
             const VT value = x2 + om * dinv[i] * acc;
             publish(&out[i], value);
-            q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(dot_with[i]) : 0.0;
+            q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i])) : 0.0;
         }
         if (dot_part != nullptr) store_partials<1>(dot_part, tile, q, lds);
         __syncthreads();
@@ -722,7 +749,7 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
 // First sweep of a V-cycle: level 0's down phase (input evaluated from
 // `src`, stored into `keep`, squares into `part`); on a one-level hierarchy
 // the input is only materialised. Ends with a barrier.
-template <class VT>
+template <class VT, bool kF = false>
 __device__ __forceinline__ void vcycle_first(const Instance& I, const Values<VT>& V, const Source<VT>& src, VT* keep, double* part,
                              int g, int G, Lds& lds, int& timeouts) {
     if (I.depth == 1) {
@@ -730,16 +757,16 @@ __device__ __forceinline__ void vcycle_first(const Instance& I, const Values<VT>
             const int i = tile * kTile + threadIdx.x;
             double sq[1] = {0.0};
             if (i < I.n) {
-                const VT wi = src.at(i);
+                const VT wi = src.template at<kF>(i);
                 publish(&keep[i], wi);
                 sq[0] = static_cast<double>(wi) * static_cast<double>(wi);
             }
             if (part != nullptr) store_partials<1>(part, tile, sq, lds);
         }
     } else {
-        down_phase<VT>(I, V, 0, src, keep, static_cast<VT*>(I.lvl_b[1]), part, g, G, lds);
+        down_phase<VT, kF>(I, V, 0, src, keep, static_cast<VT*>(I.lvl_b[1]), part, g, G, lds);
     }
-    group_sync(timeouts);
+    group_sync<kF>(timeouts);
 }
 
 // Tiles the first sweep's partials cover.
@@ -750,29 +777,30 @@ __device__ inline int first_sweep_tiles(const Instance& I) { return I.depth == 1
 template <class VT, int kW>
 __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>& V, const VT* w0, VT* out0, int g, int G, Lds& lds,
                             int& timeouts, double* dot_part = nullptr, const VT* dot_with = nullptr) {
+    constexpr bool kF = kW == 1;
     const int J = I.depth;
     if (J == 1) {
         for (int tile = g; tile < I.tiles; tile += G) {
             const int first = tile * kTile, rows = min(kTile, I.n - first);
-            coarse_rows_lds<VT>(I, w0, nullptr, first, rows, lds);
+            coarse_rows_lds<VT, kF>(I, w0, nullptr, first, rows, lds);
             const int i = first + threadIdx.x;
             double q[1] = {0.0};
             if (i < I.n) {
                 const VT value = static_cast<VT>(lds.cvec[i]);
                 publish(&out0[i], value);
-                if (dot_with != nullptr) q[0] = static_cast<double>(value) * static_cast<double>(dot_with[i]);
+                if (dot_with != nullptr) q[0] = static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i]));
             }
             if (dot_part != nullptr) store_partials<1>(dot_part, tile, q, lds);
         }
-        group_sync(timeouts);
+        group_sync<kF>(timeouts);
         return;
     }
     const Source<VT> none{nullptr, nullptr, nullptr, 0.0, 0.0, 0};
     for (int j = 1; j + 1 < J; ++j) {
         Source<VT> src = none;
         src.a = static_cast<const VT*>(I.lvl_b[j]);
-        down_phase<VT>(I, V, j, src, nullptr, static_cast<VT*>(I.lvl_b[j + 1]), nullptr, g, G, lds);
-        group_sync(timeouts);
+        down_phase<VT, kF>(I, V, j, src, nullptr, static_cast<VT*>(I.lvl_b[j + 1]), nullptr, g, G, lds);
+        group_sync<kF>(timeouts);
     }
     // with about a workgroup per tile, the last two prolongations are one
     // sweep (merged_up_phase); with fewer workgroups each tile would redo
@@ -786,9 +814,9 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
         if (merge && j == jm)
             merged_up_phase<VT, kW>(&I, j, rhs, out, g, G, lds, j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr);
         else
-            up_phase<VT>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds, j == 0 ? dot_part : nullptr,
-                         j == 0 ? dot_with : nullptr);
-        group_sync(timeouts);
+            up_phase<VT, kF>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds,
+                             j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr);
+        group_sync<kF>(timeouts);
     }
 }
 
@@ -1134,13 +1162,15 @@ __device__ __noinline__ SyncState set_matrix_call(const Instance& I, const Launc
 // ------------------------------------------------------------ BiCGSTAB
 
 // FP64 true residual row of x over level 0 (ELL): b_i - (diag_i x_i + sum).
+template <bool kF = false>
 __device__ inline double residual_row(const Instance& I, const double* b, const double* x, int i) {
     const LevelView& L = I.lv[0];
     const int n = L.n, W = L.width;
-    double ax = I.diag64[0][i] * x[i];
+    double ax = I.diag64[0][i] * fetch<kF>(&x[i]);
     ell_row(
-        L.col, I.off64[0], n, W, i, [&](int, int c) { return x[c]; }, [&](double o, double v) { ax += o * v; });
-    return b[i] - ax;
+        L.col, I.off64[0], n, W, i, [&](int, int c) { return fetch<kF>(&x[c]); },
+        [&](double o, double v) { ax += o * v; });
+    return fetch<kF>(&b[i]) - ax;
 }
 
 template <class VT, int kW>
@@ -1157,6 +1187,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                              const double* next_forcing, double* t_new, int g, int G, Lds& lds, int& timeouts,
                              int& parity, int& xi, int& rp, double*& prepared) {
     constexpr bool kPrep = kW == 1;
+    constexpr bool kF = kW == 1;   // lean barriers, coherent loads
     const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
     const Values<VT> V{I};
     const LevelView& L0 = I.lv[0];
@@ -1200,9 +1231,9 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     // acc[r] = (A in)_i[r] for R rows at once
     const auto spmv_rows = [&]<int R>(const VT* in, const int (&i)[R], VT (&acc)[R]) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = diag[i[r]] * in[i[r]];
+        for (int r = 0; r < R; ++r) acc[r] = diag[i[r]] * fetch<kF>(&in[i[r]]);
         ell_rows<R>(
-            L0.col, off, n, W, i, [&](int, int, int c) { return in[c]; },
+            L0.col, off, n, W, i, [&](int, int, int c) { return fetch<kF>(&in[c]); },
             [&](int r, VT o, VT v) { acc[r] += o * v; });
     };
 
@@ -1212,7 +1243,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     double tot2[2];
     bool zero_x = swap;
     if (swap) {
-        group_totals<2>(prepared + 2, tiles, tot2, lds);
+        group_totals<2, kF>(prepared + 2, tiles, tot2, lds);
     } else {
         part = next_partials();
         {
@@ -1225,7 +1256,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 for (int k = 0; k < R; ++k) i[k] = tile_row(tl[k], n, ok[k]);
 #pragma unroll
                 for (int k = 0; k < R; ++k)
-                    bi[k] = (P.op & kOpHeatRhs) ? P.capacity_area[i[k]] * t_old[i[k]] + P.dt * I.area[i[k]] * forcing[i[k]]
+                    bi[k] = (P.op & kOpHeatRhs) ? P.capacity_area[i[k]] * fetch<kF>(&t_old[i[k]]) + P.dt * I.area[i[k]] * forcing[i[k]]
                                                 : P.in[i[k]];
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
@@ -1243,8 +1274,8 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 store_partials_n<2, R>(part, tl, q, lds);
             });
         }
-        group_sync(timeouts);
-        group_totals<2>(part, tiles, tot2, lds);
+        group_sync<kF>(timeouts);
+        group_totals<2, kF>(part, tiles, tot2, lds);
     }
     prepared = nullptr;
     const double b_norm = sqrt(tot2[0]);
@@ -1263,7 +1294,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
             const int i = tile * kTile + threadIdx.x;
             if (i < n) publish(&x[i], 0.0);
         }
-        group_sync(timeouts);
+        group_sync<kF>(timeouts);
         zero_x = false;
     }
 
@@ -1278,19 +1309,19 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         const int i = tile * kTile + threadIdx.x;
                         double q[KQ] = {};
                         if (i < n) {
-                            const double ri = residual_row(I, b, x, i);
+                            const double ri = residual_row<kF>(I, b, x, i);
                             const VT rv = static_cast<VT>(ri);
                             publish(&r[i], rv);
                             publish(&r0[i], rv);
                             q[0] = ri * ri;
                             q[1] = static_cast<double>(rv) * static_cast<double>(rv);
-                            if (kPrep && next_forcing != nullptr) prepare_row(i, x[i], q);
+                            if (kPrep && next_forcing != nullptr) prepare_row(i, fetch<kF>(&x[i]), q);
                         }
                         store_partials<KQ>(part, tile, q, lds);
                     }
                     if (kPrep) prep = next_forcing != nullptr ? part : nullptr;
-                    group_sync(timeouts);
-                    group_totals<2>(part, tiles, tot2, lds);
+                    group_sync<kF>(timeouts);
+                    group_totals<2, kF>(part, tiles, tot2, lds);
                     r_norm = sqrt(tot2[0]);
                     rho1 = tot2[1];
                 }
@@ -1315,7 +1346,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 if (!first_iteration) src = Source<VT>{r, static_cast<const VT*>(I.p[pi]), v, beta, omega, 1};
                 VT* p_new = static_cast<VT*>(I.p[pi ^ 1]);
                 pi ^= 1;
-                vcycle_first<VT>(I, V, src, p_new, nullptr, g, G, lds, timeouts);
+                vcycle_first<VT, kF>(I, V, src, p_new, nullptr, g, G, lds, timeouts);
                 vcycle_rest<VT, kW>(I, V, p_new, ph, g, G, lds, timeouts);
                 ++v_cycles;
                 first_iteration = false;
@@ -1329,7 +1360,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
 #pragma unroll
                     for (int k = 0; k < R; ++k) i[k] = tile_row(tl[k], n, ok[k]);
 #pragma unroll
-                    for (int k = 0; k < R; ++k) rz[k] = r0[i[k]];
+                    for (int k = 0; k < R; ++k) rz[k] = fetch<kF>(&r0[i[k]]);
                     spmv_rows(ph, i, acc);
 #pragma unroll
                     for (int k = 0; k < R; ++k) {
@@ -1343,16 +1374,16 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     store_partials_n<1, R>(part, tl, q, lds);
                 });
                 zero_x = false;
-                group_sync(timeouts);
+                group_sync<kF>(timeouts);
                 double r0v[1];
-                group_totals<1>(part, tiles, r0v, lds);
+                group_totals<1, kF>(part, tiles, r0v, lds);
                 if (r0v[0] == 0.0) break;
                 alpha = rho / r0v[0];
                 // s = r - alpha v inside the next V-cycle's first sweep, (s, s)
                 part = next_partials();
-                vcycle_first<VT>(I, V, Source<VT>{r, v, nullptr, alpha, 0.0, 2}, s, part, g, G, lds, timeouts);
+                vcycle_first<VT, kF>(I, V, Source<VT>{r, v, nullptr, alpha, 0.0, 2}, s, part, g, G, lds, timeouts);
                 double ss[1];
-                group_totals<1>(part, first_sweep_tiles(I), ss, lds);
+                group_totals<1, kF>(part, first_sweep_tiles(I), ss, lds);
                 ++iterations;
                 if (sqrt(ss[0]) <= inner_target) {
                     // x' = x + alpha ph fused with the FP64 true residual of x'
@@ -1363,15 +1394,15 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         const int i = tile * kTile + threadIdx.x;
                         double q[KQ] = {};
                         if (i < n) {
-                            const double x_new = x[i] + alpha * static_cast<double>(ph[i]);
+                            const double x_new = fetch<kF>(&x[i]) + alpha * static_cast<double>(fetch<kF>(&ph[i]));
                             publish(&xn[i], x_new);
                             if (kPrep && next_forcing != nullptr) prepare_row(i, x_new, q);
                             double ax = I.diag64[0][i] * x_new;
                             ell_row(
                                 L0.col, I.off64[0], n, W, i,
-                                [&](int, int c) { return x[c] + alpha * static_cast<double>(ph[c]); },
+                                [&](int, int c) { return fetch<kF>(&x[c]) + alpha * static_cast<double>(fetch<kF>(&ph[c])); },
                                 [&](double o, double v) { ax += o * v; });
-                            const double ri = b[i] - ax;
+                            const double ri = fetch<kF>(&b[i]) - ax;
                             const VT rv = static_cast<VT>(ri);
                             publish(&r[i], rv);
                             publish(&r0[i], rv);
@@ -1381,9 +1412,9 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         store_partials<KQ>(part, tile, q, lds);
                     }
                     if (kPrep) prep = next_forcing != nullptr ? part : nullptr;
-                    group_sync(timeouts);
+                    group_sync<kF>(timeouts);
                     xi ^= 1;
-                    group_totals<2>(part, tiles, tot2, lds);
+                    group_totals<2, kF>(part, tiles, tot2, lds);
                     r_norm = sqrt(tot2[0]);
                     rho1 = tot2[1];
                     have_residual = true;
@@ -1401,7 +1432,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
 #pragma unroll
                     for (int k = 0; k < R; ++k) i[k] = tile_row(tl[k], n, ok[k]);
 #pragma unroll
-                    for (int k = 0; k < R; ++k) sv[k] = s[i[k]];
+                    for (int k = 0; k < R; ++k) sv[k] = fetch<kF>(&s[i[k]]);
                     spmv_rows(sh, i, acc);
 #pragma unroll
                     for (int k = 0; k < R; ++k) {
@@ -1415,9 +1446,9 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     }
                     store_partials_n<2, R>(part, tl, q, lds);
                 });
-                group_sync(timeouts);
+                group_sync<kF>(timeouts);
                 double tt[2];
-                group_totals<2>(part, tiles, tt, lds);
+                group_totals<2, kF>(part, tiles, tt, lds);
                 omega = tt[0] > 0.0 ? tt[1] / tt[0] : 0.0;
                 // x += alpha ph + omega sh, r = s - omega t, (r, r), (r0, r)
                 part = next_partials();
@@ -1432,12 +1463,12 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         for (int k = 0; k < R; ++k) i[k] = tile_row(tl[k], n, ok[k]);
 #pragma unroll
                         for (int k = 0; k < R; ++k) {
-                            xv[k] = x[i[k]];
-                            pv[k] = static_cast<double>(ph[i[k]]);
-                            shv[k] = static_cast<double>(sh[i[k]]);
-                            sv[k] = static_cast<double>(s[i[k]]);
-                            tv[k] = static_cast<double>(t[i[k]]);
-                            rz[k] = static_cast<double>(r0[i[k]]);
+                            xv[k] = fetch<kF>(&x[i[k]]);
+                            pv[k] = static_cast<double>(fetch<kF>(&ph[i[k]]));
+                            shv[k] = static_cast<double>(fetch<kF>(&sh[i[k]]));
+                            sv[k] = static_cast<double>(fetch<kF>(&s[i[k]]));
+                            tv[k] = static_cast<double>(fetch<kF>(&t[i[k]]));
+                            rz[k] = static_cast<double>(fetch<kF>(&r0[i[k]]));
                         }
 #pragma unroll
                         for (int k = 0; k < R; ++k) {
@@ -1454,9 +1485,9 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         store_partials_n<2, R>(part, tl, q, lds);
                     });
                 }
-                group_sync(timeouts);
+                group_sync<kF>(timeouts);
                 double rr[2];
-                group_totals<2>(part, tiles, rr, lds);
+                group_totals<2, kF>(part, tiles, rr, lds);
                 rho1 = rr[1];
                 if (sqrt(rr[0]) <= inner_target) break;
             }
@@ -1466,7 +1497,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
         const double* x = I.x[xi];
         for (int tile = g; tile < tiles; tile += G) {
             const int i = tile * kTile + threadIdx.x;
-            if (i < n) publish(&t_new[i], x[i]);
+            if (i < n) publish(&t_new[i], fetch<kF>(&x[i]));
         }
     }
     prepared = prep;
@@ -1502,6 +1533,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
 template <int kW>
 __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts,
                                            int& parity) {
+    constexpr bool kF = kW == 1;   // lean barriers, coherent loads
     const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
     const Values<double> V{I};
     const LevelView& L0 = I.lv[0];
@@ -1532,20 +1564,20 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
         }
         store_partials<1>(part, tile, v, lds);
     }
-    group_sync(timeouts);
+    group_sync<kF>(timeouts);
     double tot[1];
-    group_totals<1>(part, tiles, tot, lds);
+    group_totals<1, kF>(part, tiles, tot, lds);
     const double b_norm = sqrt(tot[0]);
     const double target = P.tolerance * b_norm;
     double r_norm = b_norm;
     // z = M r with (r, z)
     const auto precondition = [&]() {
         const Source<double> src{r, nullptr, nullptr, 0.0, 0.0, 0};
-        vcycle_first<double>(I, V, src, kept, nullptr, g, G, lds, timeouts);
+        vcycle_first<double, kF>(I, V, src, kept, nullptr, g, G, lds, timeouts);
         double* dot = next_partials();
         vcycle_rest<double, kW>(I, V, r, z, g, G, lds, timeouts, dot, r);
         double rz[1];
-        group_totals<1>(dot, tiles, rz, lds);   // one partial per level-0 tile
+        group_totals<1, kF>(dot, tiles, rz, lds);   // one partial per level-0 tile
         return rz[0];
     };
     double rho = b_norm > 0.0 ? precondition() : 0.0;
@@ -1563,10 +1595,10 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
             double v[1] = {0.0};
             if (i < n) {
                 // held rows keep d = 0, so x, r and q stay exactly 0 there
-                const double di_value = P.hold != nullptr && P.hold[i] != 0 ? 0.0 : src.at(i);
+                const double di_value = P.hold != nullptr && P.hold[i] != 0 ? 0.0 : src.template at<kF>(i);
                 double acc = diag[i] * di_value;
                 ell_row(
-                    L0.col, off, n, W, i, [&](int, int c) { return src.at(c); },
+                    L0.col, off, n, W, i, [&](int, int c) { return src.template at<kF>(c); },
                     [&](double o, double w) { acc += o * w; });
                 publish(&dn[i], di_value);
                 publish(&q[i], acc);
@@ -1574,8 +1606,8 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
             }
             store_partials<1>(part, tile, v, lds);
         }
-        group_sync(timeouts);
-        group_totals<1>(part, tiles, tot, lds);
+        group_sync<kF>(timeouts);
+        group_totals<1, kF>(part, tiles, tot, lds);
         di ^= 1;
         first = false;
         const double alpha = rho / tot[0];
@@ -1585,15 +1617,15 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
             const int i = tile * kTile + threadIdx.x;
             double v[1] = {0.0};
             if (i < n) {
-                const double ri = r[i] - alpha * q[i];
-                publish(&x[i], x[i] + alpha * dn[i]);
+                const double ri = fetch<kF>(&r[i]) - alpha * fetch<kF>(&q[i]);
+                publish(&x[i], fetch<kF>(&x[i]) + alpha * fetch<kF>(&dn[i]));
                 publish(&r[i], ri);
                 v[0] = ri * ri;
             }
             store_partials<1>(part, tile, v, lds);
         }
-        group_sync(timeouts);
-        group_totals<1>(part, tiles, tot, lds);
+        group_sync<kF>(timeouts);
+        group_totals<1, kF>(part, tiles, tot, lds);
         r_norm = sqrt(tot[0]);
         ++iterations;
         if (!(iterations < P.max_iterations && r_norm > target)) break;
@@ -1604,7 +1636,7 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
     }
     for (int tile = g; tile < tiles; tile += G) {
         const int i = tile * kTile + threadIdx.x;
-        if (i < n) publish(&P.out[i], x[i]);
+        if (i < n) publish(&P.out[i], fetch<kF>(&x[i]));
     }
     SolveResult out{};
     out.iterations = iterations;
