@@ -698,7 +698,7 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
     const VT* dinv_m = V.dinv(j + 1);
     const VT* off_m = V.off(j + 1);
     const VT* rhs_m = static_cast<const VT*>(I.lvl_b[j + 1]);
-    constexpr bool kF = kW == 1;
+    constexpr bool kF = kW >= 1;
     double* mid = lds.tile[0];
     const int tiles = (n + kTile - 1) / kTile;
     for (int tile = g; tile < tiles; tile += G) {
@@ -777,7 +777,7 @@ __device__ inline int first_sweep_tiles(const Instance& I) { return I.depth == 1
 template <class VT, int kW>
 __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>& V, const VT* w0, VT* out0, int g, int G, Lds& lds,
                             int& timeouts, double* dot_part = nullptr, const VT* dot_with = nullptr) {
-    constexpr bool kF = kW == 1;
+    constexpr bool kF = kW >= 1;
     const int J = I.depth;
     if (J == 1) {
         for (int tile = g; tile < I.tiles; tile += G) {
@@ -1187,7 +1187,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                              const double* next_forcing, double* t_new, int g, int G, Lds& lds, int& timeouts,
                              int& parity, int& xi, int& rp, double*& prepared) {
     constexpr bool kPrep = kW == 1;
-    constexpr bool kF = kW == 1;   // lean barriers, coherent loads
+    constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
     const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
     const Values<VT> V{I};
     const LevelView& L0 = I.lv[0];
@@ -1533,7 +1533,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
 template <int kW>
 __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts,
                                            int& parity) {
-    constexpr bool kF = kW == 1;   // lean barriers, coherent loads
+    constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
     const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
     const Values<double> V{I};
     const LevelView& L0 = I.lv[0];
@@ -1751,7 +1751,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
             vcycle_rest<float, kWaves>(I, Values<float>{I}, static_cast<float*>(I.p[1]), pout, g, G, lds, timeouts);
             for (int tile = g; tile < I.tiles; tile += G) {
                 const int i = tile * kTile + threadIdx.x;
-                if (i < I.n) P.out[i] = static_cast<double>(pout[i]);
+                if (i < I.n) P.out[i] = static_cast<double>(fetch<true>(&pout[i]));   // after a lean barrier
             }
         }
     }
