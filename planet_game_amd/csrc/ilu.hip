@@ -118,6 +118,84 @@ __global__ __launch_bounds__(kIluThreads) void ilu_solve_kernel(IluDev F, double
     ilu_solve_block(F, static_cast<int>(blockIdx.x), x);
 }
 
+// Static LDS of the staged cooperative solve; blocks that need more use the
+// unstaged form.
+constexpr size_t kIluStagedBytes = 61440;
+
+// Block b's factors, level lists and solve vector staged in LDS for
+// the cooperative solve (the pattern holds in-block columns only, so every
+// index is local to the block and fits 16 bits; the host checks both).
+// Layout: value[E], x[R], then column[E], start[R + 1], diag[R], lrow[R],
+// urow[R] as int16.
+struct IluLds {
+    double* value;
+    double* x;
+    int16_t* column;
+    int16_t* start;
+    int16_t* diag;
+    int16_t* lrow;
+    int16_t* urow;
+};
+
+__device__ IluLds ilu_stage(const IluDev& F, int b, char* lds) {
+    const int tid = static_cast<int>(threadIdx.x);
+    const int first = F.block_start[b], rows = F.block_start[b + 1] - first;
+    const int e0 = F.start[first], entries = F.start[first + rows] - e0;
+    const int lbase = F.llev[F.lblk[b]], ubase = F.ulev[F.ublk[b]];
+    IluLds L;
+    L.value = reinterpret_cast<double*>(lds);
+    L.x = L.value + entries;
+    L.column = reinterpret_cast<int16_t*>(L.x + rows);
+    L.start = L.column + entries;
+    L.diag = L.start + rows + 1;
+    L.lrow = L.diag + rows;
+    L.urow = L.lrow + rows;
+    for (int q = tid; q < entries; q += kIluThreads) {
+        L.value[q] = F.value[e0 + q];
+        L.column[q] = static_cast<int16_t>(F.column[e0 + q] - first);
+    }
+    for (int k = tid; k <= rows; k += kIluThreads) L.start[k] = static_cast<int16_t>(F.start[first + k] - e0);
+    for (int k = tid; k < rows; k += kIluThreads) {
+        L.diag[k] = static_cast<int16_t>(F.diag[first + k] - e0);
+        L.lrow[k] = static_cast<int16_t>(F.lrow[lbase + k] - first);
+        L.urow[k] = static_cast<int16_t>(F.urow[ubase + k] - first);
+    }
+    __syncthreads();
+    return L;
+}
+
+// ilu_solve_block on the staged block: the same rows per level, the same
+// operations in the same order, with every operand in LDS. x moves in and
+// out of LDS around the walk. Ends with a workgroup barrier.
+__device__ void ilu_solve_block_lds(const IluDev& F, const IluLds& L, int b, double* x) {
+    const int tid = static_cast<int>(threadIdx.x);
+    const int first = F.block_start[b], rows = F.block_start[b + 1] - first;
+    for (int k = tid; k < rows; k += kIluThreads) L.x[k] = x[first + k];
+    __syncthreads();
+    const int lbase = F.llev[F.lblk[b]];
+    for (int lv = F.lblk[b]; lv < F.lblk[b + 1]; ++lv) {
+        for (int k = F.llev[lv] - lbase + tid; k < F.llev[lv + 1] - lbase; k += kIluThreads) {
+            const int i = L.lrow[k];
+            double sum = L.x[i];
+            for (int q = L.start[i]; q < L.diag[i]; ++q) sum -= L.value[q] * L.x[L.column[q]];
+            L.x[i] = sum;
+        }
+        __syncthreads();
+    }
+    const int ubase = F.ulev[F.ublk[b]];
+    for (int lv = F.ublk[b]; lv < F.ublk[b + 1]; ++lv) {
+        for (int k = F.ulev[lv] - ubase + tid; k < F.ulev[lv + 1] - ubase; k += kIluThreads) {
+            const int i = L.urow[k];
+            double sum = L.x[i];
+            for (int q = L.diag[i] + 1; q < L.start[i + 1]; ++q) sum -= L.value[q] * L.x[L.column[q]];
+            L.x[i] = sum / L.value[L.diag[i]];
+        }
+        __syncthreads();
+    }
+    for (int k = tid; k < rows; k += kIluThreads) x[first + k] = L.x[k];
+    __syncthreads();
+}
+
 // Grid barrier of a cooperative launch. Each workgroup issues one
 // agent-scope release fence and a relaxed arrival; the last arrival adds an
 // acq_rel fence (ordering every arrival's writes before its own generation
@@ -218,9 +296,29 @@ __device__ bool reduce(const Krylov& S, int& parity, double (&v)[K], double* red
 
 // Right-preconditioned BiCGSTAB from x = 0 (heat_transport.cpp:820-897)
 // with block ILU(0): the reference's recurrences and stopping tests.
+template <bool kStaged> struct IluShared {
+    double red[8];
+};
+template <> struct IluShared<true> {
+    double red[8];
+    alignas(16) char staged[kIluStagedBytes];
+};
+
+// kStaged: the block's factors live in LDS for the whole launch
+// (ilu_stage), and each preconditioner application walks them there.
+template <bool kStaged>
 __global__ __launch_bounds__(kIluThreads) void ilu_bicgstab_kernel(IluDev F, CsrOp A, Krylov S) {
-    __shared__ double red[8];
+    __shared__ IluShared<kStaged> shared;
+    double* red = shared.red;
     const int b = static_cast<int>(blockIdx.x), blocks = static_cast<int>(gridDim.x);
+    IluLds staged{};
+    if constexpr (kStaged) staged = ilu_stage(F, b, shared.staged);
+    const auto precondition = [&](double* x) {
+        if constexpr (kStaged)
+            ilu_solve_block_lds(F, staged, b, x);
+        else
+            ilu_solve_block(F, b, x);
+    };
     const int first = F.block_start[b], last = F.block_start[b + 1];
     const int tid = static_cast<int>(threadIdx.x);
     int parity = 0;
@@ -252,7 +350,7 @@ __global__ __launch_bounds__(kIluThreads) void ilu_bicgstab_kernel(IluDev F, Csr
                 S.ph[i] = pi;
             }
             __syncthreads();
-            ilu_solve_block(F, b, S.ph);
+            precondition(S.ph);
             if (!(ok = grid_sync(S.words, blocks, S.limit))) break;
             d1[0] = 0.0;
             for (int i = first + tid; i < last; i += kIluThreads) {
@@ -276,7 +374,7 @@ __global__ __launch_bounds__(kIluThreads) void ilu_bicgstab_kernel(IluDev F, Csr
                 for (int i = first + tid; i < last; i += kIluThreads) S.x[i] += alpha * S.ph[i];
                 break;
             }
-            ilu_solve_block(F, b, S.sh);
+            precondition(S.sh);
             if (!(ok = grid_sync(S.words, blocks, S.limit))) break;
             double d2[2] = {0.0, 0.0};
             for (int i = first + tid; i < last; i += kIluThreads) {
@@ -314,6 +412,7 @@ __global__ __launch_bounds__(kIluThreads) void ilu_bicgstab_kernel(IluDev F, Csr
 struct psim_ilu {
     psim_ctx* ctx = nullptr;
     int32_t n = 0, blocks = 0, entries = 0, off_count = 0;
+    size_t staged_lds = 0;   // LDS the staged cooperative solve needs (0: indices exceed 16 bits)
     psim::Arena arena;
     int32_t *block_start = nullptr, *start = nullptr, *column = nullptr, *diag = nullptr, *source = nullptr;
     double* value = nullptr;
@@ -422,6 +521,17 @@ std::unique_ptr<psim_ilu> make_ilu(psim_ctx* ctx, size_t n, const uint64_t* offs
     f->n = static_cast<int32_t>(n);
     f->blocks = static_cast<int32_t>(block_start.size()) - 1;
     f->entries = static_cast<int32_t>(column.size());
+    for (size_t b = 0; b + 1 < block_start.size(); ++b) {
+        const size_t rows = static_cast<size_t>(block_start[b + 1] - block_start[b]);
+        const size_t ents = static_cast<size_t>(start[static_cast<size_t>(block_start[b + 1])] -
+                                                start[static_cast<size_t>(block_start[b])]);
+        if (rows >= 32767 || ents >= 32767) {
+            f->staged_lds = 0;
+            break;
+        }
+        const size_t bytes = 8 * (ents + rows) + 2 * (ents + 4 * rows + 1);
+        f->staged_lds = std::max(f->staged_lds, (bytes + 255) / 256 * 256);
+    }
     f->off_count = static_cast<int32_t>(offset[n]);
     psim::Arena& A = f->arena;
     A.plan(&f->block_start, block_start.size());
@@ -614,8 +724,11 @@ int ilu_bicgstab(psim_ctx* ctx, Stacked& s, double tolerance, int max_iterations
     hipStream_t stream = ctx->stream;
     psim::hip_check(hipMemsetAsync(s.out, 0, 8, stream), "tracer result");
     void* args[] = {&F, &A, &K};
-    psim::hip_check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&psim::dev::ilu_bicgstab_kernel),
-                                               dim3(f.blocks), dim3(psim::dev::kIluThreads), args, 0, stream),
+    // the staged form when the largest block's factors fit its LDS area
+    const bool staged = f.staged_lds > 0 && f.staged_lds <= psim::dev::kIluStagedBytes;
+    const void* kernel = staged ? reinterpret_cast<const void*>(&psim::dev::ilu_bicgstab_kernel<true>)
+                                : reinterpret_cast<const void*>(&psim::dev::ilu_bicgstab_kernel<false>);
+    psim::hip_check(hipLaunchCooperativeKernel(kernel, dim3(f.blocks), dim3(psim::dev::kIluThreads), args, 0, stream),
                     "tracer solve launch");
     int32_t out[2] = {0, 0};
     psim::hip_check(hipMemcpyAsync(out, s.out, 8, hipMemcpyDeviceToHost, stream), "tracer result");
