@@ -560,7 +560,7 @@ def transport_rows_section(P, ctx, meshes, with_oracle, repeats=3):
     if with_oracle:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import support as S
-    out = {"config": {"workload": "F3/F4 consumers on the L6 mesh (40,962 cells) and its agglomerated groups "
+    out = {"config": {"workload": "consumers of the path (tracer step, block ILU, advection-diffusion, operator solves) on the L6 mesh (40,962 cells) and its agglomerated groups "
                                   "(10,242); host arrays, timings per call", "repeats": repeats}}
     m6 = meshes[6]
     g6 = P.Graph(ctx, m6)
@@ -666,13 +666,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--workgroups", type=int, default=0, help="override the solver's workgroup count")
     ap.add_argument("--only-consumers", action="store_true",
-                    help="run only the F3/F4 consumer measurements (tracer, ILU, advection, operator solves)")
+                    help="run only the consumer measurements (tracer, ILU, advection, operator solves)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed-precision, batched and served measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (instances shard "
-              "across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+              "across GPUs by instance, without communication)", file=sys.stderr)
         sys.exit(2)
 
     import planet_game_amd as P
