@@ -304,8 +304,9 @@ static_assert(96 + 64 * kLeaves == kBarrierWordCount, "barrier layout");
 // whose values for other workgroups are all written with coherent stores and
 // read with coherent loads): each thread waits for its own stores instead of
 // a release fence per workgroup, and no acquire fence follows the wait, so
-// this XCD's L2 keeps the read-only operator data across phases. The last
-// arrivals of a leaf and of the root keep their acq_rel fences.
+// this XCD's L2 keeps the read-only operator data across phases. With no
+// plain store to publish, the last arrivals of a leaf and of the root need
+// no fence either; the root's last arrival only waits for its clears.
 template <bool kLean = false>
 __device__ __forceinline__ void group_sync(int& timeouts) {
     if constexpr (kLean) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -328,7 +329,7 @@ __device__ __forceinline__ void group_sync(int& timeouts) {
         uint32_t* mine = grp.barrier + 96 + 64 * leaf + 32 * p;
         bool last = false;
         if (__hip_atomic_fetch_add(mine, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == leaf_size) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            if constexpr (!kLean) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
             if (__hip_atomic_fetch_add(root, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 ==
                 static_cast<uint32_t>(leaves)) {
                 // clear the whole idle set, every leaf included: a leaf this
@@ -337,7 +338,12 @@ __device__ __forceinline__ void group_sync(int& timeouts) {
                 for (int l = 0; l < kLeaves; ++l)
                     __hip_atomic_store(grp.barrier + 96 + 64 * l + 32 * (p ^ 1u), 0u, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+                // lean: the clears complete before the generation moves (an
+                // arrival of the next episode must not meet a late clear)
+                if constexpr (kLean)
+                    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                else
+                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
                 __hip_atomic_store(gen, seen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 last = true;
             }
