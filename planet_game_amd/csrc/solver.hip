@@ -53,9 +53,9 @@ template <class T> __device__ inline T* uniform_ptr(T* p) {
 
 // Values that a later phase reads from other workgroups are written with
 // agent-scope (coherent) stores: they go through to memory instead of
-// staying dirty in this XCD's L2, so the release fence of the next group
-// barrier has (almost) nothing left to write back. The barrier's fences
-// still order them; only where the bytes wait changes.
+// staying dirty in this XCD's L2. A fenced group barrier then has (almost)
+// nothing left to write back; a lean one (group_sync<true>) only waits for
+// each thread's stores, and the readers load them with fetch<true>.
 template <class T> __device__ __forceinline__ void publish(T* p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
