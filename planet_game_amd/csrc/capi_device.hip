@@ -469,7 +469,9 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
                 A.plan(reinterpret_cast<char**>(&I.lvl_b[j]), nj * vt);
                 A.plan(reinterpret_cast<char**>(&I.lvl_e[j]), nj * vt);
             }
+            if (j + 1 < g->depth) A.plan(reinterpret_cast<char**>(&I.lvl_x[j]), nj * vt);
         }
+        I.pre2 = (1 << std::max(g->depth - 1, 0)) - 1;   // two pre-sweeps on every smoothed level
         const size_t nc = static_cast<size_t>(g->views[g->depth - 1].n);
         A.plan(&I.omega, kMaxLevels);
         A.plan(&I.cinv, nc * nc);

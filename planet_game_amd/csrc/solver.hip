@@ -485,13 +485,16 @@ __device__ inline int tile_row(int tl, int n, bool& ok) {
 
 // ------------------------------------------------------------ V-cycle phases
 
-// Pre-smoothing from zero, residual and restriction of level j in one sweep
-// over the level's restriction tiles. On level 0 the input comes from a
-// Source; each row stores its value in `keep` (if set) and adds its square
-// to the tile partial (if `part` is set).
+// Pre-smoothing, residual and restriction of level j in one sweep over the
+// level's restriction tiles. The pre-smoothed iterate is one Jacobi sweep
+// from zero (x = om D^-1 w, evaluated where it is read), or, with `pre`
+// set, the two-sweep iterate jacobi_phase stored there. On level 0 the input
+// comes from a Source; each row stores its value in `keep` and `keep2` (if
+// set) and adds its square to the tile partial (if `part` is set).
 template <class VT, bool kF = false>
 __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& V, int j, const Source<VT>& src, VT* keep,
-                           VT* coarse_rhs, double* part, int g, int G, Lds& lds) {
+                           VT* coarse_rhs, double* part, int g, int G, Lds& lds, const VT* pre = nullptr,
+                           VT* keep2 = nullptr) {
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
     const VT om = static_cast<VT>(I.omega[j]);
@@ -526,16 +529,18 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             sq[r][0] = ok[r] ? static_cast<double>(wi[r]) * static_cast<double>(wi[r]) : 0.0;
-            const VT xi = om * dinv[i[r]] * wi[r];
+            const VT xi = pre != nullptr ? fetch<kF>(pre + i[r]) : om * dinv[i[r]] * wi[r];
             acc[r] = wi[r] - diag[i[r]] * xi;
         }
         ell_rows<R>(
-            L.col, off, n, W, i, [&](int, int, int c) { return om * dinv[c] * src.template at<kF>(c); },
+            L.col, off, n, W, i,
+            [&](int, int, int c) { return pre != nullptr ? fetch<kF>(pre + c) : om * dinv[c] * src.template at<kF>(c); },
             [&](int r, VT o, VT v) { acc[r] -= o * v; });
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (ok[r]) {
                 if (keep != nullptr) publish(&keep[i[r]], wi[r]);
+                if (keep2 != nullptr) publish(&keep2[i[r]], wi[r]);
                 lds.tile[r][threadIdx.x] = static_cast<double>(acc[r]);
             }
         if (part != nullptr) store_partials_n<1, R>(part, tl, sq, lds);
@@ -548,6 +553,51 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
                 publish(&coarse_rhs[c0[r] + threadIdx.x], static_cast<VT>(sum));
             }
         __syncthreads();
+    });
+}
+
+// First of two pre-smoothing sweeps on level j, over natural 256-row tiles:
+// with w the level's input, x1 = om D^-1 w (evaluated where it is read),
+// r1 = w - A x1 and x2 = x1 + om D^-1 r1 into `pre`, which down_phase and
+// the prolongation then read as the pre-smoothed iterate. Each row stores
+// w in `keep` and `keep2` (if set) and adds w^2 to its tile partial (if
+// `part` is set).
+template <class VT, bool kF = false>
+__device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>& V, int j, const Source<VT>& src,
+                                             VT* keep, VT* keep2, VT* pre, double* part, int g, int G, Lds& lds) {
+    const LevelView& L = I.lv[j];
+    const int n = L.n, W = L.width;
+    const VT om = static_cast<VT>(I.omega[j]);
+    const VT* diag = V.diag(j);
+    const VT* dinv = V.dinv(j);
+    const VT* off = V.off(j);
+    const int tiles = (n + kTile - 1) / kTile;
+    tile_loop<sizeof(VT) == 4>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+        int i[R];
+        bool ok[R];
+        VT wi[R], acc[R];
+        double sq[R][1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) i[r] = tile_row(tl[r], n, ok[r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) wi[r] = src.template at<kF>(i[r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            sq[r][0] = ok[r] ? static_cast<double>(wi[r]) * static_cast<double>(wi[r]) : 0.0;
+            acc[r] = wi[r] - diag[i[r]] * (om * dinv[i[r]] * wi[r]);
+        }
+        ell_rows<R>(
+            L.col, off, n, W, i, [&](int, int, int c) { return om * dinv[c] * src.template at<kF>(c); },
+            [&](int r, VT o, VT v) { acc[r] -= o * v; });
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (ok[r]) {
+                const VT x1 = om * dinv[i[r]] * wi[r];
+                publish(&pre[i[r]], static_cast<VT>(x1 + om * dinv[i[r]] * acc[r]));
+                if (keep != nullptr) publish(&keep[i[r]], wi[r]);
+                if (keep2 != nullptr) publish(&keep2[i[r]], wi[r]);
+            }
+        if (part != nullptr) store_partials_n<1, R>(part, tl, sq, lds);
     });
 }
 
@@ -599,10 +649,12 @@ __device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs
 
 // Prolongation of the coarse correction fused with post-smoothing on level
 // j. The correction comes from `corr`, or, on the level above the coarsest,
-// from the coarse solve each workgroup recomputes into LDS.
+// from the coarse solve each workgroup recomputes into LDS. The pre-smoothed
+// iterate is om D^-1 rhs, or the two-sweep iterate in `pre` when set.
 template <class VT, bool kF = false>
 __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V, int j, const VT* rhs, const VT* corr, VT* out, int g,
-                         int G, Lds& lds, double* dot_part = nullptr, const VT* dot_with = nullptr) {
+                         int G, Lds& lds, double* dot_part = nullptr, const VT* dot_with = nullptr,
+                         const VT* pre = nullptr) {
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
     const VT om = static_cast<VT>(I.omega[j]);
@@ -621,7 +673,8 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
                                     L.need_stride, lds);
             if (i < n) {
                 const VT rhs_i = fetch<kF>(&rhs[i]);
-                const VT x2 = om * dinv[i] * rhs_i + static_cast<VT>(lds.cvec[L.agg[i]]);
+                const VT x2 = (pre != nullptr ? fetch<kF>(pre + i) : om * dinv[i] * rhs_i) +
+                              static_cast<VT>(lds.cvec[L.agg[i]]);
                 VT acc = rhs_i - diag[i] * x2;
                 // the neighbour's aggregate comes from the slot's own map, so the
                 // correction read does not wait for the column
@@ -629,7 +682,8 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
                     L.col, off, n, W, i,
                     [&](int s, int c) {
                         const int a = L.agg_col[s * n + i];
-                        return om * dinv[c] * fetch<kF>(&rhs[c]) + static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
+                        return (pre != nullptr ? fetch<kF>(pre + c) : om * dinv[c] * fetch<kF>(&rhs[c])) +
+                               static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
                     },
                     [&](VT o, VT v) { acc -= o * v; });
                 const VT value = x2 + om * dinv[i] * acc;
@@ -649,14 +703,15 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const VT rhs_i = fetch<kF>(&rhs[i[r]]);
-            x2[r] = om * dinv[i[r]] * rhs_i + fetch<kF>(&corr[L.agg[i[r]]]);
+            x2[r] = (pre != nullptr ? fetch<kF>(pre + i[r]) : om * dinv[i[r]] * rhs_i) + fetch<kF>(&corr[L.agg[i[r]]]);
             acc[r] = rhs_i - diag[i[r]] * x2[r];
         }
         ell_rows<R>(
             L.col, off, n, W, i,
             [&](int r, int s, int c) {
                 const int a = L.agg_col[s * n + i[r]];
-                return om * dinv[c] * fetch<kF>(&rhs[c]) + fetch<kF>(&corr[a >= 0 ? a : 0]);
+                return (pre != nullptr ? fetch<kF>(pre + c) : om * dinv[c] * fetch<kF>(&rhs[c])) +
+                       fetch<kF>(&corr[a >= 0 ? a : 0]);
             },
             [&](int r, VT o, VT v) { acc[r] -= o * v; });
         double q[R][1];
@@ -685,14 +740,18 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
 // registers into scratch.
 template <class VT, int kW>
 __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, const VT* rhs, VT* out, int g, int G,
-                                             Lds& lds, double* dot_part, const VT* dot_with) {
+                                             Lds& lds, double* dot_part, const VT* dot_with, int mask) {
     const Instance& I = *uniform_ptr(instance);
     const Values<VT> V{I};
     j = __builtin_amdgcn_readfirstlane(j);
     g = __builtin_amdgcn_readfirstlane(g);
     G = __builtin_amdgcn_readfirstlane(G);
+    mask = __builtin_amdgcn_readfirstlane(mask);
     rhs = uniform_ptr(rhs);
     out = uniform_ptr(out);
+    // the two-sweep pre-smoothed iterates of level j and j + 1, where used
+    const VT* pre = (mask >> j) & 1 ? uniform_ptr(static_cast<const VT*>(I.lvl_x[j])) : nullptr;
+    const VT* pre_m = (mask >> (j + 1)) & 1 ? uniform_ptr(static_cast<const VT*>(I.lvl_x[j + 1])) : nullptr;
     const LevelView& L = I.lv[j];
     const LevelView& M = I.lv[j + 1];
     const int n = L.n, W = L.width, nm = M.n, Wm = M.width;
@@ -717,13 +776,15 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
         {
             if (m >= 0) {
                 const VT rhs_mm = fetch<kF>(&rhs_m[m]);
-                const VT x2 = omm * dinv_m[m] * rhs_mm + static_cast<VT>(lds.cvec[M.agg[m]]);
+                const VT x2 = (pre_m != nullptr ? fetch<kF>(pre_m + m) : omm * dinv_m[m] * rhs_mm) +
+                              static_cast<VT>(lds.cvec[M.agg[m]]);
                 VT acc = rhs_mm - diag_m[m] * x2;
                 ell_row(
                     M.col, off_m, nm, Wm, m,
                     [&](int s, int c) {
                         const int a = M.agg_col[s * nm + m];
-                        return omm * dinv_m[c] * fetch<kF>(&rhs_m[c]) + static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
+                        return (pre_m != nullptr ? fetch<kF>(pre_m + c) : omm * dinv_m[c] * fetch<kF>(&rhs_m[c])) +
+                               static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
                     },
                     [&](VT o, VT v) { acc -= o * v; });
                 mid[mq] = static_cast<double>(static_cast<VT>(x2 + omm * dinv_m[m] * acc));
@@ -734,13 +795,14 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
         double q[1] = {0.0};
         if (i < n) {
             const VT rhs_i = fetch<kF>(&rhs[i]);
-            const VT x2 = om * dinv[i] * rhs_i + static_cast<VT>(mid[L.agg_loc[i]]);
+            const VT x2 = (pre != nullptr ? fetch<kF>(pre + i) : om * dinv[i] * rhs_i) + static_cast<VT>(mid[L.agg_loc[i]]);
             VT acc = rhs_i - diag[i] * x2;
             ell_row(
                 L.col, off, n, W, i,
                 [&](int s, int c) {
                     const int a = L.agg_col_loc[s * n + i];
-                    return om * dinv[c] * fetch<kF>(&rhs[c]) + static_cast<VT>(mid[a >= 0 ? a : 0]);
+                    return (pre != nullptr ? fetch<kF>(pre + c) : om * dinv[c] * fetch<kF>(&rhs[c])) +
+                           static_cast<VT>(mid[a >= 0 ? a : 0]);
                 },
                 [&](VT o, VT v) { acc -= o * v; });
             const VT value = x2 + om * dinv[i] * acc;
@@ -752,37 +814,47 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
     }
 }
 
-// First sweep of a V-cycle: level 0's down phase (input evaluated from
-// `src`, stored into `keep`, squares into `part`); on a one-level hierarchy
-// the input is only materialised. Ends with a barrier.
+// First sweep of a V-cycle: level 0's down phase, or, when level 0 takes
+// two pre-sweeps (bit 0 of `mask`), its first Jacobi sweep (input
+// evaluated from `src`, stored into `keep` and `keep2`, squares into
+// `part`); on a one-level hierarchy the input is only materialised. Ends
+// with a barrier.
 template <class VT, bool kF = false>
 __device__ __forceinline__ void vcycle_first(const Instance& I, const Values<VT>& V, const Source<VT>& src, VT* keep, double* part,
-                             int g, int G, Lds& lds, int& timeouts) {
+                             int g, int G, Lds& lds, int& timeouts, int mask = 0, VT* keep2 = nullptr) {
     if (I.depth == 1) {
         for (int tile = g; tile < I.tiles; tile += G) {
             const int i = tile * kTile + threadIdx.x;
             double sq[1] = {0.0};
             if (i < I.n) {
                 const VT wi = src.template at<kF>(i);
-                publish(&keep[i], wi);
+                if (keep != nullptr) publish(&keep[i], wi);
+                if (keep2 != nullptr) publish(&keep2[i], wi);
                 sq[0] = static_cast<double>(wi) * static_cast<double>(wi);
             }
             if (part != nullptr) store_partials<1>(part, tile, sq, lds);
         }
+    } else if (mask & 1) {
+        jacobi_phase<VT, kF>(I, V, 0, src, keep, keep2, static_cast<VT*>(I.lvl_x[0]), part, g, G, lds);
     } else {
-        down_phase<VT, kF>(I, V, 0, src, keep, static_cast<VT*>(I.lvl_b[1]), part, g, G, lds);
+        down_phase<VT, kF>(I, V, 0, src, keep, static_cast<VT*>(I.lvl_b[1]), part, g, G, lds, nullptr, keep2);
     }
     group_sync<kF>(timeouts);
 }
 
 // Tiles the first sweep's partials cover.
-__device__ inline int first_sweep_tiles(const Instance& I) { return I.depth == 1 ? I.tiles : I.lv[0].down_tiles; }
+__device__ inline int first_sweep_tiles(const Instance& I, int mask) {
+    return I.depth == 1 || (mask & 1) ? I.tiles : I.lv[0].down_tiles;
+}
 
 // The rest of the V-cycle after vcycle_first; w0 is the materialised level-0
-// input, out0 the level-0 output. Ends with a barrier.
+// input, out0 the level-0 output. Levels whose bit is set in `mask` take a
+// second pre-smoothing sweep: a Jacobi phase (level 0's ran in
+// vcycle_first), then the restriction sweep from its iterate. Ends with a
+// barrier.
 template <class VT, int kW>
 __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>& V, const VT* w0, VT* out0, int g, int G, Lds& lds,
-                            int& timeouts, double* dot_part = nullptr, const VT* dot_with = nullptr) {
+                            int& timeouts, double* dot_part = nullptr, const VT* dot_with = nullptr, int mask = 0) {
     constexpr bool kF = kW >= 1;
     const int J = I.depth;
     if (J == 1) {
@@ -802,10 +874,21 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
         return;
     }
     const Source<VT> none{nullptr, nullptr, nullptr, 0.0, 0.0, 0};
+    const auto pre_of = [&](int j) { return (mask >> j) & 1 ? static_cast<VT*>(I.lvl_x[j]) : nullptr; };
+    if (mask & 1) {
+        Source<VT> src = none;
+        src.a = w0;
+        down_phase<VT, kF>(I, V, 0, src, nullptr, static_cast<VT*>(I.lvl_b[1]), nullptr, g, G, lds, pre_of(0));
+        group_sync<kF>(timeouts);
+    }
     for (int j = 1; j + 1 < J; ++j) {
         Source<VT> src = none;
         src.a = static_cast<const VT*>(I.lvl_b[j]);
-        down_phase<VT, kF>(I, V, j, src, nullptr, static_cast<VT*>(I.lvl_b[j + 1]), nullptr, g, G, lds);
+        if ((mask >> j) & 1) {
+            jacobi_phase<VT, kF>(I, V, j, src, nullptr, nullptr, pre_of(j), nullptr, g, G, lds);
+            group_sync<kF>(timeouts);
+        }
+        down_phase<VT, kF>(I, V, j, src, nullptr, static_cast<VT*>(I.lvl_b[j + 1]), nullptr, g, G, lds, pre_of(j));
         group_sync<kF>(timeouts);
     }
     // with about a workgroup per tile, the last two prolongations are one
@@ -818,10 +901,11 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
         VT* out = j == 0 ? out0 : static_cast<VT*>(I.lvl_e[j]);
         if (merge && j == jm + 1) continue;
         if (merge && j == jm)
-            merged_up_phase<VT, kW>(&I, j, rhs, out, g, G, lds, j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr);
+            merged_up_phase<VT, kW>(&I, j, rhs, out, g, G, lds, j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr,
+                                    mask);
         else
             up_phase<VT, kF>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds,
-                             j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr);
+                             j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr, pre_of(j));
         group_sync<kF>(timeouts);
     }
 }
@@ -1289,6 +1373,9 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     double r_norm = b_norm;
     int iterations = 0, restarts = 0, v_cycles = 0;
     int status = 0;
+    const int mask = I.pre2;      // levels with two pre-smoothing sweeps
+    bool shadow_stale = false;    // r0 must be set from r at the next pass's start
+    bool x_zero = true;           // x is known to be 0 (no x update since the start)
     const double target = P.tolerance * b_norm;
     bool first = true;            // first pass: r = b exactly
     bool have_residual = false;   // the true residual of x is in r, r_norm, rho1
@@ -1347,15 +1434,17 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 if (rho1 == 0.0 || omega == 0.0) break;
                 const double beta = rho1 / rho * (alpha / omega);
                 rho = rho1;
-                // V-cycle on p = r + beta (p - omega v); p = r on a fresh start
+                // V-cycle on p = r + beta (p - omega v); p = r on a fresh start,
+                // whose sweep also refreshes the shadow residual after a restart
                 Source<VT> src{r, nullptr, nullptr, 0.0, 0.0, 0};
                 if (!first_iteration) src = Source<VT>{r, static_cast<const VT*>(I.p[pi]), v, beta, omega, 1};
                 VT* p_new = static_cast<VT*>(I.p[pi ^ 1]);
                 pi ^= 1;
-                vcycle_first<VT, kF>(I, V, src, p_new, nullptr, g, G, lds, timeouts);
-                vcycle_rest<VT, kW>(I, V, p_new, ph, g, G, lds, timeouts);
+                vcycle_first<VT, kF>(I, V, src, p_new, nullptr, g, G, lds, timeouts, mask,
+                                     first_iteration && shadow_stale ? r0 : nullptr);
+                shadow_stale = false;
+                vcycle_rest<VT, kW>(I, V, p_new, ph, g, G, lds, timeouts, nullptr, nullptr, mask);
                 ++v_cycles;
-                first_iteration = false;
                 // v = A ph, (r0, v)
                 part = next_partials();
                 tile_loop<sizeof(VT) == 4>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
@@ -1385,14 +1474,12 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 group_totals<1, kF>(part, tiles, r0v, lds);
                 if (r0v[0] == 0.0) break;
                 alpha = rho / r0v[0];
-                // s = r - alpha v inside the next V-cycle's first sweep, (s, s)
-                part = next_partials();
-                vcycle_first<VT, kF>(I, V, Source<VT>{r, v, nullptr, alpha, 0.0, 2}, s, part, g, G, lds, timeouts);
-                double ss[1];
-                group_totals<1, kF>(part, first_sweep_tiles(I), ss, lds);
-                ++iterations;
-                if (sqrt(ss[0]) <= inner_target) {
-                    // x' = x + alpha ph fused with the FP64 true residual of x'
+                bool alpha_in_x = false;   // x already holds x + alpha ph
+                if (first_iteration) {
+                    // r is the true residual of x here, so s = r - alpha v is
+                    // b - A (x + alpha ph) up to rounding: compute it as that
+                    // FP64 true residual with x' = x + alpha ph, in the phase
+                    // that a half-step exit would run anyway, and test it
                     part = next_partials();
                     const double* x = I.x[xi];
                     double* xn = I.x[xi ^ 1];
@@ -1400,33 +1487,90 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         const int i = tile * kTile + threadIdx.x;
                         double q[KQ] = {};
                         if (i < n) {
-                            const double x_new = fetch<kF>(&x[i]) + alpha * static_cast<double>(fetch<kF>(&ph[i]));
+                            const double x_new =
+                                (x_zero ? 0.0 : fetch<kF>(&x[i])) + alpha * static_cast<double>(fetch<kF>(&ph[i]));
                             publish(&xn[i], x_new);
                             if (kPrep && next_forcing != nullptr) prepare_row(i, x_new, q);
                             double ax = I.diag64[0][i] * x_new;
                             ell_row(
                                 L0.col, I.off64[0], n, W, i,
-                                [&](int, int c) { return fetch<kF>(&x[c]) + alpha * static_cast<double>(fetch<kF>(&ph[c])); },
-                                [&](double o, double v) { ax += o * v; });
-                            const double ri = fetch<kF>(&b[i]) - ax;
-                            const VT rv = static_cast<VT>(ri);
-                            publish(&r[i], rv);
-                            publish(&r0[i], rv);
-                            q[0] = ri * ri;
-                            q[1] = static_cast<double>(rv) * static_cast<double>(rv);
+                                [&](int, int c) {
+                                    return (x_zero ? 0.0 : fetch<kF>(&x[c])) + alpha * static_cast<double>(fetch<kF>(&ph[c]));
+                                },
+                                [&](double o, double w) { ax += o * w; });
+                            const double si = fetch<kF>(&b[i]) - ax;
+                            const VT sv = static_cast<VT>(si);
+                            publish(&s[i], sv);
+                            publish(&r[i], sv);
+                            q[0] = si * si;
+                            q[1] = static_cast<double>(sv) * static_cast<double>(sv);
                         }
                         store_partials<KQ>(part, tile, q, lds);
                     }
                     if (kPrep) prep = next_forcing != nullptr ? part : nullptr;
                     group_sync<kF>(timeouts);
                     xi ^= 1;
+                    x_zero = false;
                     group_totals<2, kF>(part, tiles, tot2, lds);
-                    r_norm = sqrt(tot2[0]);
-                    rho1 = tot2[1];
-                    have_residual = true;
-                    break;
+                    ++iterations;
+                    if (sqrt(tot2[0]) <= inner_target) {
+                        // r holds the true residual; a restart takes it as its
+                        // shadow residual too
+                        r_norm = sqrt(tot2[0]);
+                        rho1 = tot2[1];
+                        have_residual = true;
+                        shadow_stale = true;
+                        break;
+                    }
+                    alpha_in_x = true;
+                    vcycle_first<VT, kF>(I, V, Source<VT>{s, nullptr, nullptr, 0.0, 0.0, 0}, nullptr, nullptr, g, G, lds,
+                                         timeouts, mask);
+                } else {
+                    // s = r - alpha v inside the next V-cycle's first sweep, (s, s)
+                    part = next_partials();
+                    vcycle_first<VT, kF>(I, V, Source<VT>{r, v, nullptr, alpha, 0.0, 2}, s, part, g, G, lds, timeouts,
+                                         mask);
+                    double ss[1];
+                    group_totals<1, kF>(part, first_sweep_tiles(I, mask), ss, lds);
+                    ++iterations;
+                    if (sqrt(ss[0]) <= inner_target) {
+                        // x' = x + alpha ph fused with the FP64 true residual of x'
+                        part = next_partials();
+                        const double* x = I.x[xi];
+                        double* xn = I.x[xi ^ 1];
+                        for (int tile = g; tile < tiles; tile += G) {
+                            const int i = tile * kTile + threadIdx.x;
+                            double q[KQ] = {};
+                            if (i < n) {
+                                const double x_new = fetch<kF>(&x[i]) + alpha * static_cast<double>(fetch<kF>(&ph[i]));
+                                publish(&xn[i], x_new);
+                                if (kPrep && next_forcing != nullptr) prepare_row(i, x_new, q);
+                                double ax = I.diag64[0][i] * x_new;
+                                ell_row(
+                                    L0.col, I.off64[0], n, W, i,
+                                    [&](int, int c) { return fetch<kF>(&x[c]) + alpha * static_cast<double>(fetch<kF>(&ph[c])); },
+                                    [&](double o, double w) { ax += o * w; });
+                                const double ri = fetch<kF>(&b[i]) - ax;
+                                const VT rv = static_cast<VT>(ri);
+                                publish(&r[i], rv);
+                                publish(&r0[i], rv);
+                                q[0] = ri * ri;
+                                q[1] = static_cast<double>(rv) * static_cast<double>(rv);
+                            }
+                            store_partials<KQ>(part, tile, q, lds);
+                        }
+                        if (kPrep) prep = next_forcing != nullptr ? part : nullptr;
+                        group_sync<kF>(timeouts);
+                        xi ^= 1;
+                        group_totals<2, kF>(part, tiles, tot2, lds);
+                        r_norm = sqrt(tot2[0]);
+                        rho1 = tot2[1];
+                        have_residual = true;
+                        break;
+                    }
                 }
-                vcycle_rest<VT, kW>(I, V, s, sh, g, G, lds, timeouts);
+                first_iteration = false;
+                vcycle_rest<VT, kW>(I, V, s, sh, g, G, lds, timeouts, nullptr, nullptr, mask);
                 ++v_cycles;
                 // t = A sh, (t, t), (t, s)
                 part = next_partials();
@@ -1456,7 +1600,8 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 double tt[2];
                 group_totals<2, kF>(part, tiles, tt, lds);
                 omega = tt[0] > 0.0 ? tt[1] / tt[0] : 0.0;
-                // x += alpha ph + omega sh, r = s - omega t, (r, r), (r0, r)
+                // x += alpha ph + omega sh (omega sh when x holds x + alpha ph
+                // already), r = s - omega t, (r, r), (r0, r)
                 part = next_partials();
                 prep = nullptr;
                 {
@@ -1470,7 +1615,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
 #pragma unroll
                         for (int k = 0; k < R; ++k) {
                             xv[k] = fetch<kF>(&x[i[k]]);
-                            pv[k] = static_cast<double>(fetch<kF>(&ph[i[k]]));
+                            pv[k] = alpha_in_x ? 0.0 : static_cast<double>(fetch<kF>(&ph[i[k]]));
                             shv[k] = static_cast<double>(fetch<kF>(&sh[i[k]]));
                             sv[k] = static_cast<double>(fetch<kF>(&s[i[k]]));
                             tv[k] = static_cast<double>(fetch<kF>(&t[i[k]]));
@@ -1480,7 +1625,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         for (int k = 0; k < R; ++k) {
                             q[k][0] = q[k][1] = 0.0;
                             if (ok[k]) {
-                                publish(&x[i[k]], xv[k] + (alpha * pv[k] + omega * shv[k]));
+                                publish(&x[i[k]], alpha_in_x ? xv[k] + omega * shv[k] : xv[k] + (alpha * pv[k] + omega * shv[k]));
                                 const double ri = sv[k] - omega * tv[k];
                                 const VT rv = static_cast<VT>(ri);
                                 publish(&r[i[k]], rv);
@@ -1706,6 +1851,22 @@ __device__ __forceinline__ void run_solve(const Instance& I, const Launch& P, in
     }
 }
 
+// The solve out of line, for the three-workgroups-per-CU program: inlined
+// there, the V-cycle variants of both precisions push the kernel's live
+// state into scratch; a call keeps the solve's registers its own.
+template <int kW>
+__device__ __noinline__ SyncState run_solve_call(const Instance* instance, const Launch& P, int g, int G, Lds& lds,
+                                                 SyncState st) {
+    const Instance& I = *uniform_ptr(instance);
+    g = __builtin_amdgcn_readfirstlane(g);
+    G = __builtin_amdgcn_readfirstlane(G);
+    if (I.precision == 0)
+        run_solve<double, kW>(I, P, g, G, lds, st.timeouts, st.parity);
+    else
+        run_solve<float, kW>(I, P, g, G, lds, st.timeouts, st.parity);
+    return st;
+}
+
 // Three workgroups per CU (12 waves): the register budget that keeps the
 // solver's live state out of scratch (a tighter cap spills it) while
 // leaving batches 768 co-resident workgroups.
@@ -1776,10 +1937,16 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
         }
     }
     if (P.op & kOpSolve) {
-        if (I.precision == 0)
-            run_solve<double, kWaves>(I, P, g, G, lds, timeouts, parity);
-        else
-            run_solve<float, kWaves>(I, P, g, G, lds, timeouts, parity);
+        if constexpr (kWaves == 1) {
+            if (I.precision == 0)
+                run_solve<double, kWaves>(I, P, g, G, lds, timeouts, parity);
+            else
+                run_solve<float, kWaves>(I, P, g, G, lds, timeouts, parity);
+        } else {
+            const SyncState st = run_solve_call<kWaves>(&I, P, g, G, lds, SyncState{timeouts, parity});
+            timeouts = st.timeouts;
+            parity = st.parity;
+        }
     }
     if (P.op & kOpPcg) {
         SyncState st{timeouts, parity};
