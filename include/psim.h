@@ -230,6 +230,17 @@ void psim_system_destroy(psim_system* system);
  * reference's worker_count plays this role, deterministic_executor.hpp:14). */
 int psim_system_set_workgroups(psim_system* system, int32_t workgroups);
 int32_t psim_system_workgroups(const psim_system* system);
+/* Pre-smoothing of the V-cycle inside BiCGSTAB solves: bit j of
+ * level_mask set = level j takes two weighted-Jacobi sweeps before its
+ * restriction (each sweep a barrier-separated phase), clear = one; -1 (the
+ * default) = automatic: every level whose Jacobi weight is damped below 1
+ * (Gershgorin ratio of D^-1 A above 0.6), plus level 0 in the FP64 form.
+ * precondition and the PCG always use the symmetric one-sweep V(1,1). No
+ * reference counterpart (the reference smooths with one symmetric
+ * Gauss-Seidel sweep, heat_transport.cpp:252-326; the second Jacobi sweep
+ * brings the BiCGSTAB iteration counts to the reference's). */
+int psim_system_set_pre_sweeps(psim_system* system, int32_t level_mask);
+int32_t psim_system_pre_sweeps(const psim_system* system);
 /* Phase trace (diagnostics): with capacity > 0, every later launch on the
  * system stamps the device wall clock (100 MHz) when it starts and after
  * each device-wide barrier; psim_system_trace_read returns the stamps of

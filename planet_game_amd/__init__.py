@@ -151,6 +151,8 @@ _sig("psim_system_create", C.c_int, _vp, C.c_int, C.POINTER(_vp))
 _sig("psim_system_destroy", None, _vp)
 _sig("psim_system_set_workgroups", C.c_int, _vp, _i32)
 _sig("psim_system_workgroups", _i32, _vp)
+_sig("psim_system_set_pre_sweeps", C.c_int, _vp, _i32)
+_sig("psim_system_pre_sweeps", _i32, _vp)
 _sig("psim_system_level_values", C.c_int, _vp, _u64, _vp, _vp)
 _sig("psim_system_trace", C.c_int, _vp, _i32)
 _sig("psim_system_trace_read", C.c_int, _vp, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS"), _i32,
@@ -614,6 +616,15 @@ class System:
     @workgroups.setter
     def workgroups(self, value):
         check(lib.psim_system_set_workgroups(self.handle, value))
+
+    @property
+    def pre_sweeps(self):
+        """Bit j set: level j pre-smooths with two Jacobi sweeps in solves."""
+        return lib.psim_system_pre_sweeps(self.handle)
+
+    @pre_sweeps.setter
+    def pre_sweeps(self, mask):
+        check(lib.psim_system_set_pre_sweeps(self.handle, mask))
 
     def trace(self, capacity):
         check(lib.psim_system_trace(self.handle, capacity))

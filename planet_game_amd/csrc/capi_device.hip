@@ -471,7 +471,6 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
             }
             if (j + 1 < g->depth) A.plan(reinterpret_cast<char**>(&I.lvl_x[j]), nj * vt);
         }
-        I.pre2 = (1 << std::max(g->depth - 1, 0)) - 1;   // two pre-sweeps on every smoothed level
         const size_t nc = static_cast<size_t>(g->views[g->depth - 1].n);
         A.plan(&I.omega, kMaxLevels);
         A.plan(&I.cinv, nc * nc);
@@ -529,6 +528,20 @@ int psim_system_set_workgroups(psim_system* s, int32_t workgroups) {
 }
 
 int32_t psim_system_workgroups(const psim_system* s) { return s ? s->workgroups : 0; }
+
+int psim_system_set_pre_sweeps(psim_system* s, int32_t level_mask) {
+    return guarded([&] {
+        if (s == nullptr) throw std::invalid_argument("null system");
+        const int32_t smoothed = (1 << std::max(s->graph->depth - 1, 0)) - 1;
+        if (level_mask < -1 || (level_mask >= 0 && (level_mask & ~smoothed) != 0))
+            throw std::invalid_argument("pre-sweep mask names a level that is not smoothed");
+        psim::hip_check(hipStreamSynchronize(s->graph->ctx->stream), "synchronize");
+        s->host.pre2 = level_mask;
+        psim::put(s->d_inst, &s->host, sizeof(Instance));
+    });
+}
+
+int32_t psim_system_pre_sweeps(const psim_system* s) { return s ? s->host.pre2 : 0; }
 
 int psim_system_trace(psim_system* s, int32_t capacity) {
     return guarded([&] {

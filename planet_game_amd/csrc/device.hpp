@@ -131,8 +131,9 @@ struct Instance {
     void* lvl_x[kMaxLevels] = {};   // pre-smoothed iterate after two Jacobi sweeps, levels < depth - 1
     // BiCGSTAB solves: bit j set = level j pre-smooths with two Jacobi
     // sweeps (each its own phase) instead of one; 0 = the symmetric V(1,1)
-    // that precondition and PCG always use
-    int32_t pre2 = 0;
+    // that precondition and PCG always use; -1 = chosen per solve from the
+    // levels' smoother weights
+    int32_t pre2 = -1;
     double* tri = nullptr;          // coarsest working matrix: block sweep, or packed when not in LDS
     int32_t tri_lds = 0;            // 1: set_matrix launches carry it in dynamic LDS
     double* partial = nullptr;      // 2 x partial_slots x kPartials (double-buffered)

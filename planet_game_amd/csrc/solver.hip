@@ -1263,6 +1263,19 @@ __device__ inline double residual_row(const Instance& I, const double* b, const 
     return fetch<kF>(&b[i]) - ax;
 }
 
+// Levels that pre-smooth with two Jacobi sweeps in a solve: the system's
+// mask, or, when it is negative (automatic), every level whose Jacobi
+// weight is damped below 1 (Gershgorin ratio above 0.6: the smoother then
+// reduces the error slowly), plus level 0 in the FP64 form, where one
+// V-cycle must carry a half-step to the tolerance on its own.
+__device__ inline int pre_sweep_mask(const Instance& I) {
+    if (I.pre2 >= 0) return I.pre2;
+    int mask = I.precision == 0 && I.depth > 1 ? 1 : 0;
+    for (int j = 0; j + 1 < I.depth; ++j)
+        if (I.omega[j] != 1.0) mask |= 1 << j;
+    return mask;
+}
+
 template <class VT, int kW>
 //
 // Chained steps in the full-register-file program (kW == 1): with
@@ -1373,7 +1386,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     double r_norm = b_norm;
     int iterations = 0, restarts = 0, v_cycles = 0;
     int status = 0;
-    const int mask = I.pre2;      // levels with two pre-smoothing sweeps
+    const int mask = pre_sweep_mask(I);   // levels with two pre-smoothing sweeps
     bool shadow_stale = false;    // r0 must be set from r at the next pass's start
     bool x_zero = true;           // x is known to be 0 (no x update since the start)
     const double target = P.tolerance * b_norm;
