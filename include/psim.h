@@ -313,6 +313,23 @@ int psim_heat_chain(psim_system* system, const double* conductance_per_entry, co
                     double tolerance, int32_t max_iterations, double* temperature_new, psim_solve_info* step_info,
                     int where);
 
+/* psim_heat_step on device pointers with no allocation in the call: the
+ * solver's workspace (every BiCGSTAB and V-cycle vector, the reduction
+ * partials, the group barrier words, the result record, the instance
+ * descriptor) lives in one static device array of 8 MiB per GPU, held until
+ * the process ends, instead of the system's own buffers; the matrix and
+ * hierarchy come from `system` (psim_heat_set_matrix first). Results are
+ * bit-identical to psim_heat_step. Systems whose workspace exceeds 8 MiB are
+ * rejected with PSIM_INVALID_ARGUMENT (FP64 systems up to 40,962 cells, L6,
+ * fit). The array keeps the last call's contents and is shared by every call
+ * on the GPU: calls that overlap on one GPU (other streams, other threads)
+ * race. info, if not NULL, receives the result (synchronises).
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops). */
+int psim_heat_step_noalloc(psim_system* system, const double* temperature_old, const double* forcing_W_m2,
+                           double dt_s, double tolerance, int32_t max_iterations, double* temperature_new,
+                           psim_solve_info* info);
+
 /* ---- The implicit transport (heat_transport.hpp:78-138,
  * solve_implicit_transport; heat_transport.cpp:595-772). Newton on
  * F(h) = h - K lap T(h) (+ latent) with the reference's backtracking line

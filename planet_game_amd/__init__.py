@@ -164,6 +164,7 @@ _sig("psim_system_solve", C.c_int, _vp, _vp, _vp, _dbl, _i32, C.POINTER(SolveInf
 _sig("psim_heat_set_matrix", C.c_int, _vp, _vp, _vp, _dbl, C.c_int)
 _sig("psim_heat_step", C.c_int, _vp, _vp, _vp, _dbl, _dbl, _i32, _vp, C.POINTER(SolveInfo), C.c_int)
 _sig("psim_heat_chain", C.c_int, _vp, _vp, _vp, _vp, _vp, _i32, _dbl, _dbl, _i32, _vp, _vp, C.c_int)
+_sig("psim_heat_step_noalloc", C.c_int, _vp, _vp, _vp, _dbl, _dbl, _i32, _vp, _vp)
 _pp = C.POINTER(_vp)
 
 
@@ -688,6 +689,14 @@ class System:
         """Queue one solve on device-resident vectors (no synchronisation)."""
         check(lib.psim_heat_step(self.handle, t_old, forcing, dt, tolerance, max_iterations, t_new,
                                  None, DEVICE))
+
+    def heat_step_noalloc(self, t_old, forcing, t_new, dt=2.63e6, tolerance=1e-10, max_iterations=1000):
+        """One solve on device-resident vectors with the solver's workspace in
+        the library's static device array (no allocation); returns SolveInfo."""
+        info = SolveInfo()
+        check(lib.psim_heat_step_noalloc(self.handle, t_old, forcing, dt, tolerance, max_iterations, t_new,
+                                         C.byref(info)))
+        return info
 
     def heat_chain(self, t_old, forcings, dt=2.63e6, tolerance=1e-10, max_iterations=1000,
                    conductance_per_entry=None, capacity_area=None):

@@ -158,6 +158,7 @@ int psim_ctx_create(int device, psim_ctx** out) {
         ctx->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
         ctx->lds_per_block =
             prop.sharedMemPerBlock > ctx->static_lds ? prop.sharedMemPerBlock - ctx->static_lds : 0;
+        psim::noalloc::init_device(device, ctx->stream);
         *out = ctx.release();
     });
 }

@@ -114,3 +114,9 @@ dev::Launch solve_params(const psim_system* s, int op, double tolerance, int32_t
 void require_matrix(const psim_system* s);
 
 }  // namespace psim::sys
+
+namespace psim::noalloc {
+// Device address and first clear of the allocation-free heat step's
+// workspace array on `device` (current), once per device.
+void init_device(int device, hipStream_t stream);
+}  // namespace psim::noalloc
