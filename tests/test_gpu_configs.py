@@ -1,0 +1,268 @@
+"""The measured configurations at their own sizes and tolerance, against the
+CPU oracle.
+
+- Config 1: the L6 120-step chain at tolerance 1e-10, each step's error
+  against the oracle solving that step (BiCGSTAB to 1e-14) from the same T_k;
+  the single-launch chain equal bit for bit to the step-by-step launches;
+  the trajectory against the oracle's own chain solved to 1e-14.
+- Config 3: the full 256-instance batch (64 each of L3-L6, own seeds), every
+  member against the FP64 oracle, mixed and plain.
+- Config 4: a seeded Poisson stream over L3-L6 (40:30:20:10), sessions of 1
+  and 12 steps sharing a small bank of matrices so that pooled systems are
+  reused across sessions; every result bit-equal to its stand-alone run, and
+  the sessions' last steps against the FP64 oracle.
+- The stiff variant (K = 0.5 R^2): BiCGSTAB iteration counts within two of
+  the reference's own (tests/golden/reference_fixtures.json, 9 / 12 / 19 at
+  L3 / L4 / L5 at 1e-10; 16 at L4 at 1e-13).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import planet_game_amd as P
+import support as S
+from test_gpu_parity import announce, bits_equal, csr_apply, heat_case, product_mesh
+
+pytestmark = pytest.mark.gpu
+
+DT = 2.63e6
+TOL = 1e-10
+FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_fixtures.json")
+_graphs = {}
+
+
+def graph(ctx, level):
+    if level not in _graphs:
+        _graphs[level] = P.Graph(ctx, product_mesh(level))
+    return _graphs[level]
+
+
+def oracle_multigrid(level, cond, ca):
+    om = S.mesh("oracle", level)
+    diag, off = S.heat_matrix(om, cond, ca)
+    mg = S.Multigrid(S.oracle(), om)
+    mg.set_matrix(diag, off)
+    return om, diag, off, mg
+
+
+# ---------------------------------------------------------------- stiff
+
+
+@pytest.mark.parametrize("level,tol,key", [(3, 1e-10, "iterations"), (4, 1e-10, "iterations"),
+                                           (5, 1e-10, "iterations"), (4, 1e-13, "iterations_tight")])
+def test_stiff_iterations_match_reference(ctx, level, tol, key):
+    """The reference's Gauss-Seidel V-cycle (heat_transport.cpp:252-326) and
+    this Jacobi one with two pre-sweeps on damped levels: BiCGSTAB needs
+    within two iterations of the reference's count on the fixture's inputs
+    (instance 0, sigma 0.3, forcing of step 0)."""
+    ref_iterations = json.load(open(FIXTURES))["solves"][str(level)]["stiff"][key]
+    announce("stiff iterations", level, tol, "reference", ref_iterations)
+    m, seed, cond, ca, land, t_old, forcing = heat_case(level, k_base=0.5 * 6371000.0 ** 2)
+    s = P.System(graph(ctx, level))
+    s.heat_set_matrix(m.entry_values(cond), ca)
+    x, info = s.heat_step(t_old, forcing, tolerance=tol)
+    print("gpu iterations", info.iterations, "v-cycles", info.v_cycles, "reference", ref_iterations, flush=True)
+    assert info.status == 0 and info.relative_residual <= tol
+    assert info.iterations <= ref_iterations + 2, (info.iterations, ref_iterations)
+    om, diag, off, _ = oracle_multigrid(level, cond, ca)
+    b = S.heat_rhs(om, ca, t_old, forcing)
+    true_res = np.linalg.norm(b - csr_apply(m, diag, off, x)) / np.linalg.norm(b)
+    assert true_res <= tol, true_res
+
+
+# ---------------------------------------------------------------- config 1
+
+
+def test_l6_chain_each_step_against_oracle(ctx):
+    """Config 1 at the metric's tolerance: 120 chained L6 steps. Each GPU
+    step T_{k+1} is compared with the oracle's solve of step k from the
+    GPU's own T_k (BiCGSTAB to 1e-14): the per-step error is bounded by
+    1e-12 relative L2, the north_star's parity bar. The single launch of the
+    whole chain (psim_heat_chain) equals the step-by-step launches bit for
+    bit, and the trajectory stays within 120 x 1e-12 of the oracle's own
+    chain solved to 1e-14."""
+    level, steps = 6, 120
+    announce("config 1 chain", level, steps)
+    m, seed, cond, ca, land, t0, _ = heat_case(level)
+    forcings = [P.heat_forcing(m, seed, k) for k in range(steps)]
+    s = P.System(graph(ctx, level))
+    d_cond = P.DeviceArray.of(ctx, m.entry_values(cond))
+    d_ca = P.DeviceArray.of(ctx, ca)
+    d_f = P.DeviceArray.of(ctx, np.concatenate(forcings))
+    d_t0 = P.DeviceArray.of(ctx, t0)
+    d_out = P.DeviceArray(ctx, m.cells)
+    infos = (P.SolveInfo * steps)()
+    s.heat_chain_device(d_t0.ptr, d_f.ptr, steps, d_out.ptr, DT, TOL, 1000, step_infos=infos,
+                        conductance_per_entry=d_cond.ptr, capacity_area=d_ca.ptr)
+    chain_final = d_out.download()
+    assert all(i.status == 0 and i.relative_residual <= TOL for i in infos)
+    # the same steps one launch each, keeping every T_k
+    s.heat_set_matrix_device(d_cond, d_ca, DT)
+    temps = [t0]
+    bufs = [P.DeviceArray(ctx, m.cells), P.DeviceArray(ctx, m.cells)]
+    src = d_t0
+    for k in range(steps):
+        dst = bufs[k % 2]
+        s.heat_step_device(src.ptr, d_f.offset(k * m.cells), dst.ptr, DT, TOL, 1000)
+        temps.append(dst.download())
+        src = dst
+    assert bits_equal(temps[-1], chain_final)
+    om, diag, off, mg = oracle_multigrid(level, cond, ca)
+    errors, residuals = [], []
+    for k in range(steps):
+        b = S.heat_rhs(om, ca, temps[k], forcings[k])
+        x_ref, _ = mg.bicgstab(diag, off, b, 1e-14, 100)
+        errors.append(np.linalg.norm(temps[k + 1] - x_ref) / np.linalg.norm(x_ref))
+        residuals.append(np.linalg.norm(b - csr_apply(m, diag, off, temps[k + 1])) / np.linalg.norm(b))
+    print(f"per-step error vs oracle: max {max(errors):.3e} median {np.median(errors):.3e}; "
+          f"max true residual {max(residuals):.3e}", flush=True)
+    assert max(residuals) <= TOL
+    assert max(errors) <= 1e-12, max(errors)
+    # the trajectory against the oracle's own chain solved to 1e-14: each
+    # step adds at most the per-step bound, so 120 x 1e-12 after 120 steps
+    t_cpu = t0
+    for k in range(steps):
+        t_cpu, _ = mg.bicgstab(diag, off, S.heat_rhs(om, ca, t_cpu, forcings[k]), 1e-14, 100)
+    drift = np.linalg.norm(chain_final - t_cpu) / np.linalg.norm(t_cpu)
+    print(f"trajectory vs the oracle's 1e-14 chain after {steps} steps: {drift:.3e}", flush=True)
+    assert drift <= steps * 1e-12, drift
+
+
+# ---------------------------------------------------------------- config 3
+
+
+@pytest.mark.parametrize("precision", [P.MIXED, P.PLAIN_FP64], ids=["mixed", "fp64"])
+def test_full_config3_batch_against_oracle(ctx, precision):
+    """Config 3 at its full size: 256 instances, 64 each of L3/L4/L5/L6 in the
+    bench's order and seeds, set_matrix + one solve in ONE launch. Mixed:
+    tolerance 1e-10, every member <= 1e-8 relative L2 from the FP64 oracle;
+    plain: tolerance 1e-13, every member <= 1e-12."""
+    tol, bound = (TOL, 1e-8) if precision == P.MIXED else (1e-13, 1e-12)
+    spec = [3, 4, 5, 6] * 64
+    announce("config 3 batch", len(spec), "precision", precision)
+    members = []
+    for i, level in enumerate(spec):
+        m = product_mesh(level)
+        seed = P.instance_seed(1 + i)
+        cond, ca, _, t0 = P.heat_instance(m, seed, 1.0e6, 0.3)
+        forcing = P.heat_forcing(m, seed, 0)
+        members.append(dict(level=level, cond=cond, ca=ca, t0=t0, forcing=forcing,
+                            system=P.System(graph(ctx, level), precision),
+                            d_cond=P.DeviceArray.of(ctx, m.entry_values(cond)), d_ca=P.DeviceArray.of(ctx, ca),
+                            d_t0=P.DeviceArray.of(ctx, t0), d_f=P.DeviceArray.of(ctx, forcing),
+                            d_out=P.DeviceArray(ctx, m.cells)))
+    batch = P.Batch(ctx, [x["system"] for x in members])
+    infos = batch.heat_step([x["d_t0"] for x in members], [x["d_f"] for x in members],
+                            [x["d_out"] for x in members], DT, tol, 1000,
+                            conductance=[x["d_cond"] for x in members], capacity=[x["d_ca"] for x in members])
+    worst = {}
+    for x, info in zip(members, infos):
+        assert info.status == 0 and info.relative_residual <= tol, (x["level"], info.status, info.relative_residual)
+        om, diag, off, mg = oracle_multigrid(x["level"], x["cond"], x["ca"])
+        x_ref, _ = mg.bicgstab(diag, off, S.heat_rhs(om, x["ca"], x["t0"], x["forcing"]), 1e-13, 200)
+        got = x["d_out"].download()
+        err = np.linalg.norm(got - x_ref) / np.linalg.norm(x_ref)
+        worst[x["level"]] = max(worst.get(x["level"], 0.0), err)
+        assert err <= bound, (x["level"], err)
+    print("worst error per level", {k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
+
+
+# ---------------------------------------------------------------- config 4
+
+
+class StreamSession:
+    def __init__(self, ctx, level, bank_entry, steps, seed):
+        m = product_mesh(level)
+        self.level, self.steps, self.bank = level, steps, bank_entry
+        self.forcing = [P.heat_forcing(m, seed, s) for s in range(steps)]
+        self.d_f = [P.DeviceArray.of(ctx, f) for f in self.forcing]
+        self.d_out = [P.DeviceArray(ctx, m.cells) for _ in range(steps)]
+
+
+def test_poisson_stream_equal_alone_and_oracle(ctx):
+    """A seeded Poisson stream (2,000 req/s offered, 360 requests) over
+    L3:L4:L5:L6 = 40:30:20:10, half the arrivals continuing a live 12-step
+    session, matrices drawn from a bank of 3 per level, mixed precision, a
+    pool small enough that systems are reused across sessions."""
+    rng = np.random.default_rng(20261204)
+    levels, shares = [3, 4, 5, 6], [0.4, 0.3, 0.2, 0.1]
+    bank = {}
+    for L in levels:
+        m = product_mesh(L)
+        bank[L] = []
+        for b in range(3):
+            seed = P.instance_seed(9000 + 10 * L + b)
+            cond, ca, _, t0 = P.heat_instance(m, seed, 1.0e6, 0.3)
+            bank[L].append(dict(cond=cond, ca=ca, t0=t0, d_cond=P.DeviceArray.of(ctx, m.entry_values(cond)),
+                                d_ca=P.DeviceArray.of(ctx, ca), d_t0=P.DeviceArray.of(ctx, t0)))
+    count = 360
+    announce("config 4 stream", count, "requests")
+    times = np.cumsum(rng.exponential(1.0 / 2000.0, count))
+    times -= times[0]
+    sessions, live, requests, order = [], [], [], []
+    for k in range(count):
+        if live and rng.random() < 0.5:
+            sid = live[int(rng.integers(len(live)))]
+        else:
+            L = levels[int(rng.choice(4, p=shares))]
+            steps = 12 if rng.random() < 0.25 else 1
+            sid = len(sessions)
+            sessions.append(StreamSession(ctx, L, bank[L][int(rng.integers(3))], steps, 31000 + sid))
+            sessions[sid].done = 0
+            if steps > 1:
+                live.append(sid)
+        s = sessions[sid]
+        step = s.done
+        r = P.ServeRequest()
+        r.arrival_s = float(times[k])
+        r.graph = levels.index(s.level)
+        r.session = sid
+        r.first = int(step == 0)
+        r.last = int(step == s.steps - 1)
+        r.conductance_per_entry = s.bank["d_cond"].ptr if step == 0 else None
+        r.capacity_area = s.bank["d_ca"].ptr if step == 0 else None
+        r.temperature_old = (s.bank["d_t0"] if step == 0 else s.d_out[step - 1]).ptr
+        r.forcing_W_m2 = s.d_f[step].ptr
+        r.temperature_new = s.d_out[step].ptr
+        requests.append(r)
+        order.append((sid, step))
+        s.done += 1
+        if s.done == s.steps and sid in live:
+            live.remove(sid)
+    # sessions still open at the end are dropped with their requests
+    open_ids = {sid for sid in live}
+    keep = [i for i, (sid, _) in enumerate(order) if sid not in open_ids]
+    requests = [requests[i] for i in keep]
+    order = [order[i] for i in keep]
+    server = P.Server(ctx, [graph(ctx, L) for L in levels], share=shares, precision=P.MIXED,
+                      memory_budget=0)
+    print("plan", server.plan(), flush=True)
+    records = server.run(requests, DT, TOL, 1000)
+    for (sid, step), rec in zip(order, records):
+        assert rec.status == 0 and rec.relative_residual <= TOL, (sid, step, rec.status)
+        assert rec.completed_s >= rec.submitted_s >= rec.arrival_s
+    done = sorted({sid for sid, _ in order})
+    for sid in done:
+        s = sessions[sid]
+        alone = P.System(graph(ctx, s.level), P.MIXED)
+        alone.heat_set_matrix_device(s.bank["d_cond"], s.bank["d_ca"], DT)
+        src = s.bank["d_t0"]
+        for step in range(s.steps):
+            dst = P.DeviceArray(ctx, product_mesh(s.level).cells)
+            alone.heat_step_device(src.ptr, s.d_f[step].ptr, dst.ptr, DT, TOL, 1000)
+            assert bits_equal(s.d_out[step].download(), dst.download()), (sid, s.level, step)
+            src = dst
+    # the last step of every served session against the FP64 oracle (from
+    # the served T of the step before)
+    worst = 0.0
+    for sid in done:
+        s = sessions[sid]
+        om, diag, off, mg = oracle_multigrid(s.level, s.bank["cond"], s.bank["ca"])
+        t_prev = s.bank["t0"] if s.steps == 1 else s.d_out[s.steps - 2].download()
+        x_ref, _ = mg.bicgstab(diag, off, S.heat_rhs(om, s.bank["ca"], t_prev, s.forcing[-1]), 1e-13, 200)
+        err = np.linalg.norm(s.d_out[-1].download() - x_ref) / np.linalg.norm(x_ref)
+        worst = max(worst, err)
+        assert err <= 1e-8, (sid, s.level, err)
+    print(f"{len(done)} sessions, {len(records)} requests; worst error vs FP64 oracle {worst:.2e}", flush=True)
