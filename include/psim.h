@@ -526,8 +526,9 @@ int psim_batch_autotune(psim_batch* batch, const double* const* conductance_per_
  * and keeps, per graph, a pool of systems sized by an HBM budget: the
  * graphs' hierarchies and every live session's matrix and solver state stay
  * resident. A session's first request assembles its matrix on a free
- * system of its graph (admission waits for one); its later steps reuse it;
- * its last request frees it. Requests that arrive while others run start as
+ * system of its graph (admission waits for one) unless a free system
+ * already holds the matrix its key names; its later steps reuse it; its
+ * last request frees it. Requests that arrive while others run start as
  * soon as a slot of their graph is free, and run exactly as they would
  * alone (the same result bits). Graphs must have a coarsest level of at
  * most 168 rows (all icosphere levels do). */
@@ -543,6 +544,12 @@ typedef struct {
     const double* temperature_old;         /* device (or device-visible pinned host) memory */
     const double* forcing_W_m2;
     double* temperature_new;
+    /* first request only: 0, or a key naming the matrix (the contents of
+     * conductance_per_entry and capacity_area) that the caller guarantees
+     * for every session it is given to. A session whose key equals the one
+     * a free pooled system of the graph last assembled with the same dt
+     * takes that system and skips the assembly (kept across runs). */
+    uint64_t matrix_key;
 } psim_serve_request;
 
 typedef struct {

@@ -171,7 +171,8 @@ _pp = C.POINTER(_vp)
 class ServeRequest(C.Structure):
     _fields_ = [("arrival_s", C.c_double), ("graph", C.c_int32), ("session", C.c_int32), ("first", C.c_int32),
                 ("last", C.c_int32), ("conductance_per_entry", _vp), ("capacity_area", _vp),
-                ("temperature_old", _vp), ("forcing_W_m2", _vp), ("temperature_new", _vp)]
+                ("temperature_old", _vp), ("forcing_W_m2", _vp), ("temperature_new", _vp),
+                ("matrix_key", C.c_uint64)]
 
 
 class ServeRecord(C.Structure):
@@ -500,6 +501,16 @@ class FV:
         out = np.empty_like(s)
         check(lib.psim_laplacian(self.handle, self._dtype(s), fields, _ptr(s), _ptr(out), HOST))
         return out
+
+    def gradient_device(self, dtype, fields, scalar, east, north):
+        """Queue the gradient of `fields` fields on device pointers (no synchronisation)."""
+        check(lib.psim_gradient(self.handle, dtype, fields, scalar, east, north, DEVICE))
+
+    def divergence_device(self, dtype, fields, flux, out):
+        check(lib.psim_divergence(self.handle, dtype, fields, flux, out, DEVICE))
+
+    def laplacian_device(self, dtype, fields, scalar, out):
+        check(lib.psim_laplacian(self.handle, dtype, fields, scalar, out, DEVICE))
 
     def orientation(self, cell, edge):
         out = _dbl()

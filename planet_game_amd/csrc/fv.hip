@@ -3,14 +3,19 @@
 // sim/planet/operators/finite_volume.cpp:34-111, and the diffusion source
 // of sim/planet/surface/heat_transport.cpp:28-36,579-593.
 //
-// One thread per cell, cells in mesh order (the space-filling order keeps
-// neighbour gathers inside a few KB of the cell's own row, so they hit L2),
-// per-entry geometry stored in CSR entry order so a wavefront's loads of
-// slot k are contiguous. Each cell accumulates in double, in the mesh's
-// cell-edge order, with the reference's operation order and no FMA
-// contraction (-ffp-contract=off), so results are bit-identical to the
-// reference for float and double fields. grid.y walks the fields of a
-// multi-field call; the geometry of a cell is re-read per field from L2.
+// Layout: the per-entry geometry is slot-major (ELL): slot s of cell i at
+// s * n + i, so a wavefront's loads of one slot are consecutive; slots past
+// a cell's degree hold neighbour -1 and are skipped. One thread per cell
+// and per group of kFieldGroup fields: the thread loads its cell's geometry
+// once, then runs the group's fields with all their loads in flight, each
+// field accumulated in double in the mesh's cell-edge order with the
+// reference's operation order and no FMA contraction (-ffp-contract=off),
+// so results are bit-identical to the reference for float and double.
+//
+// Workgroup order is XCD-aware: consecutive workgroups go to the eight XCDs
+// in turn, so the field groups of one 256-cell tile are given workgroup
+// indices 8 apart (the same XCD, dispatched close together) and re-read the
+// tile's geometry from that XCD's L2 instead of HBM.
 #include "device.hpp"
 #include "internal.hpp"
 
@@ -21,64 +26,154 @@
 namespace psim::dev {
 namespace {
 
-template <class T>
-__global__ __launch_bounds__(kTile) void gradient_kernel(int n, const int32_t* __restrict__ row,
-                                                         const int32_t* __restrict__ nb, const float* __restrict__ ne,
-                                                         const float* __restrict__ nn, const double* __restrict__ dist,
-                                                         const T* __restrict__ phi, T* __restrict__ east,
-                                                         T* __restrict__ north) {
-    const int i = blockIdx.x * kTile + threadIdx.x;
-    if (i >= n) return;
-    const size_t base = static_cast<size_t>(blockIdx.y) * n;
-    const T* f = phi + base;
-    const double own = static_cast<double>(f[i]);
-    double mee = 0.0, men = 0.0, mnn = 0.0, re = 0.0, rn = 0.0;
-    for (int k = row[i]; k < row[i + 1]; ++k) {
-        const double e = static_cast<double>(ne[k]);
-        const double q = static_cast<double>(nn[k]);
-        const double slope = (static_cast<double>(f[nb[k]]) - own) / dist[k];
-        mee += e * e;
-        men += e * q;
-        mnn += q * q;
-        re += slope * e;
-        rn += slope * q;
-    }
-    const double det = mee * mnn - men * men;
-    east[base + i] = static_cast<T>((mnn * re - men * rn) / det);
-    north[base + i] = static_cast<T>((mee * rn - men * re) / det);
+constexpr int kFieldGroup = 4;   // fields per thread
+constexpr int kXcds = 8;
+
+// Tile and field group of this workgroup (grid: ceil(tiles / 8) * 8 x groups
+// workgroups); false for the padding tiles past the last.
+__device__ inline bool tile_and_group(int tiles, int groups, int& tile, int& group) {
+    const int b = static_cast<int>(blockIdx.x);
+    const int x = b % kXcds, rest = b / kXcds;
+    group = rest % groups;
+    tile = rest / groups * kXcds + x;
+    return tile < tiles;
 }
 
-template <class T>
-__global__ __launch_bounds__(kTile) void divergence_kernel(int n, int edges, const int32_t* __restrict__ row,
+template <class T, int W>
+__global__ __launch_bounds__(kTile) void gradient_kernel(int n, int fields, const int32_t* __restrict__ nb,
+                                                         const float* __restrict__ ne, const float* __restrict__ nn,
+                                                         const double* __restrict__ dist, const T* __restrict__ phi,
+                                                         T* __restrict__ east, T* __restrict__ north) {
+    int tile, group;
+    const int groups = (fields + kFieldGroup - 1) / kFieldGroup;
+    if (!tile_and_group((n + kTile - 1) / kTile, groups, tile, group)) return;
+    const int i = tile * kTile + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    int c[W];
+    double e[W], q[W], d[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+        c[s] = nb[s * n + i];
+        e[s] = static_cast<double>(ne[s * n + i]);
+        q[s] = static_cast<double>(nn[s * n + i]);
+        d[s] = dist[s * n + i];
+    }
+    // M = sum n n^T in edge order, once for the group's fields
+    double mee = 0.0, men = 0.0, mnn = 0.0;
+#pragma unroll
+    for (int s = 0; s < W; ++s)
+        if (c[s] >= 0) {
+            mee += e[s] * e[s];
+            men += e[s] * q[s];
+            mnn += q[s] * q[s];
+        }
+    const double det = mee * mnn - men * men;
+    const int f0 = group * kFieldGroup;
+    T own[kFieldGroup], v[kFieldGroup][W];
+#pragma unroll
+    for (int g = 0; g < kFieldGroup; ++g) {
+        const T* f = phi + static_cast<size_t>(min(f0 + g, fields - 1)) * n;
+        own[g] = f[i];
+#pragma unroll
+        for (int s = 0; s < W; ++s) v[g][s] = f[c[s] >= 0 ? c[s] : i];
+    }
+#pragma unroll
+    for (int g = 0; g < kFieldGroup; ++g) {
+        if (f0 + g >= fields) break;
+        const double o = static_cast<double>(own[g]);
+        double re = 0.0, rn = 0.0;
+#pragma unroll
+        for (int s = 0; s < W; ++s)
+            if (c[s] >= 0) {
+                const double slope = (static_cast<double>(v[g][s]) - o) / d[s];
+                re += slope * e[s];
+                rn += slope * q[s];
+            }
+        const size_t at = static_cast<size_t>(f0 + g) * n + i;
+        east[at] = static_cast<T>((mnn * re - men * rn) / det);
+        north[at] = static_cast<T>((mee * rn - men * re) / det);
+    }
+}
+
+template <class T, int W>
+__global__ __launch_bounds__(kTile) void divergence_kernel(int n, int edges, int fields,
                                                            const int32_t* __restrict__ edge,
                                                            const double* __restrict__ signed_length,
                                                            const double* __restrict__ area,
                                                            const T* __restrict__ flux, T* __restrict__ out) {
-    const int i = blockIdx.x * kTile + threadIdx.x;
+    int tile, group;
+    const int groups = (fields + kFieldGroup - 1) / kFieldGroup;
+    if (!tile_and_group((n + kTile - 1) / kTile, groups, tile, group)) return;
+    const int i = tile * kTile + static_cast<int>(threadIdx.x);
     if (i >= n) return;
-    const T* f = flux + static_cast<size_t>(blockIdx.y) * edges;
-    double net = 0.0;
-    // orientation (+-1) x flux x length == flux x (+-length): a sign flip
-    // commutes with round-to-nearest multiplication.
-    for (int k = row[i]; k < row[i + 1]; ++k) net += static_cast<double>(f[edge[k]]) * signed_length[k];
-    out[static_cast<size_t>(blockIdx.y) * n + i] = static_cast<T>(net / area[i]);
+    int k[W];
+    double sl[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+        k[s] = edge[s * n + i];
+        sl[s] = signed_length[s * n + i];
+    }
+    const double a = area[i];
+    const int f0 = group * kFieldGroup;
+    T v[kFieldGroup][W];
+#pragma unroll
+    for (int g = 0; g < kFieldGroup; ++g) {
+        const T* f = flux + static_cast<size_t>(min(f0 + g, fields - 1)) * edges;
+#pragma unroll
+        for (int s = 0; s < W; ++s) v[g][s] = k[s] >= 0 ? f[k[s]] : T(0);
+    }
+#pragma unroll
+    for (int g = 0; g < kFieldGroup; ++g) {
+        if (f0 + g >= fields) break;
+        // orientation (+-1) x flux x length == flux x (+-length): a sign flip
+        // commutes with round-to-nearest multiplication
+        double net = 0.0;
+#pragma unroll
+        for (int s = 0; s < W; ++s)
+            if (k[s] >= 0) net += static_cast<double>(v[g][s]) * sl[s];
+        out[static_cast<size_t>(f0 + g) * n + i] = static_cast<T>(net / a);
+    }
 }
 
-template <class T>
-__global__ __launch_bounds__(kTile) void laplacian_kernel(int n, const int32_t* __restrict__ row,
-                                                          const int32_t* __restrict__ nb,
+template <class T, int W>
+__global__ __launch_bounds__(kTile) void laplacian_kernel(int n, int fields, const int32_t* __restrict__ nb,
                                                           const double* __restrict__ length,
                                                           const double* __restrict__ dist,
                                                           const double* __restrict__ area, const T* __restrict__ phi,
                                                           T* __restrict__ out) {
-    const int i = blockIdx.x * kTile + threadIdx.x;
+    int tile, group;
+    const int groups = (fields + kFieldGroup - 1) / kFieldGroup;
+    if (!tile_and_group((n + kTile - 1) / kTile, groups, tile, group)) return;
+    const int i = tile * kTile + static_cast<int>(threadIdx.x);
     if (i >= n) return;
-    const size_t base = static_cast<size_t>(blockIdx.y) * n;
-    const T* f = phi + base;
-    const double own = static_cast<double>(f[i]);
-    double sum = 0.0;
-    for (int k = row[i]; k < row[i + 1]; ++k) sum += (static_cast<double>(f[nb[k]]) - own) * length[k] / dist[k];
-    out[base + i] = static_cast<T>(sum / area[i]);
+    int c[W];
+    double l[W], d[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+        c[s] = nb[s * n + i];
+        l[s] = length[s * n + i];
+        d[s] = dist[s * n + i];
+    }
+    const double a = area[i];
+    const int f0 = group * kFieldGroup;
+    T own[kFieldGroup], v[kFieldGroup][W];
+#pragma unroll
+    for (int g = 0; g < kFieldGroup; ++g) {
+        const T* f = phi + static_cast<size_t>(min(f0 + g, fields - 1)) * n;
+        own[g] = f[i];
+#pragma unroll
+        for (int s = 0; s < W; ++s) v[g][s] = f[c[s] >= 0 ? c[s] : i];
+    }
+#pragma unroll
+    for (int g = 0; g < kFieldGroup; ++g) {
+        if (f0 + g >= fields) break;
+        const double o = static_cast<double>(own[g]);
+        double sum = 0.0;
+#pragma unroll
+        for (int s = 0; s < W; ++s)
+            if (c[s] >= 0) sum += (static_cast<double>(v[g][s]) - o) * l[s] / d[s];
+        out[static_cast<size_t>(f0 + g) * n + i] = static_cast<T>(sum / a);
+    }
 }
 
 __global__ __launch_bounds__(kTile) void diffusion_kernel(int n, const int32_t* __restrict__ row,
@@ -110,9 +205,10 @@ void launch_diffusion(int n, const int32_t* row, const int32_t* col, const doubl
 struct psim_fv {
     psim_ctx* ctx = nullptr;
     int32_t n = 0, edges = 0, directed = 0;
+    int32_t width = 0;                     // slots per cell in the device layout (6, 8 or 16)
     std::vector<uint32_t> first, second;   // host copies for edge_orientation
     void* block = nullptr;                 // one device allocation for all arrays
-    int32_t* row = nullptr;
+    // slot-major: slot s of cell i at s * n + i; padding: nb = edge = -1
     int32_t* nb = nullptr;
     int32_t* edge = nullptr;
     float* ne = nullptr;
@@ -147,57 +243,63 @@ psim_fv* upload_fv(psim_ctx* ctx, uint64_t cells, uint64_t edges, const double* 
     fv->ctx = ctx;
     fv->n = static_cast<int32_t>(cells);
     fv->edges = static_cast<int32_t>(edges);
-    std::vector<int32_t> row(cells + 1);
-    row[0] = 0;
+    uint64_t d = 0;
+    int degree = 0;
     for (uint64_t c = 0; c < cells; ++c) {
-        if (edge_offset[c] != static_cast<uint32_t>(row[c])) throw std::invalid_argument("cell edges are not contiguous");
-        row[c + 1] = row[c] + edge_count[c];
+        if (edge_offset[c] != static_cast<uint32_t>(d)) throw std::invalid_argument("cell edges are not contiguous");
+        d += edge_count[c];
+        degree = std::max<int>(degree, edge_count[c]);
     }
-    const uint64_t d = static_cast<uint64_t>(row[cells]);
+    if (degree > 16) throw std::invalid_argument("cells with more than 16 edges are not supported");
+    fv->width = degree <= 6 ? 6 : degree <= 8 ? 8 : 16;
     fv->directed = static_cast<int32_t>(d);
-    std::vector<int32_t> nb(d), ed(d);
-    std::vector<double> dist(d), len(d), slen(d);
+    const size_t W = static_cast<size_t>(fv->width), slots = W * cells;
+    std::vector<int32_t> nb(slots, -1), ed(slots, -1);
+    std::vector<float> e_ne(slots, 0.0f), e_nn(slots, 0.0f);
+    std::vector<double> dist(slots, 1.0), len(slots, 0.0), slen(slots, 0.0);
     for (uint64_t c = 0; c < cells; ++c)
-        for (int32_t k = row[c]; k < row[c + 1]; ++k) {
+        for (uint32_t q = 0; q < edge_count[c]; ++q) {
+            const uint64_t k = edge_offset[c] + q;
+            const size_t at = q * cells + c;
             if (neighbor[k] >= cells || edge[k] >= edges) throw std::invalid_argument("mesh index out of range");
-            nb[k] = static_cast<int32_t>(neighbor[k]);
-            ed[k] = static_cast<int32_t>(edge[k]);
-            dist[k] = distance[edge[k]];
-            len[k] = length[edge[k]];
+            nb[at] = static_cast<int32_t>(neighbor[k]);
+            ed[at] = static_cast<int32_t>(edge[k]);
+            e_ne[at] = ne[k];
+            e_nn[at] = nn[k];
+            dist[at] = distance[edge[k]];
+            len[at] = length[edge[k]];
             // edge_orientation: +1 from the first cell, -1 from the second
             if (first[edge[k]] == c)
-                slen[k] = length[edge[k]];
+                slen[at] = length[edge[k]];
             else if (second[edge[k]] == c)
-                slen[k] = -length[edge[k]];
+                slen[at] = -length[edge[k]];
             else
                 throw std::invalid_argument("cell is not incident to edge");
         }
     fv->first.assign(first, first + edges);
     fv->second.assign(second, second + edges);
-    const size_t bytes = 256 * 16 + (cells + 1) * 4 + 2 * d * 4 + 2 * d * 4 + 3 * d * 8 + cells * 8;
+    const size_t bytes = 256 * 16 + slots * (4 + 4 + 4 + 4 + 8 + 8 + 8) + cells * 8;
     psim::hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     psim::hip_check(hipMalloc(&fv->block, bytes), "mesh upload allocation");
     char* cur = static_cast<char*>(fv->block);
-    fv->row = carve<int32_t>(cur, cells + 1);
-    fv->nb = carve<int32_t>(cur, d);
-    fv->edge = carve<int32_t>(cur, d);
-    fv->ne = carve<float>(cur, d);
-    fv->nn = carve<float>(cur, d);
-    fv->dist = carve<double>(cur, d);
-    fv->length = carve<double>(cur, d);
-    fv->signed_length = carve<double>(cur, d);
+    fv->nb = carve<int32_t>(cur, slots);
+    fv->edge = carve<int32_t>(cur, slots);
+    fv->ne = carve<float>(cur, slots);
+    fv->nn = carve<float>(cur, slots);
+    fv->dist = carve<double>(cur, slots);
+    fv->length = carve<double>(cur, slots);
+    fv->signed_length = carve<double>(cur, slots);
     fv->area = carve<double>(cur, cells);
     const auto put = [&](void* dst, const void* src, size_t b) {
         psim::hip_check(hipMemcpy(dst, src, b, hipMemcpyHostToDevice), "mesh upload copy");
     };
-    put(fv->row, row.data(), (cells + 1) * 4);
-    put(fv->nb, nb.data(), d * 4);
-    put(fv->edge, ed.data(), d * 4);
-    put(fv->ne, ne, d * 4);
-    put(fv->nn, nn, d * 4);
-    put(fv->dist, dist.data(), d * 8);
-    put(fv->length, len.data(), d * 8);
-    put(fv->signed_length, slen.data(), d * 8);
+    put(fv->nb, nb.data(), slots * 4);
+    put(fv->edge, ed.data(), slots * 4);
+    put(fv->ne, e_ne.data(), slots * 4);
+    put(fv->nn, e_nn.data(), slots * 4);
+    put(fv->dist, dist.data(), slots * 8);
+    put(fv->length, len.data(), slots * 8);
+    put(fv->signed_length, slen.data(), slots * 8);
     put(fv->area, area, cells * 8);
     return fv.release();
 }
@@ -208,7 +310,26 @@ void check_fields(const psim_fv* fv, int dtype, uint64_t fields) {
     if (fields == 0 || fields > 65535) throw std::invalid_argument("field count out of range");
 }
 
-dim3 grid_of(int n, uint64_t fields) { return dim3((n + psim::dev::kTile - 1) / psim::dev::kTile, static_cast<unsigned>(fields)); }
+// Workgroups of a sweep: the tiles rounded up to a multiple of the XCD
+// count, times the field groups (decoded by tile_and_group).
+dim3 grid_of(int n, uint64_t fields) {
+    const uint64_t tiles = (static_cast<uint64_t>(n) + psim::dev::kTile - 1) / psim::dev::kTile;
+    const uint64_t groups = (fields + psim::dev::kFieldGroup - 1) / psim::dev::kFieldGroup;
+    const uint64_t blocks = (tiles + psim::dev::kXcds - 1) / psim::dev::kXcds * psim::dev::kXcds * groups;
+    if (blocks > 0x7fffffffULL) throw std::invalid_argument("cells x fields too large for one sweep");
+    return dim3(static_cast<unsigned>(blocks));
+}
+
+// Launch `kernel<W>` for the layout's slot count W (6, 8 or 16).
+#define PSIM_FV_LAUNCH(KERNEL, T, W, ...)                                                                       \
+    do {                                                                                                        \
+        if ((W) == 6)                                                                                           \
+            hipLaunchKernelGGL((KERNEL<T, 6>), __VA_ARGS__);                                                    \
+        else if ((W) == 8)                                                                                      \
+            hipLaunchKernelGGL((KERNEL<T, 8>), __VA_ARGS__);                                                    \
+        else                                                                                                    \
+            hipLaunchKernelGGL((KERNEL<T, 16>), __VA_ARGS__);                                                   \
+    } while (0)
 
 }  // namespace
 
@@ -265,14 +386,15 @@ int psim_gradient(psim_fv* fv, int dtype, uint64_t fields, const void* scalar, v
         void* e = where == PSIM_HOST ? st.out(bytes) : east;
         void* no = where == PSIM_HOST ? st.out(bytes) : north;
         hipStream_t stream = fv->ctx->stream;
+        const int F = static_cast<int>(fields);
         if (dtype == PSIM_F64)
-            hipLaunchKernelGGL(psim::dev::gradient_kernel<double>, grid_of(fv->n, fields), dim3(psim::dev::kTile), 0,
-                               stream, fv->n, fv->row, fv->nb, fv->ne, fv->nn, fv->dist, static_cast<const double*>(s),
-                               static_cast<double*>(e), static_cast<double*>(no));
+            PSIM_FV_LAUNCH(psim::dev::gradient_kernel, double, fv->width, grid_of(fv->n, fields), dim3(psim::dev::kTile),
+                           0, stream, fv->n, F, fv->nb, fv->ne, fv->nn, fv->dist, static_cast<const double*>(s),
+                           static_cast<double*>(e), static_cast<double*>(no));
         else
-            hipLaunchKernelGGL(psim::dev::gradient_kernel<float>, grid_of(fv->n, fields), dim3(psim::dev::kTile), 0,
-                               stream, fv->n, fv->row, fv->nb, fv->ne, fv->nn, fv->dist, static_cast<const float*>(s),
-                               static_cast<float*>(e), static_cast<float*>(no));
+            PSIM_FV_LAUNCH(psim::dev::gradient_kernel, float, fv->width, grid_of(fv->n, fields), dim3(psim::dev::kTile),
+                           0, stream, fv->n, F, fv->nb, fv->ne, fv->nn, fv->dist, static_cast<const float*>(s),
+                           static_cast<float*>(e), static_cast<float*>(no));
         psim::hip_check(hipGetLastError(), "gradient launch");
         if (where == PSIM_HOST) {
             psim::hip_check(hipMemcpyAsync(east, e, bytes, hipMemcpyDeviceToHost, stream), "gradient copy");
@@ -292,14 +414,15 @@ int psim_divergence(psim_fv* fv, int dtype, uint64_t fields, const void* flux, v
         const void* f = where == PSIM_HOST ? st.in(flux, in_bytes) : flux;
         void* o = where == PSIM_HOST ? st.out(out_bytes) : result;
         hipStream_t stream = fv->ctx->stream;
+        const int F = static_cast<int>(fields);
         if (dtype == PSIM_F64)
-            hipLaunchKernelGGL(psim::dev::divergence_kernel<double>, grid_of(fv->n, fields), dim3(psim::dev::kTile), 0,
-                               stream, fv->n, fv->edges, fv->row, fv->edge, fv->signed_length, fv->area,
-                               static_cast<const double*>(f), static_cast<double*>(o));
+            PSIM_FV_LAUNCH(psim::dev::divergence_kernel, double, fv->width, grid_of(fv->n, fields),
+                           dim3(psim::dev::kTile), 0, stream, fv->n, fv->edges, F, fv->edge, fv->signed_length, fv->area,
+                           static_cast<const double*>(f), static_cast<double*>(o));
         else
-            hipLaunchKernelGGL(psim::dev::divergence_kernel<float>, grid_of(fv->n, fields), dim3(psim::dev::kTile), 0,
-                               stream, fv->n, fv->edges, fv->row, fv->edge, fv->signed_length, fv->area,
-                               static_cast<const float*>(f), static_cast<float*>(o));
+            PSIM_FV_LAUNCH(psim::dev::divergence_kernel, float, fv->width, grid_of(fv->n, fields),
+                           dim3(psim::dev::kTile), 0, stream, fv->n, fv->edges, F, fv->edge, fv->signed_length, fv->area,
+                           static_cast<const float*>(f), static_cast<float*>(o));
         psim::hip_check(hipGetLastError(), "divergence launch");
         if (where == PSIM_HOST) {
             psim::hip_check(hipMemcpyAsync(result, o, out_bytes, hipMemcpyDeviceToHost, stream), "divergence copy");
@@ -316,14 +439,15 @@ int psim_laplacian(psim_fv* fv, int dtype, uint64_t fields, const void* scalar, 
         const void* s = where == PSIM_HOST ? st.in(scalar, bytes) : scalar;
         void* o = where == PSIM_HOST ? st.out(bytes) : result;
         hipStream_t stream = fv->ctx->stream;
+        const int F = static_cast<int>(fields);
         if (dtype == PSIM_F64)
-            hipLaunchKernelGGL(psim::dev::laplacian_kernel<double>, grid_of(fv->n, fields), dim3(psim::dev::kTile), 0,
-                               stream, fv->n, fv->row, fv->nb, fv->length, fv->dist, fv->area,
-                               static_cast<const double*>(s), static_cast<double*>(o));
+            PSIM_FV_LAUNCH(psim::dev::laplacian_kernel, double, fv->width, grid_of(fv->n, fields),
+                           dim3(psim::dev::kTile), 0, stream, fv->n, F, fv->nb, fv->length, fv->dist, fv->area,
+                           static_cast<const double*>(s), static_cast<double*>(o));
         else
-            hipLaunchKernelGGL(psim::dev::laplacian_kernel<float>, grid_of(fv->n, fields), dim3(psim::dev::kTile), 0,
-                               stream, fv->n, fv->row, fv->nb, fv->length, fv->dist, fv->area,
-                               static_cast<const float*>(s), static_cast<float*>(o));
+            PSIM_FV_LAUNCH(psim::dev::laplacian_kernel, float, fv->width, grid_of(fv->n, fields),
+                           dim3(psim::dev::kTile), 0, stream, fv->n, F, fv->nb, fv->length, fv->dist, fv->area,
+                           static_cast<const float*>(s), static_cast<float*>(o));
         psim::hip_check(hipGetLastError(), "laplacian launch");
         if (where == PSIM_HOST) {
             psim::hip_check(hipMemcpyAsync(result, o, bytes, hipMemcpyDeviceToHost, stream), "laplacian copy");

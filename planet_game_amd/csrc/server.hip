@@ -14,6 +14,7 @@
 // and a late arrival does not wait for a batch to drain.
 #include "system.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <cmath>
@@ -59,6 +60,10 @@ struct psim_server {
     // pool systems, per graph, and their index in the instance table
     std::vector<std::vector<psim_system*>> pool;
     std::vector<std::vector<int32_t>> pool_state;
+    // per table index: the matrix key and dt its system last assembled (key 0:
+    // none that can be reused); kept across runs
+    std::vector<uint64_t> matrix_key;
+    std::vector<double> matrix_dt;
     std::vector<psim_system*> owned;
     int32_t blocks = 0;
     psim::Arena arena;
@@ -209,6 +214,8 @@ int psim_server_create(psim_ctx* ctx, psim_graph* const* graphs, const double* r
                 table.push_back(I);
             }
         }
+        s->matrix_key.assign(table.size(), 0);
+        s->matrix_dt.assign(table.size(), 0.0);
         int32_t slots = 0;
         for (size_t q = 0; q < k; ++q) slots += s->slot_count[q];
         auto& A = s->arena;
@@ -341,11 +348,64 @@ int psim_server_run(psim_server* s, const psim_serve_request* requests, int32_t 
                 (void)hipStreamSynchronize(s->ctx->stream);
             }
         } drain{s};
+        // a run that does not complete forgets every kept matrix: an
+        // assembly may have been cut short
+        struct ForgetUnlessDone {
+            psim_server* s;
+            bool done = false;
+            ~ForgetUnlessDone() {
+                if (!done) std::fill(s->matrix_key.begin(), s->matrix_key.end(), uint64_t{0});
+            }
+        } forget{s};
 
         // host loop: arrivals -> admission -> queues; completions -> records
-        std::vector<std::vector<int32_t>> free_states(k);
+        //
+        // Free pooled systems per graph: a queue in release order (a session
+        // without a reusable matrix takes the system freed longest ago, which
+        // evicts the coldest matrix) and, per matrix key, the free systems
+        // that hold it. Entries go stale when their system is taken through
+        // the other list; is_free and the system's key tell.
+        std::vector<std::deque<int32_t>> free_any(k);
+        std::vector<std::unordered_map<uint64_t, std::vector<int32_t>>> free_keyed(k);
+        std::vector<int32_t> free_count(k, 0);
+        std::vector<char> is_free(s->matrix_key.size(), 0);
+        const auto release = [&](size_t q, int32_t st) {
+            is_free[static_cast<size_t>(st)] = 1;
+            ++free_count[q];
+            free_any[q].push_back(st);
+            const uint64_t key = s->matrix_key[static_cast<size_t>(st)];
+            if (key != 0 && s->matrix_dt[static_cast<size_t>(st)] == dt) free_keyed[q][key].push_back(st);
+        };
+        const auto take_keyed = [&](size_t q, uint64_t key) -> int32_t {
+            if (key == 0) return -1;
+            const auto found = free_keyed[q].find(key);
+            if (found == free_keyed[q].end()) return -1;
+            auto& list = found->second;
+            while (!list.empty()) {
+                const int32_t st = list.back();
+                list.pop_back();
+                if (is_free[static_cast<size_t>(st)] && s->matrix_key[static_cast<size_t>(st)] == key) {
+                    is_free[static_cast<size_t>(st)] = 0;
+                    --free_count[q];
+                    return st;
+                }
+            }
+            return -1;
+        };
+        const auto take_any = [&](size_t q) -> int32_t {
+            while (!free_any[q].empty()) {
+                const int32_t st = free_any[q].front();
+                free_any[q].pop_front();
+                if (is_free[static_cast<size_t>(st)]) {
+                    is_free[static_cast<size_t>(st)] = 0;
+                    --free_count[q];
+                    return st;
+                }
+            }
+            return -1;
+        };
         for (size_t q = 0; q < k; ++q)
-            for (auto it = s->pool_state[q].rbegin(); it != s->pool_state[q].rend(); ++it) free_states[q].push_back(*it);
+            for (const int32_t st : s->pool_state[q]) release(q, st);
         std::unordered_map<int32_t, int32_t> session_state;   // session -> table index
         std::unordered_map<int32_t, bool> session_busy;
         std::vector<uint64_t> published(k, 0);
@@ -385,7 +445,7 @@ int psim_server_run(psim_server* s, const psim_serve_request* requests, int32_t 
                 const size_t q = static_cast<size_t>(r.graph);
                 int32_t st = -1;
                 if (r.first) {
-                    if (free_states[q].empty()) {
+                    if (free_count[q] == 0) {
                         ++it;
                         continue;   // admission: no system free in this graph's pool
                     }
@@ -403,15 +463,21 @@ int psim_server_run(psim_server* s, const psim_serve_request* requests, int32_t 
                     ++it;
                     continue;
                 }
+                bool assemble = false;
                 if (r.first) {
-                    st = free_states[q].back();
-                    free_states[q].pop_back();
+                    st = take_keyed(q, r.matrix_key);
+                    if (st < 0) {
+                        st = take_any(q);
+                        assemble = true;
+                        s->matrix_key[static_cast<size_t>(st)] = r.matrix_key;
+                        s->matrix_dt[static_cast<size_t>(st)] = dt;
+                    }
                     session_state[r.session] = st;
                 }
                 session_busy[r.session] = true;
                 psim_system* sys = system_of[st];
                 const int op = psim::dev::kOpSolve | psim::dev::kOpHeatRhs |
-                               (r.first ? psim::dev::kOpSetMatrix | psim::dev::kOpHeatAssemble : 0);
+                               (assemble ? psim::dev::kOpSetMatrix | psim::dev::kOpHeatAssemble : 0);
                 const psim::dev::Launch lp = psim::sys::solve_params(sys, op, tolerance, max_iterations);
                 ServeRequest e{};
                 e.state = st;
@@ -421,9 +487,9 @@ int psim_server_run(psim_server* s, const psim_serve_request* requests, int32_t 
                 e.dt = dt;
                 e.tolerance = tolerance;
                 e.inner_relative = lp.inner_relative;
-                e.entry_conductance = r.first ? r.conductance_per_entry : nullptr;
-                e.capacity_area = r.first ? r.capacity_area : sys->capacity_area;
-                e.capacity_keep = r.first ? sys->capacity_area : nullptr;
+                e.entry_conductance = assemble ? r.conductance_per_entry : nullptr;
+                e.capacity_area = assemble ? r.capacity_area : sys->capacity_area;
+                e.capacity_keep = assemble ? sys->capacity_area : nullptr;
                 e.t_old = r.temperature_old;
                 e.forcing = r.forcing_W_m2;
                 e.t_new = r.temperature_new;
@@ -468,7 +534,7 @@ int psim_server_run(psim_server* s, const psim_serve_request* requests, int32_t 
                 free_completion.push_back(c);
                 session_busy[r.session] = false;
                 if (r.last) {
-                    free_states[q].push_back(session_state[r.session]);
+                    release(q, session_state[r.session]);
                     session_state.erase(r.session);
                 }
                 in_flight[f] = in_flight.back();
@@ -483,6 +549,7 @@ int psim_server_run(psim_server* s, const psim_serve_request* requests, int32_t 
         for (int32_t i = 0; i < count; ++i)
             if (records[i].status != PSIM_OK && records[i].status != PSIM_NOT_CONVERGED)
                 throw psim::DeviceFailure("a served request failed on the device");
+        forget.done = true;
     });
 }
 
