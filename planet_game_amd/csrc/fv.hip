@@ -13,9 +13,13 @@
 // so results are bit-identical to the reference for float and double.
 //
 // Workgroup order is XCD-aware: consecutive workgroups go to the eight XCDs
-// in turn, so the field groups of one 256-cell tile are given workgroup
-// indices 8 apart (the same XCD, dispatched close together) and re-read the
-// tile's geometry from that XCD's L2 instead of HBM.
+// in turn, so workgroup b runs on XCD b % 8. Each XCD is given one
+// contiguous eighth of the tiles, and its workgroups walk those tiles in
+// order with all field groups of a tile in a row: the field groups re-read
+// the tile's geometry from that XCD's L2, and the neighbour gathers, which
+// reach a few hundred cells either side in the mesh's space-filling order,
+// find the neighbouring tiles' values in the same L2 instead of each XCD
+// fetching them again from HBM.
 #include "device.hpp"
 #include "internal.hpp"
 
@@ -29,14 +33,16 @@ namespace {
 constexpr int kFieldGroup = 4;   // fields per thread
 constexpr int kXcds = 8;
 
-// Tile and field group of this workgroup (grid: ceil(tiles / 8) * 8 x groups
-// workgroups); false for the padding tiles past the last.
+// Tile and field group of this workgroup (grid: 8 x ceil(tiles / 8) x
+// groups workgroups; XCD x takes tiles [x * per_xcd, (x + 1) * per_xcd));
+// false for the padding past the last tile.
 __device__ inline bool tile_and_group(int tiles, int groups, int& tile, int& group) {
     const int b = static_cast<int>(blockIdx.x);
-    const int x = b % kXcds, rest = b / kXcds;
-    group = rest % groups;
-    tile = rest / groups * kXcds + x;
-    return tile < tiles;
+    const int per_xcd = (tiles + kXcds - 1) / kXcds;
+    const int x = b % kXcds, m = b / kXcds;
+    group = m % groups;
+    tile = x * per_xcd + m / groups;
+    return m / groups < per_xcd && tile < tiles;
 }
 
 template <class T, int W>
