@@ -504,16 +504,23 @@ int psim_batch_set_groups(psim_batch* batch, const int32_t* groups);
  * otherwise its matrix is assembled first in the same launch. infos (host,
  * count records) synchronises; NULL queues the work and returns (up to four
  * calls may be in flight; psim_batch_synchronize waits for them). */
+/* Program instance of the batch's launches: 3 (default) or 2 workgroups per
+ * CU (register budgets of 168 or 256 VGPRs per thread); setting it goes back
+ * to the default allocation of that occupancy's capacity. Results are
+ * bit-identical at either. */
+int psim_batch_set_occupancy(psim_batch* batch, int32_t workgroups_per_cu);
+int32_t psim_batch_occupancy(const psim_batch* batch);
 int psim_batch_heat_step(psim_batch* batch, const double* const* conductance_per_entry,
                          const double* const* capacity_area, const double* const* temperature_old,
                          const double* const* forcing_W_m2, double dt_s, double tolerance, int32_t max_iterations,
                          double* const* temperature_new, psim_solve_info* infos);
 int psim_batch_synchronize(psim_batch* batch);
 
-/* Autotuner: times candidate workgroup allocations (shares of the device and
- * tile-count weightings) on this exact step, `repeats` launches each, and
- * keeps the fastest (its mean launch time in best_ms). Outputs are written
- * as by psim_batch_heat_step. */
+/* Autotuner: times candidate launch shapes on this exact step, `repeats`
+ * launches each, and keeps the fastest (its mean launch time in best_ms):
+ * the occupancy (2 or 3 workgroups per CU) and, for each, workgroup
+ * allocations (shares of the device's co-resident workgroups and tile-count
+ * weightings). Outputs are written as by psim_batch_heat_step. */
 int psim_batch_autotune(psim_batch* batch, const double* const* conductance_per_entry,
                         const double* const* capacity_area, const double* const* temperature_old,
                         const double* const* forcing_W_m2, double dt_s, double tolerance, int32_t max_iterations,
@@ -568,6 +575,12 @@ typedef struct {
 int psim_server_create(psim_ctx* ctx, psim_graph* const* graphs, const double* request_share,
                        const int32_t* slot_sizes, int32_t graph_count, int precision, uint64_t memory_budget_bytes,
                        psim_server** out);
+/* The same with the server program's occupancy: 2 or 3 workgroups per CU
+ * (register budgets of 256 or 168 VGPRs; 0 = 2, the default of
+ * psim_server_create). */
+int psim_server_create_occupancy(psim_ctx* ctx, psim_graph* const* graphs, const double* request_share,
+                                 const int32_t* slot_sizes, int32_t graph_count, int precision,
+                                 uint64_t memory_budget_bytes, int32_t workgroups_per_cu, psim_server** out);
 void psim_server_destroy(psim_server* server);
 /* Per graph: slots, workgroups per slot, pool systems, bytes per system
  * (any output may be NULL). */

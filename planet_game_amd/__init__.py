@@ -211,6 +211,8 @@ class ResponseTable(C.Structure):
 _sig("psim_implicit_transport_table", C.c_int, _vp, _dbl, _vp, C.POINTER(ResponseTable), C.POINTER(TransportSettings),
      _vp, _vp, _vp, C.POINTER(TransportResult), C.c_int)
 _sig("psim_server_create", C.c_int, _vp, _pp, C.POINTER(_dbl), C.POINTER(_i32), _i32, C.c_int, _u64, C.POINTER(_vp))
+_sig("psim_server_create_occupancy", C.c_int, _vp, _pp, C.POINTER(_dbl), C.POINTER(_i32), _i32, C.c_int, _u64, _i32,
+     C.POINTER(_vp))
 _sig("psim_server_destroy", None, _vp)
 _sig("psim_server_plan", C.c_int, _vp, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_u64))
 _sig("psim_server_run", C.c_int, _vp, C.POINTER(ServeRequest), _i32, _dbl, _dbl, _i32, C.POINTER(ServeRecord))
@@ -218,6 +220,8 @@ _sig("psim_batch_create", C.c_int, _vp, _pp, _i32, C.POINTER(_vp))
 _sig("psim_batch_destroy", None, _vp)
 _sig("psim_batch_size", C.c_int, _vp, C.POINTER(_i32), C.POINTER(_i32))
 _sig("psim_batch_groups", C.c_int, _vp, C.POINTER(_i32))
+_sig("psim_batch_set_occupancy", C.c_int, _vp, _i32)
+_sig("psim_batch_occupancy", _i32, _vp)
 _sig("psim_batch_set_groups", C.c_int, _vp, C.POINTER(_i32))
 _sig("psim_batch_heat_step", C.c_int, _vp, _pp, _pp, _pp, _pp, _dbl, _dbl, _i32, _pp, C.POINTER(SolveInfo))
 _sig("psim_batch_synchronize", C.c_int, _vp)
@@ -796,6 +800,15 @@ class Batch:
         return cap.value
 
     @property
+    def occupancy(self):
+        """Workgroups per CU of the batch's program instance (2 or 3)."""
+        return lib.psim_batch_occupancy(self.handle)
+
+    @occupancy.setter
+    def occupancy(self, workgroups_per_cu):
+        check(lib.psim_batch_set_occupancy(self.handle, workgroups_per_cu))
+
+    @property
     def groups(self):
         g = (_i32 * len(self.systems))()
         check(lib.psim_batch_groups(self.handle, g))
@@ -849,14 +862,15 @@ class Server:
     arrivals. `graphs` are Graph objects; `share` the expected fraction of
     requests per graph."""
 
-    def __init__(self, ctx: Context, graphs, share=None, precision=MIXED, memory_budget=4 << 30, slot_sizes=None):
+    def __init__(self, ctx: Context, graphs, share=None, precision=MIXED, memory_budget=4 << 30, slot_sizes=None,
+                 occupancy=0):
         self.ctx, self.graphs = ctx, list(graphs)
         arr = (_vp * len(self.graphs))(*[g.handle for g in self.graphs])
         sh = None if share is None else (_dbl * len(self.graphs))(*share)
         sz = None if slot_sizes is None else (_i32 * len(self.graphs))(*slot_sizes)
         h = _vp()
-        check(lib.psim_server_create(ctx.handle, arr, sh, sz, len(self.graphs), precision, memory_budget,
-                                     C.byref(h)))
+        check(lib.psim_server_create_occupancy(ctx.handle, arr, sh, sz, len(self.graphs), precision, memory_budget,
+                                               occupancy, C.byref(h)))
         self.handle = h
         _track(self)
 

@@ -7,8 +7,11 @@
 //
 // Workgroup allocation: every instance gets one workgroup, then the rest of
 // the device's co-resident workgroups go, one at a time, to the instance
-// with the most tiles per workgroup (its phase length). The autotuner times
-// alternative allocations on the caller's real workload and keeps the best.
+// with the most tiles per workgroup (its phase length). The program runs at
+// 3 workgroups per CU (168 VGPRs each) or 2 (256 VGPRs, fewer spills and less
+// contention per CU, fewer workgroups). The autotuner times alternative
+// occupancies and allocations on the caller's real workload and keeps the
+// best.
 #include "system.hpp"
 
 #include <array>
@@ -42,7 +45,9 @@ struct psim_batch {
     std::vector<psim_system*> systems;
     std::vector<int32_t> groups;   // workgroups per instance
     std::vector<int32_t> work;     // level-0 tiles per instance
-    int32_t capacity = 0;          // co-resident program workgroups
+    int32_t capacity = 0;          // co-resident program workgroups at this occupancy
+    int32_t waves = 3;             // program instance: 2 or 3 workgroups per CU
+    int32_t map_capacity = 0;      // workgroup-map entries allocated (the largest capacity)
     int32_t blocks = 0;            // sum of groups
     psim::Arena arena;
     Instance* d_inst = nullptr;
@@ -190,7 +195,7 @@ Slot& enqueue(psim_batch* b, const StepArgs& a) {
     psim::hip_check(hipEventRecord(slot.copied, stream), "batch event");
     slot.used = true;
     psim::hip_check(psim::dev::launch_program(b->d_inst, b->d_map, slot.d_launch, Launch{}, b->blocks, 0, stream,
-                                              b->blocks > 1),
+                                              b->blocks > 1, b->waves),
                     "batch launch");
     for (size_t i = 0; i < count; ++i)
         if (a.conductance != nullptr && a.conductance[i] != nullptr) {
@@ -198,6 +203,23 @@ Slot& enqueue(psim_batch* b, const StepArgs& a) {
             b->systems[i]->heat_matrix = true;
         }
     return slot;
+}
+
+// Co-resident program workgroups on the device at `waves` workgroups per CU.
+int32_t capacity_at(const psim_ctx* ctx, int waves) {
+    return ctx->compute_units * std::min(waves, psim::dev::program_blocks_per_cu(0, waves));
+}
+
+// Switch the program instance (2 or 3 workgroups per CU) and go back to the
+// default allocation of the new capacity.
+void set_waves(psim_batch* b, int waves) {
+    if (waves != 2 && waves != 3) throw std::invalid_argument("occupancy must be 2 or 3 workgroups per CU");
+    const int32_t capacity = capacity_at(b->ctx, waves);
+    if (static_cast<int32_t>(b->systems.size()) > capacity)
+        throw std::invalid_argument("batch larger than the device's co-resident workgroups at that occupancy");
+    b->waves = waves;
+    b->capacity = capacity;
+    set_groups(b, default_groups(b, 1.0, 1.0));
 }
 
 // Wait for the stream, check the barrier words, copy the slot's records.
@@ -239,7 +261,8 @@ int psim_batch_create(psim_ctx* ctx, psim_system* const* systems, int32_t count,
         psim::hip_check(hipSetDevice(ctx->device), "hipSetDevice");
         auto b = std::make_unique<psim_batch>();
         b->ctx = ctx;
-        b->capacity = ctx->compute_units * ctx->blocks_per_cu;
+        b->capacity = capacity_at(ctx, 3);
+        b->map_capacity = std::max(b->capacity, capacity_at(ctx, 2));
         if (count > b->capacity) throw std::invalid_argument("batch larger than the device's co-resident workgroups");
         for (int32_t i = 0; i < count; ++i) {
             psim_system* s = systems[i];
@@ -253,7 +276,7 @@ int psim_batch_create(psim_ctx* ctx, psim_system* const* systems, int32_t count,
         auto& A = b->arena;
         const size_t n = static_cast<size_t>(count);
         A.plan(&b->d_inst, n);
-        A.plan(&b->d_map, static_cast<size_t>(b->capacity));
+        A.plan(&b->d_map, static_cast<size_t>(b->map_capacity));
         A.plan(&b->d_sticky, n);
         b->tri.assign(n, nullptr);
         for (size_t i = 0; i < n; ++i)
@@ -302,6 +325,15 @@ int psim_batch_set_groups(psim_batch* batch, const int32_t* groups) {
     });
 }
 
+int psim_batch_set_occupancy(psim_batch* batch, int32_t workgroups_per_cu) {
+    return guarded([&] {
+        if (batch == nullptr) throw std::invalid_argument("null batch");
+        set_waves(batch, workgroups_per_cu);
+    });
+}
+
+int32_t psim_batch_occupancy(const psim_batch* batch) { return batch ? batch->waves : 0; }
+
 int psim_batch_heat_step(psim_batch* batch, const double* const* conductance_per_entry,
                          const double* const* capacity_area, const double* const* t_old, const double* const* forcing,
                          double dt, double tolerance, int32_t max_iterations, double* const* t_new,
@@ -332,15 +364,22 @@ int psim_batch_autotune(psim_batch* batch, const double* const* conductance_per_
         if (repeats < 1) throw std::invalid_argument("repeats must be positive");
         const StepArgs a{conductance_per_entry, capacity_area, t_old, forcing, dt, tolerance, max_iterations, t_new};
         check_args(batch, a);
-        // candidates: weight = tiles^e over a share of the co-resident workgroups
-        std::vector<std::vector<int32_t>> candidates;
-        for (double fraction : {1.0, 0.75, 0.5, 0.25})
-            for (double exponent : {1.0, 0.75, 0.5}) {
-                auto g = default_groups(batch, exponent, fraction);
-                bool seen = false;
-                for (const auto& c : candidates) seen = seen || c == g;
-                if (!seen) candidates.push_back(std::move(g));
-            }
+        // candidates: the program instance (2 or 3 workgroups per CU: a
+        // register budget of 256 or 168 VGPRs), and, for each, weight =
+        // tiles^e over a share of its co-resident workgroups
+        std::vector<std::pair<int, std::vector<int32_t>>> candidates;
+        for (int waves : {3, 2}) {
+            if (static_cast<int32_t>(batch->systems.size()) > capacity_at(batch->ctx, waves)) continue;
+            batch->waves = waves;
+            batch->capacity = capacity_at(batch->ctx, waves);
+            for (double fraction : {1.0, 0.75, 0.5, 0.25})
+                for (double exponent : {1.0, 0.75, 0.5}) {
+                    auto g = default_groups(batch, exponent, fraction);
+                    bool seen = false;
+                    for (const auto& c : candidates) seen = seen || (c.first == waves && c.second == g);
+                    if (!seen) candidates.emplace_back(waves, std::move(g));
+                }
+        }
         hipStream_t stream = batch->ctx->stream;
         hipEvent_t start = nullptr, stop = nullptr;
         psim::hip_check(hipEventCreate(&start), "event");
@@ -354,8 +393,13 @@ int psim_batch_autotune(psim_batch* batch, const double* const* conductance_per_
         } release{start, stop};
         double best = 0.0;
         size_t best_index = 0;
+        const auto apply = [&](size_t c) {
+            batch->waves = candidates[c].first;
+            batch->capacity = capacity_at(batch->ctx, batch->waves);
+            set_groups(batch, candidates[c].second);
+        };
         for (size_t c = 0; c < candidates.size(); ++c) {
-            set_groups(batch, candidates[c]);
+            apply(c);
             collect(batch, enqueue(batch, a), nullptr);   // warm
             psim::hip_check(hipEventRecord(start, stream), "event");
             for (int32_t r = 0; r < repeats; ++r) (void)enqueue(batch, a);
@@ -369,7 +413,7 @@ int psim_batch_autotune(psim_batch* batch, const double* const* conductance_per_
                 best_index = c;
             }
         }
-        set_groups(batch, candidates[best_index]);
+        apply(best_index);
         if (best_ms != nullptr) *best_ms = best;
     });
 }

@@ -62,7 +62,7 @@ void launch(psim_system* s, const Launch& p) {
     // full-register-file instance of the program
     const bool wide = cooperative && s->workgroups <= ctx->compute_units;
     psim::hip_check(psim::dev::launch_program(s->d_inst, s->d_map, nullptr, p, s->workgroups, lds, ctx->stream,
-                                              cooperative, wide),
+                                              cooperative, wide ? 1 : 3),
                     "solver launch");
 }
 
@@ -479,7 +479,7 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         A.plan(&I.b_alt, n);
         A.plan(&I.x[0], n);
         A.plan(&I.x[1], n);
-        void** vecs[11] = {&I.r, &I.r0, &I.r_alt, &I.r0_alt, &I.p[0], &I.p[1], &I.v, &I.ph, &I.s, &I.sh, &I.t};
+        void** vecs[12] = {&I.r, &I.r0, &I.r_alt, &I.r0_alt, &I.p[0], &I.p[1], &I.v, &I.ph, &I.s, &I.sh, &I.t, &I.u};
         for (void** v : vecs) A.plan(reinterpret_cast<char**>(v), static_cast<size_t>(n) * vt);
         I.partial_slots = std::max(tiles, g->depth > 1 ? g->views[0].down_tiles : 0);
         A.plan(&I.partial, 2 * static_cast<size_t>(I.partial_slots) * kPartials);

@@ -126,6 +126,7 @@ struct Instance {
     void* s = nullptr;
     void* sh = nullptr;
     void* t = nullptr;
+    void* u = nullptr;           // A r0 of a pass's first iteration (symmetric A: (r0, A ph) = (A r0, ph))
     void* lvl_b[kMaxLevels] = {};   // V-cycle right-hand sides, levels >= 1
     void* lvl_e[kMaxLevels] = {};   // V-cycle corrections, levels >= 1
     void* lvl_x[kMaxLevels] = {};   // pre-smoothed iterate after two Jacobi sweeps, levels < depth - 1
@@ -170,16 +171,17 @@ struct Launch {
 
 // Launch the persistent program. Workgroup w serves instance
 // block_instance[w]; its parameters are launches[that] when `launches` is
-// non-null, else `single`. `wide`: the one-wave-per-SIMD instance of the
-// program (one workgroup per CU, the full register file), for a single
-// instance whose group fits one workgroup per CU.
+// non-null, else `single`. `waves`: the instance of the program compiled
+// for 1, 2 or 3 workgroups per CU (the register budget of 512, 256 or 168
+// VGPRs): 1 for a single instance whose group fits one workgroup per CU, 2
+// or 3 for batches.
 hipError_t launch_program(const Instance* instances, const int32_t* block_instance, const Launch* launches,
                           const Launch& single, int32_t blocks, size_t dynamic_lds, hipStream_t stream,
-                          bool cooperative, bool wide = false);
+                          bool cooperative, int waves = 3);
 
 // Co-resident program workgroups per compute unit with `dynamic_lds`
 // bytes of dynamic LDS (0 if the kernel cannot launch with that much).
-int program_blocks_per_cu(size_t dynamic_lds, bool wide = false);
+int program_blocks_per_cu(size_t dynamic_lds, int waves = 3);
 // The program kernel's static LDS bytes.
 size_t program_static_lds();
 
@@ -250,10 +252,11 @@ struct ServeParams {
     uint64_t lifetime;          // wall-clock ticks after its start that an idle slot gives up
 };
 
-// Launch the server (cooperative, `blocks` = sum of slot sizes).
-hipError_t launch_server(const ServeParams& params, int32_t blocks, hipStream_t stream);
+// Launch the server (cooperative, `blocks` = sum of slot sizes), compiled
+// for `waves` (2 or 3) workgroups per CU.
+hipError_t launch_server(const ServeParams& params, int32_t blocks, hipStream_t stream, int waves = 3);
 // Co-resident server workgroups per compute unit.
-int server_blocks_per_cu();
+int server_blocks_per_cu(int waves = 3);
 
 void launch_diffusion(int n, const int32_t* row, const int32_t* col, const double* weight, const double* area,
                       double conductance, const double* t, double* out, hipStream_t stream);

@@ -87,7 +87,7 @@ size_t carve(const psim_system* s, unsigned char* base, Instance& I) {
     I.b_alt = reinterpret_cast<double*>(take(n * 8));
     I.x[0] = reinterpret_cast<double*>(take(n * 8));
     I.x[1] = reinterpret_cast<double*>(take(n * 8));
-    void** vecs[11] = {&I.r, &I.r0, &I.r_alt, &I.r0_alt, &I.p[0], &I.p[1], &I.v, &I.ph, &I.s, &I.sh, &I.t};
+    void** vecs[12] = {&I.r, &I.r0, &I.r_alt, &I.r0_alt, &I.p[0], &I.p[1], &I.v, &I.ph, &I.s, &I.sh, &I.t, &I.u};
     for (void** v : vecs) *v = take(n * vt);
     for (int j = 0; j < I.depth && j < kMaxLevels; ++j) {
         const size_t nj = static_cast<size_t>(I.lv[j].n);
@@ -164,7 +164,7 @@ int psim_heat_step_noalloc(psim_system* s, const double* t_old, const double* fo
         const bool wide = cooperative && s->workgroups <= ctx->compute_units;
         psim::hip_check(psim::dev::launch_program(reinterpret_cast<const Instance*>(d.base + kDescriptorOffset),
                                                   reinterpret_cast<const int32_t*>(d.base + kMapOffset), nullptr, p,
-                                                  s->workgroups, 0, stream, cooperative, wide),
+                                                  s->workgroups, 0, stream, cooperative, wide ? 1 : 3),
                         "solver launch");
         if (info != nullptr) {
             SolveResult r{};

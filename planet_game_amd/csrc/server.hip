@@ -54,6 +54,7 @@ template <class T> T* pinned(size_t count) {
 struct psim_server {
     psim_ctx* ctx = nullptr;
     int precision = PSIM_MIXED;
+    int waves = 2;                     // server program instance: workgroups per CU
     std::vector<psim_graph*> graphs;
     std::vector<int32_t> slot_count, slot_size, pool_size;
     std::vector<size_t> footprint;
@@ -138,7 +139,17 @@ extern "C" {
 
 int psim_server_create(psim_ctx* ctx, psim_graph* const* graphs, const double* request_share, const int32_t* slot_sizes,
                        int32_t graph_count, int precision, uint64_t memory_budget, psim_server** out) {
+    return psim_server_create_occupancy(ctx, graphs, request_share, slot_sizes, graph_count, precision, memory_budget, 0,
+                                        out);
+}
+
+int psim_server_create_occupancy(psim_ctx* ctx, psim_graph* const* graphs, const double* request_share,
+                                 const int32_t* slot_sizes, int32_t graph_count, int precision, uint64_t memory_budget,
+                                 int32_t workgroups_per_cu, psim_server** out) {
     return guarded([&] {
+        if (workgroups_per_cu == 0) workgroups_per_cu = 2;
+        if (workgroups_per_cu != 2 && workgroups_per_cu != 3)
+            throw std::invalid_argument("occupancy must be 2 or 3 workgroups per CU");
         if (ctx == nullptr || graphs == nullptr || out == nullptr) throw std::invalid_argument("null argument");
         if (graph_count < 1 || graph_count > kMaxQueues) throw std::invalid_argument("graph count out of range");
         if (precision != PSIM_PLAIN_FP64 && precision != PSIM_MIXED) throw std::invalid_argument("unknown precision");
@@ -166,7 +177,8 @@ int psim_server_create(psim_ctx* ctx, psim_graph* const* graphs, const double* r
         s->slot_size.assign(k, 0);
         s->pool_size.assign(k, 0);
         s->footprint.assign(k, 0);
-        const int per_cu = std::min(ctx->blocks_per_cu, psim::dev::server_blocks_per_cu());
+        s->waves = workgroups_per_cu;
+        const int per_cu = std::min(workgroups_per_cu, psim::dev::server_blocks_per_cu(workgroups_per_cu));
         if (per_cu < 1) throw psim::DeviceFailure("the server kernel cannot be resident");
         plan_slots(s.get(), share, slot_sizes, ctx->compute_units * per_cu);
         // pools: one system per slot at least, the rest of the budget split by
@@ -338,7 +350,7 @@ int psim_server_run(psim_server* s, const psim_serve_request* requests, int32_t 
         P.stop = s->stop;
         const double span = count > 0 ? requests[count - 1].arrival_s : 0.0;
         P.lifetime = static_cast<uint64_t>((span + 120.0) * s->ctx->wall_clock_hz);
-        psim::hip_check(psim::dev::launch_server(P, s->blocks, stream), "server launch");
+        psim::hip_check(psim::dev::launch_server(P, s->blocks, stream, s->waves), "server launch");
         // whatever happens below, the server is told to stop and drained
         // before this call returns
         struct Drain {

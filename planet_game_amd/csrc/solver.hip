@@ -171,8 +171,9 @@ __device__ inline void store_partials_n(double* part, const int (&tl)[R], const 
 // This workgroup's tiles of a sweep (g, g + G, g + 2G, ...), two at a time
 // while two remain, so that each thread keeps two rows' loads in flight;
 // f is called with the tile list (one or two tiles). Pairs hold two rows'
-// slots in registers, which FP32 sweeps afford and FP64 sweeps do not (the
-// solver would spill), so FP64 sweeps go one tile at a time.
+// slots in registers, which FP32 sweeps afford and FP64 sweeps afford only
+// in the program instance with a 256-VGPR budget (two workgroups per CU:
+// kW == 2); elsewhere FP64 sweeps go one tile at a time.
 template <bool Pairs = true, class F> __device__ inline void tile_loop(int tiles, int g, int G, const F& f) {
     if constexpr (!Pairs) {
         for (int t = g; t < tiles; t += G) {
@@ -398,6 +399,11 @@ template <class VT> struct Source {
     }
 };
 
+// Two values one gather returns.
+template <class VT> struct Pair {
+    VT first, second;
+};
+
 // Row i of an ELL level: fold(off_s, gather(s, c_s)) over the row's slots
 // in order, skipping padding (c_s < 0). All column and value loads are
 // issued first, then every slot's gather (a padding slot gathers at row i,
@@ -491,17 +497,18 @@ __device__ inline int tile_row(int tl, int n, bool& ok) {
 // set, the two-sweep iterate jacobi_phase stored there. On level 0 the input
 // comes from a Source; each row stores its value in `keep` and `keep2` (if
 // set) and adds its square to the tile partial (if `part` is set).
-template <class VT, bool kF = false>
+template <class VT, int kW = 0>
 __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& V, int j, const Source<VT>& src, VT* keep,
                            VT* coarse_rhs, double* part, int g, int G, Lds& lds, const VT* pre = nullptr,
-                           VT* keep2 = nullptr) {
+                           VT* keep2 = nullptr, VT* au = nullptr) {
+    constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
     const VT om = static_cast<VT>(I.omega[j]);
     const VT* diag = V.diag(j);
     const VT* dinv = V.dinv(j);
     const VT* off = V.off(j);
-    tile_loop<sizeof(VT) == 4>(L.down_tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+    tile_loop<sizeof(VT) == 4 || kW == 2>(L.down_tiles, g, G, [&]<int R>(const int (&tl)[R]) {
         // the tile's rows come from its padded row list (one load); the
         // aggregate bounds the restriction needs load alongside
         int c0[R], c1[R], base[R], i[R];
@@ -522,7 +529,7 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
             q0[r] = c < L.n_next ? L.agg_start[c] : 0;
             q1[r] = c < L.n_next ? L.agg_start[c + 1] : 0;
         }
-        VT wi[R], acc[R];
+        VT wi[R], acc[R], aw[R];
         double sq[R][1];
 #pragma unroll
         for (int r = 0; r < R; ++r) wi[r] = src.template at<kF>(i[r]);
@@ -531,16 +538,33 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
             sq[r][0] = ok[r] ? static_cast<double>(wi[r]) * static_cast<double>(wi[r]) : 0.0;
             const VT xi = pre != nullptr ? fetch<kF>(pre + i[r]) : om * dinv[i[r]] * wi[r];
             acc[r] = wi[r] - diag[i[r]] * xi;
+            aw[r] = diag[i[r]] * wi[r];
         }
-        ell_rows<R>(
-            L.col, off, n, W, i,
-            [&](int, int, int c) { return pre != nullptr ? fetch<kF>(pre + c) : om * dinv[c] * src.template at<kF>(c); },
-            [&](int r, VT o, VT v) { acc[r] -= o * v; });
+        if (au != nullptr) {
+            // (A w)_i beside the residual: the neighbours' inputs and their
+            // Jacobi iterates from the same gathers
+            ell_rows<R>(
+                L.col, off, n, W, i,
+                [&](int, int, int c) {
+                    const VT wc = src.template at<kF>(c);
+                    return Pair<VT>{wc, om * dinv[c] * wc};
+                },
+                [&](int r, VT o, Pair<VT> v) {
+                    acc[r] -= o * v.second;
+                    aw[r] += o * v.first;
+                });
+        } else {
+            ell_rows<R>(
+                L.col, off, n, W, i,
+                [&](int, int, int c) { return pre != nullptr ? fetch<kF>(pre + c) : om * dinv[c] * src.template at<kF>(c); },
+                [&](int r, VT o, VT v) { acc[r] -= o * v; });
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (ok[r]) {
                 if (keep != nullptr) publish(&keep[i[r]], wi[r]);
                 if (keep2 != nullptr) publish(&keep2[i[r]], wi[r]);
+                if (au != nullptr) publish(&au[i[r]], aw[r]);
                 lds.tile[r][threadIdx.x] = static_cast<double>(acc[r]);
             }
         if (part != nullptr) store_partials_n<1, R>(part, tl, sq, lds);
@@ -562,9 +586,11 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
 // the prolongation then read as the pre-smoothed iterate. Each row stores
 // w in `keep` and `keep2` (if set) and adds w^2 to its tile partial (if
 // `part` is set).
-template <class VT, bool kF = false>
+template <class VT, int kW = 0>
 __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>& V, int j, const Source<VT>& src,
-                                             VT* keep, VT* keep2, VT* pre, double* part, int g, int G, Lds& lds) {
+                                             VT* keep, VT* keep2, VT* pre, double* part, int g, int G, Lds& lds,
+                                             VT* au = nullptr) {
+    constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
     const VT om = static_cast<VT>(I.omega[j]);
@@ -572,10 +598,10 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
     const VT* dinv = V.dinv(j);
     const VT* off = V.off(j);
     const int tiles = (n + kTile - 1) / kTile;
-    tile_loop<sizeof(VT) == 4>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+    tile_loop<sizeof(VT) == 4 || kW == 2>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
         int i[R];
         bool ok[R];
-        VT wi[R], acc[R];
+        VT wi[R], acc[R], aw[R];
         double sq[R][1];
 #pragma unroll
         for (int r = 0; r < R; ++r) i[r] = tile_row(tl[r], n, ok[r]);
@@ -585,10 +611,25 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
         for (int r = 0; r < R; ++r) {
             sq[r][0] = ok[r] ? static_cast<double>(wi[r]) * static_cast<double>(wi[r]) : 0.0;
             acc[r] = wi[r] - diag[i[r]] * (om * dinv[i[r]] * wi[r]);
+            aw[r] = diag[i[r]] * wi[r];
         }
-        ell_rows<R>(
-            L.col, off, n, W, i, [&](int, int, int c) { return om * dinv[c] * src.template at<kF>(c); },
-            [&](int r, VT o, VT v) { acc[r] -= o * v; });
+        if (au != nullptr) {
+            // (A w)_i beside the residual, from the same gathers
+            ell_rows<R>(
+                L.col, off, n, W, i,
+                [&](int, int, int c) {
+                    const VT wc = src.template at<kF>(c);
+                    return Pair<VT>{wc, om * dinv[c] * wc};
+                },
+                [&](int r, VT o, Pair<VT> v) {
+                    acc[r] -= o * v.second;
+                    aw[r] += o * v.first;
+                });
+        } else {
+            ell_rows<R>(
+                L.col, off, n, W, i, [&](int, int, int c) { return om * dinv[c] * src.template at<kF>(c); },
+                [&](int r, VT o, VT v) { acc[r] -= o * v; });
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (ok[r]) {
@@ -596,6 +637,7 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
                 publish(&pre[i[r]], static_cast<VT>(x1 + om * dinv[i[r]] * acc[r]));
                 if (keep != nullptr) publish(&keep[i[r]], wi[r]);
                 if (keep2 != nullptr) publish(&keep2[i[r]], wi[r]);
+                if (au != nullptr) publish(&au[i[r]], aw[r]);
             }
         if (part != nullptr) store_partials_n<1, R>(part, tl, sq, lds);
     });
@@ -651,10 +693,11 @@ __device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs
 // j. The correction comes from `corr`, or, on the level above the coarsest,
 // from the coarse solve each workgroup recomputes into LDS. The pre-smoothed
 // iterate is om D^-1 rhs, or the two-sweep iterate in `pre` when set.
-template <class VT, bool kF = false>
+template <class VT, int kW = 0>
 __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V, int j, const VT* rhs, const VT* corr, VT* out, int g,
                          int G, Lds& lds, double* dot_part = nullptr, const VT* dot_with = nullptr,
                          const VT* pre = nullptr) {
+    constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
     const VT om = static_cast<VT>(I.omega[j]);
@@ -694,7 +737,7 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
         }
         return;
     }
-    tile_loop<sizeof(VT) == 4>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+    tile_loop<sizeof(VT) == 4 || kW == 2>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
         int i[R];
         bool ok[R];
         VT x2[R], acc[R];
@@ -819,9 +862,11 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
 // evaluated from `src`, stored into `keep` and `keep2`, squares into
 // `part`); on a one-level hierarchy the input is only materialised. Ends
 // with a barrier.
-template <class VT, bool kF = false>
+template <class VT, int kW = 0>
 __device__ __forceinline__ void vcycle_first(const Instance& I, const Values<VT>& V, const Source<VT>& src, VT* keep, double* part,
-                             int g, int G, Lds& lds, int& timeouts, int mask = 0, VT* keep2 = nullptr) {
+                             int g, int G, Lds& lds, int& timeouts, int mask = 0, VT* keep2 = nullptr,
+                             VT* au = nullptr) {
+    constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
     if (I.depth == 1) {
         for (int tile = g; tile < I.tiles; tile += G) {
             const int i = tile * kTile + threadIdx.x;
@@ -835,9 +880,9 @@ __device__ __forceinline__ void vcycle_first(const Instance& I, const Values<VT>
             if (part != nullptr) store_partials<1>(part, tile, sq, lds);
         }
     } else if (mask & 1) {
-        jacobi_phase<VT, kF>(I, V, 0, src, keep, keep2, static_cast<VT*>(I.lvl_x[0]), part, g, G, lds);
+        jacobi_phase<VT, kW>(I, V, 0, src, keep, keep2, static_cast<VT*>(I.lvl_x[0]), part, g, G, lds, au);
     } else {
-        down_phase<VT, kF>(I, V, 0, src, keep, static_cast<VT*>(I.lvl_b[1]), part, g, G, lds, nullptr, keep2);
+        down_phase<VT, kW>(I, V, 0, src, keep, static_cast<VT*>(I.lvl_b[1]), part, g, G, lds, nullptr, keep2, au);
     }
     group_sync<kF>(timeouts);
 }
@@ -878,17 +923,17 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
     if (mask & 1) {
         Source<VT> src = none;
         src.a = w0;
-        down_phase<VT, kF>(I, V, 0, src, nullptr, static_cast<VT*>(I.lvl_b[1]), nullptr, g, G, lds, pre_of(0));
+        down_phase<VT, kW>(I, V, 0, src, nullptr, static_cast<VT*>(I.lvl_b[1]), nullptr, g, G, lds, pre_of(0));
         group_sync<kF>(timeouts);
     }
     for (int j = 1; j + 1 < J; ++j) {
         Source<VT> src = none;
         src.a = static_cast<const VT*>(I.lvl_b[j]);
         if ((mask >> j) & 1) {
-            jacobi_phase<VT, kF>(I, V, j, src, nullptr, nullptr, pre_of(j), nullptr, g, G, lds);
+            jacobi_phase<VT, kW>(I, V, j, src, nullptr, nullptr, pre_of(j), nullptr, g, G, lds);
             group_sync<kF>(timeouts);
         }
-        down_phase<VT, kF>(I, V, j, src, nullptr, static_cast<VT*>(I.lvl_b[j + 1]), nullptr, g, G, lds, pre_of(j));
+        down_phase<VT, kW>(I, V, j, src, nullptr, static_cast<VT*>(I.lvl_b[j + 1]), nullptr, g, G, lds, pre_of(j));
         group_sync<kF>(timeouts);
     }
     // with about a workgroup per tile, the last two prolongations are one
@@ -904,7 +949,7 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
             merged_up_phase<VT, kW>(&I, j, rhs, out, g, G, lds, j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr,
                                     mask);
         else
-            up_phase<VT, kF>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds,
+            up_phase<VT, kW>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds,
                              j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr, pre_of(j));
         group_sync<kF>(timeouts);
     }
@@ -1387,6 +1432,8 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     int iterations = 0, restarts = 0, v_cycles = 0;
     int status = 0;
     const int mask = pre_sweep_mask(I);   // levels with two pre-smoothing sweeps
+    // heat steps solve the assembled heat matrix, which is symmetric
+    const bool symmetric = (P.op & kOpHeatRhs) != 0;
     bool shadow_stale = false;    // r0 must be set from r at the next pass's start
     bool x_zero = true;           // x is known to be 0 (no x update since the start)
     const double target = P.tolerance * b_norm;
@@ -1453,14 +1500,28 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 if (!first_iteration) src = Source<VT>{r, static_cast<const VT*>(I.p[pi]), v, beta, omega, 1};
                 VT* p_new = static_cast<VT*>(I.p[pi ^ 1]);
                 pi ^= 1;
-                vcycle_first<VT, kF>(I, V, src, p_new, nullptr, g, G, lds, timeouts, mask,
-                                     first_iteration && shadow_stale ? r0 : nullptr);
+                // A symmetric (heat systems): (r0, A ph) = (A r0, ph), with A r0
+                // from the first sweep's gathers and the product from the
+                // V-cycle's last sweep, so alpha is known when the V-cycle
+                // ends and the SpMV joins the true-residual phase below. FP64
+                // inner vectors only: A r0 (about |C A| |b|) overflows FP32.
+                const bool fuse = sizeof(VT) == 8 && first_iteration && symmetric && I.depth > 1;
+                VT* u = static_cast<VT*>(I.u);
+                vcycle_first<VT, kW>(I, V, src, p_new, nullptr, g, G, lds, timeouts, mask,
+                                     first_iteration && shadow_stale ? r0 : nullptr, fuse ? u : nullptr);
                 shadow_stale = false;
-                vcycle_rest<VT, kW>(I, V, p_new, ph, g, G, lds, timeouts, nullptr, nullptr, mask);
+                double* dot = fuse ? next_partials() : nullptr;
+                vcycle_rest<VT, kW>(I, V, p_new, ph, g, G, lds, timeouts, dot, fuse ? u : nullptr, mask);
                 ++v_cycles;
+                if (fuse) {
+                    double uph[1];
+                    group_totals<1, kF>(dot, tiles, uph, lds);
+                    if (uph[0] == 0.0) break;
+                    alpha = rho / uph[0];
+                } else {
                 // v = A ph, (r0, v)
                 part = next_partials();
-                tile_loop<sizeof(VT) == 4>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+                tile_loop<sizeof(VT) == 4 || kW == 2>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
                     int i[R];
                     bool ok[R];
                     VT acc[R], rz[R];
@@ -1487,6 +1548,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 group_totals<1, kF>(part, tiles, r0v, lds);
                 if (r0v[0] == 0.0) break;
                 alpha = rho / r0v[0];
+                }
                 bool alpha_in_x = false;   // x already holds x + alpha ph
                 if (first_iteration) {
                     // r is the true residual of x here, so s = r - alpha v is
@@ -1500,21 +1562,26 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         const int i = tile * kTile + threadIdx.x;
                         double q[KQ] = {};
                         if (i < n) {
-                            const double x_new =
-                                (x_zero ? 0.0 : fetch<kF>(&x[i])) + alpha * static_cast<double>(fetch<kF>(&ph[i]));
+                            const double ph_i = static_cast<double>(fetch<kF>(&ph[i]));
+                            const double x_new = (x_zero ? 0.0 : fetch<kF>(&x[i])) + alpha * ph_i;
                             publish(&xn[i], x_new);
                             if (kPrep && next_forcing != nullptr) prepare_row(i, x_new, q);
                             double ax = I.diag64[0][i] * x_new;
+                            double av = I.diag64[0][i] * ph_i;   // (A ph)_i when the SpMV phase was skipped
                             ell_row(
                                 L0.col, I.off64[0], n, W, i,
                                 [&](int, int c) {
-                                    return (x_zero ? 0.0 : fetch<kF>(&x[c])) + alpha * static_cast<double>(fetch<kF>(&ph[c]));
+                                    return Pair<double>{x_zero ? 0.0 : fetch<kF>(&x[c]), static_cast<double>(fetch<kF>(&ph[c]))};
                                 },
-                                [&](double o, double w) { ax += o * w; });
+                                [&](double o, Pair<double> w) {
+                                    ax += o * (w.first + alpha * w.second);
+                                    av += o * w.second;
+                                });
                             const double si = fetch<kF>(&b[i]) - ax;
                             const VT sv = static_cast<VT>(si);
                             publish(&s[i], sv);
                             publish(&r[i], sv);
+                            if (fuse) publish(&v[i], static_cast<VT>(av));
                             q[0] = si * si;
                             q[1] = static_cast<double>(sv) * static_cast<double>(sv);
                         }
@@ -1524,6 +1591,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     group_sync<kF>(timeouts);
                     xi ^= 1;
                     x_zero = false;
+                    zero_x = false;   // x[xi] now holds x + alpha ph
                     group_totals<2, kF>(part, tiles, tot2, lds);
                     ++iterations;
                     if (sqrt(tot2[0]) <= inner_target) {
@@ -1536,12 +1604,12 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         break;
                     }
                     alpha_in_x = true;
-                    vcycle_first<VT, kF>(I, V, Source<VT>{s, nullptr, nullptr, 0.0, 0.0, 0}, nullptr, nullptr, g, G, lds,
+                    vcycle_first<VT, kW>(I, V, Source<VT>{s, nullptr, nullptr, 0.0, 0.0, 0}, nullptr, nullptr, g, G, lds,
                                          timeouts, mask);
                 } else {
                     // s = r - alpha v inside the next V-cycle's first sweep, (s, s)
                     part = next_partials();
-                    vcycle_first<VT, kF>(I, V, Source<VT>{r, v, nullptr, alpha, 0.0, 2}, s, part, g, G, lds, timeouts,
+                    vcycle_first<VT, kW>(I, V, Source<VT>{r, v, nullptr, alpha, 0.0, 2}, s, part, g, G, lds, timeouts,
                                          mask);
                     double ss[1];
                     group_totals<1, kF>(part, first_sweep_tiles(I, mask), ss, lds);
@@ -1587,7 +1655,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 ++v_cycles;
                 // t = A sh, (t, t), (t, s)
                 part = next_partials();
-                tile_loop<sizeof(VT) == 4>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
+                tile_loop<sizeof(VT) == 4 || kW == 2>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
                     int i[R];
                     bool ok[R];
                     VT acc[R], sv[R];
@@ -1656,6 +1724,17 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 if (sqrt(rr[0]) <= inner_target) break;
             }
         }
+    }
+    if (zero_x) {
+        // a prepared step that broke down before any x update: x still
+        // holds the previous step's T
+        double* x = I.x[xi];
+        for (int tile = g; tile < tiles; tile += G) {
+            const int i = tile * kTile + threadIdx.x;
+            if (i < n) publish(&x[i], 0.0);
+        }
+        group_sync<kF>(timeouts);
+        zero_x = false;
     }
     if (t_new != nullptr) {
         const double* x = I.x[xi];
@@ -1737,7 +1816,7 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
     // z = M r with (r, z)
     const auto precondition = [&]() {
         const Source<double> src{r, nullptr, nullptr, 0.0, 0.0, 0};
-        vcycle_first<double, kF>(I, V, src, kept, nullptr, g, G, lds, timeouts);
+        vcycle_first<double, kW>(I, V, src, kept, nullptr, g, G, lds, timeouts);
         double* dot = next_partials();
         vcycle_rest<double, kW>(I, V, r, z, g, G, lds, timeouts, dot, r);
         double rz[1];
@@ -1981,6 +2060,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
 }
 
 template __global__ void program_kernel_t<3>(const Instance*, const int32_t*, const Launch*, Launch);
+template __global__ void program_kernel_t<2>(const Instance*, const int32_t*, const Launch*, Launch);
 template __global__ void program_kernel_t<1>(const Instance*, const int32_t*, const Launch*, Launch);
 
 // ------------------------------------------------------------ the server
@@ -2019,6 +2099,7 @@ __device__ int64_t claim_request(const ServeParams& S, const ServeQueue& Q, long
 
 // One request on the slot's group, out of line so that the server loop's
 // own state does not add to the solver's register pressure.
+template <int kW>
 __device__ __noinline__ int serve_one(const Instance* instance, const Launch& P, int g, int G, Lds& lds,
                                       int timeouts) {
     const Instance& I = *uniform_ptr(instance);
@@ -2026,14 +2107,14 @@ __device__ __noinline__ int serve_one(const Instance* instance, const Launch& P,
     G = __builtin_amdgcn_readfirstlane(G);
     int parity = 0;
     if (P.op & kOpSetMatrix) {
-        const SyncState st = set_matrix_call<3>(I, P, g, G, lds, SyncState{timeouts, parity});
+        const SyncState st = set_matrix_call<kW>(I, P, g, G, lds, SyncState{timeouts, parity});
         timeouts = st.timeouts;
         parity = st.parity;
     }
     if (I.precision == 0)
-        run_solve<double, 3>(I, P, g, G, lds, timeouts, parity);
+        run_solve<double, kW>(I, P, g, G, lds, timeouts, parity);
     else
-        run_solve<float, 3>(I, P, g, G, lds, timeouts, parity);
+        run_solve<float, kW>(I, P, g, G, lds, timeouts, parity);
     // every workgroup's rows of T are written before the completion
     group_sync(timeouts);
     return timeouts;
@@ -2046,7 +2127,8 @@ __device__ __noinline__ int serve_one(const Instance* instance, const Launch& P,
 // and reductions over the slot's workgroups), so results do not depend on
 // what else is being served. Every wait is bounded by the host's stop flag
 // or the launch's lifetime.
-__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void server_kernel(ServeParams S) {
+template <int kWaves>
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves))) void server_kernel(ServeParams S) {
     __shared__ Lds lds;
     __shared__ Launch params;
     __shared__ int32_t state;
@@ -2119,7 +2201,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
         }
         __syncthreads();
         if (claimed < 0) break;
-        timeouts = serve_one(S.table + state, params, g, G, lds, timeouts);
+        timeouts = serve_one<kWaves>(S.table + state, params, g, G, lds, timeouts);
         const Launch& P = params;
         if (g == 0 && threadIdx.x == 0) {
             const SolveResult& res = *slot.result;
@@ -2146,26 +2228,37 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(3))) void
     }
 }
 
-}  // namespace
-
-hipError_t launch_server(const ServeParams& params, int32_t blocks, hipStream_t stream) {
-    ServeParams copy = params;
-    void* args[] = {&copy};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&server_kernel), dim3(blocks), dim3(kTile), args,
-                                      0, stream);
+// The kernels of one occupancy: 1, 2 or 3 workgroups per CU.
+const void* program_kernel(int waves) {
+    if (waves == 1) return reinterpret_cast<const void*>(&program_kernel_t<1>);
+    if (waves == 2) return reinterpret_cast<const void*>(&program_kernel_t<2>);
+    return reinterpret_cast<const void*>(&program_kernel_t<3>);
 }
 
-int server_blocks_per_cu() {
+const void* server_kernel_of(int waves) {
+    if (waves == 2) return reinterpret_cast<const void*>(&server_kernel<2>);
+    return reinterpret_cast<const void*>(&server_kernel<3>);
+}
+
+}  // namespace
+
+hipError_t launch_server(const ServeParams& params, int32_t blocks, hipStream_t stream, int waves) {
+    ServeParams copy = params;
+    void* args[] = {&copy};
+    return hipLaunchCooperativeKernel(server_kernel_of(waves), dim3(blocks), dim3(kTile), args, 0, stream);
+}
+
+int server_blocks_per_cu(int waves) {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, server_kernel, kTile, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, server_kernel_of(waves), kTile, 0) != hipSuccess)
+        return 0;
     return blocks;
 }
 
 hipError_t launch_program(const Instance* instances, const int32_t* block_instance, const Launch* launches,
                           const Launch& single, int32_t blocks, size_t dynamic_lds, hipStream_t stream,
-                          bool cooperative, bool wide) {
-    const void* kernel = wide ? reinterpret_cast<const void*>(&program_kernel_t<1>)
-                              : reinterpret_cast<const void*>(&program_kernel_t<3>);
+                          bool cooperative, int waves) {
+    const void* kernel = program_kernel(waves);
     if (dynamic_lds > 0) {
         const hipError_t e =
             hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamic_lds));
@@ -2182,24 +2275,18 @@ hipError_t launch_program(const Instance* instances, const int32_t* block_instan
 
 size_t program_static_lds() {
     hipFuncAttributes attributes{};
-    if (hipFuncGetAttributes(&attributes, reinterpret_cast<const void*>(&program_kernel_t<3>)) != hipSuccess)
-        return sizeof(Lds) + sizeof(Launch) + 256;
+    if (hipFuncGetAttributes(&attributes, program_kernel(3)) != hipSuccess) return sizeof(Lds) + sizeof(Launch) + 256;
     return attributes.sharedSizeBytes;
 }
 
-int program_blocks_per_cu(size_t dynamic_lds, bool wide) {
-    const void* kernel = wide ? reinterpret_cast<const void*>(&program_kernel_t<1>)
-                              : reinterpret_cast<const void*>(&program_kernel_t<3>);
+int program_blocks_per_cu(size_t dynamic_lds, int waves) {
+    const void* kernel = program_kernel(waves);
     int blocks = 0;
     if (dynamic_lds > 0 &&
         hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamic_lds)) !=
             hipSuccess)
         return 0;
-    const hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, program_kernel_t<1>, kTile,
-                                                                              dynamic_lds)
-                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, program_kernel_t<3>, kTile,
-                                                                              dynamic_lds);
-    return e == hipSuccess ? blocks : 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kTile, dynamic_lds) == hipSuccess ? blocks : 0;
 }
 
 }  // namespace psim::dev
