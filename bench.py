@@ -537,9 +537,15 @@ def served_section(P, ctx, meshes, graphs, plain_value, sweep=SWEEP):
     plan = server.plan()
     out["config"]["slots"] = {str(L): [p["slots"], p["slot_size"], p["pool"]] for L, p in zip(levels, plan)}
     server.close()
+    # L6 only, all requests at t = 0: a cold run (every session assembles its
+    # matrix) and the same stream again on the same server (the matrices of
+    # the bank stay by key, as a running server keeps them); pool sized for
+    # 32 GiB of HBM
     for name, precision in (("l6_only_saturated", P.MIXED), ("l6_only_saturated_plain_fp64", P.PLAIN_FP64)):
-        l6 = P.Server(ctx, [graphs[6]], precision=precision, memory_budget=8 << 30, slot_sizes=[4])
-        out[name] = run(l6, [6], [1.0], float("inf"), 4000, 99)
+        l6 = P.Server(ctx, [graphs[6]], precision=precision, memory_budget=32 << 30, slot_sizes=[4])
+        cold = run(l6, [6], [1.0], float("inf"), 6000, 99)
+        out[name] = run(l6, [6], [1.0], float("inf"), 6000, 99)
+        out[name]["cold_start_solves_per_s"] = cold["solves_per_s"]
         l6.close()
     out["l6_served_mixed_vs_plain_fp64_chain"] = round(out["l6_only_saturated"]["solves_per_s"] / plain_value, 2)
     out["l6_served_plain_vs_plain_fp64_chain"] = round(

@@ -576,7 +576,7 @@ int psim_server_create(psim_ctx* ctx, psim_graph* const* graphs, const double* r
                        const int32_t* slot_sizes, int32_t graph_count, int precision, uint64_t memory_budget_bytes,
                        psim_server** out);
 /* The same with the server program's occupancy: 2 or 3 workgroups per CU
- * (register budgets of 256 or 168 VGPRs; 0 = 2, the default of
+ * (register budgets of 256 or 168 VGPRs; 0 = 3, the default of
  * psim_server_create). */
 int psim_server_create_occupancy(psim_ctx* ctx, psim_graph* const* graphs, const double* request_share,
                                  const int32_t* slot_sizes, int32_t graph_count, int precision,

@@ -54,7 +54,7 @@ template <class T> T* pinned(size_t count) {
 struct psim_server {
     psim_ctx* ctx = nullptr;
     int precision = PSIM_MIXED;
-    int waves = 2;                     // server program instance: workgroups per CU
+    int waves = 3;                     // server program instance: workgroups per CU
     std::vector<psim_graph*> graphs;
     std::vector<int32_t> slot_count, slot_size, pool_size;
     std::vector<size_t> footprint;
@@ -147,7 +147,7 @@ int psim_server_create_occupancy(psim_ctx* ctx, psim_graph* const* graphs, const
                                  const int32_t* slot_sizes, int32_t graph_count, int precision, uint64_t memory_budget,
                                  int32_t workgroups_per_cu, psim_server** out) {
     return guarded([&] {
-        if (workgroups_per_cu == 0) workgroups_per_cu = 2;
+        if (workgroups_per_cu == 0) workgroups_per_cu = 3;
         if (workgroups_per_cu != 2 && workgroups_per_cu != 3)
             throw std::invalid_argument("occupancy must be 2 or 3 workgroups per CU");
         if (ctx == nullptr || graphs == nullptr || out == nullptr) throw std::invalid_argument("null argument");
