@@ -26,8 +26,6 @@ restatement of the reference algorithm) on a bounded sample on this host.
 from __future__ import annotations
 
 import argparse
-import contextlib
-import ctypes
 import json
 import os
 import platform
@@ -47,45 +45,18 @@ CHAIN = 120
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "program_kernel_l6_chain_traffic.json")
+BATCH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "program_kernel_batch256_mixed_traffic.json")
 COPY_BYTES = 256 << 20
 
 
-class Markers:
-    """roctx ranges around each kind of launch, so that a profile taken with
-    `rocprofv3 --kernel-rename` reports the timed passes apart from the
-    census, latency and one-launch-per-solve launches. No-op without roctx."""
-
-    def __init__(self):
-        self.lib = None
-        for name in ("librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1"):
-            try:
-                self.lib = ctypes.CDLL(name)
-                break
-            except OSError:
-                continue
-        if self.lib is not None:
-            self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
-
-    @contextlib.contextmanager
-    def range(self, name):
-        if self.lib is not None:
-            self.lib.roctxRangePushA(name.encode())
-        try:
-            yield
-        finally:
-            if self.lib is not None:
-                self.lib.roctxRangePop()
 
 
-MARKS = Markers()
-
-
-def measured_traffic():
-    """HBM bytes per chained-pass launch from the committed counter profile
-    (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration),
-    or None when that profile is absent."""
+def measured_traffic(path=TRAFFIC_FILE):
+    """HBM bytes per launch from a committed counter profile (FETCH_SIZE and
+    WRITE_SIZE, each scaled by the copy-kernel calibration), or None when
+    that profile is absent."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             return json.load(f)
     except (OSError, ValueError):
         return None
@@ -253,12 +224,11 @@ def fv_section(P, ctx, mesh, with_cpu, cpu_workers, repeats=50):
         call()
         ctx.events(repeats + 1)
         ctx.synchronize()
-        with MARKS.range(f"fv_{name}_timed"):
-            ctx.record(0)
-            for k in range(repeats):
-                call()
-                ctx.record(k + 1)
-            ctx.synchronize()
+        ctx.record(0)
+        for k in range(repeats):
+            call()
+            ctx.record(k + 1)
+        ctx.synchronize()
         ms = np.array([ctx.elapsed_ms(k, k + 1) for k in range(repeats)])
         timings[name] = float(ms.mean())
     g_bytes, d_bytes = fv_bytes(fields, n, e, mesh.directed)
@@ -361,7 +331,7 @@ def mixed_section(P, ctx, mesh, graph, levels, with_oracle):
     return out
 
 
-def batched_section(P, ctx, meshes, graphs, hierarchy_levels, precision=None):
+def batched_section(P, ctx, meshes, graphs, hierarchy_levels, precision=None, autotune=True):
     """256 independent instances, L3:L4:L5:L6 = 64 each, own seeds, matrix
     assembled in the same launch; default allocation and the autotuner's.
     Mixed precision (config 3) unless another precision is given."""
@@ -383,17 +353,19 @@ def batched_section(P, ctx, meshes, graphs, hierarchy_levels, precision=None):
         infos = batch.heat_step(*args, DT, TOL, 1000, **kw)
         ctx.events(2)
         ctx.synchronize()
-        with MARKS.range("batched_launch_timed"):
-            ctx.record(0)
-            for _ in range(reps):
-                batch.heat_step(*args, DT, TOL, 1000, collect=False, **kw)
-            ctx.record(1)
-            batch.synchronize()
+        ctx.record(0)
+        for _ in range(reps):
+            batch.heat_step(*args, DT, TOL, 1000, collect=False, **kw)
+        ctx.record(1)
+        batch.synchronize()
         return ctx.elapsed_ms(0, 1) / reps, infos
 
     ms_default, infos = timed()
-    tuned = batch.autotune(*args, DT, TOL, 1000, repeats=2, **kw)
-    ms_tuned, infos = timed()
+    if autotune:
+        batch.autotune(*args, DT, TOL, 1000, repeats=2, **kw)
+        ms_tuned, infos = timed()
+    else:
+        ms_tuned = ms_default
     nbytes = 0
     for x, info in zip(members, infos):
         lv = hierarchy_levels[x["level"]]
@@ -404,7 +376,7 @@ def batched_section(P, ctx, meshes, graphs, hierarchy_levels, precision=None):
     by_level = {}
     for x, gsize in zip(members, groups):
         by_level.setdefault(str(x["level"]), []).append(gsize)
-    return {
+    out = {
         "config": {"workload": "256 instances, L3/L4/L5/L6 x 64, " +
                                ("mixed precision" if precision == P.MIXED else "plain FP64") +
                                ", set_matrix + 1 solve each, one launch", "tolerance": TOL},
@@ -417,6 +389,17 @@ def batched_section(P, ctx, meshes, graphs, hierarchy_levels, precision=None):
         "converged": all(i.status == 0 and i.relative_residual <= TOL for i in infos),
         "launch_GBps": round(nbytes / (best / 1e3) / 1e9, 2),
     }
+    traffic = measured_traffic(BATCH_TRAFFIC_FILE) if precision == P.MIXED else None
+    achieved = nbytes / (ms_default / 1e3) / 1e9
+    out["roofline"] = {
+        "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5), "algorithmic_bytes_per_launch": int(nbytes),
+        "mean_launch_ms": round(ms_default, 5), "allocation": "default",
+        "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
+        "traffic_GBps": round(traffic["hbm_bytes_per_launch"] / (ms_default / 1e3) / 1e9, 1) if traffic else None,
+        "traffic_source": os.path.relpath(BATCH_TRAFFIC_FILE, ROOT) if traffic else None,
+    }
+    return out
 
 
 class OutputRing:
@@ -486,7 +469,7 @@ def serve_requests(P, bank, levels, shares, rate, count, rng, ring):
 SWEEP = ((100, 1000), (200, 2000), (500, 5000), (1000, 10000), (2000, 20000), (5000, 20000))
 
 
-def served_section(P, ctx, meshes, graphs, plain_value, sweep=SWEEP):
+def served_section(P, ctx, meshes, graphs, plain_value, sweep=SWEEP, profile=False):
     """Served stream: Poisson arrivals of single-step requests, levels
     L3:L4:L5:L6 = 40:30:20:10, half continuing a resident 12-step session,
     mixed precision, one persistent server launch per run; each offered rate
@@ -507,8 +490,7 @@ def served_section(P, ctx, meshes, graphs, plain_value, sweep=SWEEP):
 
     def run(server, lv, sh, rate, count, seed):
         reqs = serve_requests(P, bank, lv, sh, rate, count, np.random.default_rng(seed), ring)
-        with MARKS.range("server_run"):
-            rec = server.run(reqs, DT, TOL, 1000)
+        rec = server.run(reqs, DT, TOL, 1000)
         lat = np.array([(r.completed_s - r.arrival_s) * 1e3 for r in rec])
         queue = np.array([(r.submitted_s - r.arrival_s) * 1e3 for r in rec])
         device = np.array([r.device_ms for r in rec])
@@ -532,6 +514,13 @@ def served_section(P, ctx, meshes, graphs, plain_value, sweep=SWEEP):
                                   "resident 12-step session, mixed precision, persistent server, matrices kept "
                                   "by key (8 per level)", "tolerance": TOL}}
     server = P.Server(ctx, [graphs[L] for L in levels], share=shares, precision=P.MIXED, memory_budget=8 << 30)
+    if profile:
+        # two server launches of the same all-at-once stream: cold, then with
+        # the bank's matrices kept
+        out["cold"] = run(server, levels, shares, float("inf"), 4000, 17)
+        out["warm"] = run(server, levels, shares, float("inf"), 4000, 17)
+        server.close()
+        return out
     out["sweep"] = [run(server, levels, shares, rate, count, 11 + i) for i, (rate, count) in enumerate(sweep)]
     out["all_at_once"] = run(server, levels, shares, float("inf"), 10000, 17)
     plan = server.plan()
@@ -607,10 +596,9 @@ def transport_section(P, ctx, meshes, with_oracle, repeats=5):
         res = P.implicit_transport_table_device(graph, K, d_f, d_h, d_t, 6, d_s, d_m)
         ctx.synchronize()
         start = time.perf_counter()
-        with MARKS.range(f"transport_{name}"):
-            for _ in range(repeats):
-                res = P.implicit_transport_table_device(graph, K, d_f, d_h, d_t, 6, d_s, d_m)
-            ctx.synchronize()
+        for _ in range(repeats):
+            res = P.implicit_transport_table_device(graph, K, d_f, d_h, d_t, 6, d_s, d_m)
+        ctx.synchronize()
         device_ms = (time.perf_counter() - start) / repeats * 1e3
         fn = host_table_response(h, t)
         start = time.perf_counter()
@@ -827,9 +815,9 @@ def main():
     ap.add_argument("--workgroups", type=int, default=0, help="override the solver's workgroup count")
     ap.add_argument("--only-consumers", action="store_true",
                     help="run only the consumer measurements (tracer, ILU, advection, operator solves)")
-    ap.add_argument("--only", choices=["fv", "batched", "served"],
-                    help="run only one section (profiling runs): the 64-field FV sweeps, the 256-instance "
-                         "batch, or a short served run")
+    ap.add_argument("--only", choices=["chain", "fv", "batched", "served"],
+                    help="run only one section (profiling runs): the headline chain's census, warm-up and timed "
+                         "passes, the 64-field FV sweeps, the 256-instance batch, or a short served run")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed-precision, batched and served measurements")
     args = ap.parse_args()
@@ -846,7 +834,7 @@ def main():
         meshes = {L: P.Mesh(L) for L in (5, 6)}
         print(json.dumps({"transport_consumers": transport_rows_section(P, ctx, meshes, not args.no_cpu)}))
         return
-    if args.only:
+    if args.only and args.only != "chain":
         ctx = P.Context(0)
         mesh = P.Mesh(level)
         # one 256 MiB streaming copy first: the calibration kernel of the
@@ -861,9 +849,9 @@ def main():
         graphs = {L: P.Graph(ctx, meshes[L]) for L in meshes}
         hierarchy_levels = {L: [(lv["size"], lv["nnz"]) for lv in P.Hierarchy(meshes[L]).levels] for L in meshes}
         if args.only == "batched":
-            print(json.dumps({"batched": batched_section(P, ctx, meshes, graphs, hierarchy_levels)}))
+            print(json.dumps({"batched": batched_section(P, ctx, meshes, graphs, hierarchy_levels, autotune=False)}))
         else:
-            print(json.dumps({"served": served_section(P, ctx, meshes, graphs, 1.0, sweep=((2000, 4000),))}))
+            print(json.dumps({"served": served_section(P, ctx, meshes, graphs, 1.0, profile=True)}))
         return
     mesh = P.Mesh(level)
     seed = P.instance_seed(0)
@@ -886,8 +874,6 @@ def main():
     d_out = P.DeviceArray(ctx, n)
     infos = (P.SolveInfo * CHAIN)()
 
-    marks = MARKS
-
     def run_pass(collect=True, tol=TOL):
         """One launch: set the matrix, then 120 chained steps T_k -> T_{k+1}."""
         system.heat_chain_device(d_t0.ptr, d_forcing.ptr, CHAIN, d_out.ptr, DT, tol, 1000,
@@ -896,43 +882,46 @@ def main():
         return [(i.iterations, i.v_cycles, i.restarts, i.relative_residual, i.status, i.device_ms)
                 for i in infos] if collect else None
 
-    with marks.range("chain_pass_census"):
-        census = run_pass()
+    census = run_pass()
     for step, (_, _, _, res, status, _) in enumerate(census):
         if status != 0 or not res <= TOL:
             raise SystemExit(f"step {step} did not converge: status {status}, residual {res:.3e}")
     gpu_final = d_out.download()
 
-    with marks.range("chain_pass_warmup"):
-        for _ in range(args.warmup):
-            run_pass(collect=False)
-        ctx.synchronize()
+    for _ in range(args.warmup):
+        run_pass(collect=False)
+    ctx.synchronize()
 
     ctx.events(2 * args.steps)
     ctx.synchronize()
-    with marks.range("chain_pass_timed"):
-        t_start = time.perf_counter()
-        for k in range(args.steps):
-            ctx.record(2 * k)
-            run_pass(collect=False)
-            ctx.record(2 * k + 1)
-        ctx.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ctx.record(2 * k)
+        run_pass(collect=False)
+        ctx.record(2 * k + 1)
+    ctx.synchronize()
     wall = time.perf_counter() - t_start
     launch_ms = np.array([ctx.elapsed_ms(2 * k, 2 * k + 1) for k in range(args.steps)])
     solves = CHAIN * args.steps
     value = solves / wall
+    if args.only == "chain":
+        # the counter runs' calibration kernel: one 256 MiB streaming copy
+        calibrate = P.DeviceArray(ctx, COPY_BYTES // 8), P.DeviceArray(ctx, COPY_BYTES // 8)
+        ctx.copy(calibrate[1].ptr, calibrate[0].ptr, COPY_BYTES)
+        ctx.synchronize()
+        print(json.dumps({"chain_passes": {"solves_per_s": round(value, 2),
+                                           "mean_launch_ms": round(float(launch_ms.mean()), 5)}}))
+        return
     # per-solve latencies: the same passes again, with per-step records
     step_ms = []
-    with marks.range("chain_pass_latency"):
-        for _ in range(args.steps):
-            records = run_pass(collect=True)
-            if any(r[4] != 0 or not r[3] <= TOL for r in records):
-                raise SystemExit("a timed step did not converge")
-            step_ms.extend(r[5] for r in records)
+    for _ in range(args.steps):
+        records = run_pass(collect=True)
+        if any(r[4] != 0 or not r[3] <= TOL for r in records):
+            raise SystemExit("a timed step did not converge")
+        step_ms.extend(r[5] for r in records)
     step_ms = np.array(step_ms)
     # the same chain converged to 1e-13, for the parity check against the oracle
-    with marks.range("chain_pass_parity"):
-        run_pass(collect=True, tol=1e-13)
+    run_pass(collect=True, tol=1e-13)
     tight_final = d_out.download()
 
     bytes_per_launch = set_matrix_bytes(levels, mesh.edges) + sum(
@@ -944,16 +933,15 @@ def main():
     d_t = [P.DeviceArray(ctx, n), P.DeviceArray(ctx, n)]
     ctx.events(CHAIN + 1)
     ctx.synchronize()
-    with marks.range("single_solve"):
-        t1 = time.perf_counter()
-        ctx.record(0)
-        src = d_t0.ptr
-        for step in range(CHAIN):
-            dst = d_t[step % 2]
-            system.heat_step_device(src, d_forcing.offset(step * n), dst.ptr, DT, TOL, 1000)
-            ctx.record(step + 1)
-            src = dst.ptr
-        ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.record(0)
+    src = d_t0.ptr
+    for step in range(CHAIN):
+        dst = d_t[step % 2]
+        system.heat_step_device(src, d_forcing.offset(step * n), dst.ptr, DT, TOL, 1000)
+        ctx.record(step + 1)
+        src = dst.ptr
+    ctx.synchronize()
     single_wall = time.perf_counter() - t1
     single_ms = np.array([ctx.elapsed_ms(s, s + 1) for s in range(CHAIN)])
     single_err = float(np.abs(d_t[(CHAIN - 1) % 2].download() - gpu_final).max())
@@ -972,12 +960,11 @@ def main():
     ctx.copy(copy_dst.ptr, copy_src.ptr, COPY_BYTES)   # first touch of both buffers
     ctx.events(2)
     ctx.synchronize()
-    with marks.range("hbm_copy_256MiB"):
-        ctx.record(0)
-        for _ in range(4):
-            ctx.copy(copy_dst.ptr, copy_src.ptr, COPY_BYTES)
-        ctx.record(1)
-        ctx.synchronize()
+    ctx.record(0)
+    for _ in range(4):
+        ctx.copy(copy_dst.ptr, copy_src.ptr, COPY_BYTES)
+    ctx.record(1)
+    ctx.synchronize()
     copy_gbs = 4 * 2 * COPY_BYTES / (ctx.elapsed_ms(0, 1) / 1e3) / 1e9
     traffic = measured_traffic()
 
