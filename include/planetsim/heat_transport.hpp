@@ -33,7 +33,11 @@ struct TransportGraph {
     [[nodiscard]] std::size_t size() const noexcept { return area_m2.size(); }
 
     // Device copy with its multigrid hierarchy structure, made on first
-    // device use and remade if the arrays are reallocated or resized.
+    // device use and remade if the arrays are reallocated or resized. The
+    // copy is keyed on the arrays' addresses and sizes only: values changed
+    // in place (element writes, assign() with the same size) are not seen
+    // until invalidate() is called. (The reference's TransportGraph is a plain
+    // struct with no device copy; this cache is the drop-in's own.)
     [[nodiscard]] psim_graph* device() const {
         const Key key{area_m2.data(), offset.data(), neighbour.data(), weight.data(), weight_sum.data(),
                       area_m2.size(), neighbour.size()};
@@ -48,6 +52,9 @@ struct TransportGraph {
         }
         return cache_->graph;
     }
+
+    // Drop the device copy: the next device() call uploads the arrays again.
+    void invalidate() const noexcept { cache_.reset(); }
 
   private:
     struct Key {

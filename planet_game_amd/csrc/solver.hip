@@ -222,10 +222,14 @@ __device__ inline void group_totals(const double* partial, int tiles, double (&o
     __syncthreads();
 }
 
+// Like group_totals, the partials are read with fetch<kF>: after a lean
+// barrier (kF) they must be coherent loads; set_matrix, the only caller,
+// meets at the fenced barrier and reads them with plain loads.
+template <bool kF>
 __device__ inline double group_max(const double* partial, int tiles, Lds& lds) {
     if (threadIdx.x < 64) {
         double m = 0.0;
-        for (int t = threadIdx.x; t < tiles; t += 64) m = fmax(m, partial[t * kPartials]);
+        for (int t = threadIdx.x; t < tiles; t += 64) m = fmax(m, fetch<kF>(&partial[t * kPartials]));
         m = wave_max(m);
         if (threadIdx.x == 0) lds.bcast[0] = m;
     }
@@ -1266,7 +1270,7 @@ __device__ __forceinline__ void set_matrix(const Instance& I, const Launch& P, i
             }
         }
         group_sync(timeouts);
-        const double rho = group_max(part, tiles, lds);
+        const double rho = group_max<false>(part, tiles, lds);   // after the fenced group_sync above
         // Chebyshev degree-1 weight on [a, b] = [max(1 - rho, (1 + rho) / 4), 1 + rho]
         const double hi = 1.0 + rho;
         const double lo = fmax(1.0 - rho, 0.25 * hi);
