@@ -440,7 +440,12 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
             }
         }
         const int32_t cap = std::max(1, ctx->compute_units * per_cu);
-        s->workgroups = std::min(tiles, cap);
+        // one workgroup per tile of the widest level-0 sweep: the restriction
+        // sweep packs whole aggregates into its tiles, so it has a few more
+        // tiles than the natural 256-row split, and a workgroup with two of
+        // them would hold up the phase
+        const int32_t widest = g->depth > 1 ? std::max(tiles, g->views[0].down_tiles) : tiles;
+        s->workgroups = std::min(widest, cap);
         Instance& I = s->host;
         I.precision = precision == PSIM_MIXED ? 1 : 0;
         I.depth = g->depth;

@@ -33,6 +33,8 @@
 //     restarts from x (the mixed-precision form relies on this refinement).
 #include "device.hpp"
 
+#include <type_traits>
+
 namespace psim::dev {
 namespace {
 
@@ -49,6 +51,22 @@ template <class T> __device__ inline T* uniform_ptr(T* p) {
     return const_cast<T*>(uniform_ptr(static_cast<const T*>(p)));
 }
 
+// Every array the program touches lives in device global memory. Pointers
+// that come out of a descriptor are generic (flat) pointers to the
+// compiler, and flat accesses cost twice: they count against the LDS and
+// scalar-load counter too, so each wait for an LDS or descriptor load also
+// waits for every flat load in flight, and atomic flat loads are issued one
+// wait at a time. The program therefore reads and writes its arrays through
+// global-address-space pointers (global_load / global_store): independent
+// loads stay in flight together.
+template <class T> using gptr_t = __attribute__((address_space(1))) T*;
+template <class T> __device__ __forceinline__ const __attribute__((address_space(1))) T* gptr(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T> __device__ __forceinline__ __attribute__((address_space(1))) T* gptr(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 // ------------------------------------------------------------ reductions
 
 // Values that a later phase reads from other workgroups are written with
@@ -57,7 +75,7 @@ template <class T> __device__ inline T* uniform_ptr(T* p) {
 // nothing left to write back; a lean one (group_sync<true>) only waits for
 // each thread's stores, and the readers load them with fetch<true>.
 template <class T> __device__ __forceinline__ void publish(T* p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // A value another workgroup wrote before the last group barrier. kFresh:
@@ -67,9 +85,20 @@ template <class T> __device__ __forceinline__ void publish(T* p, T v) {
 // the caches).
 template <bool kFresh, class T> __device__ __forceinline__ T fetch(const T* p) {
     if constexpr (kFresh)
-        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return __hip_atomic_load(gptr(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else
-        return *p;
+        return *gptr(p);
+}
+
+// The value type behind a pointer of any address space (int32, float and
+// double arrays only: the unary plus would promote narrower integers).
+template <class P> using elem_t = decltype(+(*static_cast<P>(nullptr)));
+
+// A plain load of element i (read-only operator and graph data).
+template <class P> __device__ __forceinline__ elem_t<P> ld(P p, int i) {
+    using T = elem_t<P>;
+    static_assert(sizeof(T) == sizeof(*p), "array element type");
+    return ((const __attribute__((address_space(1))) T*)p)[i];
 }
 
 __device__ inline double wave_sum(double v) {
@@ -205,15 +234,36 @@ __device__ inline double tile_max(double v, Lds& lds) {
 }
 
 // Totals of slots 0..K-1 over `tiles` tiles (fixed order), to every thread.
+// Lane l adds tiles l, l + 64, l + 128, ... in ascending order; the loads of
+// kU such tiles and all K slots go out together before any is added (a lane
+// past the last tile loads the last tile again and discards it, so no load
+// sits behind a branch).
 template <int K, bool kF = false>
 __device__ inline void group_totals(const double* partial, int tiles, double (&out)[K], Lds& lds) {
     if (threadIdx.x < 64) {
+        constexpr int kU = 4;
+        double s[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) s[q] = 0.0;
+        for (int t0 = threadIdx.x; t0 < tiles; t0 += 64 * kU) {
+            double v[kU][K];
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const int t = min(t0 + 64 * u, tiles - 1);
+                    v[u][q] = fetch<kF>(&partial[t * kPartials + q]);
+                }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+#pragma unroll
+                for (int q = 0; q < K; ++q)
+                    if (t0 + 64 * u < tiles) s[q] += v[u][q];
+        }
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            double s = 0.0;
-            for (int t = threadIdx.x; t < tiles; t += 64) s += fetch<kF>(&partial[t * kPartials + q]);
-            s = wave_sum(s);
-            if (threadIdx.x == 0) lds.bcast[q] = s;
+            const double w = wave_sum(s[q]);
+            if (threadIdx.x == 0) lds.bcast[q] = w;
         }
     }
     __syncthreads();
@@ -374,15 +424,15 @@ __device__ __forceinline__ void group_sync(int& timeouts) {
 template <class VT> struct Values;
 template <> struct Values<double> {
     const Instance& I;
-    __device__ const double* diag(int j) const { return I.diag64[j]; }
-    __device__ const double* dinv(int j) const { return I.dinv64[j]; }
-    __device__ const double* off(int j) const { return I.off64[j]; }
+    __device__ auto diag(int j) const { return gptr(static_cast<const double*>(I.diag64[j])); }
+    __device__ auto dinv(int j) const { return gptr(static_cast<const double*>(I.dinv64[j])); }
+    __device__ auto off(int j) const { return gptr(static_cast<const double*>(I.off64[j])); }
 };
 template <> struct Values<float> {
     const Instance& I;
-    __device__ const float* diag(int j) const { return I.diag32[j]; }
-    __device__ const float* dinv(int j) const { return I.dinv32[j]; }
-    __device__ const float* off(int j) const { return I.off32[j]; }
+    __device__ auto diag(int j) const { return gptr(static_cast<const float*>(I.diag32[j])); }
+    __device__ auto dinv(int j) const { return gptr(static_cast<const float*>(I.dinv32[j])); }
+    __device__ auto off(int j) const { return gptr(static_cast<const float*>(I.off32[j])); }
 };
 
 // The level-0 input of a V-cycle, evaluated where it is read:
@@ -416,9 +466,11 @@ template <class VT> struct Pair {
 // every rounding are those of the plain slot loop.
 constexpr int kMaxWidth = 7;   // widest icosphere level (hexagons + aggregation: 6, 7)
 
-template <class T, class Gather, class Fold>
-__device__ inline void ell_row(const int32_t* col, const T* off, int n, int W, int i, const Gather& gather,
-                               const Fold& fold) {
+template <class CP, class OP, class Gather, class Fold>
+__device__ inline void ell_row(CP col_in, OP off_in, int n, int W, int i, const Gather& gather, const Fold& fold) {
+    using T = elem_t<OP>;
+    const auto col = (const __attribute__((address_space(1))) int32_t*)col_in;
+    const auto off = (const __attribute__((address_space(1))) T*)off_in;
     if (W <= kMaxWidth) {
         int c[kMaxWidth];
         T o[kMaxWidth];
@@ -446,9 +498,12 @@ __device__ inline void ell_row(const int32_t* col, const T* off, int n, int W, i
 // The same for R rows of a thread at once (rows of different tiles): all
 // rows' column and value loads, then all their gathers, then each row's
 // folds in slot order. fold(r, off, value) and gather(r, slot, column).
-template <int R, class T, class Gather, class Fold>
-__device__ inline void ell_rows(const int32_t* col, const T* off, int n, int W, const int (&i)[R], const Gather& gather,
+template <int R, class CP, class OP, class Gather, class Fold>
+__device__ inline void ell_rows(CP col_in, OP off_in, int n, int W, const int (&i)[R], const Gather& gather,
                                 const Fold& fold) {
+    using T = elem_t<OP>;
+    const auto col = (const __attribute__((address_space(1))) int32_t*)col_in;
+    const auto off = (const __attribute__((address_space(1))) T*)off_in;
     if (W <= kMaxWidth) {
         int c[R][kMaxWidth];
         T o[R][kMaxWidth];
@@ -509,9 +564,9 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
     const VT om = static_cast<VT>(I.omega[j]);
-    const VT* diag = V.diag(j);
-    const VT* dinv = V.dinv(j);
-    const VT* off = V.off(j);
+    const auto diag = V.diag(j);
+    const auto dinv = V.dinv(j);
+    const auto off = V.off(j);
     tile_loop<sizeof(VT) == 4 || kW == 2>(L.down_tiles, g, G, [&]<int R>(const int (&tl)[R]) {
         // the tile's rows come from its padded row list (one load); the
         // aggregate bounds the restriction needs load alongside
@@ -519,19 +574,19 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
         bool ok[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            i[r] = L.down_rows[tl[r] * kTile + threadIdx.x];
-            c0[r] = L.tile_start[tl[r]];
-            c1[r] = L.tile_start[tl[r] + 1];
+            i[r] = ld(L.down_rows, tl[r] * kTile + threadIdx.x);
+            c0[r] = ld(L.tile_start, tl[r]);
+            c1[r] = ld(L.tile_start, tl[r] + 1);
         }
         int q0[R], q1[R];   // this thread's coarse row's fine-row range, for the restriction
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             ok[r] = i[r] >= 0;
             i[r] = ok[r] ? i[r] : 0;
-            base[r] = L.agg_start[c0[r]];
+            base[r] = ld(L.agg_start, c0[r]);
             const int c = c0[r] + static_cast<int>(threadIdx.x);
-            q0[r] = c < L.n_next ? L.agg_start[c] : 0;
-            q1[r] = c < L.n_next ? L.agg_start[c + 1] : 0;
+            q0[r] = c < L.n_next ? ld(L.agg_start, c) : 0;
+            q1[r] = c < L.n_next ? ld(L.agg_start, c + 1) : 0;
         }
         VT wi[R], acc[R], aw[R];
         double sq[R][1];
@@ -540,9 +595,9 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             sq[r][0] = ok[r] ? static_cast<double>(wi[r]) * static_cast<double>(wi[r]) : 0.0;
-            const VT xi = pre != nullptr ? fetch<kF>(pre + i[r]) : om * dinv[i[r]] * wi[r];
-            acc[r] = wi[r] - diag[i[r]] * xi;
-            aw[r] = diag[i[r]] * wi[r];
+            const VT xi = pre != nullptr ? fetch<kF>(pre + i[r]) : om * ld(dinv, i[r]) * wi[r];
+            acc[r] = wi[r] - ld(diag, i[r]) * xi;
+            aw[r] = ld(diag, i[r]) * wi[r];
         }
         if (au != nullptr) {
             // (A w)_i beside the residual: the neighbours' inputs and their
@@ -551,7 +606,7 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
                 L.col, off, n, W, i,
                 [&](int, int, int c) {
                     const VT wc = src.template at<kF>(c);
-                    return Pair<VT>{wc, om * dinv[c] * wc};
+                    return Pair<VT>{wc, om * ld(dinv, c) * wc};
                 },
                 [&](int r, VT o, Pair<VT> v) {
                     acc[r] -= o * v.second;
@@ -560,7 +615,7 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
         } else {
             ell_rows<R>(
                 L.col, off, n, W, i,
-                [&](int, int, int c) { return pre != nullptr ? fetch<kF>(pre + c) : om * dinv[c] * src.template at<kF>(c); },
+                [&](int, int, int c) { return pre != nullptr ? fetch<kF>(pre + c) : om * ld(dinv, c) * src.template at<kF>(c); },
                 [&](int r, VT o, VT v) { acc[r] -= o * v; });
         }
 #pragma unroll
@@ -598,9 +653,9 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
     const VT om = static_cast<VT>(I.omega[j]);
-    const VT* diag = V.diag(j);
-    const VT* dinv = V.dinv(j);
-    const VT* off = V.off(j);
+    const auto diag = V.diag(j);
+    const auto dinv = V.dinv(j);
+    const auto off = V.off(j);
     const int tiles = (n + kTile - 1) / kTile;
     tile_loop<sizeof(VT) == 4 || kW == 2>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
         int i[R];
@@ -614,8 +669,8 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             sq[r][0] = ok[r] ? static_cast<double>(wi[r]) * static_cast<double>(wi[r]) : 0.0;
-            acc[r] = wi[r] - diag[i[r]] * (om * dinv[i[r]] * wi[r]);
-            aw[r] = diag[i[r]] * wi[r];
+            acc[r] = wi[r] - ld(diag, i[r]) * (om * ld(dinv, i[r]) * wi[r]);
+            aw[r] = ld(diag, i[r]) * wi[r];
         }
         if (au != nullptr) {
             // (A w)_i beside the residual, from the same gathers
@@ -623,7 +678,7 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
                 L.col, off, n, W, i,
                 [&](int, int, int c) {
                     const VT wc = src.template at<kF>(c);
-                    return Pair<VT>{wc, om * dinv[c] * wc};
+                    return Pair<VT>{wc, om * ld(dinv, c) * wc};
                 },
                 [&](int r, VT o, Pair<VT> v) {
                     acc[r] -= o * v.second;
@@ -631,14 +686,14 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
                 });
         } else {
             ell_rows<R>(
-                L.col, off, n, W, i, [&](int, int, int c) { return om * dinv[c] * src.template at<kF>(c); },
+                L.col, off, n, W, i, [&](int, int, int c) { return om * ld(dinv, c) * src.template at<kF>(c); },
                 [&](int r, VT o, VT v) { acc[r] -= o * v; });
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (ok[r]) {
-                const VT x1 = om * dinv[i[r]] * wi[r];
-                publish(&pre[i[r]], static_cast<VT>(x1 + om * dinv[i[r]] * acc[r]));
+                const VT x1 = om * ld(dinv, i[r]) * wi[r];
+                publish(&pre[i[r]], static_cast<VT>(x1 + om * ld(dinv, i[r]) * acc[r]));
                 if (keep != nullptr) publish(&keep[i[r]], wi[r]);
                 if (keep2 != nullptr) publish(&keep2[i[r]], wi[r]);
                 if (au != nullptr) publish(&au[i[r]], aw[r]);
@@ -672,17 +727,32 @@ __device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs
     row_ids(wave * kRows);
     for (int k = threadIdx.x; k < n; k += blockDim.x) lds.crhs[k] = static_cast<double>(fetch<kF>(&rhs[k]));
     __syncthreads();
-    const double* inv = I.cinv;
+    const auto inv = gptr(static_cast<const double*>(I.cinv));
     for (int q0 = wave * kRows; q0 < count; q0 += 4 * kRows) {
         if (q0 != wave * kRows) row_ids(q0);
         double acc[kRows];
 #pragma unroll
-        for (int u = 0; u < kRows; ++u) {
-            acc[u] = 0.0;
-            if (a[u] >= 0) {
-                const double* row = inv + static_cast<size_t>(a[u]) * n;
-                for (int k = lane; k < n; k += 64) acc[u] += row[k] * lds.crhs[k];
+        for (int u = 0; u < kRows; ++u) acc[u] = 0.0;
+        // kChunks columns per lane and row at a time, every load issued
+        // before the first product (an index past the row or a skipped row
+        // loads a valid element and is discarded); each lane still adds its
+        // columns in ascending order
+        constexpr int kChunks = 4;
+        for (int k0 = lane; k0 < n; k0 += 64 * kChunks) {
+            double m[kRows][kChunks];
+#pragma unroll
+            for (int u = 0; u < kRows; ++u) {
+                const auto row = inv + static_cast<size_t>(a[u] >= 0 ? a[u] : 0) * n;
+#pragma unroll
+                for (int c = 0; c < kChunks; ++c) m[u][c] = row[min(k0 + 64 * c, n - 1)];
             }
+#pragma unroll
+            for (int u = 0; u < kRows; ++u)
+#pragma unroll
+                for (int c = 0; c < kChunks; ++c) {
+                    const int k = k0 + 64 * c;
+                    if (a[u] >= 0 && k < n) acc[u] += m[u][c] * lds.crhs[k];
+                }
         }
 #pragma unroll
         for (int u = 0; u < kRows; ++u) {
@@ -705,9 +775,9 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
     const VT om = static_cast<VT>(I.omega[j]);
-    const VT* diag = V.diag(j);
-    const VT* dinv = V.dinv(j);
-    const VT* off = V.off(j);
+    const auto diag = V.diag(j);
+    const auto dinv = V.dinv(j);
+    const auto off = V.off(j);
     const bool fused = j + 2 == I.depth;
     const int tiles = (n + kTile - 1) / kTile;
     if (fused) {
@@ -720,20 +790,20 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
                                     L.need_stride, lds);
             if (i < n) {
                 const VT rhs_i = fetch<kF>(&rhs[i]);
-                const VT x2 = (pre != nullptr ? fetch<kF>(pre + i) : om * dinv[i] * rhs_i) +
-                              static_cast<VT>(lds.cvec[L.agg[i]]);
-                VT acc = rhs_i - diag[i] * x2;
+                const VT x2 = (pre != nullptr ? fetch<kF>(pre + i) : om * ld(dinv, i) * rhs_i) +
+                              static_cast<VT>(lds.cvec[ld(L.agg, i)]);
+                VT acc = rhs_i - ld(diag, i) * x2;
                 // the neighbour's aggregate comes from the slot's own map, so the
                 // correction read does not wait for the column
                 ell_row(
                     L.col, off, n, W, i,
                     [&](int s, int c) {
-                        const int a = L.agg_col[s * n + i];
-                        return (pre != nullptr ? fetch<kF>(pre + c) : om * dinv[c] * fetch<kF>(&rhs[c])) +
+                        const int a = ld(L.agg_col, s * n + i);
+                        return (pre != nullptr ? fetch<kF>(pre + c) : om * ld(dinv, c) * fetch<kF>(&rhs[c])) +
                                static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
                     },
                     [&](VT o, VT v) { acc -= o * v; });
-                const VT value = x2 + om * dinv[i] * acc;
+                const VT value = x2 + om * ld(dinv, i) * acc;
                 publish(&out[i], value);
                 q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i])) : 0.0;
             }
@@ -750,14 +820,14 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const VT rhs_i = fetch<kF>(&rhs[i[r]]);
-            x2[r] = (pre != nullptr ? fetch<kF>(pre + i[r]) : om * dinv[i[r]] * rhs_i) + fetch<kF>(&corr[L.agg[i[r]]]);
-            acc[r] = rhs_i - diag[i[r]] * x2[r];
+            x2[r] = (pre != nullptr ? fetch<kF>(pre + i[r]) : om * ld(dinv, i[r]) * rhs_i) + fetch<kF>(&corr[ld(L.agg, i[r])]);
+            acc[r] = rhs_i - ld(diag, i[r]) * x2[r];
         }
         ell_rows<R>(
             L.col, off, n, W, i,
             [&](int r, int s, int c) {
-                const int a = L.agg_col[s * n + i[r]];
-                return (pre != nullptr ? fetch<kF>(pre + c) : om * dinv[c] * fetch<kF>(&rhs[c])) +
+                const int a = ld(L.agg_col, s * n + i[r]);
+                return (pre != nullptr ? fetch<kF>(pre + c) : om * ld(dinv, c) * fetch<kF>(&rhs[c])) +
                        fetch<kF>(&corr[a >= 0 ? a : 0]);
             },
             [&](int r, VT o, VT v) { acc[r] -= o * v; });
@@ -766,7 +836,7 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
         for (int r = 0; r < R; ++r) {
             q[r][0] = 0.0;
             if (ok[r]) {
-                const VT value = x2[r] + om * dinv[i[r]] * acc[r];
+                const VT value = x2[r] + om * ld(dinv, i[r]) * acc[r];
                 publish(&out[i[r]], value);
                 if (dot_with != nullptr) q[r][0] = static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i[r]]));
             }
@@ -803,12 +873,12 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
     const LevelView& M = I.lv[j + 1];
     const int n = L.n, W = L.width, nm = M.n, Wm = M.width;
     const VT om = static_cast<VT>(I.omega[j]), omm = static_cast<VT>(I.omega[j + 1]);
-    const VT* diag = V.diag(j);
-    const VT* dinv = V.dinv(j);
-    const VT* off = V.off(j);
-    const VT* diag_m = V.diag(j + 1);
-    const VT* dinv_m = V.dinv(j + 1);
-    const VT* off_m = V.off(j + 1);
+    const auto diag = V.diag(j);
+    const auto dinv = V.dinv(j);
+    const auto off = V.off(j);
+    const auto diag_m = V.diag(j + 1);
+    const auto dinv_m = V.dinv(j + 1);
+    const auto off_m = V.off(j + 1);
     const VT* rhs_m = static_cast<const VT*>(I.lvl_b[j + 1]);
     constexpr bool kF = kW >= 1;
     double* mid = lds.tile[0];
@@ -817,24 +887,24 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
         // this thread's mid row id (a tile reads at most kTile of them) goes
         // out before the coarse solve
         const int mq = threadIdx.x;
-        const int m = mq < L.mid_stride ? L.mid[tile * L.mid_stride + mq] : -1;
+        const int m = mq < L.mid_stride ? ld(L.mid, tile * L.mid_stride + mq) : -1;
         coarse_rows_lds<VT, kF>(I, static_cast<const VT*>(I.lvl_b[j + 2]), L.mid_coarse + tile * L.mid_coarse_stride, 0,
                                 L.mid_coarse_stride, lds);
         {
             if (m >= 0) {
                 const VT rhs_mm = fetch<kF>(&rhs_m[m]);
-                const VT x2 = (pre_m != nullptr ? fetch<kF>(pre_m + m) : omm * dinv_m[m] * rhs_mm) +
-                              static_cast<VT>(lds.cvec[M.agg[m]]);
-                VT acc = rhs_mm - diag_m[m] * x2;
+                const VT x2 = (pre_m != nullptr ? fetch<kF>(pre_m + m) : omm * ld(dinv_m, m) * rhs_mm) +
+                              static_cast<VT>(lds.cvec[ld(M.agg, m)]);
+                VT acc = rhs_mm - ld(diag_m, m) * x2;
                 ell_row(
                     M.col, off_m, nm, Wm, m,
                     [&](int s, int c) {
-                        const int a = M.agg_col[s * nm + m];
-                        return (pre_m != nullptr ? fetch<kF>(pre_m + c) : omm * dinv_m[c] * fetch<kF>(&rhs_m[c])) +
+                        const int a = ld(M.agg_col, s * nm + m);
+                        return (pre_m != nullptr ? fetch<kF>(pre_m + c) : omm * ld(dinv_m, c) * fetch<kF>(&rhs_m[c])) +
                                static_cast<VT>(lds.cvec[a >= 0 ? a : 0]);
                     },
                     [&](VT o, VT v) { acc -= o * v; });
-                mid[mq] = static_cast<double>(static_cast<VT>(x2 + omm * dinv_m[m] * acc));
+                mid[mq] = static_cast<double>(static_cast<VT>(x2 + omm * ld(dinv_m, m) * acc));
             }
         }
         __syncthreads();
@@ -842,17 +912,17 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
         double q[1] = {0.0};
         if (i < n) {
             const VT rhs_i = fetch<kF>(&rhs[i]);
-            const VT x2 = (pre != nullptr ? fetch<kF>(pre + i) : om * dinv[i] * rhs_i) + static_cast<VT>(mid[L.agg_loc[i]]);
-            VT acc = rhs_i - diag[i] * x2;
+            const VT x2 = (pre != nullptr ? fetch<kF>(pre + i) : om * ld(dinv, i) * rhs_i) + static_cast<VT>(mid[ld(L.agg_loc, i)]);
+            VT acc = rhs_i - ld(diag, i) * x2;
             ell_row(
                 L.col, off, n, W, i,
                 [&](int s, int c) {
-                    const int a = L.agg_col_loc[s * n + i];
-                    return (pre != nullptr ? fetch<kF>(pre + c) : om * dinv[c] * fetch<kF>(&rhs[c])) +
+                    const int a = ld(L.agg_col_loc, s * n + i);
+                    return (pre != nullptr ? fetch<kF>(pre + c) : om * ld(dinv, c) * fetch<kF>(&rhs[c])) +
                            static_cast<VT>(mid[a >= 0 ? a : 0]);
                 },
                 [&](VT o, VT v) { acc -= o * v; });
-            const VT value = x2 + om * dinv[i] * acc;
+            const VT value = x2 + om * ld(dinv, i) * acc;
             publish(&out[i], value);
             q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i])) : 0.0;
         }
@@ -980,7 +1050,7 @@ template <int kW> __device__ void coarse_inverse(const Instance& I, double* tri,
     for (int e = threadIdx.x; e < packed; e += blockDim.x) tri[e] = 0.0;
     __syncthreads();
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        tri[tri_index(r, r)] = diag[r];
+        tri[tri_index(r, r)] = ld(diag, r);
         for (int s = 0; s < W; ++s) {
             const int c = L.col[s * n + r];
             if (c >= 0 && c < r) tri[tri_index(r, c)] = off[s * n + r];
@@ -1180,17 +1250,17 @@ __device__ __forceinline__ void set_matrix(const Instance& I, const Launch& P, i
                     double value = 0.0;
                     if (s < deg) {
                         if (P.op & kOpHeatAssemble) {
-                            const double gk = P.entry_conductance[k0 + s] * I.weight[k0 + s];
+                            const double gk = ld(P.entry_conductance, k0 + s) * ld(I.weight, k0 + s);
                             sum += gk;
                             value = -P.dt * gk;
                         } else {
-                            value = P.in[k0 + s];
+                            value = ld(P.in, k0 + s);
                         }
                     }
                     I.off64[0][s * n + i] = value;
                 }
                 if (P.op & kOpHeatAssemble) {
-                    const double ca = P.capacity_area[i];
+                    const double ca = ld(P.capacity_area, i);
                     I.diag64[0][i] = ca + P.dt * sum;
                     if (P.capacity_keep != nullptr) P.capacity_keep[i] = ca;
                 }
@@ -1325,26 +1395,34 @@ __device__ inline int pre_sweep_mask(const Instance& I) {
     return mask;
 }
 
+// A chained step's right-hand side prepared by the previous solve: set when
+// the phase that left that solve's x final also assembled the next b, r
+// and r0, with the totals (b, b) and (r, r) the next solve starts from.
+struct Prepared {
+    bool ready = false;
+    double totals[2] = {0.0, 0.0};
+};
+
 template <class VT, int kW>
 //
 // Chained steps in the full-register-file program (kW == 1): with
 // next_forcing set, every phase that leaves x final (the half-step exit and
 // the FP64 true residual) also assembles the next step's right-hand side
 // b' = C A x + dt A S' into the other (b, r, r0) pair (rp selects the pair in
-// use), with its tile partials beside the phase's own; `prepared` then points
-// at that partial buffer and the next solve starts from it instead of an
-// assembly phase (the same values and partial sums, so the same bits),
-// zeroing its x in its first SpMV sweep.
+// use), with its tile partials beside the phase's own, summed with them;
+// `prepared` then holds those totals and the next solve starts from them
+// instead of an assembly phase (the same values and partial sums, so the
+// same bits), zeroing its x in its first SpMV sweep.
 __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P, const double* t_old, const double* forcing,
                              const double* next_forcing, double* t_new, int g, int G, Lds& lds, int& timeouts,
-                             int& parity, int& xi, int& rp, double*& prepared) {
+                             int& parity, int& xi, int& rp, Prepared& prepared) {
     constexpr bool kPrep = kW == 1;
     constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
     const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
     const Values<VT> V{I};
     const LevelView& L0 = I.lv[0];
     const int n = I.n, W = L0.width, tiles = I.tiles;
-    const bool swap = kPrep && prepared != nullptr;
+    const bool swap = kPrep && prepared.ready;
     if (swap) rp ^= 1;
     const bool alt = kPrep && rp != 0;
     VT* r = static_cast<VT*>(alt ? I.r_alt : I.r);
@@ -1363,7 +1441,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
             double* bn = alt ? I.b : I.b_alt;
             VT* rn = static_cast<VT*>(alt ? I.r : I.r_alt);
             VT* rn0 = static_cast<VT*>(alt ? I.r0 : I.r0_alt);
-            const double bi = P.capacity_area[i] * x_final + P.dt * I.area[i] * next_forcing[i];
+            const double bi = ld(P.capacity_area, i) * x_final + P.dt * ld(I.area, i) * ld(next_forcing, i);
             const VT rv = static_cast<VT>(bi);
             publish(&bn[i], bi);
             publish(&rn[i], rv);
@@ -1372,9 +1450,25 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
             q[3] = static_cast<double>(rv) * static_cast<double>(rv);
         }
     };
-    double* prep = nullptr;   // partial buffer of the last phase that prepared the next step
-    const VT* diag = V.diag(0);
-    const VT* off = V.off(0);
+    Prepared prep;   // set by the last phase that prepared the next step
+    // totals of a phase that leaves x final: its slots 0 and 1 into tot, and
+    // with next_forcing set, the prepared right-hand side's (b', b') and
+    // (r', r') (slots 2 and 3) into prep, from one read of the partials
+    const auto final_totals = [&](const double* part_in, double (&tot)[2]) {
+        if (kPrep && next_forcing != nullptr) {
+            double t4[4];
+            group_totals<4, kF>(part_in, tiles, t4, lds);
+            tot[0] = t4[0];
+            tot[1] = t4[1];
+            prep.ready = true;
+            prep.totals[0] = t4[2];
+            prep.totals[1] = t4[3];
+        } else {
+            group_totals<2, kF>(part_in, tiles, tot, lds);
+        }
+    };
+    const auto diag = V.diag(0);
+    const auto off = V.off(0);
     const auto next_partials = [&]() {
         double* part = I.partial + parity * I.partial_slots * kPartials;
         parity ^= 1;
@@ -1383,7 +1477,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     // acc[r] = (A in)_i[r] for R rows at once
     const auto spmv_rows = [&]<int R>(const VT* in, const int (&i)[R], VT (&acc)[R]) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = diag[i[r]] * fetch<kF>(&in[i[r]]);
+        for (int r = 0; r < R; ++r) acc[r] = ld(diag, i[r]) * fetch<kF>(&in[i[r]]);
         ell_rows<R>(
             L0.col, off, n, W, i, [&](int, int, int c) { return fetch<kF>(&in[c]); },
             [&](int r, VT o, VT v) { acc[r] += o * v; });
@@ -1395,7 +1489,8 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     double tot2[2];
     bool zero_x = swap;
     if (swap) {
-        group_totals<2, kF>(prepared + 2, tiles, tot2, lds);
+        tot2[0] = prepared.totals[0];
+        tot2[1] = prepared.totals[1];
     } else {
         part = next_partials();
         {
@@ -1408,8 +1503,8 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 for (int k = 0; k < R; ++k) i[k] = tile_row(tl[k], n, ok[k]);
 #pragma unroll
                 for (int k = 0; k < R; ++k)
-                    bi[k] = (P.op & kOpHeatRhs) ? P.capacity_area[i[k]] * fetch<kF>(&t_old[i[k]]) + P.dt * I.area[i[k]] * forcing[i[k]]
-                                                : P.in[i[k]];
+                    bi[k] = (P.op & kOpHeatRhs) ? ld(P.capacity_area, i[k]) * fetch<kF>(&t_old[i[k]]) + P.dt * ld(I.area, i[k]) * ld(forcing, i[k])
+                                                : ld(P.in, i[k]);
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     q[k][0] = q[k][1] = 0.0;
@@ -1429,7 +1524,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
         group_sync<kF>(timeouts);
         group_totals<2, kF>(part, tiles, tot2, lds);
     }
-    prepared = nullptr;
+    prepared = Prepared{};
     const double b_norm = sqrt(tot2[0]);
     double rho1 = tot2[1];
     double r_norm = b_norm;
@@ -1476,9 +1571,8 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         }
                         store_partials<KQ>(part, tile, q, lds);
                     }
-                    if (kPrep) prep = next_forcing != nullptr ? part : nullptr;
                     group_sync<kF>(timeouts);
-                    group_totals<2, kF>(part, tiles, tot2, lds);
+                    final_totals(part, tot2);
                     r_norm = sqrt(tot2[0]);
                     rho1 = tot2[1];
                 }
@@ -1591,12 +1685,11 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         }
                         store_partials<KQ>(part, tile, q, lds);
                     }
-                    if (kPrep) prep = next_forcing != nullptr ? part : nullptr;
                     group_sync<kF>(timeouts);
                     xi ^= 1;
                     x_zero = false;
                     zero_x = false;   // x[xi] now holds x + alpha ph
-                    group_totals<2, kF>(part, tiles, tot2, lds);
+                    final_totals(part, tot2);
                     ++iterations;
                     if (sqrt(tot2[0]) <= inner_target) {
                         // r holds the true residual; a restart takes it as its
@@ -1644,10 +1737,9 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                             }
                             store_partials<KQ>(part, tile, q, lds);
                         }
-                        if (kPrep) prep = next_forcing != nullptr ? part : nullptr;
                         group_sync<kF>(timeouts);
                         xi ^= 1;
-                        group_totals<2, kF>(part, tiles, tot2, lds);
+                        final_totals(part, tot2);
                         r_norm = sqrt(tot2[0]);
                         rho1 = tot2[1];
                         have_residual = true;
@@ -1688,7 +1780,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 // x += alpha ph + omega sh (omega sh when x holds x + alpha ph
                 // already), r = s - omega t, (r, r), (r0, r)
                 part = next_partials();
-                prep = nullptr;
+                prep = Prepared{};
                 {
                     double* x = I.x[xi];
                     tile_loop(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
@@ -1804,7 +1896,7 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
         const int i = tile * kTile + threadIdx.x;
         double v[1] = {0.0};
         if (i < n) {
-            const double bi = P.in[i];
+            const double bi = ld(P.in, i);
             publish(&r[i], bi);
             publish(&x[i], 0.0);
             v[0] = bi * bi;
@@ -1843,7 +1935,7 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
             if (i < n) {
                 // held rows keep d = 0, so x, r and q stay exactly 0 there
                 const double di_value = P.hold != nullptr && P.hold[i] != 0 ? 0.0 : src.template at<kF>(i);
-                double acc = diag[i] * di_value;
+                double acc = ld(diag, i) * di_value;
                 ell_row(
                     L0.col, off, n, W, i, [&](int, int c) { return src.template at<kF>(c); },
                     [&](double o, double w) { acc += o * w; });
@@ -1913,7 +2005,7 @@ __device__ __noinline__ SolveResult pcg_call(const Instance* instance, const Lau
 template <class VT, int kW>
 __device__ __forceinline__ void run_solve(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts, int& parity) {
     int xi = 0, rp = 0;
-    double* prepared = nullptr;
+    Prepared prepared;
     const int steps = (P.op & kOpChain) ? P.steps : 1;
     SolveResult last{};
     for (int step = 0; step < steps; ++step) {
@@ -1977,8 +2069,10 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
     const Instance* instances, const int32_t* block_instance, const Launch* launches, Launch single) {
     __shared__ Lds lds;
     __shared__ Launch params;
-    const int index = block_instance[blockIdx.x];
-    const Instance& I = instances[index];
+    // the workgroup's instance, uniform: its descriptor fields load as scalar
+    // loads into SGPRs instead of per-lane vector loads
+    const int index = __builtin_amdgcn_readfirstlane(block_instance[blockIdx.x]);
+    const Instance& I = *uniform_ptr(instances + index);
     if (threadIdx.x == 0) params = launches != nullptr ? launches[index] : single;
     set_group(I.group_first, I.group_size, I.barrier, I.result, I.trace, I.trace_count, I.trace_capacity);
     const Launch& P = params;
@@ -2006,7 +2100,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
             float* pout = static_cast<float*>(I.ph);
             for (int tile = g; tile < I.tiles; tile += G) {
                 const int i = tile * kTile + threadIdx.x;
-                if (i < I.n) pin[i] = static_cast<float>(P.in[i]);
+                if (i < I.n) pin[i] = static_cast<float>(ld(P.in, i));
             }
             group_sync(timeouts);
             const Source<float> src{pin, nullptr, nullptr, 0.0, 0.0, 0};
@@ -2024,9 +2118,9 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
         for (int tile = g; tile < I.tiles; tile += G) {
             const int i = tile * kTile + threadIdx.x;
             if (i < n) {
-                double acc = I.diag64[0][i] * P.in[i];
+                double acc = I.diag64[0][i] * ld(P.in, i);
                 ell_row(
-                    L.col, I.off64[0], n, W, i, [&](int, int c) { return P.in[c]; },
+                    L.col, I.off64[0], n, W, i, [&](int, int c) { return ld(P.in, c); },
                     [&](double o, double v) { acc += o * v; });
                 P.out[i] = acc;
             }
