@@ -302,12 +302,13 @@ struct Group {
     uint64_t* trace;
     uint32_t* trace_count;
     int32_t trace_capacity;
+    uint32_t episode;   // group barriers this workgroup has entered in this launch
 };
 __shared__ Group grp;
 
 __device__ inline void set_group(int32_t first, int32_t size, uint32_t* barrier, SolveResult* result, uint64_t* trace,
                                  uint32_t* trace_count, int32_t trace_capacity) {
-    if (threadIdx.x == 0) grp = Group{first, size, barrier, result, trace, trace_count, trace_capacity};
+    if (threadIdx.x == 0) grp = Group{first, size, barrier, result, trace, trace_count, trace_capacity, 0u};
     __syncthreads();
 }
 
@@ -321,47 +322,45 @@ __device__ inline void trace_stamp() {
     }
 }
 
-// Two-level counting barrier over the instance's workgroups. Workgroup g
-// arrives on leaf counter g % kLeaves (separate 128-byte lines, so the
-// arrivals do not serialise on one address); the last arrival of a leaf
-// arrives at the root; the last root arrival bumps the generation word that
-// every thread 0 polls. The wait is bounded by the 100 MHz wall clock
-// (about one second); a timeout is recorded and is sticky, so a broken
-// group drains within seconds and the host reports it.
+// Counting barrier over the instance's workgroups. Every barrier of a
+// launch is an episode k = 1, 2, ... that all workgroups of the group enter
+// in the same order. Workgroup g adds 1 to leaf counter g % kLeaves (eight
+// counters on separate 256-byte lines, so arrivals do not serialise on one
+// address) with an atomic whose result it does not wait for; lanes 0..7 of
+// its first wave then poll the eight leaves until each holds at least
+// k x (the leaf's share of the group). The counters only grow within a
+// launch, so no arrival is waited on for more than its own trip to memory
+// and back: one round trip from the last arrival to every waiter's release,
+// where a tree with a generation word takes four (leaf, root, clear,
+// generation). A counter is at most one episode ahead of a waiter (a
+// workgroup enters episode k + 1 only after k completes, which needs this
+// waiter's arrival), so the comparison is wrap-safe. The last workgroup to
+// leave the launch (group_exit) clears the counters, so every launch starts
+// from zero whatever the previous group size was. The wait is bounded by
+// the 100 MHz wall clock (about one second); a timeout is recorded and is
+// sticky, so a broken group drains within seconds and the host reports it.
 //
-// Ordering, with the fewest cache-maintenance fences: every workgroup
-// issues one agent-scope release fence before its (relaxed) leaf arrival
-// and one acquire fence after its (relaxed) poll; only the last arrival of
-// a leaf and the last root arrival add an acq_rel fence between the RMW
-// they observed and the one they perform. Fence-fence synchronisation
-// through each counter's RMW chain then orders every workgroup's writes
-// before every workgroup's reads after the barrier. (An acq_rel RMW would
-// add a second L2 write-back and invalidate per workgroup per barrier.)
+// Ordering (fenced form): thread 0 issues one agent-scope release fence
+// before its relaxed arrival and the polling lanes one acquire fence after
+// the wait. Every arrival is an RMW, so the value a waiter reads continues
+// the release sequence of every earlier arrival on that leaf, and reading
+// all eight leaves at their targets synchronises with every workgroup's
+// release: fence-fence synchronisation orders all writes before the
+// barrier before all reads after it.
 //
-// Counters come in two sets used by alternate episodes (the parity of the
-// generation observed on entry). The last root arrival clears the whole
-// other set (root and all eight leaves) before its acq_rel fence, which
-// orders those stores before the generation bump: that set is idle (every
-// workgroup has passed the episode that used it) and is next used only
-// after this episode completes. Between launches the set of the current
-// parity is therefore all zero, whatever the group size was or will be.
+// Lean form (kLean, the solve and PCG phases, whose values for other
+// workgroups are all written with coherent stores and read with coherent
+// loads): each thread waits for its own stores (s_waitcnt vmcnt(0)) instead
+// of the release fence, and no acquire fence follows the wait, so this
+// XCD's L2 keeps the read-only operator data across phases.
 //
-// Barrier words (uint32, zeroed at creation):
-//   [0]                  generation
-//   [32 + 32 p]          root arrivals, parity p
-//   [96 + 64 l + 32 p]   arrivals of leaf l, parity p
+// Barrier words (uint32, zeroed at creation and at every launch's end):
+//   [32]         workgroups that have left the launch (group_exit)
+//   [96 + 64 l]  arrivals on leaf l in this launch
 constexpr long long kBarrierTicks = 100000000;
 constexpr int kLeaves = 8;
 static_assert(96 + 64 * kLeaves == kBarrierWordCount, "barrier layout");
 
-//
-// Lean form (kLean, the full-register-file program's solve and PCG phases,
-// whose values for other workgroups are all written with coherent stores and
-// read with coherent loads): each thread waits for its own stores instead of
-// a release fence per workgroup, and no acquire fence follows the wait, so
-// this XCD's L2 keeps the read-only operator data across phases. With no
-// plain store to publish, the last arrivals of a leaf and of the root need
-// no fence either; the root's last arrival only waits for its clears.
 template <bool kLean = false>
 __device__ __forceinline__ void group_sync(int& timeouts) {
     if constexpr (kLean) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -371,52 +370,53 @@ __device__ __forceinline__ void group_sync(int& timeouts) {
         trace_stamp();
         return;
     }
-    if (threadIdx.x == 0 && timeouts == 0) {
-        uint32_t* gen = grp.barrier;
+    if (threadIdx.x < 64) {
+        const uint32_t k = grp.episode + 1;
         const int g = static_cast<int>(blockIdx.x) - grp.first;
-        const int leaf = g % kLeaves;
-        const int leaves = G < kLeaves ? G : kLeaves;
-        const uint32_t leaf_size = static_cast<uint32_t>((G - leaf + kLeaves - 1) / kLeaves);
-        if constexpr (!kLean) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const uint32_t seen = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t p = seen & 1u;
-        uint32_t* root = grp.barrier + 32 + 32 * p;
-        uint32_t* mine = grp.barrier + 96 + 64 * leaf + 32 * p;
-        bool last = false;
-        if (__hip_atomic_fetch_add(mine, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == leaf_size) {
-            if constexpr (!kLean) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-            if (__hip_atomic_fetch_add(root, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 ==
-                static_cast<uint32_t>(leaves)) {
-                // clear the whole idle set, every leaf included: a leaf this
-                // group does not use now may hold counts from a larger group
-                __hip_atomic_store(grp.barrier + 32 + 32 * (p ^ 1u), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                for (int l = 0; l < kLeaves; ++l)
-                    __hip_atomic_store(grp.barrier + 96 + 64 * l + 32 * (p ^ 1u), 0u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                // lean: the clears complete before the generation moves (an
-                // arrival of the next episode must not meet a late clear)
-                if constexpr (kLean)
-                    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                else
-                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-                __hip_atomic_store(gen, seen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                last = true;
-            }
+        const int lane = static_cast<int>(threadIdx.x);
+        if (lane == 0) {
+            if constexpr (!kLean) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            (void)__hip_atomic_fetch_add(grp.barrier + 96 + 64 * (g % kLeaves), 1u, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+            grp.episode = k;
         }
-        if (!last) {
+        bool late = false;
+        if (lane < kLeaves && lane < G && timeouts == 0) {
+            const uint32_t target = k * static_cast<uint32_t>((G - lane + kLeaves - 1) / kLeaves);
+            const uint32_t* leaf = grp.barrier + 96 + 64 * lane;
             const long long start = wall_clock64();
-            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == seen) {
+            while (static_cast<int32_t>(__hip_atomic_load(const_cast<uint32_t*>(leaf), __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT) -
+                                        target) < 0) {
                 __builtin_amdgcn_s_sleep(1);
                 if (wall_clock64() - start > kBarrierTicks) {
-                    timeouts = 1;
+                    late = true;
                     break;
                 }
             }
-            if constexpr (!kLean) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
+        if (__any(late)) timeouts = 1;
+        if constexpr (!kLean) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     trace_stamp();
     __syncthreads();
+}
+
+// A workgroup leaves the launch: the last of its group to leave clears the
+// barrier counters for the next launch (every other workgroup of the group
+// has passed its last barrier by then, so no wait can meet the clear).
+__device__ inline void group_exit() {
+    __syncthreads();
+    const int G = grp.size;
+    if (G > 1 && threadIdx.x == 0) {
+        uint32_t* left = grp.barrier + 32;
+        if (__hip_atomic_fetch_add(left, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 ==
+            static_cast<uint32_t>(G)) {
+            for (int l = 0; l < kLeaves; ++l)
+                __hip_atomic_store(grp.barrier + 96 + 64 * l, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(left, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 // ------------------------------------------------------------ typed views
@@ -2155,6 +2155,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
         }
     }
     if (timeouts != 0 && threadIdx.x == 0) atomicAdd(&grp.result->barrier_timeouts, timeouts);
+    group_exit();
 }
 
 template __global__ void program_kernel_t<3>(const Instance*, const int32_t*, const Launch*, Launch);
@@ -2324,6 +2325,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
         }
         __syncthreads();
     }
+    group_exit();
 }
 
 // The kernels of one occupancy: 1, 2 or 3 workgroups per CU.

@@ -1,23 +1,24 @@
-// CPU model of the solver program's group barrier (group_sync in
-// planet_game_amd/csrc/solver.hip): the same barrier words and the same
-// arrival logic, run by std::thread "workgroups" on std::atomic words.
+// CPU model of the solver program's group barrier (group_sync and
+// group_exit in planet_game_amd/csrc/solver.hip): the same barrier words and
+// the same arrival, wait and exit logic, run by std::thread "workgroups" on
+// std::atomic words.
 //
-// Barrier words (uint32): [0] generation, [32 + 32 p] root arrivals of
-// parity p, [96 + 64 l + 32 p] arrivals of leaf l for parity p; workgroup g
-// arrives on leaf g % 8, the last arrival of a leaf arrives at the root, the
-// last root arrival clears the whole idle set (root and every leaf of the
-// other parity) and then moves the generation that every waiter polls.
+// Barrier words (uint32): [32] workgroups that have left the launch,
+// [96 + 64 l] arrivals on leaf l in this launch. Workgroup g adds 1 to leaf
+// g % 8 at every barrier (episode k = 1, 2, ... of the launch) and waits
+// until every leaf l holds at least k x (the number of workgroups on leaf l);
+// the counters only grow within a launch. The last workgroup to leave the
+// launch clears them.
 //
 // What the model checks (exit status 0 = all passed):
 //   * every workgroup leaves every barrier, for group sizes 1..64 that change
-//     from launch to launch, with odd and even numbers of barriers per
-//     launch (the idle set must be clean whatever the previous group size);
+//     from launch to launch, with 1 to 10 barriers per launch (the counters
+//     must be clean at every launch's start whatever the previous group size);
 //   * a value a workgroup publishes before a barrier is what every other
 //     workgroup reads after it (each episode overwrites the same slots);
 //   * in both forms: fenced (release fence before the arrival, acquire fence
-//     after the wait, acq_rel fences between the counters a last arrival
-//     observes and updates) and lean (no fences: the published values and
-//     the counters are relaxed atomics).
+//     after the wait) and lean (no fences: the published values and the
+//     counters are relaxed atomics).
 // The lean form's ordering on the GPU rests on the hardware (a thread's
 // s_waitcnt vmcnt(0) returns once its write-through stores are acknowledged,
 // and loads that bypass the L2 read memory); a CPU model cannot check that
@@ -43,42 +44,43 @@ struct Barrier {
     }
 };
 
-// One arrival of workgroup g of a group of G; returns false on a timeout.
-bool group_sync(Barrier& b, int g, int G, bool lean) {
+// One arrival of workgroup g of a group of G at its episode-th barrier
+// (`episode` counts this workgroup's barriers in the launch, as grp.episode
+// does); returns false on a timeout.
+bool group_sync(Barrier& b, int g, int G, uint32_t& episode, bool lean) {
     if (G == 1) return true;
-    const int leaf = g % kLeaves;
-    const int leaves = G < kLeaves ? G : kLeaves;
-    const uint32_t leaf_size = static_cast<uint32_t>((G - leaf + kLeaves - 1) / kLeaves);
+    const uint32_t k = ++episode;
     if (!lean) std::atomic_thread_fence(std::memory_order_release);
-    const uint32_t seen = b.w[0].load(std::memory_order_relaxed);
-    const uint32_t p = seen & 1u;
-    std::atomic<uint32_t>& root = b.w[32 + 32 * p];
-    std::atomic<uint32_t>& mine = b.w[96 + 64 * leaf + 32 * p];
-    if (mine.fetch_add(1, std::memory_order_relaxed) + 1 == leaf_size) {
-        if (!lean) std::atomic_thread_fence(std::memory_order_acq_rel);
-        if (root.fetch_add(1, std::memory_order_relaxed) + 1 == static_cast<uint32_t>(leaves)) {
-            b.w[32 + 32 * (p ^ 1u)].store(0, std::memory_order_relaxed);
-#ifdef CLEAR_OWN_LEAF_ONLY
-            // a planted mistake (the tests build it to show the model catches
-            // it): only this leaf's idle counter is cleared
-            b.w[96 + 64 * leaf + 32 * (p ^ 1u)].store(0, std::memory_order_relaxed);
-#else
-            for (int l = 0; l < kLeaves; ++l) b.w[96 + 64 * l + 32 * (p ^ 1u)].store(0, std::memory_order_relaxed);
-#endif
-            // the clears complete before the generation moves (lean: the
-            // thread's own wait for its stores; fenced: the acq_rel fence)
-            std::atomic_thread_fence(lean ? std::memory_order_seq_cst : std::memory_order_acq_rel);
-            b.w[0].store(seen + 1, std::memory_order_relaxed);
-            return true;
+    b.w[96 + 64 * (g % kLeaves)].fetch_add(1, std::memory_order_relaxed);
+    for (int l = 0; l < kLeaves && l < G; ++l) {
+        const uint32_t target = k * static_cast<uint32_t>((G - l + kLeaves - 1) / kLeaves);
+        long spins = 0;
+        while (static_cast<int32_t>(b.w[96 + 64 * l].load(std::memory_order_relaxed) - target) < 0) {
+            if (++spins > kSpinLimit) return false;
+            std::this_thread::yield();
         }
-    }
-    long spins = 0;
-    while (b.w[0].load(std::memory_order_relaxed) == seen) {
-        if (++spins > kSpinLimit) return false;
-        std::this_thread::yield();
     }
     if (!lean) std::atomic_thread_fence(std::memory_order_acquire);
     return true;
+}
+
+// Workgroup g leaves the launch; the last to leave clears the counters.
+void group_exit(Barrier& b, int G) {
+    if (G == 1) return;
+    const uint32_t before = b.w[32].fetch_add(1, std::memory_order_relaxed);
+#ifdef CLEAR_AT_FIRST_EXIT
+    // a planted mistake (the tests build it to show the model catches it):
+    // the first workgroup to leave clears the leaves, while others may still
+    // wait on them
+    if (before == 0)
+        for (int l = 0; l < kLeaves; ++l) b.w[96 + 64 * l].store(0, std::memory_order_relaxed);
+    if (before + 1 == static_cast<uint32_t>(G)) b.w[32].store(0, std::memory_order_relaxed);
+#else
+    if (before + 1 == static_cast<uint32_t>(G)) {
+        for (int l = 0; l < kLeaves; ++l) b.w[96 + 64 * l].store(0, std::memory_order_relaxed);
+        b.w[32].store(0, std::memory_order_relaxed);
+    }
+#endif
 }
 
 // One launch: G workgroups run `episodes` barriers; in each they publish a
@@ -88,10 +90,11 @@ bool launch(Barrier& b, std::vector<std::atomic<uint64_t>>& slots, int G, int ep
     std::vector<std::thread> threads;
     for (int g = 0; g < G; ++g)
         threads.emplace_back([&, g] {
+            uint32_t episode = 0;
             for (int e = 0; e < episodes && ok.load(std::memory_order_relaxed); ++e) {
                 const uint64_t mark = (tag << 32) | (static_cast<uint64_t>(e) << 8);
                 slots[static_cast<size_t>(g)].store(mark | static_cast<uint64_t>(g), std::memory_order_relaxed);
-                if (!group_sync(b, g, G, lean)) {
+                if (!group_sync(b, g, G, episode, lean)) {
                     std::fprintf(stderr, "timeout: G=%d episode %d workgroup %d\n", G, e, g);
                     ok = false;
                     return;
@@ -106,12 +109,13 @@ bool launch(Barrier& b, std::vector<std::atomic<uint64_t>>& slots, int G, int ep
                 }
                 // the second barrier of the episode keeps the next episode's
                 // writes after every read of this one
-                if (!group_sync(b, g, G, lean)) {
+                if (!group_sync(b, g, G, episode, lean)) {
                     std::fprintf(stderr, "timeout: G=%d episode %d workgroup %d (second)\n", G, e, g);
                     ok = false;
                     return;
                 }
             }
+            group_exit(b, G);
         });
     for (auto& t : threads) t.join();
     return ok.load();
@@ -130,26 +134,29 @@ int main(int argc, char** argv) {
         for (int k = 0; k < launches; ++k) {
             state = state * 1103515245u + 12345u;
             const int G = 1 + static_cast<int>((state >> 16) % 64);
-            // odd episodes: an odd number of barriers, so the next launch
-            // starts on the other parity
+            // 2..10 barriers per launch (two per episode)
             const int episodes = 1 + static_cast<int>((state >> 8) % 5);
             if (!launch(b, slots, G, episodes, tag++, lean)) {
                 ++failures;
                 std::fprintf(stderr, "%s form failed at launch %d (G=%d)\n", lean ? "lean" : "fenced", k, G);
                 break;
             }
-            // an extra single barrier flips the parity for the next launch
+            // now and then a launch of a single barrier: an odd count, whose
+            // counters must be clean for the next launch as well
             if ((state >> 4) & 1) {
                 std::vector<std::thread> threads;
                 std::atomic<bool> ok{true};
                 for (int g = 0; g < G; ++g)
                     threads.emplace_back([&, g] {
-                        if (!group_sync(b, g, G, lean)) ok = false;
+                        uint32_t episode = 0;
+                        if (!group_sync(b, g, G, episode, lean)) ok = false;
+                        group_exit(b, G);
                     });
                 for (auto& t : threads) t.join();
                 if (!ok) {
                     ++failures;
-                    std::fprintf(stderr, "%s form: parity flip timed out (G=%d)\n", lean ? "lean" : "fenced", G);
+                    std::fprintf(stderr, "%s form: single-barrier launch timed out (G=%d)\n", lean ? "lean" : "fenced",
+                                 G);
                     break;
                 }
             }

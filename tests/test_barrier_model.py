@@ -1,8 +1,9 @@
 """The group barrier's counting protocol on the CPU (tests/cpp/barrier_model.cpp):
 a std::thread model of group_sync with the device's barrier-word layout,
 fenced and lean forms, group sizes changing between launches. A planted
-mistake (the last arrival clearing only its own leaf of the idle set) must
-make the model fail, so the model is shown to see that class of bug."""
+mistake (the first workgroup to leave a launch clearing the counters while
+others may still wait on them) must make the model fail, so the model is
+shown to see that class of bug."""
 import os
 import subprocess
 
@@ -25,8 +26,8 @@ def test_barrier_protocol_model():
     assert "fenced form: ok" in r.stdout and "lean form: ok" in r.stdout
 
 
-def test_barrier_model_catches_partial_clear():
-    r = subprocess.run([_build("barrier_model_mutant", "-DCLEAR_OWN_LEAF_ONLY"), "60"], capture_output=True,
+def test_barrier_model_catches_early_clear():
+    r = subprocess.run([_build("barrier_model_mutant", "-DCLEAR_AT_FIRST_EXIT"), "60"], capture_output=True,
                        text=True, timeout=300)
     print(r.stdout + r.stderr)
     assert r.returncode != 0
