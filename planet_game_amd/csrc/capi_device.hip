@@ -120,8 +120,9 @@ Launch solve_params(const psim_system* s, int op, double tolerance, int32_t max_
     p.op = op;
     p.tolerance = tolerance;
     p.max_iterations = max_iterations;
-    p.max_restarts = s->precision == PSIM_MIXED ? 40 : 8;
-    p.inner_relative = s->precision == PSIM_MIXED ? 1.0e-5 : 0.0;
+    // both forms iterate FP64 vectors: no inner target below the tolerance
+    p.max_restarts = 8;
+    p.inner_relative = 0.0;
     return p;
 }
 
@@ -484,8 +485,10 @@ int psim_system_create(psim_graph* g, int precision, psim_system** out) {
         A.plan(&I.b_alt, n);
         A.plan(&I.x[0], n);
         A.plan(&I.x[1], n);
+        // the BiCGSTAB vectors are FP64 in both forms (the mixed form runs the
+        // V-cycle's interior in FP32, on the level vectors above)
         void** vecs[12] = {&I.r, &I.r0, &I.r_alt, &I.r0_alt, &I.p[0], &I.p[1], &I.v, &I.ph, &I.s, &I.sh, &I.t, &I.u};
-        for (void** v : vecs) A.plan(reinterpret_cast<char**>(v), static_cast<size_t>(n) * vt);
+        for (void** v : vecs) A.plan(reinterpret_cast<char**>(v), static_cast<size_t>(n) * 8);
         I.partial_slots = std::max(tiles, g->depth > 1 ? g->views[0].down_tiles : 0);
         A.plan(&I.partial, 2 * static_cast<size_t>(I.partial_slots) * kPartials);
         A.plan(&s->csr_stage, static_cast<size_t>(g->nnz));

@@ -88,7 +88,7 @@ size_t carve(const psim_system* s, unsigned char* base, Instance& I) {
     I.x[0] = reinterpret_cast<double*>(take(n * 8));
     I.x[1] = reinterpret_cast<double*>(take(n * 8));
     void** vecs[12] = {&I.r, &I.r0, &I.r_alt, &I.r0_alt, &I.p[0], &I.p[1], &I.v, &I.ph, &I.s, &I.sh, &I.t, &I.u};
-    for (void** v : vecs) *v = take(n * vt);
+    for (void** v : vecs) *v = take(n * 8);   // FP64 in both forms
     for (int j = 0; j < I.depth && j < kMaxLevels; ++j) {
         const size_t nj = static_cast<size_t>(I.lv[j].n);
         I.lvl_b[j] = j > 0 ? take(nj * vt) : nullptr;

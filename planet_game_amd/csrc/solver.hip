@@ -644,11 +644,12 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
 // r1 = w - A x1 and x2 = x1 + om D^-1 r1 into `pre`, which down_phase and
 // the prolongation then read as the pre-smoothed iterate. Each row stores
 // w in `keep` and `keep2` (if set) and adds w^2 to its tile partial (if
-// `part` is set).
+// `part` is set). With `r1_out` set (the split V-cycle of the mixed form),
+// r1, computed in VT, is stored there rounded to FP32 instead of x2.
 template <class VT, int kW = 0>
 __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>& V, int j, const Source<VT>& src,
                                              VT* keep, VT* keep2, VT* pre, double* part, int g, int G, Lds& lds,
-                                             VT* au = nullptr) {
+                                             VT* au = nullptr, float* r1_out = nullptr) {
     constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
@@ -692,8 +693,12 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (ok[r]) {
-                const VT x1 = om * ld(dinv, i[r]) * wi[r];
-                publish(&pre[i[r]], static_cast<VT>(x1 + om * ld(dinv, i[r]) * acc[r]));
+                if (r1_out != nullptr) {
+                    publish(&r1_out[i[r]], static_cast<float>(acc[r]));
+                } else {
+                    const VT x1 = om * ld(dinv, i[r]) * wi[r];
+                    publish(&pre[i[r]], static_cast<VT>(x1 + om * ld(dinv, i[r]) * acc[r]));
+                }
                 if (keep != nullptr) publish(&keep[i[r]], wi[r]);
                 if (keep2 != nullptr) publish(&keep2[i[r]], wi[r]);
                 if (au != nullptr) publish(&au[i[r]], aw[r]);
@@ -763,6 +768,23 @@ __device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs
     __syncthreads();
 }
 
+// Level-0 output of the split V-cycle of the mixed form: its FP32 part
+// works on the residual r1 = w - A x1 of the FP64 first sweep
+// x1 = om D^-1 w, so the V-cycle's output is x1 + y, with x1 recomputed
+// from the FP64 input w in the FP64 operations of that sweep and y the FP32
+// part's level-0 result. out == nullptr: not split.
+struct SplitOut {
+    const double* w;     // the V-cycle's FP64 input
+    double* out;         // x1 + y
+    const double* dot;   // FP64 vector the output is dotted with (tile partials), or null
+};
+
+template <bool kF>
+__device__ __forceinline__ double split_value(const Instance& I, const SplitOut& so, int i, float y) {
+    const double om = I.omega[0];
+    return om * ld(gptr(static_cast<const double*>(I.dinv64[0])), i) * fetch<kF>(&so.w[i]) + static_cast<double>(y);
+}
+
 // Prolongation of the coarse correction fused with post-smoothing on level
 // j. The correction comes from `corr`, or, on the level above the coarsest,
 // from the coarse solve each workgroup recomputes into LDS. The pre-smoothed
@@ -770,7 +792,7 @@ __device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs
 template <class VT, int kW = 0>
 __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V, int j, const VT* rhs, const VT* corr, VT* out, int g,
                          int G, Lds& lds, double* dot_part = nullptr, const VT* dot_with = nullptr,
-                         const VT* pre = nullptr) {
+                         const VT* pre = nullptr, SplitOut so = SplitOut{}) {
     constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
     const LevelView& L = I.lv[j];
     const int n = L.n, W = L.width;
@@ -804,8 +826,17 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
                     },
                     [&](VT o, VT v) { acc -= o * v; });
                 const VT value = x2 + om * ld(dinv, i) * acc;
-                publish(&out[i], value);
-                q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i])) : 0.0;
+                if constexpr (sizeof(VT) == 4) {
+                    if (so.out != nullptr) {
+                        const double v = split_value<kF>(I, so, i, value);
+                        publish(&so.out[i], v);
+                        q[0] = so.dot != nullptr ? v * fetch<kF>(&so.dot[i]) : 0.0;
+                    }
+                }
+                if (so.out == nullptr) {
+                    publish(&out[i], value);
+                    q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i])) : 0.0;
+                }
             }
             if (dot_part != nullptr) store_partials<1>(dot_part, tile, q, lds);
         }
@@ -837,8 +868,18 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
             q[r][0] = 0.0;
             if (ok[r]) {
                 const VT value = x2[r] + om * ld(dinv, i[r]) * acc[r];
-                publish(&out[i[r]], value);
-                if (dot_with != nullptr) q[r][0] = static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i[r]]));
+                if constexpr (sizeof(VT) == 4) {
+                    if (so.out != nullptr) {
+                        const double v = split_value<kF>(I, so, i[r], value);
+                        publish(&so.out[i[r]], v);
+                        if (so.dot != nullptr) q[r][0] = v * fetch<kF>(&so.dot[i[r]]);
+                    }
+                }
+                if (so.out == nullptr) {
+                    publish(&out[i[r]], value);
+                    if (dot_with != nullptr)
+                        q[r][0] = static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i[r]]));
+                }
             }
         }
         if (dot_part != nullptr) store_partials_n<1, R>(dot_part, tl, q, lds);
@@ -857,7 +898,8 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
 // registers into scratch.
 template <class VT, int kW>
 __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, const VT* rhs, VT* out, int g, int G,
-                                             Lds& lds, double* dot_part, const VT* dot_with, int mask) {
+                                             Lds& lds, double* dot_part, const VT* dot_with, int mask,
+                                             SplitOut so = SplitOut{}) {
     const Instance& I = *uniform_ptr(instance);
     const Values<VT> V{I};
     j = __builtin_amdgcn_readfirstlane(j);
@@ -866,6 +908,7 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
     mask = __builtin_amdgcn_readfirstlane(mask);
     rhs = uniform_ptr(rhs);
     out = uniform_ptr(out);
+    so = SplitOut{uniform_ptr(so.w), uniform_ptr(so.out), uniform_ptr(so.dot)};
     // the two-sweep pre-smoothed iterates of level j and j + 1, where used
     const VT* pre = (mask >> j) & 1 ? uniform_ptr(static_cast<const VT*>(I.lvl_x[j])) : nullptr;
     const VT* pre_m = (mask >> (j + 1)) & 1 ? uniform_ptr(static_cast<const VT*>(I.lvl_x[j + 1])) : nullptr;
@@ -923,8 +966,17 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
                 },
                 [&](VT o, VT v) { acc -= o * v; });
             const VT value = x2 + om * ld(dinv, i) * acc;
-            publish(&out[i], value);
-            q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i])) : 0.0;
+            if constexpr (sizeof(VT) == 4) {
+                if (so.out != nullptr) {
+                    const double v = split_value<kF>(I, so, i, value);
+                    publish(&so.out[i], v);
+                    q[0] = so.dot != nullptr ? v * fetch<kF>(&so.dot[i]) : 0.0;
+                }
+            }
+            if (so.out == nullptr) {
+                publish(&out[i], value);
+                q[0] = dot_with != nullptr ? static_cast<double>(value) * static_cast<double>(fetch<kF>(&dot_with[i])) : 0.0;
+            }
         }
         if (dot_part != nullptr) store_partials<1>(dot_part, tile, q, lds);
         __syncthreads();
@@ -939,9 +991,14 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
 template <class VT, int kW = 0>
 __device__ __forceinline__ void vcycle_first(const Instance& I, const Values<VT>& V, const Source<VT>& src, VT* keep, double* part,
                              int g, int G, Lds& lds, int& timeouts, int mask = 0, VT* keep2 = nullptr,
-                             VT* au = nullptr) {
+                             VT* au = nullptr, bool split = false) {
     constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
-    if (I.depth == 1) {
+    if (split && I.depth > 1) {
+        // the split V-cycle's FP64 first sweep: r1 = w - A om D^-1 w into
+        // level 0's FP32 iterate buffer, the input of the FP32 part
+        jacobi_phase<VT, kW>(I, V, 0, src, keep, keep2, nullptr, part, g, G, lds, au,
+                             static_cast<float*>(I.lvl_x[0]));
+    } else if (I.depth == 1) {
         for (int tile = g; tile < I.tiles; tile += G) {
             const int i = tile * kTile + threadIdx.x;
             double sq[1] = {0.0};
@@ -971,9 +1028,15 @@ __device__ inline int first_sweep_tiles(const Instance& I, int mask) {
 // second pre-smoothing sweep: a Jacobi phase (level 0's ran in
 // vcycle_first), then the restriction sweep from its iterate. Ends with a
 // barrier.
+//
+// Split (the mixed form, VT = float, `so.out` set): vcycle_first ran the
+// FP64 first sweep and left r1 = w - A x1 in level 0's FP32 iterate buffer;
+// this FP32 V-cycle then runs on r1 with one pre-sweep from zero on level 0
+// (the second of its two) and writes x1 + y to so.out in FP64.
 template <class VT, int kW>
 __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>& V, const VT* w0, VT* out0, int g, int G, Lds& lds,
-                            int& timeouts, double* dot_part = nullptr, const VT* dot_with = nullptr, int mask = 0) {
+                            int& timeouts, double* dot_part = nullptr, const VT* dot_with = nullptr, int mask = 0,
+                            SplitOut so = SplitOut{}) {
     constexpr bool kF = kW >= 1;
     const int J = I.depth;
     if (J == 1) {
@@ -992,9 +1055,15 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
         group_sync<kF>(timeouts);
         return;
     }
+    const bool split = so.out != nullptr;
+    if (split) {
+        // level 0 of the FP32 part: input r1, one pre-sweep from zero
+        w0 = static_cast<const VT*>(I.lvl_x[0]);
+        mask &= ~1;
+    }
     const Source<VT> none{nullptr, nullptr, nullptr, 0.0, 0.0, 0};
     const auto pre_of = [&](int j) { return (mask >> j) & 1 ? static_cast<VT*>(I.lvl_x[j]) : nullptr; };
-    if (mask & 1) {
+    if (split || (mask & 1)) {
         Source<VT> src = none;
         src.a = w0;
         down_phase<VT, kW>(I, V, 0, src, nullptr, static_cast<VT*>(I.lvl_b[1]), nullptr, g, G, lds, pre_of(0));
@@ -1018,15 +1087,34 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
     for (int j = J - 2; j >= 0; --j) {
         const VT* rhs = j == 0 ? w0 : static_cast<const VT*>(I.lvl_b[j]);
         VT* out = j == 0 ? out0 : static_cast<VT*>(I.lvl_e[j]);
+        const SplitOut level_so = j == 0 ? so : SplitOut{};
         if (merge && j == jm + 1) continue;
         if (merge && j == jm)
             merged_up_phase<VT, kW>(&I, j, rhs, out, g, G, lds, j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr,
-                                    mask);
+                                    mask, level_so);
         else
             up_phase<VT, kW>(I, V, j, rhs, static_cast<const VT*>(I.lvl_e[j + 1]), out, g, G, lds,
-                             j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr, pre_of(j));
+                             j == 0 ? dot_part : nullptr, j == 0 ? dot_with : nullptr, pre_of(j), level_so);
         group_sync<kF>(timeouts);
     }
+}
+
+// The FP32 part of the split V-cycle (mixed form) out of line: its phases
+// are the FP32 instances of the V-cycle's, which would otherwise be inlined
+// beside the FP64 ones into the solve and add to its registers. Returns the
+// barrier timeout state.
+template <int kW>
+__device__ __noinline__ int vcycle_rest_split(const Instance* instance, const double* w0, double* out0, int g, int G,
+                                              Lds& lds, int timeouts, double* dot_part, const double* dot_with,
+                                              int mask) {
+    const Instance& I = *uniform_ptr(instance);
+    g = __builtin_amdgcn_readfirstlane(g);
+    G = __builtin_amdgcn_readfirstlane(G);
+    mask = __builtin_amdgcn_readfirstlane(mask);
+    const SplitOut so{uniform_ptr(w0), uniform_ptr(out0), uniform_ptr(dot_with)};
+    vcycle_rest<float, kW>(I, Values<float>{I}, nullptr, nullptr, g, G, lds, timeouts, uniform_ptr(dot_part), nullptr,
+                           mask, so);
+    return timeouts;
 }
 
 // ------------------------------------------------------------ set_matrix
@@ -1530,7 +1618,19 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     double r_norm = b_norm;
     int iterations = 0, restarts = 0, v_cycles = 0;
     int status = 0;
-    const int mask = pre_sweep_mask(I);   // levels with two pre-smoothing sweeps
+    // the mixed form: FP64 vectors and first level-0 sweep, the rest of the
+    // V-cycle in FP32 (vcycle_rest_split)
+    const bool split = sizeof(VT) == 8 && I.precision == 1 && I.depth > 1;
+    const int mask = pre_sweep_mask(I) | (split ? 1 : 0);   // levels with two pre-smoothing sweeps
+    const auto vcycle_rest_any = [&](const VT* w, VT* out, double* dp, const VT* dw) {
+        if constexpr (sizeof(VT) == 8) {
+            if (split) {
+                timeouts = vcycle_rest_split<kW>(&I, w, out, g, G, lds, timeouts, dp, dw, mask);
+                return;
+            }
+        }
+        vcycle_rest<VT, kW>(I, V, w, out, g, G, lds, timeouts, dp, dw, mask);
+    };
     // heat steps solve the assembled heat matrix, which is symmetric
     const bool symmetric = (P.op & kOpHeatRhs) != 0;
     bool shadow_stale = false;    // r0 must be set from r at the next pass's start
@@ -1606,10 +1706,10 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 const bool fuse = sizeof(VT) == 8 && first_iteration && symmetric && I.depth > 1;
                 VT* u = static_cast<VT*>(I.u);
                 vcycle_first<VT, kW>(I, V, src, p_new, nullptr, g, G, lds, timeouts, mask,
-                                     first_iteration && shadow_stale ? r0 : nullptr, fuse ? u : nullptr);
+                                     first_iteration && shadow_stale ? r0 : nullptr, fuse ? u : nullptr, split);
                 shadow_stale = false;
                 double* dot = fuse ? next_partials() : nullptr;
-                vcycle_rest<VT, kW>(I, V, p_new, ph, g, G, lds, timeouts, dot, fuse ? u : nullptr, mask);
+                vcycle_rest_any(p_new, ph, dot, fuse ? u : nullptr);
                 ++v_cycles;
                 if (fuse) {
                     double uph[1];
@@ -1702,12 +1802,12 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     }
                     alpha_in_x = true;
                     vcycle_first<VT, kW>(I, V, Source<VT>{s, nullptr, nullptr, 0.0, 0.0, 0}, nullptr, nullptr, g, G, lds,
-                                         timeouts, mask);
+                                         timeouts, mask, nullptr, nullptr, split);
                 } else {
                     // s = r - alpha v inside the next V-cycle's first sweep, (s, s)
                     part = next_partials();
                     vcycle_first<VT, kW>(I, V, Source<VT>{r, v, nullptr, alpha, 0.0, 2}, s, part, g, G, lds, timeouts,
-                                         mask);
+                                         mask, nullptr, nullptr, split);
                     double ss[1];
                     group_totals<1, kF>(part, first_sweep_tiles(I, mask), ss, lds);
                     ++iterations;
@@ -1747,7 +1847,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     }
                 }
                 first_iteration = false;
-                vcycle_rest<VT, kW>(I, V, s, sh, g, G, lds, timeouts, nullptr, nullptr, mask);
+                vcycle_rest_any(s, sh, nullptr, nullptr);
                 ++v_cycles;
                 // t = A sh, (t, t), (t, s)
                 part = next_partials();
@@ -2048,10 +2148,7 @@ __device__ __noinline__ SyncState run_solve_call(const Instance* instance, const
     const Instance& I = *uniform_ptr(instance);
     g = __builtin_amdgcn_readfirstlane(g);
     G = __builtin_amdgcn_readfirstlane(G);
-    if (I.precision == 0)
-        run_solve<double, kW>(I, P, g, G, lds, st.timeouts, st.parity);
-    else
-        run_solve<float, kW>(I, P, g, G, lds, st.timeouts, st.parity);
+    run_solve<double, kW>(I, P, g, G, lds, st.timeouts, st.parity);
     return st;
 }
 
@@ -2128,10 +2225,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
     }
     if (P.op & kOpSolve) {
         if constexpr (kWaves == 1) {
-            if (I.precision == 0)
-                run_solve<double, kWaves>(I, P, g, G, lds, timeouts, parity);
-            else
-                run_solve<float, kWaves>(I, P, g, G, lds, timeouts, parity);
+            run_solve<double, kWaves>(I, P, g, G, lds, timeouts, parity);
         } else {
             const SyncState st = run_solve_call<kWaves>(&I, P, g, G, lds, SyncState{timeouts, parity});
             timeouts = st.timeouts;
@@ -2210,10 +2304,7 @@ __device__ __noinline__ int serve_one(const Instance* instance, const Launch& P,
         timeouts = st.timeouts;
         parity = st.parity;
     }
-    if (I.precision == 0)
-        run_solve<double, kW>(I, P, g, G, lds, timeouts, parity);
-    else
-        run_solve<float, kW>(I, P, g, G, lds, timeouts, parity);
+    run_solve<double, kW>(I, P, g, G, lds, timeouts, parity);
     // every workgroup's rows of T are written before the completion
     group_sync(timeouts);
     return timeouts;
