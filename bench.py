@@ -62,15 +62,19 @@ def measured_traffic(path=TRAFFIC_FILE):
         return None
 
 
-def algorithmic_bytes(levels, v_cycles, restarts, vb=8):
+def algorithmic_bytes(levels, v_cycles, restarts, mixed=False):
     """Bytes one solve must move at least (int32 indices, each array once per
-    operation, neighbour gathers from cache), with vb-byte values in the
-    V-cycle, SpMV and inner vectors (8 plain, 4 mixed) and FP64 elsewhere:
-      SpMV at level j    (3 vb + 4) n_j + (vb + 4) nnz_j + 4
-      V-cycle            sum_{j<coarsest} (3 B_A(j) + vb n_j + 2 vb n_{j+1}) + 8 n_c^2
-      half-iteration     B_A(0) + B_V + 11 vb n (the halves' vector updates)
+    operation, neighbour gathers from cache). The BiCGSTAB vectors are FP64
+    in both forms; the V-cycle's values are FP64 (plain) or, in the mixed
+    form's split V-cycle, FP64 for the first level-0 sweep and FP32 after it:
+      SpMV at level j, v-byte values   B_A(j, v) = (3 v + 4) n_j + (v + 4) nnz_j + 4
+      V-cycle, plain     sum_{j<coarsest} (3 B_A(j, 8) + 8 n_j + 16 n_{j+1}) + 8 n_c^2
+      V-cycle, split     B_A(0, 8) + 2 B_A(0, 4) + 28 n_0 + 8 n_1     (r1 out and in, w and
+                                                                       D^-1 again, x1 + y out)
+                         + sum_{1<=j<coarsest} (3 B_A(j, 4) + 4 n_j + 8 n_{j+1}) + 8 n_c^2
+      half-iteration     B_A(0, 8) + B_V + 88 n (the halves' vector updates)
       per solve          48 n (RHS assembly and ||b||) + halves
-                         + (1 + restarts)(B_A(0) in FP64 + 16 n) (true residuals)
+                         + (1 + restarts)(B_A(0, 8) + 16 n) (true residuals)
     """
     def b_a(n, nnz, w):
         return (3 * w + 4) * n + (w + 4) * nnz + 4
@@ -79,20 +83,29 @@ def algorithmic_bytes(levels, v_cycles, restarts, vb=8):
     b_v = 8 * levels[-1][0] ** 2
     for j in range(len(levels) - 1):
         n, nnz = levels[j]
-        b_v += 3 * b_a(n, nnz, vb) + vb * n + 2 * vb * levels[j + 1][0]
-    half = b_a(n0, nnz0, vb) + b_v + 11 * vb * n0
+        nn = levels[j + 1][0]
+        if not mixed:
+            b_v += 3 * b_a(n, nnz, 8) + 8 * n + 16 * nn
+        elif j == 0:
+            b_v += b_a(n, nnz, 8) + 2 * b_a(n, nnz, 4) + 28 * n + 8 * nn
+        else:
+            b_v += 3 * b_a(n, nnz, 4) + 4 * n + 8 * nn
+    half = b_a(n0, nnz0, 8) + b_v + 88 * n0
     return 48 * n0 + v_cycles * half + (1 + restarts) * (b_a(n0, nnz0, 8) + 16 * n0)
 
 
-def set_matrix_bytes(levels, edges):
+def set_matrix_bytes(levels, edges, mixed=False):
     """Assembly (read C A, K_e, w; write diag and off), Galerkin sweeps and
-    the dense coarsest level."""
+    the dense coarsest level; the mixed form also writes FP32 copies of every
+    level's diagonal, inverse diagonal and off-diagonals."""
     n0, nnz0 = levels[0]
     total = 16 * n0 + 8 * edges + 16 * nnz0 + 8 * levels[-1][0] ** 2
     for j in range(len(levels) - 1):
         n, nnz = levels[j]
         nc, nnzc = levels[j + 1]
         total += 12 * n + 12 * nnz + 8 * nc + 8 * nnzc
+    if mixed:
+        total += sum(8 * n + 4 * nnz for n, nnz in levels)
     return total
 
 
@@ -287,9 +300,9 @@ def mixed_section(P, ctx, mesh, graph, levels, with_oracle):
         ctx.record(1)
         ctx.synchronize()
         ms = ctx.elapsed_ms(0, 1) / 3
-        vb = 4 if precision == P.MIXED else 8
-        nbytes = set_matrix_bytes(levels, mesh.edges) + sum(
-            algorithmic_bytes(levels, i.v_cycles, i.restarts, vb) for i in census)
+        mixed = precision == P.MIXED
+        nbytes = set_matrix_bytes(levels, mesh.edges, mixed) + sum(
+            algorithmic_bytes(levels, i.v_cycles, i.restarts, mixed) for i in census)
         out[name] = {
             "solves_per_s": round(CHAIN / ms * 1e3, 2),
             "iterations_per_solve": float(np.mean([i.iterations for i in census])),
@@ -369,8 +382,8 @@ def batched_section(P, ctx, meshes, graphs, hierarchy_levels, precision=None, au
     nbytes = 0
     for x, info in zip(members, infos):
         lv = hierarchy_levels[x["level"]]
-        nbytes += set_matrix_bytes(lv, meshes[x["level"]].edges) + algorithmic_bytes(
-            lv, info.v_cycles, info.restarts, 4 if precision == P.MIXED else 8)
+        nbytes += set_matrix_bytes(lv, meshes[x["level"]].edges, precision == P.MIXED) + algorithmic_bytes(
+            lv, info.v_cycles, info.restarts, precision == P.MIXED)
     best = min(ms_default, ms_tuned)
     groups = batch.groups
     by_level = {}

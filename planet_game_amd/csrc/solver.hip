@@ -1491,7 +1491,9 @@ struct Prepared {
     double totals[2] = {0.0, 0.0};
 };
 
-template <class VT, int kW>
+// kForm: 0 the plain form, 1 the mixed form's split V-cycle, 2 either, by
+// the instance's precision at run time.
+template <class VT, int kW, int kForm = 0>
 //
 // Chained steps in the full-register-file program (kW == 1): with
 // next_forcing set, every phase that leaves x final (the half-step exit and
@@ -1620,12 +1622,19 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     int status = 0;
     // the mixed form: FP64 vectors and first level-0 sweep, the rest of the
     // V-cycle in FP32 (vcycle_rest_split)
-    const bool split = sizeof(VT) == 8 && I.precision == 1 && I.depth > 1;
+    const bool split = sizeof(VT) == 8 && I.depth > 1 && (kForm == 1 || (kForm == 2 && I.precision == 1));
     const int mask = pre_sweep_mask(I) | (split ? 1 : 0);   // levels with two pre-smoothing sweeps
     const auto vcycle_rest_any = [&](const VT* w, VT* out, double* dp, const VT* dw) {
-        if constexpr (sizeof(VT) == 8) {
+        if constexpr (kForm != 0 && sizeof(VT) == 8) {
             if (split) {
-                timeouts = vcycle_rest_split<kW>(&I, w, out, g, G, lds, timeouts, dp, dw, mask);
+                if constexpr (kW == 1) {
+                    // the whole register file: the FP32 part inlined (a
+                    // separate out-of-line solve instance, run_heat_solve)
+                    vcycle_rest<float, kW>(I, Values<float>{I}, nullptr, nullptr, g, G, lds, timeouts, dp, nullptr, mask,
+                                           SplitOut{w, out, dw});
+                } else {
+                    timeouts = vcycle_rest_split<kW>(&I, w, out, g, G, lds, timeouts, dp, dw, mask);
+                }
                 return;
             }
         }
@@ -2102,7 +2111,7 @@ __device__ __noinline__ SolveResult pcg_call(const Instance* instance, const Lau
 
 // ------------------------------------------------------------ the kernel
 
-template <class VT, int kW>
+template <class VT, int kW, int kForm>
 __device__ __forceinline__ void run_solve(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts, int& parity) {
     int xi = 0, rp = 0;
     Prepared prepared;
@@ -2123,7 +2132,7 @@ __device__ __forceinline__ void run_solve(const Instance& I, const Launch& P, in
             if (kW == 1 && step + 1 < steps) next_forcing = forcing + I.n;
             t_new = step + 1 == steps ? P.t_new : nullptr;
         }
-        last = solve<VT, kW>(I, P, t_old, forcing, next_forcing, t_new, g, G, lds, timeouts, parity, xi, rp,
+        last = solve<VT, kW, kForm>(I, P, t_old, forcing, next_forcing, t_new, g, G, lds, timeouts, parity, xi, rp,
                              prepared);
         if (g == 0 && threadIdx.x == 0 && P.step_results != nullptr) P.step_results[step] = last;
     }
@@ -2139,6 +2148,40 @@ __device__ __forceinline__ void run_solve(const Instance& I, const Launch& P, in
     }
 }
 
+// The heat solve in the system's form: FP64 vectors in both, the mixed form
+// with the split V-cycle. Where the split lives was chosen by measurement
+// (the FP32 part's registers and its call affect the FP64 phases' spills):
+// with the whole register file (kW == 1, single instances) the mixed form is
+// its own instance of the solve, out of line, with the FP32 part inlined, and
+// the plain solve compiles as if the mixed form did not exist; in the
+// programs of two and three workgroups per CU (batches, servers) one solve
+// picks the split at run time and calls the FP32 part out of line.
+template <int kW>
+__device__ __noinline__ SyncState run_split_solve_call(const Instance* instance, const Launch& P, int g, int G, Lds& lds,
+                                                       SyncState st) {
+    const Instance& I = *uniform_ptr(instance);
+    g = __builtin_amdgcn_readfirstlane(g);
+    G = __builtin_amdgcn_readfirstlane(G);
+    run_solve<double, kW, 1>(I, P, g, G, lds, st.timeouts, st.parity);
+    return st;
+}
+
+template <int kW>
+__device__ __forceinline__ void run_heat_solve(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts,
+                                               int& parity) {
+    if constexpr (kW == 1) {
+        if (I.precision == 1) {
+            const SyncState st = run_split_solve_call<kW>(&I, P, g, G, lds, SyncState{timeouts, parity});
+            timeouts = st.timeouts;
+            parity = st.parity;
+        } else {
+            run_solve<double, kW, 0>(I, P, g, G, lds, timeouts, parity);
+        }
+    } else {
+        run_solve<double, kW, 2>(I, P, g, G, lds, timeouts, parity);
+    }
+}
+
 // The solve out of line, for the three-workgroups-per-CU program: inlined
 // there, the V-cycle variants of both precisions push the kernel's live
 // state into scratch; a call keeps the solve's registers its own.
@@ -2148,7 +2191,7 @@ __device__ __noinline__ SyncState run_solve_call(const Instance* instance, const
     const Instance& I = *uniform_ptr(instance);
     g = __builtin_amdgcn_readfirstlane(g);
     G = __builtin_amdgcn_readfirstlane(G);
-    run_solve<double, kW>(I, P, g, G, lds, st.timeouts, st.parity);
+    run_heat_solve<kW>(I, P, g, G, lds, st.timeouts, st.parity);
     return st;
 }
 
@@ -2225,7 +2268,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
     }
     if (P.op & kOpSolve) {
         if constexpr (kWaves == 1) {
-            run_solve<double, kWaves>(I, P, g, G, lds, timeouts, parity);
+            run_heat_solve<kWaves>(I, P, g, G, lds, timeouts, parity);
         } else {
             const SyncState st = run_solve_call<kWaves>(&I, P, g, G, lds, SyncState{timeouts, parity});
             timeouts = st.timeouts;
@@ -2304,7 +2347,7 @@ __device__ __noinline__ int serve_one(const Instance* instance, const Launch& P,
         timeouts = st.timeouts;
         parity = st.parity;
     }
-    run_solve<double, kW>(I, P, g, G, lds, timeouts, parity);
+    run_heat_solve<kW>(I, P, g, G, lds, timeouts, parity);
     // every workgroup's rows of T are written before the completion
     group_sync(timeouts);
     return timeouts;
