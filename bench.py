@@ -219,7 +219,8 @@ def fv_section(P, ctx, mesh, with_cpu, cpu_workers, repeats=50):
     """Config 1's operator half: 64 seeded FP64 fields through the LSQ
     gradient and the edge-flux divergence at L6 (cell scalars and edge fluxes
     u - 0.5, as test_operator_determinism.cpp:15-26), one launch per operator
-    for all 64 fields; HIP events on the library's stream."""
+    for all 64 fields; HIP events on the library's stream around `repeats`
+    back-to-back launches (the mean launch-to-launch time)."""
     fields = 64
     n, e = mesh.cells, mesh.edges
     rng = np.random.default_rng(20261205)
@@ -235,15 +236,14 @@ def fv_section(P, ctx, mesh, with_cpu, cpu_workers, repeats=50):
     for name, call in (("gradient", lambda: fv.gradient_device(P.F64, fields, d_phi.ptr, d_east.ptr, d_north.ptr)),
                        ("divergence", lambda: fv.divergence_device(P.F64, fields, d_flux.ptr, d_div.ptr))):
         call()
-        ctx.events(repeats + 1)
+        ctx.events(2)
         ctx.synchronize()
         ctx.record(0)
         for k in range(repeats):
             call()
-            ctx.record(k + 1)
+        ctx.record(1)
         ctx.synchronize()
-        ms = np.array([ctx.elapsed_ms(k, k + 1) for k in range(repeats)])
-        timings[name] = float(ms.mean())
+        timings[name] = ctx.elapsed_ms(0, 1) / repeats
     g_bytes, d_bytes = fv_bytes(fields, n, e, mesh.directed)
     for name, nbytes in (("gradient", g_bytes), ("divergence", d_bytes)):
         gbs = nbytes / (timings[name] / 1e3) / 1e9
@@ -526,28 +526,46 @@ def served_section(P, ctx, meshes, graphs, plain_value, sweep=SWEEP, profile=Fal
     out = {"config": {"workload": "Poisson single-step requests, L3:L4:L5:L6 = 40:30:20:10, half continuing a "
                                   "resident 12-step session, mixed precision, persistent server, matrices kept "
                                   "by key (8 per level)", "tolerance": TOL}}
-    server = P.Server(ctx, [graphs[L] for L in levels], share=shares, precision=P.MIXED, memory_budget=8 << 30)
     if profile:
         # two server launches of the same all-at-once stream: cold, then with
         # the bank's matrices kept
+        server = P.Server(ctx, [graphs[L] for L in levels], share=shares, precision=P.MIXED, memory_budget=8 << 30)
         out["cold"] = run(server, levels, shares, float("inf"), 4000, 17)
         out["warm"] = run(server, levels, shares, float("inf"), 4000, 17)
         server.close()
         return out
+    # The slot plan is the deployment's latency/throughput choice: slots of
+    # ceil(tiles / 2) workgroups (two 256-row tiles per workgroup per phase)
+    # for the offered-rate sweep, whose load needs few slots; the default plan
+    # (about 12 tiles per workgroup, many small slots) for saturation.
+    latency_sizes = [max(1, -(-((meshes[L].cells + 255) // 256) // 2)) for L in levels]
+    server = P.Server(ctx, [graphs[L] for L in levels], share=shares, precision=P.MIXED, memory_budget=8 << 30,
+                      slot_sizes=latency_sizes)
+    # first an all-at-once stream, which also assembles every bank matrix
+    # once (a running server holds them by key), then the offered rates
+    out["all_at_once_on_sweep_plan"] = run(server, levels, shares, float("inf"), 10000, 17)
     out["sweep"] = [run(server, levels, shares, rate, count, 11 + i) for i, (rate, count) in enumerate(sweep)]
+    out["sweep_plan"] = {str(L): [p["slots"], p["slot_size"]] for L, p in zip(levels, server.plan())}
+    server.close()
+    server = P.Server(ctx, [graphs[L] for L in levels], share=shares, precision=P.MIXED, memory_budget=8 << 30)
     out["all_at_once"] = run(server, levels, shares, float("inf"), 10000, 17)
     plan = server.plan()
     out["config"]["slots"] = {str(L): [p["slots"], p["slot_size"], p["pool"]] for L, p in zip(levels, plan)}
+    out["config"]["system_bytes"] = {str(L): p["system_bytes"] for L, p in zip(levels, plan)}
     server.close()
     # L6 only, all requests at t = 0: a cold run (every session assembles its
     # matrix) and the same stream again on the same server (the matrices of
     # the bank stay by key, as a running server keeps them); pool sized for
-    # 32 GiB of HBM
-    for name, precision in (("l6_only_saturated", P.MIXED), ("l6_only_saturated_plain_fp64", P.PLAIN_FP64)):
-        l6 = P.Server(ctx, [graphs[6]], precision=precision, memory_budget=32 << 30, slot_sizes=[4])
+    # 32 GiB of HBM; slots of 4 workgroups, at the occupancy that measured
+    # fastest for each form (3 workgroups per CU mixed, 2 plain)
+    for name, precision, occupancy in (("l6_only_saturated", P.MIXED, 3),
+                                       ("l6_only_saturated_plain_fp64", P.PLAIN_FP64, 2)):
+        l6 = P.Server(ctx, [graphs[6]], precision=precision, memory_budget=32 << 30, slot_sizes=[4],
+                      occupancy=occupancy)
         cold = run(l6, [6], [1.0], float("inf"), 6000, 99)
         out[name] = run(l6, [6], [1.0], float("inf"), 6000, 99)
         out[name]["cold_start_solves_per_s"] = cold["solves_per_s"]
+        out[name]["workgroups_per_cu"] = occupancy
         l6.close()
     out["l6_served_mixed_vs_plain_fp64_chain"] = round(out["l6_only_saturated"]["solves_per_s"] / plain_value, 2)
     out["l6_served_plain_vs_plain_fp64_chain"] = round(
