@@ -82,7 +82,8 @@ def test_fv_operators_bit_exact(ctx, level, dtype):
 
 
 def test_fv_sixty_four_fields_l6(ctx):
-    """Config 1's operator sweep: 64 seeded FP64 fields at L6 in one call."""
+    """Config 1's operator sweep: 64 seeded FP64 fields at L6 in one call,
+    every field bit-identical to the oracle."""
     level, fields = 6, 64
     announce("fv 64 fields", level)
     m = product_mesh(level)
@@ -95,7 +96,7 @@ def test_fv_sixty_four_fields_l6(ctx):
     div = fv.divergence(flux)
     o = S.oracle()
     n, e = m.cells, m.edges
-    for f in (0, 17, 63):
+    for f in range(fields):
         re, rn, rd = np.zeros(n), np.zeros(n), np.zeros(n)
         o.gradient_f64(om.handle, phi[f * n:(f + 1) * n], re, rn, 8)
         o.divergence_f64(om.handle, flux[f * e:(f + 1) * e], rd, 8)
@@ -282,8 +283,11 @@ def test_workgroup_count_does_not_change_bits(ctx):
 
 @pytest.mark.parametrize("level", [4, 6])
 def test_mixed_precision_error_vs_fp64_oracle(ctx, level):
-    """Mixed form: FP32 inner BiCGSTAB/V-cycle in FP64 refinement reaches the
-    FP64 true residual 1e-10 and <= 1e-8 relative error vs the FP64 oracle."""
+    """Mixed form (FP64 BiCGSTAB, split V-cycle: FP64 first level-0 sweep,
+    FP32 rest) reaches the FP64 true residual 1e-10 and <= 1e-8 relative
+    error vs the FP64 oracle, in one V-cycle and no restart, as the plain
+    form does on these inputs (an FP32 V-cycle without the FP64 first sweep
+    leaves about 6e-8 and needs a second one)."""
     announce("mixed precision", level)
     m, seed, cond, ca, land, t_old, forcing = heat_case(level, sigma=1.0)
     x_ref, _, diag, off, b = oracle_solution(level, cond, ca, t_old, forcing)
@@ -295,8 +299,13 @@ def test_mixed_precision_error_vs_fp64_oracle(ctx, level):
     true_res = np.linalg.norm(b - csr_apply(m, diag, off, x)) / np.linalg.norm(b)
     assert true_res <= 1e-10
     err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
-    print("mixed error", err, "restarts", info.restarts, "iterations", info.iterations, flush=True)
+    print("mixed error", err, "restarts", info.restarts, "iterations", info.iterations, "v-cycles", info.v_cycles,
+          flush=True)
     assert err <= 1e-8, err
+    plain = P.System(g, P.PLAIN_FP64)
+    plain.heat_set_matrix(m.entry_values(cond), ca)
+    _, plain_info = plain.heat_step(t_old, forcing, tolerance=1e-10)
+    assert info.restarts == 0 and info.v_cycles == plain_info.v_cycles, (info.v_cycles, plain_info.v_cycles)
 
 
 def test_chained_steps_track_oracle(ctx):
