@@ -556,17 +556,22 @@ def served_section(P, ctx, meshes, graphs, plain_value, sweep=SWEEP, profile=Fal
     # L6 only, all requests at t = 0: a cold run (every session assembles its
     # matrix) and the same stream again on the same server (the matrices of
     # the bank stay by key, as a running server keeps them); pool sized for
-    # 32 GiB of HBM; slots of 4 workgroups, at the occupancy that measured
-    # fastest for each form (3 workgroups per CU mixed, 2 plain)
-    for name, precision, occupancy in (("l6_only_saturated", P.MIXED, 3),
-                                       ("l6_only_saturated_plain_fp64", P.PLAIN_FP64, 2)):
-        l6 = P.Server(ctx, [graphs[6]], precision=precision, memory_budget=32 << 30, slot_sizes=[4],
-                      occupancy=occupancy)
-        cold = run(l6, [6], [1.0], float("inf"), 6000, 99)
-        out[name] = run(l6, [6], [1.0], float("inf"), 6000, 99)
-        out[name]["cold_start_solves_per_s"] = cold["solves_per_s"]
-        out[name]["workgroups_per_cu"] = occupancy
-        l6.close()
+    # 32 GiB of HBM; slots of 4 workgroups. The server program's occupancy
+    # (two or three workgroups per CU) is a deployment choice: both are run
+    # and the faster warm one is reported, with the other beside it.
+    for name, precision in (("l6_only_saturated", P.MIXED), ("l6_only_saturated_plain_fp64", P.PLAIN_FP64)):
+        runs = {}
+        for occupancy in (3, 2):
+            l6 = P.Server(ctx, [graphs[6]], precision=precision, memory_budget=32 << 30, slot_sizes=[4],
+                          occupancy=occupancy)
+            cold = run(l6, [6], [1.0], float("inf"), 6000, 99)
+            runs[occupancy] = run(l6, [6], [1.0], float("inf"), 6000, 99)
+            runs[occupancy]["cold_start_solves_per_s"] = cold["solves_per_s"]
+            runs[occupancy]["workgroups_per_cu"] = occupancy
+            l6.close()
+        best = max(runs, key=lambda o: runs[o]["solves_per_s"])
+        out[name] = runs[best]
+        out[name]["other_occupancy_solves_per_s"] = runs[5 - best]["solves_per_s"]
     out["l6_served_mixed_vs_plain_fp64_chain"] = round(out["l6_only_saturated"]["solves_per_s"] / plain_value, 2)
     out["l6_served_plain_vs_plain_fp64_chain"] = round(
         out["l6_only_saturated_plain_fp64"]["solves_per_s"] / plain_value, 2)

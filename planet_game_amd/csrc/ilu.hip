@@ -23,6 +23,7 @@
 // and the flux form of the solution run on the host, in the reference's
 // order.
 #include "system.hpp"
+#include "wave.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -239,10 +240,10 @@ __device__ bool grid_sync(uint32_t* words, uint32_t blocks, long long limit) {
     return ok != 0;
 }
 
-// Fixed-order workgroup sum of K values: wave butterflies, then waves in order.
+// Fixed-order workgroup sum of K values: wave reductions (wave_sum), then
+// waves in order.
 template <int K> __device__ void block_sums(double (&v)[K], double* red) {
-    for (int k = 0; k < K; ++k)
-        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0)
         for (int k = 0; k < K; ++k) red[k * 4 + wave] = v[k];

@@ -16,6 +16,7 @@
 // vectors: its dot products are per-256-row-block partials reduced in a
 // fixed order on the device and added in block order on the host.
 #include "system.hpp"
+#include "wave.hpp"
 
 #include <cmath>
 #include <memory>
@@ -64,13 +65,13 @@ __global__ __launch_bounds__(kOpThreads) void update_kernel(int n, double* x, co
     x[i] += b != nullptr ? c1 * a[i] + c2 * b[i] : c1 * a[i];
 }
 
-// Per block of 256 rows: sum of u v (fixed order: wave butterflies, then
+// Per block of 256 rows: sum of u v (fixed order: wave reductions, then
 // waves in order).
 __global__ __launch_bounds__(kOpThreads) void dot_kernel(int n, const double* u, const double* v, double* partial) {
     __shared__ double red[4];
     const int i = static_cast<int>(blockIdx.x) * kOpThreads + static_cast<int>(threadIdx.x);
     double s = i < n ? u[i] * v[i] : 0.0;
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];

@@ -25,13 +25,14 @@
 //     s = r - alpha v) are evaluated inside its first sweep (neighbours
 //     recompute them; each row stores its own), and the x update after a
 //     half-step exit is fused with the FP64 true-residual SpMV;
-//   * inner products reduce each tile with a fixed butterfly, then the
-//     tile partials in a fixed order, so every workgroup computes the same
+//   * inner products reduce each wave with DPP and swizzle moves in a fixed
+//     order (wave.hpp), then the waves and the tile partials in a fixed order, so every workgroup computes the same
 //     scalars and results do not depend on the group size or the batch;
 //   * BiCGSTAB stops on the recursive residual as the reference does, then
 //     the FP64 residual b - A x is recomputed; above tolerance the solve
 //     restarts from x (the mixed-precision form relies on this refinement).
 #include "device.hpp"
+#include "wave.hpp"
 
 #include <type_traits>
 
@@ -101,17 +102,6 @@ template <class P> __device__ __forceinline__ elem_t<P> ld(P p, int i) {
     return ((const __attribute__((address_space(1))) T*)p)[i];
 }
 
-__device__ inline double wave_sum(double v) {
-    // xor butterfly: every lane ends with the same bits
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ inline double wave_max(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
 // The coarsest inverse's MFMA blocks: 16 x 16, kMaxBlocks of them along a
 // side for up to kBlockedCoarse rows.
 constexpr int kMB = kSweepBlock;
@@ -148,8 +138,8 @@ __device__ inline double* dynamic_lds() {
     return dyn;
 }
 
-// Sum of K per-thread values over the workgroup, fixed order: wave
-// butterflies, then waves 0..3 left to right. Thread 0 gets the totals.
+// Sum of K per-thread values over the workgroup, fixed order: wave_sum in
+// each wave, then waves 0..3 left to right. Thread 0 gets the totals.
 template <int K> __device__ inline void tile_sums(const double (&v)[K], double (&out)[K], Lds& lds) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -712,7 +702,7 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
 // skipped), or rows first .. first + count - 1 when list is null. One wave
 // per row: lanes split the columns (row-major reads of the symmetric
 // inverse, coalesced), each lane adding its columns in ascending order from
-// 0, and a butterfly sums the lanes, so every workgroup computing a row
+// 0, and wave_sum adds the lanes in its fixed order, so every workgroup computing a row
 // gets the same bits. A wave takes kRows rows at a time; the first round's
 // row ids are loaded before the right-hand side is staged.
 template <class VT, bool kF = false>
