@@ -205,6 +205,49 @@ def cpu_baseline(mesh, cond, ca, t0, forcings, seconds, workers=1):
     return solves / elapsed, solves, chains, elapsed, final, kind
 
 
+def config0_section(P, ctx, seconds=3.0):
+    """Config 0's size (L4, 2,562 cells), the literal distributions: the
+    120-step chain on the GPU (plain FP64, one launch per pass) beside the
+    reference CPU path on a bounded sample of the same chain."""
+    mesh = P.Mesh(4)
+    seed = P.instance_seed(0)
+    cond, ca, _, t0 = P.heat_instance(mesh, seed, 1.0e6, 0.3)
+    forcings = [P.heat_forcing(mesh, seed, s) for s in range(CHAIN)]
+    graph = P.Graph(ctx, mesh)
+    system = P.System(graph, P.PLAIN_FP64)
+    d_cond = P.DeviceArray.of(ctx, mesh.entry_values(cond))
+    d_ca = P.DeviceArray.of(ctx, ca)
+    d_forcing = P.DeviceArray.of(ctx, np.concatenate(forcings))
+    d_t0 = P.DeviceArray.of(ctx, t0)
+    d_out = P.DeviceArray(ctx, mesh.cells)
+    infos = (P.SolveInfo * CHAIN)()
+
+    def run_pass(collect):
+        system.heat_chain_device(d_t0.ptr, d_forcing.ptr, CHAIN, d_out.ptr, DT, TOL, 1000,
+                                 step_infos=infos if collect else None,
+                                 conductance_per_entry=d_cond.ptr, capacity_area=d_ca.ptr)
+
+    run_pass(True)
+    converged = all(i.status == 0 and i.relative_residual <= TOL for i in infos)
+    step_ms = np.array([i.device_ms for i in infos])
+    ctx.synchronize()
+    passes = 20
+    t = time.perf_counter()
+    for _ in range(passes):
+        run_pass(False)
+    ctx.synchronize()
+    gpu_rate = passes * CHAIN / (time.perf_counter() - t)
+    rate, solves, _, elapsed, cpu_final, kind = cpu_baseline(mesh, cond, ca, t0, forcings, seconds, 1)
+    return {"config": {"workload": "L4 plain FP64: set_matrix + 120 chained monthly implicit heat steps per pass",
+                       "cells": mesh.cells, "tolerance": TOL, "workgroups": system.workgroups},
+            "gpu_solves_per_s": round(gpu_rate, 1),
+            "gpu_p50_ms": round(float(np.percentile(step_ms, 50)), 5),
+            "gpu_p99_ms": round(float(np.percentile(step_ms, 99)), 5),
+            "converged": converged,
+            "cpu_solves_per_s": round(rate, 1), "cpu_kind": kind, "cpu_threads": 1,
+            "cpu_sample": f"{solves} solves in {elapsed:.1f} s"}
+
+
 def fv_bytes(fields, n, edges, directed):
     """Algorithmic bytes of the operator sweeps for F fields (int32 indices,
     geometry read once per launch, neighbour gathers from cache):
@@ -1079,6 +1122,7 @@ def main():
             np.linalg.norm(tight_final - tight_cpu) / np.linalg.norm(tight_cpu))
         line["error_vs_fp64_oracle_per_step"] = step_errors_vs_oracle(mesh, cond, ca, step_temps, forcings)
         line["fv_sweeps"] = fv_section(P, ctx, mesh, True, best)
+        line["config0_l4"] = config0_section(P, ctx)
     else:
         line["fv_sweeps"] = fv_section(P, ctx, mesh, False, 1)
     if not args.no_extras:

@@ -590,7 +590,7 @@ struct HeatStepSettings {
     double dt_s = 2.63e6;
     double relative_tolerance = 1.0e-10;
     int max_iterations = 1000;
-    bool mixed_precision = false;   // FP32 inner solve inside FP64 refinement
+    bool mixed_precision = false;   // PSIM_MIXED: FP32 V-cycle after an FP64 first sweep, FP64 BiCGSTAB
 };
 
 struct HeatStepResult {

@@ -40,9 +40,11 @@ enum psim_status {
 enum psim_where { PSIM_HOST = 0, PSIM_DEVICE = 1 };
 enum psim_dtype { PSIM_F64 = 0, PSIM_F32 = 1 };
 /* Solver precision: PSIM_PLAIN_FP64 keeps everything in FP64;
- * PSIM_MIXED runs the inner BiCGSTAB vectors, SpMV and V-cycle in FP32
- * inside FP64 residual refinement (FP64 residuals, inner products and
- * coarsest solve). */
+ * PSIM_MIXED keeps the BiCGSTAB vectors, inner products, true residual and
+ * coarsest solve in FP64 and runs the V-cycle after its first level-0
+ * Jacobi sweep in FP32 (every level's operator is also stored in FP32): the
+ * FP64 first sweep carries the part of the preconditioned residual that
+ * FP32 could not represent to 1e-10. */
 enum psim_precision { PSIM_PLAIN_FP64 = 0, PSIM_MIXED = 1 };
 
 const char* psim_last_error(void);
