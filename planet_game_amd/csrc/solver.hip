@@ -1023,10 +1023,14 @@ __device__ inline int first_sweep_tiles(const Instance& I, int mask) {
 // FP64 first sweep and left r1 = w - A x1 in level 0's FP32 iterate buffer;
 // this FP32 V-cycle then runs on r1 with one pre-sweep from zero on level 0
 // (the second of its two) and writes x1 + y to so.out in FP64.
+//
+// keep0 (set when the previous chained step already ran level 0's first
+// Jacobi sweep, so vcycle_first did not run): the level-0 down sweep stores
+// its input w0 there, as the first sweep would have.
 template <class VT, int kW>
 __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>& V, const VT* w0, VT* out0, int g, int G, Lds& lds,
                             int& timeouts, double* dot_part = nullptr, const VT* dot_with = nullptr, int mask = 0,
-                            SplitOut so = SplitOut{}) {
+                            SplitOut so = SplitOut{}, VT* keep0 = nullptr) {
     constexpr bool kF = kW >= 1;
     const int J = I.depth;
     if (J == 1) {
@@ -1056,7 +1060,7 @@ __device__ __forceinline__ void vcycle_rest(const Instance& I, const Values<VT>&
     if (split || (mask & 1)) {
         Source<VT> src = none;
         src.a = w0;
-        down_phase<VT, kW>(I, V, 0, src, nullptr, static_cast<VT*>(I.lvl_b[1]), nullptr, g, G, lds, pre_of(0));
+        down_phase<VT, kW>(I, V, 0, src, keep0, static_cast<VT*>(I.lvl_b[1]), nullptr, g, G, lds, pre_of(0));
         group_sync<kF>(timeouts);
     }
     for (int j = 1; j + 1 < J; ++j) {
@@ -1479,6 +1483,9 @@ __device__ inline int pre_sweep_mask(const Instance& I) {
 struct Prepared {
     bool ready = false;
     double totals[2] = {0.0, 0.0};
+    // the next solve's first level-0 Jacobi sweep is done too: its two-sweep
+    // iterate is in level 0's iterate buffer and A w in u (plain form)
+    bool first_sweep = false;
 };
 
 // kForm: 0 the plain form, 1 the mixed form's split V-cycle, 2 either, by
@@ -1503,6 +1510,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     const LevelView& L0 = I.lv[0];
     const int n = I.n, W = L0.width, tiles = I.tiles;
     const bool swap = kPrep && prepared.ready;
+    bool use_prepared_sweep = swap && prepared.first_sweep;
     if (swap) rp ^= 1;
     const bool alt = kPrep && rp != 0;
     VT* r = static_cast<VT*>(alt ? I.r_alt : I.r);
@@ -1516,7 +1524,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     constexpr int KQ = kPrep ? 4 : 2;   // partials of a phase that leaves x final
     // row i's share of the next step's right-hand side from its final x:
     // q[2], q[3] get (b', b') and (r', r') as the assembly phase sums them
-    const auto prepare_row = [&](int i, double x_final, double (&q)[KQ]) {
+    const auto prepare_row = [&](int i, double x_final, double (&q)[KQ]) -> double {
         if constexpr (kPrep) {
             double* bn = alt ? I.b : I.b_alt;
             VT* rn = static_cast<VT*>(alt ? I.r : I.r_alt);
@@ -1528,13 +1536,15 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
             publish(&rn0[i], rv);
             q[2] = bi * bi;
             q[3] = static_cast<double>(rv) * static_cast<double>(rv);
+            return static_cast<double>(rv);
         }
+        return 0.0;
     };
     Prepared prep;   // set by the last phase that prepared the next step
     // totals of a phase that leaves x final: its slots 0 and 1 into tot, and
     // with next_forcing set, the prepared right-hand side's (b', b') and
     // (r', r') (slots 2 and 3) into prep, from one read of the partials
-    const auto final_totals = [&](const double* part_in, double (&tot)[2]) {
+    const auto final_totals = [&](const double* part_in, double (&tot)[2], bool swept = false) {
         if (kPrep && next_forcing != nullptr) {
             double t4[4];
             group_totals<4, kF>(part_in, tiles, t4, lds);
@@ -1543,6 +1553,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
             prep.ready = true;
             prep.totals[0] = t4[2];
             prep.totals[1] = t4[3];
+            prep.first_sweep = swept;
         } else {
             group_totals<2, kF>(part_in, tiles, tot, lds);
         }
@@ -1704,11 +1715,20 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 // inner vectors only: A r0 (about |C A| |b|) overflows FP32.
                 const bool fuse = sizeof(VT) == 8 && first_iteration && symmetric && I.depth > 1;
                 VT* u = static_cast<VT*>(I.u);
-                vcycle_first<VT, kW>(I, V, src, p_new, nullptr, g, G, lds, timeouts, mask,
-                                     first_iteration && shadow_stale ? r0 : nullptr, fuse ? u : nullptr, split);
+                // the previous chained step ran this V-cycle's first sweep on
+                // w = r (its iterate and A w are in place): start at the down
+                // sweep, which stores w into p
+                const bool prepared_sweep = use_prepared_sweep && first_iteration && fuse && !shadow_stale;
+                use_prepared_sweep = false;
+                if (!prepared_sweep)
+                    vcycle_first<VT, kW>(I, V, src, p_new, nullptr, g, G, lds, timeouts, mask,
+                                         first_iteration && shadow_stale ? r0 : nullptr, fuse ? u : nullptr, split);
                 shadow_stale = false;
                 double* dot = fuse ? next_partials() : nullptr;
-                vcycle_rest_any(p_new, ph, dot, fuse ? u : nullptr);
+                if (prepared_sweep)
+                    vcycle_rest<VT, kW>(I, V, r, ph, g, G, lds, timeouts, dot, u, mask, SplitOut{}, p_new);
+                else
+                    vcycle_rest_any(p_new, ph, dot, fuse ? u : nullptr);
                 ++v_cycles;
                 if (fuse) {
                     double uph[1];
@@ -1755,6 +1775,16 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     part = next_partials();
                     const double* x = I.x[xi];
                     double* xn = I.x[xi ^ 1];
+                    // Chained steps, plain form: this phase also runs the next
+                    // solve's first level-0 Jacobi sweep on w' = r' = b' (what
+                    // jacobi_phase would compute there, operation for
+                    // operation): each row recomputes its neighbours' b'_c from
+                    // x_c and ph_c, which it gathers anyway, and C A, A and S'
+                    // at c, so the next solve starts at its down sweep.
+                    const bool sweep_next = kPrep && next_forcing != nullptr && kForm == 0 && (mask & 1) != 0 &&
+                                            I.depth > 1 && sizeof(VT) == 8;
+                    const double om0 = I.omega[0];
+                    const auto dinv0 = gptr(static_cast<const double*>(I.dinv64[0]));
                     for (int tile = g; tile < tiles; tile += G) {
                         const int i = tile * kTile + threadIdx.x;
                         double q[KQ] = {};
@@ -1762,23 +1792,53 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                             const double ph_i = static_cast<double>(fetch<kF>(&ph[i]));
                             const double x_new = (x_zero ? 0.0 : fetch<kF>(&x[i])) + alpha * ph_i;
                             publish(&xn[i], x_new);
-                            if (kPrep && next_forcing != nullptr) prepare_row(i, x_new, q);
+                            double w_next = 0.0;
+                            if (kPrep && next_forcing != nullptr) w_next = prepare_row(i, x_new, q);
                             double ax = I.diag64[0][i] * x_new;
                             double av = I.diag64[0][i] * ph_i;   // (A ph)_i when the SpMV phase was skipped
+                            // the next sweep's residual and A w' at row i
+                            double acc_next = 0.0, aw_next = 0.0;
+                            if (sweep_next) {
+                                acc_next = w_next - I.diag64[0][i] * (om0 * ld(dinv0, i) * w_next);
+                                aw_next = I.diag64[0][i] * w_next;
+                            }
+                            struct Gathered {
+                                double x, ph, ca, area, forcing, dinv;
+                            };
                             ell_row(
                                 L0.col, I.off64[0], n, W, i,
                                 [&](int, int c) {
-                                    return Pair<double>{x_zero ? 0.0 : fetch<kF>(&x[c]), static_cast<double>(fetch<kF>(&ph[c]))};
+                                    Gathered w{x_zero ? 0.0 : fetch<kF>(&x[c]), static_cast<double>(fetch<kF>(&ph[c])),
+                                               0.0, 0.0, 0.0, 0.0};
+                                    if (sweep_next) {
+                                        w.ca = ld(P.capacity_area, c);
+                                        w.area = ld(I.area, c);
+                                        w.forcing = ld(next_forcing, c);
+                                        w.dinv = ld(dinv0, c);
+                                    }
+                                    return w;
                                 },
-                                [&](double o, Pair<double> w) {
-                                    ax += o * (w.first + alpha * w.second);
-                                    av += o * w.second;
+                                [&](double o, const Gathered& w) {
+                                    const double xc = w.x + alpha * w.ph;
+                                    ax += o * xc;
+                                    av += o * w.ph;
+                                    if (sweep_next) {
+                                        // b'_c as prepare_row forms it at row c
+                                        const double wc = w.ca * xc + P.dt * w.area * w.forcing;
+                                        acc_next -= o * (om0 * w.dinv * wc);
+                                        aw_next += o * wc;
+                                    }
                                 });
                             const double si = fetch<kF>(&b[i]) - ax;
                             const VT sv = static_cast<VT>(si);
                             publish(&s[i], sv);
                             publish(&r[i], sv);
                             if (fuse) publish(&v[i], static_cast<VT>(av));
+                            if (sweep_next) {
+                                const double x1 = om0 * ld(dinv0, i) * w_next;
+                                publish(&static_cast<double*>(I.lvl_x[0])[i], x1 + om0 * ld(dinv0, i) * acc_next);
+                                publish(&static_cast<double*>(I.u)[i], aw_next);
+                            }
                             q[0] = si * si;
                             q[1] = static_cast<double>(sv) * static_cast<double>(sv);
                         }
@@ -1788,7 +1848,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     xi ^= 1;
                     x_zero = false;
                     zero_x = false;   // x[xi] now holds x + alpha ph
-                    final_totals(part, tot2);
+                    final_totals(part, tot2, sweep_next);
                     ++iterations;
                     if (sqrt(tot2[0]) <= inner_target) {
                         // r holds the true residual; a restart takes it as its
