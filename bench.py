@@ -47,6 +47,12 @@ CHAIN = 120
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "program_kernel_l6_chain_traffic.json")
 BATCH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "program_kernel_batch256_mixed_traffic.json")
 COPY_BYTES = 256 << 20
+_T0 = time.perf_counter()
+
+
+def progress(what):
+    """One stderr line per finished section, so a long run shows its steps."""
+    print(f"bench: {what} ({time.perf_counter() - _T0:.1f} s)", file=sys.stderr, flush=True)
 
 
 
@@ -587,7 +593,10 @@ def served_section(P, ctx, meshes, graphs, plain_value, sweep=SWEEP, profile=Fal
     # first an all-at-once stream, which also assembles every bank matrix
     # once (a running server holds them by key), then the offered rates
     out["all_at_once_on_sweep_plan"] = run(server, levels, shares, float("inf"), 10000, 17)
-    out["sweep"] = [run(server, levels, shares, rate, count, 11 + i) for i, (rate, count) in enumerate(sweep)]
+    out["sweep"] = []
+    for i, (rate, count) in enumerate(sweep):
+        out["sweep"].append(run(server, levels, shares, rate, count, 11 + i))
+        progress(f"served sweep at {rate} req/s")
     out["sweep_plan"] = {str(L): [p["slots"], p["slot_size"]] for L, p in zip(levels, server.plan())}
     server.close()
     server = P.Server(ctx, [graphs[L] for L in levels], share=shares, precision=P.MIXED, memory_budget=8 << 30)
@@ -894,9 +903,10 @@ def main():
     ap.add_argument("--workgroups", type=int, default=0, help="override the solver's workgroup count")
     ap.add_argument("--only-consumers", action="store_true",
                     help="run only the consumer measurements (tracer, ILU, advection, operator solves)")
-    ap.add_argument("--only", choices=["chain", "fv", "batched", "served"],
+    ap.add_argument("--only", choices=["chain", "fv", "batched", "served", "kernels"],
                     help="run only one section (profiling runs): the headline chain's census, warm-up and timed "
-                         "passes, the 64-field FV sweeps, the 256-instance batch, or a short served run")
+                         "passes, the 64-field FV sweeps, the 256-instance batch, a short served run, or "
+                         "'kernels': the chain passes, then the batch, then the served run, in one process")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed-precision, batched and served measurements")
     args = ap.parse_args()
@@ -913,7 +923,7 @@ def main():
         meshes = {L: P.Mesh(L) for L in (5, 6)}
         print(json.dumps({"transport_consumers": transport_rows_section(P, ctx, meshes, not args.no_cpu)}))
         return
-    if args.only and args.only != "chain":
+    if args.only and args.only not in ("chain", "kernels"):
         ctx = P.Context(0)
         mesh = P.Mesh(level)
         # one 256 MiB streaming copy first: the calibration kernel of the
@@ -962,6 +972,7 @@ def main():
                 for i in infos] if collect else None
 
     census = run_pass()
+    progress("census pass")
     for step, (_, _, _, res, status, _) in enumerate(census):
         if status != 0 or not res <= TOL:
             raise SystemExit(f"step {step} did not converge: status {status}, residual {res:.3e}")
@@ -980,16 +991,29 @@ def main():
         ctx.record(2 * k + 1)
     ctx.synchronize()
     wall = time.perf_counter() - t_start
+    progress(f"{args.steps} timed passes")
     launch_ms = np.array([ctx.elapsed_ms(2 * k, 2 * k + 1) for k in range(args.steps)])
     solves = CHAIN * args.steps
     value = solves / wall
-    if args.only == "chain":
+    if args.only in ("chain", "kernels"):
         # the counter runs' calibration kernel: one 256 MiB streaming copy
         calibrate = P.DeviceArray(ctx, COPY_BYTES // 8), P.DeviceArray(ctx, COPY_BYTES // 8)
         ctx.copy(calibrate[1].ptr, calibrate[0].ptr, COPY_BYTES)
         ctx.synchronize()
         print(json.dumps({"chain_passes": {"solves_per_s": round(value, 2),
-                                           "mean_launch_ms": round(float(launch_ms.mean()), 5)}}))
+                                           "mean_launch_ms": round(float(launch_ms.mean()), 5)}}), flush=True)
+        if args.only == "kernels":
+            # the batched and served kernels in the same process (their
+            # program instances have names of their own in a trace)
+            meshes = {L: (mesh if L == level else P.Mesh(L)) for L in (3, 4, 5, 6)}
+            graphs = {L: (graph if L == level else P.Graph(ctx, meshes[L])) for L in meshes}
+            hierarchy_levels = {L: [(lv["size"], lv["nnz"]) for lv in P.Hierarchy(meshes[L]).levels]
+                                for L in meshes}
+            print(json.dumps({"batched": batched_section(P, ctx, meshes, graphs, hierarchy_levels,
+                                                         autotune=False)}), flush=True)
+            progress("batched")
+            print(json.dumps({"served": served_section(P, ctx, meshes, graphs, 1.0, profile=True)}), flush=True)
+            progress("served")
         return
     # per-solve latencies: the same passes again, with per-step records
     step_ms = []
@@ -1046,6 +1070,7 @@ def main():
     ctx.synchronize()
     copy_gbs = 4 * 2 * COPY_BYTES / (ctx.elapsed_ms(0, 1) / 1e3) / 1e9
     traffic = measured_traffic()
+    progress("latency, single-launch and copy runs")
 
     line = {
         "metric": METRIC,
@@ -1121,8 +1146,11 @@ def main():
         line["error_vs_oracle_after_120_steps_tol_1e-13"] = float(
             np.linalg.norm(tight_final - tight_cpu) / np.linalg.norm(tight_cpu))
         line["error_vs_fp64_oracle_per_step"] = step_errors_vs_oracle(mesh, cond, ca, step_temps, forcings)
+        progress("CPU baseline and oracle checks")
         line["fv_sweeps"] = fv_section(P, ctx, mesh, True, best)
+        progress("FV sweeps")
         line["config0_l4"] = config0_section(P, ctx)
+        progress("config 0")
     else:
         line["fv_sweeps"] = fv_section(P, ctx, mesh, False, 1)
     if not args.no_extras:
@@ -1130,11 +1158,16 @@ def main():
         graphs = {L: (graph if L == level else P.Graph(ctx, meshes[L])) for L in meshes}
         hierarchy_levels = {L: [(lv["size"], lv["nnz"]) for lv in P.Hierarchy(meshes[L]).levels] for L in meshes}
         line["mixed_precision"] = mixed_section(P, ctx, mesh, graph, levels, not args.no_cpu)
+        progress("mixed precision")
         line["batched"] = batched_section(P, ctx, meshes, graphs, hierarchy_levels)
         line["batched_plain_fp64"] = batched_section(P, ctx, meshes, graphs, hierarchy_levels, P.PLAIN_FP64)
+        progress("batched")
         line["served"] = served_section(P, ctx, meshes, graphs, value)
+        progress("served")
         line["implicit_transport"] = transport_section(P, ctx, meshes, not args.no_cpu)
+        progress("implicit transport")
         line["transport_consumers"] = transport_rows_section(P, ctx, meshes, not args.no_cpu)
+        progress("transport consumers")
     print(json.dumps(line))
 
 

@@ -703,59 +703,115 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
 // per row: lanes split the columns (row-major reads of the symmetric
 // inverse, coalesced), each lane adding its columns in ascending order from
 // 0, and wave_sum adds the lanes in its fixed order, so every workgroup computing a row
-// gets the same bits. A wave takes kRows rows at a time; the first round's
-// row ids are loaded before the right-hand side is staged.
-template <class VT, bool kF = false>
-__device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs, const int32_t* list, int first, int count,
-                                Lds& lds) {
-    constexpr int kRows = 4;   // rows in flight per wave
+// gets the same bits. A wave takes kCoarseRows rows at a time.
+//
+// Two steps: coarse_rows_issue loads the first round's row ids and, with
+// kEarly, the first kCoarseChunks columns of each of those rows for this
+// lane (independent of the right-hand side, so they are in flight while the
+// caller issues its own loads and the right-hand side is staged);
+// coarse_rows_finish stages the right-hand side in LDS and computes the
+// rows. coarse_rows_lds is the two in a row.
+constexpr int kCoarseRows = 4;     // rows in flight per wave
+constexpr int kCoarseChunks = 4;   // columns per lane and row loaded at a time
+struct CoarseFirst {
+    int a[kCoarseRows];                        // the wave's first-round rows (-1: none)
+    double m[kCoarseRows][kCoarseChunks];      // with kEarly: their first columns for this lane
+};
+
+// kEarly: list is set and count >= 1; the ids load without a branch (a
+// position past the list reloads its last entry, discarded)
+template <bool kEarly>
+__device__ __forceinline__ void coarse_row_ids(const int32_t* list, int first, int count, int q0, int (&a)[kCoarseRows]) {
+#pragma unroll
+    for (int u = 0; u < kCoarseRows; ++u) {
+        const int q = q0 + u;
+        if constexpr (kEarly) {
+            const int id = ld(list, min(q, count - 1));
+            a[u] = q < count ? id : -1;
+        } else {
+            a[u] = q < count ? (list != nullptr ? ld(list, q) : first + q) : -1;
+        }
+    }
+}
+
+template <bool kEarly>
+__device__ __forceinline__ CoarseFirst coarse_rows_issue(const Instance& I, const int32_t* list, int first, int count) {
+    CoarseFirst cf;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    coarse_row_ids<kEarly>(list, first, count, wave * kCoarseRows, cf.a);
+    if constexpr (kEarly) {
+        const int n = I.lv[I.depth - 1].n;
+        const auto inv = gptr(static_cast<const double*>(I.cinv));
+#pragma unroll
+        for (int u = 0; u < kCoarseRows; ++u) {
+            const auto row = inv + static_cast<size_t>(cf.a[u] >= 0 ? cf.a[u] : 0) * n;
+#pragma unroll
+            for (int c = 0; c < kCoarseChunks; ++c) cf.m[u][c] = row[min(lane + 64 * c, n - 1)];
+        }
+    }
+    return cf;
+}
+
+template <class VT, bool kF, bool kEarly>
+__device__ __forceinline__ void coarse_rows_finish(const Instance& I, const VT* rhs, const int32_t* list, int first,
+                                                   int count, Lds& lds, CoarseFirst& cf) {
     const int n = I.lv[I.depth - 1].n;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int a[kRows];
-    const auto row_ids = [&](int q0) {
-#pragma unroll
-        for (int u = 0; u < kRows; ++u) {
-            const int q = q0 + u;
-            a[u] = q < count ? (list != nullptr ? list[q] : first + q) : -1;
-        }
-    };
-    row_ids(wave * kRows);
     for (int k = threadIdx.x; k < n; k += blockDim.x) lds.crhs[k] = static_cast<double>(fetch<kF>(&rhs[k]));
     __syncthreads();
     const auto inv = gptr(static_cast<const double*>(I.cinv));
-    for (int q0 = wave * kRows; q0 < count; q0 += 4 * kRows) {
-        if (q0 != wave * kRows) row_ids(q0);
-        double acc[kRows];
+    for (int q0 = wave * kCoarseRows; q0 < count; q0 += 4 * kCoarseRows) {
+        const bool first_round = q0 == wave * kCoarseRows;
+        if (!first_round) coarse_row_ids<kEarly>(list, first, count, q0, cf.a);
+        double acc[kCoarseRows];
 #pragma unroll
-        for (int u = 0; u < kRows; ++u) acc[u] = 0.0;
-        // kChunks columns per lane and row at a time, every load issued
+        for (int u = 0; u < kCoarseRows; ++u) acc[u] = 0.0;
+        // kCoarseChunks columns per lane and row at a time, every load issued
         // before the first product (an index past the row or a skipped row
         // loads a valid element and is discarded); each lane still adds its
         // columns in ascending order
-        constexpr int kChunks = 4;
-        for (int k0 = lane; k0 < n; k0 += 64 * kChunks) {
-            double m[kRows][kChunks];
+        int k0 = lane;
+        if (kEarly && first_round) {
+            // the first columns, loaded by coarse_rows_issue
 #pragma unroll
-            for (int u = 0; u < kRows; ++u) {
-                const auto row = inv + static_cast<size_t>(a[u] >= 0 ? a[u] : 0) * n;
+            for (int u = 0; u < kCoarseRows; ++u)
 #pragma unroll
-                for (int c = 0; c < kChunks; ++c) m[u][c] = row[min(k0 + 64 * c, n - 1)];
+                for (int c = 0; c < kCoarseChunks; ++c) {
+                    const int k = k0 + 64 * c;
+                    if (cf.a[u] >= 0 && k < n) acc[u] += cf.m[u][c] * lds.crhs[k];
+                }
+            k0 += 64 * kCoarseChunks;
+        }
+        for (; k0 < n; k0 += 64 * kCoarseChunks) {
+            double m[kCoarseRows][kCoarseChunks];
+#pragma unroll
+            for (int u = 0; u < kCoarseRows; ++u) {
+                const auto row = inv + static_cast<size_t>(cf.a[u] >= 0 ? cf.a[u] : 0) * n;
+#pragma unroll
+                for (int c = 0; c < kCoarseChunks; ++c) m[u][c] = row[min(k0 + 64 * c, n - 1)];
             }
 #pragma unroll
-            for (int u = 0; u < kRows; ++u)
+            for (int u = 0; u < kCoarseRows; ++u)
 #pragma unroll
-                for (int c = 0; c < kChunks; ++c) {
+                for (int c = 0; c < kCoarseChunks; ++c) {
                     const int k = k0 + 64 * c;
-                    if (a[u] >= 0 && k < n) acc[u] += m[u][c] * lds.crhs[k];
+                    if (cf.a[u] >= 0 && k < n) acc[u] += m[u][c] * lds.crhs[k];
                 }
         }
 #pragma unroll
-        for (int u = 0; u < kRows; ++u) {
+        for (int u = 0; u < kCoarseRows; ++u) {
             const double total = wave_sum(acc[u]);
-            if (lane == 0 && a[u] >= 0) lds.cvec[a[u]] = total;
+            if (lane == 0 && cf.a[u] >= 0) lds.cvec[cf.a[u]] = total;
         }
     }
     __syncthreads();
+}
+
+template <class VT, bool kF = false, bool kEarly = false>
+__device__ __forceinline__ void coarse_rows_lds(const Instance& I, const VT* rhs, const int32_t* list, int first, int count,
+                                                Lds& lds) {
+    CoarseFirst cf = coarse_rows_issue<kEarly>(I, list, first, count);
+    coarse_rows_finish<VT, kF, kEarly>(I, rhs, list, first, count, lds, cf);
 }
 
 // Level-0 output of the split V-cycle of the mixed form: its FP32 part
@@ -876,6 +932,138 @@ __device__ __forceinline__ void up_phase(const Instance& I, const Values<VT>& V,
     });
 }
 
+// One tile of merged_up_phase (below) with every global load it makes
+// issued before the coarse solve: the coarse rows' ids and their inverse
+// columns, this thread's level-0 row (columns, values, aggregate positions,
+// its own inputs and its neighbours' gathers) and its level-(j+1) row (the
+// same). Only LDS values (coarse solution, level-(j+1) results) are waited
+// for between the steps, so the tile costs three dependent memory round
+// trips (ids -> columns -> gathers) instead of six. The loads are
+// branch-free (slots past the width reload the last slot, rows past the
+// level reload row 0, and the results are discarded by selects), so the
+// compiler keeps them all in flight instead of waiting at every join.
+// Operations, their order and every rounding are those of the loop in
+// merged_up_phase. kPre / kPreM: levels j / j + 1 have two-sweep iterates.
+template <class VT, bool kF, bool kPre, bool kPreM>
+__device__ __forceinline__ void merged_up_tile_early(const Instance& I, const Values<VT>& V, int j, int tile,
+                                                     const VT* rhs, VT* out, Lds& lds, double* dot_part,
+                                                     const VT* dot_with, const VT* pre, const VT* pre_m,
+                                                     const SplitOut& so) {
+    const LevelView& L = I.lv[j];
+    const LevelView& M = I.lv[j + 1];
+    const int n = L.n, W = L.width, nm = M.n, Wm = M.width;
+    const VT om = static_cast<VT>(I.omega[j]), omm = static_cast<VT>(I.omega[j + 1]);
+    const auto diag = V.diag(j);
+    const auto dinv = V.dinv(j);
+    const auto off = V.off(j);
+    const auto diag_m = V.diag(j + 1);
+    const auto dinv_m = V.dinv(j + 1);
+    const auto off_m = V.off(j + 1);
+    const VT* rhs_m = static_cast<const VT*>(I.lvl_b[j + 1]);
+    const auto col = gptr(L.col);
+    const auto col_m = gptr(M.col);
+    double* mid = lds.tile[0];
+    const bool split = sizeof(VT) == 4 && so.out != nullptr;
+    // the arrays of the output step, or a valid array of the same type
+    // (loaded and discarded) where there is none
+    const double* split_w = split ? so.w : static_cast<const double*>(I.dinv64[0]);
+    const double* split_dot = split && so.dot != nullptr ? so.dot : static_cast<const double*>(I.dinv64[0]);
+    const VT* dot_in = !split && dot_with != nullptr ? dot_with : rhs;
+    // round 1: ids, static row data and own inputs
+    const int mq = threadIdx.x;
+    const int m_id = ld(L.mid, tile * L.mid_stride + min(mq, L.mid_stride - 1));
+    const int m = mq < L.mid_stride ? m_id : -1;
+    const int32_t* clist = L.mid_coarse + tile * L.mid_coarse_stride;
+    CoarseFirst cf = coarse_rows_issue<true>(I, clist, 0, L.mid_coarse_stride);
+    const int i0 = tile * kTile + static_cast<int>(threadIdx.x);
+    const bool ok = i0 < n;
+    const int i = ok ? i0 : 0;
+    int c[kMaxWidth], al[kMaxWidth];
+    VT o[kMaxWidth];
+#pragma unroll
+    for (int s = 0; s < kMaxWidth; ++s) {
+        const int e = min(s, W - 1) * n + i;
+        c[s] = s < W ? col[e] : -1;
+        o[s] = off[e];
+        al[s] = ld(L.agg_col_loc, e);
+    }
+    const VT rhs_i = fetch<kF>(&rhs[i]);
+    const VT pre_i = kPre ? fetch<kF>(pre + i) : VT(0);
+    const VT dinv_i = ld(dinv, i), diag_i = ld(diag, i);
+    const int agl = ld(L.agg_loc, i);
+    const double sd = ld(gptr(static_cast<const double*>(I.dinv64[0])), i);
+    const double sw = fetch<kF>(&split_w[i]);
+    const double sdot = fetch<kF>(&split_dot[i]);
+    const double dv = static_cast<double>(fetch<kF>(&dot_in[i]));
+    // round 2: the level-(j+1) row (its id came in round 1) and the level-0
+    // gathers
+    const int mm = m >= 0 ? m : 0;
+    int cm[kMaxWidth], am[kMaxWidth];
+    VT omv[kMaxWidth];
+#pragma unroll
+    for (int s = 0; s < kMaxWidth; ++s) {
+        const int e = min(s, Wm - 1) * nm + mm;
+        const int cv = col_m[e];
+        cm[s] = s < Wm ? cv : -1;
+        omv[s] = off_m[e];
+        am[s] = ld(M.agg_col, e);
+    }
+    const VT rhs_mm = fetch<kF>(&rhs_m[mm]);
+    const VT pre_mm = kPreM ? fetch<kF>(pre_m + mm) : VT(0);
+    const VT dinv_mm = ld(dinv_m, mm), diag_mm = ld(diag_m, mm);
+    const int agg_mm = ld(M.agg, mm);
+    VT gv[kMaxWidth];
+#pragma unroll
+    for (int s = 0; s < kMaxWidth; ++s) {
+        const int cc = c[s] >= 0 ? c[s] : i;
+        if constexpr (kPre)
+            gv[s] = fetch<kF>(pre + cc);
+        else
+            gv[s] = om * ld(dinv, cc) * fetch<kF>(&rhs[cc]);
+    }
+    // round 3: the level-(j+1) gathers
+    VT gm[kMaxWidth];
+#pragma unroll
+    for (int s = 0; s < kMaxWidth; ++s) {
+        const int cc = cm[s] >= 0 ? cm[s] : mm;
+        if constexpr (kPreM)
+            gm[s] = fetch<kF>(pre_m + cc);
+        else
+            gm[s] = omm * ld(dinv_m, cc) * fetch<kF>(&rhs_m[cc]);
+    }
+    // the coarse rows into lds.cvec
+    coarse_rows_finish<VT, kF, true>(I, static_cast<const VT*>(I.lvl_b[j + 2]), clist, 0, L.mid_coarse_stride, lds, cf);
+    if (m >= 0) {
+        const VT x2 = (kPreM ? pre_mm : omm * dinv_mm * rhs_mm) + static_cast<VT>(lds.cvec[agg_mm]);
+        VT acc = rhs_mm - diag_mm * x2;
+#pragma unroll
+        for (int s = 0; s < kMaxWidth; ++s)
+            if (cm[s] >= 0) acc -= omv[s] * (gm[s] + static_cast<VT>(lds.cvec[am[s] >= 0 ? am[s] : 0]));
+        mid[mq] = static_cast<double>(static_cast<VT>(x2 + omm * dinv_mm * acc));
+    }
+    __syncthreads();
+    double q[1] = {0.0};
+    if (ok) {
+        const VT x2 = (kPre ? pre_i : om * dinv_i * rhs_i) + static_cast<VT>(mid[agl]);
+        VT acc = rhs_i - diag_i * x2;
+#pragma unroll
+        for (int s = 0; s < kMaxWidth; ++s)
+            if (c[s] >= 0) acc -= o[s] * (gv[s] + static_cast<VT>(mid[al[s] >= 0 ? al[s] : 0]));
+        const VT value = x2 + om * dinv_i * acc;
+        if (split) {
+            // split_value's operations on the loaded inputs
+            const double v = I.omega[0] * sd * sw + static_cast<double>(value);
+            publish(&so.out[i], v);
+            q[0] = so.dot != nullptr ? v * sdot : 0.0;
+        } else {
+            publish(&out[i], value);
+            q[0] = dot_with != nullptr ? static_cast<double>(value) * dv : 0.0;
+        }
+    }
+    if (dot_part != nullptr) store_partials<1>(dot_part, tile, q, lds);
+    __syncthreads();
+}
+
 // Prolongation from the coarsest level through level j + 1 and on to level
 // j (two levels above the coarsest) in one sweep, for groups with about a
 // workgroup per tile of level j: each tile solves the coarsest rows its
@@ -916,6 +1104,24 @@ __device__ __noinline__ void merged_up_phase(const Instance* instance, int j, co
     constexpr bool kF = kW >= 1;
     double* mid = lds.tile[0];
     const int tiles = (n + kTile - 1) / kTile;
+    // With the register budget of one or two workgroups per CU, a tile
+    // issues every global load of the phase before the coarse solve
+    // (merged_up_tile_early): three dependent memory round trips instead of
+    // six, the same operations in the same order.
+    if (kW <= 2 && W >= 1 && W <= kMaxWidth && Wm >= 1 && Wm <= kMaxWidth && L.mid_stride >= 1 &&
+        L.mid_coarse_stride >= 1) {
+        for (int tile = g; tile < tiles; tile += G) {
+            if (pre != nullptr && pre_m != nullptr)
+                merged_up_tile_early<VT, kF, true, true>(I, V, j, tile, rhs, out, lds, dot_part, dot_with, pre, pre_m, so);
+            else if (pre != nullptr)
+                merged_up_tile_early<VT, kF, true, false>(I, V, j, tile, rhs, out, lds, dot_part, dot_with, pre, pre_m, so);
+            else if (pre_m != nullptr)
+                merged_up_tile_early<VT, kF, false, true>(I, V, j, tile, rhs, out, lds, dot_part, dot_with, pre, pre_m, so);
+            else
+                merged_up_tile_early<VT, kF, false, false>(I, V, j, tile, rhs, out, lds, dot_part, dot_with, pre, pre_m, so);
+        }
+        return;
+    }
     for (int tile = g; tile < tiles; tile += G) {
         // this thread's mid row id (a tile reads at most kTile of them) goes
         // out before the coarse solve
