@@ -1936,11 +1936,10 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 else
                     vcycle_rest_any(p_new, ph, dot, fuse ? u : nullptr);
                 ++v_cycles;
+                // fused: alpha = rho / (A r0, ph) from the V-cycle's partials,
+                // read inside the next phase once its loads are in flight
+                bool alpha_ready = !fuse;
                 if (fuse) {
-                    double uph[1];
-                    group_totals<1, kF>(dot, tiles, uph, lds);
-                    if (uph[0] == 0.0) break;
-                    alpha = rho / uph[0];
                 } else {
                 // v = A ph, (r0, v)
                 part = next_partials();
@@ -1991,64 +1990,204 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                                             I.depth > 1 && sizeof(VT) == 8;
                     const double om0 = I.omega[0];
                     const auto dinv0 = gptr(static_cast<const double*>(I.dinv64[0]));
-                    for (int tile = g; tile < tiles; tile += G) {
-                        const int i = tile * kTile + threadIdx.x;
+                    const auto diag0 = gptr(static_cast<const double*>(I.diag64[0]));
+                    const auto off0 = gptr(static_cast<const double*>(I.off64[0]));
+                    const auto col0 = gptr(L0.col);
+                    // the next step's forcing, or a valid array of the same size
+                    // (loaded and discarded) when there is none
+                    const double* nforce = kPrep && next_forcing != nullptr ? next_forcing : I.area;
+                    // Every load of a row is issued first and without a branch
+                    // (slots past the width reload the last slot, rows past n
+                    // reload row 0), then alpha is taken from the previous
+                    // phase's partials while they are in flight; the
+                    // operations and their order are those of the fused
+                    // phase's definition: x' = x + alpha ph, s = r = b - A x',
+                    // v = A ph and, in a chain, the next b, r, r0 and the next
+                    // solve's first level-0 Jacobi sweep.
+                    const auto take_alpha = [&]() {
+                        if (!alpha_ready) {
+                            double uph[1];
+                            group_totals<1, kF>(dot, tiles, uph, lds);
+                            alpha_ready = true;
+                            if (uph[0] == 0.0) return false;
+                            alpha = rho / uph[0];
+                        }
+                        return true;
+                    };
+                    bool stop = false;
+                    if (W < 1 || W > kMaxWidth) {
+                        // rows wider than the unrolled slots: alpha first,
+                        // then the row loop
+                        stop = !take_alpha();
+                        if (!stop) {
+                        for (int tile = g; tile < tiles; tile += G) {
+                            const int i = tile * kTile + threadIdx.x;
+                            double q[KQ] = {};
+                            if (i < n) {
+                                const double ph_i = static_cast<double>(fetch<kF>(&ph[i]));
+                                const double x_new = (x_zero ? 0.0 : fetch<kF>(&x[i])) + alpha * ph_i;
+                                publish(&xn[i], x_new);
+                                double w_next = 0.0;
+                                if (kPrep && next_forcing != nullptr) w_next = prepare_row(i, x_new, q);
+                                double ax = I.diag64[0][i] * x_new;
+                                double av = I.diag64[0][i] * ph_i;   // (A ph)_i when the SpMV phase was skipped
+                                // the next sweep's residual and A w' at row i
+                                double acc_next = 0.0, aw_next = 0.0;
+                                if (sweep_next) {
+                                    acc_next = w_next - I.diag64[0][i] * (om0 * ld(dinv0, i) * w_next);
+                                    aw_next = I.diag64[0][i] * w_next;
+                                }
+                                struct Gathered {
+                                    double x, ph, ca, area, forcing, dinv;
+                                };
+                                ell_row(
+                                    L0.col, I.off64[0], n, W, i,
+                                    [&](int, int c) {
+                                        Gathered w{x_zero ? 0.0 : fetch<kF>(&x[c]), static_cast<double>(fetch<kF>(&ph[c])),
+                                                   0.0, 0.0, 0.0, 0.0};
+                                        if (sweep_next) {
+                                            w.ca = ld(P.capacity_area, c);
+                                            w.area = ld(I.area, c);
+                                            w.forcing = ld(next_forcing, c);
+                                            w.dinv = ld(dinv0, c);
+                                        }
+                                        return w;
+                                    },
+                                    [&](double o, const Gathered& w) {
+                                        const double xc = w.x + alpha * w.ph;
+                                        ax += o * xc;
+                                        av += o * w.ph;
+                                        if (sweep_next) {
+                                            // b'_c as prepare_row forms it at row c
+                                            const double wc = w.ca * xc + P.dt * w.area * w.forcing;
+                                            acc_next -= o * (om0 * w.dinv * wc);
+                                            aw_next += o * wc;
+                                        }
+                                    });
+                                const double si = fetch<kF>(&b[i]) - ax;
+                                const VT sv = static_cast<VT>(si);
+                                publish(&s[i], sv);
+                                publish(&r[i], sv);
+                                if (fuse) publish(&v[i], static_cast<VT>(av));
+                                if (sweep_next) {
+                                    const double x1 = om0 * ld(dinv0, i) * w_next;
+                                    publish(&static_cast<double*>(I.lvl_x[0])[i], x1 + om0 * ld(dinv0, i) * acc_next);
+                                    publish(&static_cast<double*>(I.u)[i], aw_next);
+                                }
+                                q[0] = si * si;
+                                q[1] = static_cast<double>(sv) * static_cast<double>(sv);
+                            }
+                            store_partials<KQ>(part, tile, q, lds);
+                        }
+                        }
+                    }
+                    for (int tile = g; tile < tiles && !stop && W >= 1 && W <= kMaxWidth; tile += G) {
+                        const int i0 = tile * kTile + static_cast<int>(threadIdx.x);
+                        const bool ok = i0 < n;
+                        const int i = ok ? i0 : 0;
+                        int c[kMaxWidth];
+                        double o[kMaxWidth];
+#pragma unroll
+                        for (int k = 0; k < kMaxWidth; ++k) {
+                            const int e = min(k, W - 1) * n + i;
+                            const int cv = col0[e];
+                            c[k] = k < W ? cv : -1;
+                            o[k] = off0[e];
+                        }
+                        const double ph_i = static_cast<double>(fetch<kF>(&ph[i]));
+                        const double x_i = fetch<kF>(&x[i]);
+                        const double b_i = fetch<kF>(&b[i]);
+                        const double diag_i = diag0[i], dinv_i = dinv0[i];
+                        double ca_i = 0.0, area_i = 0.0, nf_i = 0.0;
+                        if constexpr (kPrep) {
+                            ca_i = ld(P.capacity_area, i);
+                            area_i = ld(I.area, i);
+                            nf_i = ld(nforce, i);
+                        }
+                        double gx[kMaxWidth], gph[kMaxWidth], gca[kMaxWidth], garea[kMaxWidth], gf[kMaxWidth],
+                            gd[kMaxWidth];
+#pragma unroll
+                        for (int k = 0; k < kMaxWidth; ++k) {
+                            const int cc = c[k] >= 0 ? c[k] : i;
+                            gx[k] = fetch<kF>(&x[cc]);
+                            gph[k] = static_cast<double>(fetch<kF>(&ph[cc]));
+                            if constexpr (kPrep) {
+                                gca[k] = ld(P.capacity_area, cc);
+                                garea[k] = ld(I.area, cc);
+                                gf[k] = ld(nforce, cc);
+                                gd[k] = dinv0[cc];
+                            }
+                        }
+                        if (tile == g && !take_alpha()) {
+                            stop = true;
+                            break;
+                        }
                         double q[KQ] = {};
-                        if (i < n) {
-                            const double ph_i = static_cast<double>(fetch<kF>(&ph[i]));
-                            const double x_new = (x_zero ? 0.0 : fetch<kF>(&x[i])) + alpha * ph_i;
+                        if (ok) {
+                            const double x_new = (x_zero ? 0.0 : x_i) + alpha * ph_i;
                             publish(&xn[i], x_new);
                             double w_next = 0.0;
-                            if (kPrep && next_forcing != nullptr) w_next = prepare_row(i, x_new, q);
-                            double ax = I.diag64[0][i] * x_new;
-                            double av = I.diag64[0][i] * ph_i;   // (A ph)_i when the SpMV phase was skipped
+                            if constexpr (kPrep) {
+                                if (next_forcing != nullptr) {
+                                    // prepare_row's operations on the loaded inputs
+                                    double* bn = alt ? I.b : I.b_alt;
+                                    VT* rn = static_cast<VT*>(alt ? I.r : I.r_alt);
+                                    VT* rn0 = static_cast<VT*>(alt ? I.r0 : I.r0_alt);
+                                    const double bi = ca_i * x_new + P.dt * area_i * nf_i;
+                                    const VT rv = static_cast<VT>(bi);
+                                    publish(&bn[i], bi);
+                                    publish(&rn[i], rv);
+                                    publish(&rn0[i], rv);
+                                    q[2] = bi * bi;
+                                    q[3] = static_cast<double>(rv) * static_cast<double>(rv);
+                                    w_next = static_cast<double>(rv);
+                                }
+                            }
+                            double ax = diag_i * x_new;
+                            double av = diag_i * ph_i;   // (A ph)_i when the SpMV phase was skipped
                             // the next sweep's residual and A w' at row i
                             double acc_next = 0.0, aw_next = 0.0;
                             if (sweep_next) {
-                                acc_next = w_next - I.diag64[0][i] * (om0 * ld(dinv0, i) * w_next);
-                                aw_next = I.diag64[0][i] * w_next;
+                                acc_next = w_next - diag_i * (om0 * dinv_i * w_next);
+                                aw_next = diag_i * w_next;
                             }
-                            struct Gathered {
-                                double x, ph, ca, area, forcing, dinv;
-                            };
-                            ell_row(
-                                L0.col, I.off64[0], n, W, i,
-                                [&](int, int c) {
-                                    Gathered w{x_zero ? 0.0 : fetch<kF>(&x[c]), static_cast<double>(fetch<kF>(&ph[c])),
-                                               0.0, 0.0, 0.0, 0.0};
-                                    if (sweep_next) {
-                                        w.ca = ld(P.capacity_area, c);
-                                        w.area = ld(I.area, c);
-                                        w.forcing = ld(next_forcing, c);
-                                        w.dinv = ld(dinv0, c);
+#pragma unroll
+                            for (int k = 0; k < kMaxWidth; ++k)
+                                if (c[k] >= 0) {
+                                    const double xc = (x_zero ? 0.0 : gx[k]) + alpha * gph[k];
+                                    ax += o[k] * xc;
+                                    av += o[k] * gph[k];
+                                    if constexpr (kPrep) {
+                                        if (sweep_next) {
+                                            // b'_c as the next right-hand side forms it at row c
+                                            const double wc = gca[k] * xc + P.dt * garea[k] * gf[k];
+                                            acc_next -= o[k] * (om0 * gd[k] * wc);
+                                            aw_next += o[k] * wc;
+                                        }
                                     }
-                                    return w;
-                                },
-                                [&](double o, const Gathered& w) {
-                                    const double xc = w.x + alpha * w.ph;
-                                    ax += o * xc;
-                                    av += o * w.ph;
-                                    if (sweep_next) {
-                                        // b'_c as prepare_row forms it at row c
-                                        const double wc = w.ca * xc + P.dt * w.area * w.forcing;
-                                        acc_next -= o * (om0 * w.dinv * wc);
-                                        aw_next += o * wc;
-                                    }
-                                });
-                            const double si = fetch<kF>(&b[i]) - ax;
+                                }
+                            const double si = b_i - ax;
                             const VT sv = static_cast<VT>(si);
                             publish(&s[i], sv);
                             publish(&r[i], sv);
                             if (fuse) publish(&v[i], static_cast<VT>(av));
                             if (sweep_next) {
-                                const double x1 = om0 * ld(dinv0, i) * w_next;
-                                publish(&static_cast<double*>(I.lvl_x[0])[i], x1 + om0 * ld(dinv0, i) * acc_next);
+                                const double x1 = om0 * dinv_i * w_next;
+                                publish(&static_cast<double*>(I.lvl_x[0])[i], x1 + om0 * dinv_i * acc_next);
                                 publish(&static_cast<double*>(I.u)[i], aw_next);
                             }
                             q[0] = si * si;
                             q[1] = static_cast<double>(sv) * static_cast<double>(sv);
                         }
                         store_partials<KQ>(part, tile, q, lds);
+                    }
+                    if (!alpha_ready && !take_alpha()) stop = true;
+                    if (stop) {
+                        // breakdown: this phase's partial buffer was not used;
+                        // give it back, so the next phase writes the buffer
+                        // that no workgroup can still be reading
+                        parity ^= 1;
+                        break;
                     }
                     group_sync<kF>(timeouts);
                     xi ^= 1;
