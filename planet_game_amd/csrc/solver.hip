@@ -540,6 +540,81 @@ __device__ inline int tile_row(int tl, int n, bool& ok) {
 
 // ------------------------------------------------------------ V-cycle phases
 
+// One restriction tile of down_phase (below) for a plain input w (no
+// Source update, no A w), with every global load issued first and without
+// a branch: the tile's row id and coarse-row range, then the row's
+// columns, values and own inputs beside the aggregate bounds, then the
+// neighbour gathers (slots past the width reload the last slot, padding
+// rows reload row 0, and the results are discarded by selects), so the
+// compiler keeps them all in flight instead of waiting at every join. The
+// operations, their order and every rounding are down_phase's. kPre: the
+// level has a two-sweep iterate.
+template <class VT, bool kF, bool kPre>
+__device__ __forceinline__ void down_tile_early(const Instance& I, const Values<VT>& V, int j, int tl, const VT* w,
+                                                VT* keep, VT* keep2, VT* coarse_rhs, double* part, Lds& lds,
+                                                const VT* pre) {
+    const LevelView& L = I.lv[j];
+    const int n = L.n, W = L.width;
+    const VT om = static_cast<VT>(I.omega[j]);
+    const auto diag = V.diag(j);
+    const auto dinv = V.dinv(j);
+    const auto off = V.off(j);
+    const auto col = gptr(L.col);
+    const int row_id = ld(L.down_rows, tl * kTile + static_cast<int>(threadIdx.x));
+    const int c0 = ld(L.tile_start, tl), c1 = ld(L.tile_start, tl + 1);
+    const bool ok = row_id >= 0;
+    const int i = ok ? row_id : 0;
+    const int base = ld(L.agg_start, c0);
+    const int c = c0 + static_cast<int>(threadIdx.x);
+    const int cq = min(c, L.n_next - 1);
+    const int qa = ld(L.agg_start, cq), qb = ld(L.agg_start, cq + 1);
+    const int q0 = c < L.n_next ? qa : 0, q1 = c < L.n_next ? qb : 0;
+    const VT wi = fetch<kF>(w + i);
+    const VT pre_i = kPre ? fetch<kF>(pre + i) : VT(0);
+    const VT dinv_i = ld(dinv, i), diag_i = ld(diag, i);
+    int cs[kMaxWidth];
+    VT o[kMaxWidth];
+#pragma unroll
+    for (int s = 0; s < kMaxWidth; ++s) {
+        const int e = min(s, W - 1) * n + i;
+        const int cv = col[e];
+        cs[s] = s < W ? cv : -1;
+        o[s] = off[e];
+    }
+    VT gv[kMaxWidth];
+#pragma unroll
+    for (int s = 0; s < kMaxWidth; ++s) {
+        const int cc = cs[s] >= 0 ? cs[s] : i;
+        if constexpr (kPre)
+            gv[s] = fetch<kF>(pre + cc);
+        else
+            gv[s] = om * ld(dinv, cc) * fetch<kF>(w + cc);
+    }
+    double sq[1][1];
+    sq[0][0] = ok ? static_cast<double>(wi) * static_cast<double>(wi) : 0.0;
+    const VT xi = kPre ? pre_i : om * dinv_i * wi;
+    VT acc = wi - diag_i * xi;
+#pragma unroll
+    for (int s = 0; s < kMaxWidth; ++s)
+        if (cs[s] >= 0) acc -= o[s] * gv[s];
+    if (ok) {
+        if (keep != nullptr) publish(&keep[i], wi);
+        if (keep2 != nullptr) publish(&keep2[i], wi);
+        lds.tile[0][threadIdx.x] = static_cast<double>(acc);
+    }
+    if (part != nullptr) {
+        const int tls[1] = {tl};
+        store_partials_n<1, 1>(part, tls, sq, lds);
+    }
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < c1 - c0) {
+        double sum = 0.0;
+        for (int q = q0 - base; q < q1 - base; ++q) sum += lds.tile[0][q];
+        publish(&coarse_rhs[c0 + threadIdx.x], static_cast<VT>(sum));
+    }
+    __syncthreads();
+}
+
 // Pre-smoothing, residual and restriction of level j in one sweep over the
 // level's restriction tiles. The pre-smoothed iterate is one Jacobi sweep
 // from zero (x = om D^-1 w, evaluated where it is read), or, with `pre`
@@ -557,6 +632,16 @@ __device__ __forceinline__ void down_phase(const Instance& I, const Values<VT>& 
     const auto diag = V.diag(j);
     const auto dinv = V.dinv(j);
     const auto off = V.off(j);
+    if (kW == 1 && src.kind == 0 && au == nullptr && W >= 1 && W <= kMaxWidth && L.n_next >= 1) {
+        // the full register file: each tile's loads all issued up front
+        for (int tl = g; tl < L.down_tiles; tl += G) {
+            if (pre != nullptr)
+                down_tile_early<VT, kF, true>(I, V, j, tl, src.a, keep, keep2, coarse_rhs, part, lds, pre);
+            else
+                down_tile_early<VT, kF, false>(I, V, j, tl, src.a, keep, keep2, coarse_rhs, part, lds, pre);
+        }
+        return;
+    }
     tile_loop<sizeof(VT) == 4 || kW == 2>(L.down_tiles, g, G, [&]<int R>(const int (&tl)[R]) {
         // the tile's rows come from its padded row list (one load); the
         // aggregate bounds the restriction needs load alongside
