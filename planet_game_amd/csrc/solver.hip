@@ -1939,7 +1939,9 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     const double target = P.tolerance * b_norm;
     bool first = true;            // first pass: r = b exactly
     bool have_residual = false;   // the true residual of x is in r, r_norm, rho1
-    int pi = 0;                   // current search-direction buffer
+    // current search-direction buffer; a prepared split step's first
+    // direction was kept in p[0] by the previous step
+    int pi = use_prepared_sweep && split ? 1 : 0;
     if (zero_x && !(b_norm > 0.0 && rho1 != 0.0 && P.max_iterations > 0)) {
         // no SpMV sweep will run before x is read: zero it here
         double* x = I.x[xi];
@@ -2016,9 +2018,17 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                                          first_iteration && shadow_stale ? r0 : nullptr, fuse ? u : nullptr, split);
                 shadow_stale = false;
                 double* dot = fuse ? next_partials() : nullptr;
-                if (prepared_sweep)
-                    vcycle_rest<VT, kW>(I, V, r, ph, g, G, lds, timeouts, dot, u, mask, SplitOut{}, p_new);
-                else
+                if (prepared_sweep) {
+                    if constexpr (kForm == 1 && kW == 1 && sizeof(VT) == 8) {
+                        // the split V-cycle after its first sweep (r1 and A w
+                        // in place, w kept in p): its FP32 part on r1, output
+                        // x1 + y with x1 from w = r
+                        vcycle_rest<float, kW>(I, Values<float>{I}, nullptr, nullptr, g, G, lds, timeouts, dot, nullptr,
+                                               mask, SplitOut{r, ph, u});
+                    } else {
+                        vcycle_rest<VT, kW>(I, V, r, ph, g, G, lds, timeouts, dot, u, mask, SplitOut{}, p_new);
+                    }
+                } else
                     vcycle_rest_any(p_new, ph, dot, fuse ? u : nullptr);
                 ++v_cycles;
                 // fused: alpha = rho / (A r0, ph) from the V-cycle's partials,
@@ -2071,8 +2081,12 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     // operation): each row recomputes its neighbours' b'_c from
                     // x_c and ph_c, which it gathers anyway, and C A, A and S'
                     // at c, so the next solve starts at its down sweep.
-                    const bool sweep_next = kPrep && next_forcing != nullptr && kForm == 0 && (mask & 1) != 0 &&
-                                            I.depth > 1 && sizeof(VT) == 8;
+                    // (the split V-cycle's first sweep is the same sweep, with
+                    // its residual r1 = w' - A x1 stored in FP32 instead of the
+                    // two-sweep iterate, and w' kept as the next search
+                    // direction, which the down sweep cannot store from r1)
+                    const bool sweep_next = kPrep && next_forcing != nullptr && (kForm == 0 || split) &&
+                                            (mask & 1) != 0 && I.depth > 1 && sizeof(VT) == 8;
                     const double om0 = I.omega[0];
                     const auto dinv0 = gptr(static_cast<const double*>(I.dinv64[0]));
                     const auto diag0 = gptr(static_cast<const double*>(I.diag64[0]));
@@ -2155,8 +2169,13 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                                 publish(&r[i], sv);
                                 if (fuse) publish(&v[i], static_cast<VT>(av));
                                 if (sweep_next) {
-                                    const double x1 = om0 * ld(dinv0, i) * w_next;
-                                    publish(&static_cast<double*>(I.lvl_x[0])[i], x1 + om0 * ld(dinv0, i) * acc_next);
+                                    if (split) {
+                                        publish(&static_cast<float*>(I.lvl_x[0])[i], static_cast<float>(acc_next));
+                                        publish(&static_cast<double*>(I.p[0])[i], w_next);
+                                    } else {
+                                        const double x1 = om0 * ld(dinv0, i) * w_next;
+                                        publish(&static_cast<double*>(I.lvl_x[0])[i], x1 + om0 * ld(dinv0, i) * acc_next);
+                                    }
                                     publish(&static_cast<double*>(I.u)[i], aw_next);
                                 }
                                 q[0] = si * si;
@@ -2257,8 +2276,13 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                             publish(&r[i], sv);
                             if (fuse) publish(&v[i], static_cast<VT>(av));
                             if (sweep_next) {
-                                const double x1 = om0 * dinv_i * w_next;
-                                publish(&static_cast<double*>(I.lvl_x[0])[i], x1 + om0 * dinv_i * acc_next);
+                                if (split) {
+                                    publish(&static_cast<float*>(I.lvl_x[0])[i], static_cast<float>(acc_next));
+                                    publish(&static_cast<double*>(I.p[0])[i], w_next);
+                                } else {
+                                    const double x1 = om0 * dinv_i * w_next;
+                                    publish(&static_cast<double*>(I.lvl_x[0])[i], x1 + om0 * dinv_i * acc_next);
+                                }
                                 publish(&static_cast<double*>(I.u)[i], aw_next);
                             }
                             q[0] = si * si;

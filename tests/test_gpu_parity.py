@@ -348,24 +348,36 @@ def test_invalid_arguments_raise(ctx):
     assert fv.orientation(int(m.second[0]), 0) == -1.0
 
 
-@pytest.mark.parametrize("level", [3, 6])
-def test_chain_launch_equals_separate_steps(ctx, level):
-    """psim_heat_chain (set_matrix + N steps in one launch) gives the same
-    bits as psim_heat_set_matrix followed by N psim_heat_step launches."""
-    announce("chained launch", level)
-    m, seed, cond, ca, land, t_old, _ = heat_case(level)
+@pytest.mark.parametrize("level,precision,case", [(3, P.PLAIN_FP64, "literal"), (6, P.PLAIN_FP64, "literal"),
+                                                   (4, P.PLAIN_FP64, "stiff"), (6, P.MIXED, "sigma1"),
+                                                   (4, P.MIXED, "stiff")])
+def test_chain_launch_equals_separate_steps(ctx, level, precision, case):
+    """psim_heat_chain (set_matrix + N steps in one launch, each step's last
+    phase preparing the next step's right-hand side and first level-0 sweep)
+    gives the same bits as psim_heat_set_matrix followed by N psim_heat_step
+    launches, in both forms; the stiff inputs take several iterations per
+    step, so the prepared values are written while a step still iterates."""
+    announce("chained launch", level, "mixed" if precision == P.MIXED else "plain", case)
+    if case == "stiff":
+        m, seed, cond, ca, land, t_old, _ = heat_case(level, k_base=0.5 * 6371000.0 ** 2)
+    else:
+        m, seed, cond, ca, land, t_old, _ = heat_case(level, sigma=1.0 if case == "sigma1" else 0.3)
+    tol = 1e-10 if precision == P.MIXED else 1e-11
     steps = 6
     forcings = np.concatenate([P.heat_forcing(m, seed, k) for k in range(steps)])
     g = P.Graph(ctx, m)
-    s = P.System(g)
-    chained, infos = s.heat_chain(t_old, forcings, tolerance=1e-11,
+    s = P.System(g, precision)
+    chained, infos = s.heat_chain(t_old, forcings, tolerance=tol,
                                   conductance_per_entry=m.entry_values(cond), capacity_area=ca)
-    assert len(infos) == steps and all(i.status == 0 and i.relative_residual <= 1e-11 for i in infos)
+    assert len(infos) == steps and all(i.status == 0 and i.relative_residual <= tol for i in infos)
     assert all(i.device_ms > 0 for i in infos)
-    s2 = P.System(g)
+    if case == "stiff":
+        assert all(i.iterations > 1 for i in infos), [i.iterations for i in infos]
+    s2 = P.System(g, precision)
     s2.heat_set_matrix(m.entry_values(cond), ca)
     t = t_old.copy()
     for k in range(steps):
-        t, info = s2.heat_step(t, forcings[k * m.cells:(k + 1) * m.cells], tolerance=1e-11)
+        t, info = s2.heat_step(t, forcings[k * m.cells:(k + 1) * m.cells], tolerance=tol)
         assert info.iterations == infos[k].iterations
+        assert info.v_cycles == infos[k].v_cycles
     assert bits_equal(chained, t)
