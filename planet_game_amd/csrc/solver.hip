@@ -374,14 +374,24 @@ __device__ __forceinline__ void group_sync(int& timeouts) {
         if (lane < kLeaves && lane < G && timeouts == 0) {
             const uint32_t target = k * static_cast<uint32_t>((G - lane + kLeaves - 1) / kLeaves);
             const uint32_t* leaf = grp.barrier + 96 + 64 * lane;
-            const long long start = wall_clock64();
+            // the clock is read every kClockPolls polls only (its read waits
+            // on the scalar memory path, which the next LDS or scalar access
+            // would wait for too); a wait that ends sooner never reads it
+            constexpr uint32_t kClockPolls = 64;
+            uint32_t polls = 0;
+            long long start = 0;
             while (static_cast<int32_t>(__hip_atomic_load(const_cast<uint32_t*>(leaf), __ATOMIC_RELAXED,
                                                           __HIP_MEMORY_SCOPE_AGENT) -
                                         target) < 0) {
                 __builtin_amdgcn_s_sleep(1);
-                if (wall_clock64() - start > kBarrierTicks) {
-                    late = true;
-                    break;
+                if (++polls % kClockPolls == 0) {
+                    const long long now = wall_clock64();
+                    if (polls == kClockPolls) {
+                        start = now;
+                    } else if (now - start > kBarrierTicks) {
+                        late = true;
+                        break;
+                    }
                 }
             }
         }
@@ -1796,7 +1806,8 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                              int& parity, int& xi, int& rp, Prepared& prepared) {
     constexpr bool kPrep = kW == 1;
     constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
-    const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
+    // the solve's device clock (only the group's first workgroup reports it)
+    const uint64_t t_begin = g == 0 ? static_cast<uint64_t>(wall_clock64()) : 0;
     const Values<VT> V{I};
     const LevelView& L0 = I.lv[0];
     const int n = I.n, W = L0.width, tiles = I.tiles;
@@ -2461,7 +2472,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     out.b_norm = b_norm;
     out.v_cycles = v_cycles;
     out.t_begin = t_begin;
-    out.t_end = static_cast<uint64_t>(wall_clock64());
+    out.t_end = g == 0 ? static_cast<uint64_t>(wall_clock64()) : 0;
     return out;
 }
 
@@ -2486,7 +2497,7 @@ template <int kW>
 __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, int g, int G, Lds& lds, int& timeouts,
                                            int& parity) {
     constexpr bool kF = kW >= 1;   // lean barriers, coherent loads
-    const uint64_t t_begin = static_cast<uint64_t>(wall_clock64());
+    const uint64_t t_begin = g == 0 ? static_cast<uint64_t>(wall_clock64()) : 0;
     const Values<double> V{I};
     const LevelView& L0 = I.lv[0];
     const int n = I.n, W = L0.width, tiles = I.tiles;
@@ -2597,7 +2608,7 @@ __device__ __forceinline__ SolveResult pcg(const Instance& I, const Launch& P, i
     out.b_norm = b_norm;
     out.v_cycles = v_cycles;
     out.t_begin = t_begin;
-    out.t_end = static_cast<uint64_t>(wall_clock64());
+    out.t_end = g == 0 ? static_cast<uint64_t>(wall_clock64()) : 0;
     return out;
 }
 
