@@ -464,7 +464,9 @@ template <class VT> struct Pair {
 // a valid address, and is discarded), then the folds, so a row costs two
 // dependent memory round trips instead of two per slot. Fold order and
 // every rounding are those of the plain slot loop.
-constexpr int kMaxWidth = 7;   // widest icosphere level (hexagons + aggregation: 6, 7)
+constexpr int kMaxWidth = 7;
+// Level sweeps read in-tile neighbours from LDS (jacobi_phase).
+constexpr bool kStagedGathers = true;   // widest icosphere level (hexagons + aggregation: 6, 7)
 
 template <class CP, class OP, class Gather, class Fold>
 __device__ inline void ell_row(CP col_in, OP off_in, int n, int W, int i, const Gather& gather, const Fold& fold) {
@@ -743,6 +745,77 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
     const auto dinv = V.dinv(j);
     const auto off = V.off(j);
     const int tiles = (n + kTile - 1) / kTile;
+    // LDS-staged neighbour gathers (FP64 sweeps one tile at a time, plain
+    // inputs): each row's input and D^-1 entry go to LDS once per tile, a
+    // neighbour inside the tile (about 85 % of them on icosphere levels,
+    // whose cell order keeps neighbours close) is read from LDS and only the
+    // others from memory, so the sweep's coherent gathers drop to the tile
+    // boundary's. The values, and so every operation, are the ones the
+    // gathers from memory return.
+    if constexpr (sizeof(VT) == 8 && kW != 2) {
+        if (src.kind == 0 && kStagedGathers && W >= 1 && W <= kMaxWidth) {
+            double* sw = lds.tile[0];
+            double* sd = lds.tile[1];
+            const auto col = gptr(L.col);
+            for (int tile = g; tile < tiles; tile += G) {
+                bool okr;
+                const int ir = tile_row(tile, n, okr);
+                const int base = tile * kTile;
+                // the row's columns and values go out with its own inputs,
+                // before the staging barrier (no branch: slots past the
+                // width reload the last slot, marked padding)
+                const int Wc = W <= kMaxWidth ? W : kMaxWidth;
+                int cs[kMaxWidth];
+                VT o[kMaxWidth];
+#pragma unroll
+                for (int q = 0; q < kMaxWidth; ++q) {
+                    const int e = min(q, Wc - 1) * n + ir;
+                    const int cv = col[e];
+                    cs[q] = q < Wc ? cv : -1;
+                    o[q] = off[e];
+                }
+                const VT wr = fetch<kF>(src.a + ir);
+                const VT dr = ld(dinv, ir), gr = ld(diag, ir);
+                sw[threadIdx.x] = static_cast<double>(wr);
+                sd[threadIdx.x] = static_cast<double>(dr);
+                __syncthreads();
+                // the neighbour's input and D^-1 entry: LDS inside the tile
+                Pair<VT> v[kMaxWidth];
+#pragma unroll
+                for (int q = 0; q < kMaxWidth; ++q) {
+                    const int c = cs[q] >= 0 ? cs[q] : ir;
+                    const unsigned k = static_cast<unsigned>(c - base);
+                    if (k < static_cast<unsigned>(kTile))
+                        v[q] = Pair<VT>{static_cast<VT>(sw[k]), static_cast<VT>(sd[k])};
+                    else
+                        v[q] = Pair<VT>{fetch<kF>(src.a + c), ld(dinv, c)};
+                }
+                double sq[1] = {okr ? static_cast<double>(wr) * static_cast<double>(wr) : 0.0};
+                VT acc = wr - gr * (om * dr * wr);
+                VT aw = gr * wr;
+#pragma unroll
+                for (int q = 0; q < kMaxWidth; ++q)
+                    if (cs[q] >= 0) {
+                        acc -= o[q] * (om * v[q].second * v[q].first);
+                        if (au != nullptr) aw += o[q] * v[q].first;
+                    }
+                if (okr) {
+                    if (r1_out != nullptr) {
+                        publish(&r1_out[ir], static_cast<float>(acc));
+                    } else {
+                        const VT x1 = om * dr * wr;
+                        publish(&pre[ir], static_cast<VT>(x1 + om * dr * acc));
+                    }
+                    if (keep != nullptr) publish(&keep[ir], wr);
+                    if (keep2 != nullptr) publish(&keep2[ir], wr);
+                    if (au != nullptr) publish(&au[ir], aw);
+                }
+                if (part != nullptr) store_partials<1>(part, tile, sq, lds);
+                __syncthreads();
+            }
+            return;
+        }
+    }
     tile_loop<sizeof(VT) == 4 || kW == 2>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
         int i[R];
         bool ok[R];
