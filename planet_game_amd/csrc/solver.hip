@@ -2026,7 +2026,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     // current search-direction buffer; a prepared split step's first
     // direction was kept in p[0] by the previous step
     int pi = use_prepared_sweep && split ? 1 : 0;
-    if (zero_x && !(b_norm > 0.0 && rho1 != 0.0 && P.max_iterations > 0)) {
+    if (zero_x && !(b_norm > 0.0 && rho1 != 0.0 && P.max_iterations > 0)) [[unlikely]] {
         // no SpMV sweep will run before x is read: zero it here
         double* x = I.x[xi];
         for (int tile = g; tile < tiles; tile += G) {
@@ -2040,7 +2040,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     if (b_norm > 0.0) {
         for (;;) {
             if (!first) {
-                if (!have_residual) {
+                if (!have_residual) [[unlikely]] {
                     // FP64 true residual r = b - A x; restart from x if needed.
                     part = next_partials();
                     const double* x = I.x[xi];
@@ -2119,7 +2119,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 // read inside the next phase once its loads are in flight
                 bool alpha_ready = !fuse;
                 if (fuse) {
-                } else {
+                } else [[unlikely]] {
                 // v = A ph, (r0, v)
                 part = next_partials();
                 tile_loop<sizeof(VT) == 4 || kW == 2>(tiles, g, G, [&]<int R>(const int (&tl)[R]) {
@@ -2151,7 +2151,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 alpha = rho / r0v[0];
                 }
                 bool alpha_in_x = false;   // x already holds x + alpha ph
-                if (first_iteration) {
+                if (first_iteration) [[likely]] {
                     // r is the true residual of x here, so s = r - alpha v is
                     // b - A (x + alpha ph) up to rounding: compute it as that
                     // FP64 true residual with x' = x + alpha ph, in the phase
@@ -2198,7 +2198,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         return true;
                     };
                     bool stop = false;
-                    if (W < 1 || W > kMaxWidth) {
+                    if (W < 1 || W > kMaxWidth) [[unlikely]] {
                         // rows wider than the unrolled slots: alpha first,
                         // then the row loop
                         stop = !take_alpha();
@@ -2306,7 +2306,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                                 gd[k] = dinv0[cc];
                             }
                         }
-                        if (tile == g && !take_alpha()) {
+                        if (tile == g && !take_alpha()) [[unlikely]] {
                             stop = true;
                             break;
                         }
@@ -2375,7 +2375,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                         store_partials<KQ>(part, tile, q, lds);
                     }
                     if (!alpha_ready && !take_alpha()) stop = true;
-                    if (stop) {
+                    if (stop) [[unlikely]] {
                         // breakdown: this phase's partial buffer was not used;
                         // give it back, so the next phase writes the buffer
                         // that no workgroup can still be reading
@@ -2388,7 +2388,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     zero_x = false;   // x[xi] now holds x + alpha ph
                     final_totals(part, tot2, sweep_next);
                     ++iterations;
-                    if (sqrt(tot2[0]) <= inner_target) {
+                    if (sqrt(tot2[0]) <= inner_target) [[likely]] {
                         // r holds the true residual; a restart takes it as its
                         // shadow residual too
                         r_norm = sqrt(tot2[0]);
@@ -2400,7 +2400,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     alpha_in_x = true;
                     vcycle_first<VT, kW>(I, V, Source<VT>{s, nullptr, nullptr, 0.0, 0.0, 0}, nullptr, nullptr, g, G, lds,
                                          timeouts, mask, nullptr, nullptr, split);
-                } else {
+                } else [[unlikely]] {
                     // s = r - alpha v inside the next V-cycle's first sweep, (s, s)
                     part = next_partials();
                     vcycle_first<VT, kW>(I, V, Source<VT>{r, v, nullptr, alpha, 0.0, 2}, s, part, g, G, lds, timeouts,
@@ -2518,7 +2518,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
             }
         }
     }
-    if (zero_x) {
+    if (zero_x) [[unlikely]] {
         // a prepared step that broke down before any x update: x still
         // holds the previous step's T
         double* x = I.x[xi];
