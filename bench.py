@@ -464,6 +464,50 @@ def batched_section(P, ctx, meshes, graphs, hierarchy_levels, precision=None, au
     return out
 
 
+def autotune_section(P, ctx, meshes, graphs):
+    """The autotuner on lopsided batches (mixed precision, one heat step per
+    member): the default allocation against the tuned plan."""
+    out = []
+    for name, spec in (("8 L6", [6] * 8), ("16 L6 + 240 L3", [6] * 16 + [3] * 240)):
+        members = []
+        for i, level in enumerate(spec):
+            m = meshes[level]
+            seed = P.instance_seed(1000 + i)
+            cond, ca, _, t0 = P.heat_instance(m, seed, 1.0e6, 0.3)
+            members.append(dict(system=P.System(graphs[level], P.MIXED),
+                                cond=P.DeviceArray.of(ctx, m.entry_values(cond)), ca=P.DeviceArray.of(ctx, ca),
+                                t0=P.DeviceArray.of(ctx, t0), f=P.DeviceArray.of(ctx, P.heat_forcing(m, seed, 0)),
+                                out=P.DeviceArray(ctx, m.cells)))
+        batch = P.Batch(ctx, [x["system"] for x in members])
+        args = ([x["t0"] for x in members], [x["f"] for x in members], [x["out"] for x in members])
+        kw = dict(conductance=[x["cond"] for x in members], capacity=[x["ca"] for x in members])
+
+        def timed(reps=5):
+            batch.heat_step(*args, DT, TOL, 1000, **kw)
+            ctx.events(2)
+            ctx.synchronize()
+            ctx.record(0)
+            for _ in range(reps):
+                batch.heat_step(*args, DT, TOL, 1000, collect=False, **kw)
+            ctx.record(1)
+            batch.synchronize()
+            return ctx.elapsed_ms(0, 1) / reps
+
+        default_ms = timed()
+        default_groups = list(batch.groups)
+        batch.autotune(*args, DT, TOL, 1000, repeats=2, **kw)
+        tuned_ms = timed()
+        out.append({"batch": name, "default_ms": round(default_ms, 4), "tuned_ms": round(tuned_ms, 4),
+                    "gain": round(default_ms / tuned_ms, 3),
+                    "default_workgroups_per_instance": [min(default_groups), max(default_groups)],
+                    "tuned_workgroups_per_instance": [min(batch.groups), max(batch.groups)],
+                    "tuned_workgroups_per_cu": batch.occupancy})
+        batch.close()
+        for x in members:
+            x["system"].close()
+    return out
+
+
 class OutputRing:
     """Device output buffers for served requests: one block per level carved
     into `slots` temperature vectors, handed out in order from the start of
@@ -1161,6 +1205,7 @@ def main():
         progress("mixed precision")
         line["batched"] = batched_section(P, ctx, meshes, graphs, hierarchy_levels)
         line["batched_plain_fp64"] = batched_section(P, ctx, meshes, graphs, hierarchy_levels, P.PLAIN_FP64)
+        line["batched"]["autotuner_lopsided"] = autotune_section(P, ctx, meshes, graphs)
         progress("batched")
         line["served"] = served_section(P, ctx, meshes, graphs, value)
         progress("served")
