@@ -351,14 +351,24 @@ constexpr long long kBarrierTicks = 100000000;
 constexpr int kLeaves = 8;
 static_assert(96 + 64 * kLeaves == kBarrierWordCount, "barrier layout");
 
+template <bool kLean>
+__device__ __noinline__ int group_sync_call(int timeouts);
+
+// The barrier's code is called, not inlined: one copy of it stays in the
+// instruction cache for every phase of a step.
 template <bool kLean = false>
 __device__ __forceinline__ void group_sync(int& timeouts) {
+    timeouts = group_sync_call<kLean>(timeouts);
+}
+
+template <bool kLean>
+__device__ __noinline__ int group_sync_call(int timeouts) {
     if constexpr (kLean) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     const int G = grp.size;
     if (G == 1) {
         trace_stamp();
-        return;
+        return timeouts;
     }
     if (threadIdx.x < 64) {
         const uint32_t k = grp.episode + 1;
@@ -400,6 +410,7 @@ __device__ __forceinline__ void group_sync(int& timeouts) {
     }
     trace_stamp();
     __syncthreads();
+    return timeouts;
 }
 
 // A workgroup leaves the launch: the last of its group to leave clears the
