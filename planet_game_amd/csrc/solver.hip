@@ -475,9 +475,7 @@ template <class VT> struct Pair {
 // a valid address, and is discarded), then the folds, so a row costs two
 // dependent memory round trips instead of two per slot. Fold order and
 // every rounding are those of the plain slot loop.
-constexpr int kMaxWidth = 7;
-// Level sweeps read in-tile neighbours from LDS (jacobi_phase).
-constexpr bool kStagedGathers = true;   // widest icosphere level (hexagons + aggregation: 6, 7)
+constexpr int kMaxWidth = 7;   // widest icosphere level (hexagons + aggregation: 6, 7)
 
 template <class CP, class OP, class Gather, class Fold>
 __device__ inline void ell_row(CP col_in, OP off_in, int n, int W, int i, const Gather& gather, const Fold& fold) {
@@ -764,7 +762,7 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
     // boundary's. The values, and so every operation, are the ones the
     // gathers from memory return.
     if constexpr (sizeof(VT) == 8 && kW != 2) {
-        if (src.kind == 0 && kStagedGathers && W >= 1 && W <= kMaxWidth) {
+        if (src.kind == 0 && W >= 1 && W <= kMaxWidth) {
             double* sw = lds.tile[0];
             double* sd = lds.tile[1];
             const auto col = gptr(L.col);
