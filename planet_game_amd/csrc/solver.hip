@@ -1869,6 +1869,9 @@ struct Prepared {
     // the next solve's first level-0 Jacobi sweep is done too: its two-sweep
     // iterate is in level 0's iterate buffer and A w in u (plain form)
     bool first_sweep = false;
+    // (mixed form) the search-direction buffer that holds the next solve's
+    // first direction: the one the preparing solve's current direction is not
+    int p_next = 0;
 };
 
 // kForm: 0 the plain form, 1 the mixed form's split V-cycle, 2 either, by
@@ -1958,14 +1961,97 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
             [&](int r, VT o, VT v) { acc[r] += o * v; });
     };
 
+    // the mixed form: FP64 vectors and first level-0 sweep, the rest of the
+    // V-cycle in FP32 (vcycle_rest_split)
+    const bool split = sizeof(VT) == 8 && I.depth > 1 && (kForm == 1 || (kForm == 2 && I.precision == 1));
+    const int mask = pre_sweep_mask(I) | (split ? 1 : 0);   // levels with two pre-smoothing sweeps
+
     // b (assembled or given), x = 0, r = r0 = b; or, after a prepared
     // step, b, r and r0 are in place and x is zeroed in the first SpMV sweep.
     double* part = nullptr;
     double tot2[2];
     bool zero_x = swap;
+    // A heat step whose V-cycle starts with a level-0 Jacobi sweep (in the
+    // full-register-file program; the batch and server programs measured
+    // slower with it) assembles its right-hand side in that sweep's phase: each row forms its b and
+    // recomputes its neighbours' from T, C A, A and S (what the assembly
+    // would store there, operation for operation), then runs the sweep on
+    // w = r = b, as the previous step's last phase does in a chain. x is
+    // left to the first x update (known to be zero until then).
+    const bool assemble_swept = kW == 1 && !swap && sizeof(VT) == 8 && (P.op & kOpHeatRhs) != 0 && I.depth > 1 &&
+                                ((mask & 1) != 0 || split) && W >= 1 && W <= kMaxWidth && P.max_iterations > 0;
     if (swap) {
         tot2[0] = prepared.totals[0];
         tot2[1] = prepared.totals[1];
+    } else if (assemble_swept) {
+        part = next_partials();
+        const double om0 = I.omega[0];
+        const auto dinv0 = gptr(static_cast<const double*>(I.dinv64[0]));
+        const auto diag0 = gptr(static_cast<const double*>(I.diag64[0]));
+        const auto off0 = gptr(static_cast<const double*>(I.off64[0]));
+        const auto col0 = gptr(L0.col);
+        for (int tile = g; tile < tiles; tile += G) {
+            const int i0 = tile * kTile + static_cast<int>(threadIdx.x);
+            const bool ok = i0 < n;
+            const int i = ok ? i0 : 0;
+            int c[kMaxWidth];
+            double o[kMaxWidth];
+#pragma unroll
+            for (int k = 0; k < kMaxWidth; ++k) {
+                const int e = min(k, W - 1) * n + i;
+                const int cv = col0[e];
+                c[k] = k < W ? cv : -1;
+                o[k] = off0[e];
+            }
+            const double t_i = fetch<kF>(&t_old[i]);
+            const double ca_i = ld(P.capacity_area, i), area_i = ld(I.area, i), f_i = ld(forcing, i);
+            const double diag_i = diag0[i], dinv_i = dinv0[i];
+            double tc[kMaxWidth], cac[kMaxWidth], areac[kMaxWidth], fc[kMaxWidth], dc[kMaxWidth];
+#pragma unroll
+            for (int k = 0; k < kMaxWidth; ++k) {
+                const int cc = c[k] >= 0 ? c[k] : i;
+                tc[k] = fetch<kF>(&t_old[cc]);
+                cac[k] = ld(P.capacity_area, cc);
+                areac[k] = ld(I.area, cc);
+                fc[k] = ld(forcing, cc);
+                dc[k] = dinv0[cc];
+            }
+            double q[2] = {0.0, 0.0};
+            if (ok) {
+                const double bi = ca_i * t_i + P.dt * area_i * f_i;
+                const VT rv = static_cast<VT>(bi);
+                publish(&b[i], bi);
+                publish(&r[i], rv);
+                publish(&r0[i], rv);
+                q[0] = bi * bi;
+                q[1] = static_cast<double>(rv) * static_cast<double>(rv);
+                const double w = static_cast<double>(rv);
+                double acc = w - diag_i * (om0 * dinv_i * w);
+                double aw = diag_i * w;
+#pragma unroll
+                for (int k = 0; k < kMaxWidth; ++k)
+                    if (c[k] >= 0) {
+                        const double wc = static_cast<double>(static_cast<VT>(cac[k] * tc[k] + P.dt * areac[k] * fc[k]));
+                        acc -= o[k] * (om0 * dc[k] * wc);
+                        aw += o[k] * wc;
+                    }
+                if (split) {
+                    // the split V-cycle's first sweep: r1 in FP32, w kept as
+                    // the first search direction
+                    publish(&static_cast<float*>(I.lvl_x[0])[i], static_cast<float>(acc));
+                    publish(&static_cast<double*>(I.p[1])[i], w);
+                } else {
+                    const double x1 = om0 * dinv_i * w;
+                    publish(&static_cast<double*>(I.lvl_x[0])[i], x1 + om0 * dinv_i * acc);
+                }
+                publish(&static_cast<double*>(I.u)[i], aw);
+            }
+            store_partials<2>(part, tile, q, lds);
+        }
+        group_sync<kF>(timeouts);
+        group_totals<2, kF>(part, tiles, tot2, lds);
+        zero_x = true;
+        use_prepared_sweep = true;
     } else {
         part = next_partials();
         {
@@ -1999,16 +2085,13 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
         group_sync<kF>(timeouts);
         group_totals<2, kF>(part, tiles, tot2, lds);
     }
+    const int prepared_p_next = prepared.p_next;
     prepared = Prepared{};
     const double b_norm = sqrt(tot2[0]);
     double rho1 = tot2[1];
     double r_norm = b_norm;
     int iterations = 0, restarts = 0, v_cycles = 0;
     int status = 0;
-    // the mixed form: FP64 vectors and first level-0 sweep, the rest of the
-    // V-cycle in FP32 (vcycle_rest_split)
-    const bool split = sizeof(VT) == 8 && I.depth > 1 && (kForm == 1 || (kForm == 2 && I.precision == 1));
-    const int mask = pre_sweep_mask(I) | (split ? 1 : 0);   // levels with two pre-smoothing sweeps
     const auto vcycle_rest_any = [&](const VT* w, VT* out, double* dp, const VT* dw) {
         if constexpr (kForm != 0 && sizeof(VT) == 8) {
             if (split) {
@@ -2033,8 +2116,9 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
     bool first = true;            // first pass: r = b exactly
     bool have_residual = false;   // the true residual of x is in r, r_norm, rho1
     // current search-direction buffer; a prepared split step's first
-    // direction was kept in p[0] by the previous step
-    int pi = use_prepared_sweep && split ? 1 : 0;
+    // direction was kept by the previous step in the buffer that was not its
+    // own direction (the assembled form keeps it in p[1], the default)
+    int pi = swap && use_prepared_sweep && split ? (prepared_p_next ^ 1) : 0;
     if (zero_x && !(b_norm > 0.0 && rho1 != 0.0 && P.max_iterations > 0)) [[unlikely]] {
         // no SpMV sweep will run before x is read: zero it here
         double* x = I.x[xi];
@@ -2112,12 +2196,11 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                 shadow_stale = false;
                 double* dot = fuse ? next_partials() : nullptr;
                 if (prepared_sweep) {
-                    if constexpr (kForm == 1 && kW == 1 && sizeof(VT) == 8) {
+                    if (split) {
                         // the split V-cycle after its first sweep (r1 and A w
                         // in place, w kept in p): its FP32 part on r1, output
                         // x1 + y with x1 from w = r
-                        vcycle_rest<float, kW>(I, Values<float>{I}, nullptr, nullptr, g, G, lds, timeouts, dot, nullptr,
-                                               mask, SplitOut{r, ph, u});
+                        vcycle_rest_any(r, ph, dot, u);
                     } else {
                         vcycle_rest<VT, kW>(I, V, r, ph, g, G, lds, timeouts, dot, u, mask, SplitOut{}, p_new);
                     }
@@ -2264,7 +2347,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                                 if (sweep_next) {
                                     if (split) {
                                         publish(&static_cast<float*>(I.lvl_x[0])[i], static_cast<float>(acc_next));
-                                        publish(&static_cast<double*>(I.p[0])[i], w_next);
+                                        publish(&static_cast<double*>(I.p[pi ^ 1])[i], w_next);
                                     } else {
                                         const double x1 = om0 * ld(dinv0, i) * w_next;
                                         publish(&static_cast<double*>(I.lvl_x[0])[i], x1 + om0 * ld(dinv0, i) * acc_next);
@@ -2371,7 +2454,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                             if (sweep_next) {
                                 if (split) {
                                     publish(&static_cast<float*>(I.lvl_x[0])[i], static_cast<float>(acc_next));
-                                    publish(&static_cast<double*>(I.p[0])[i], w_next);
+                                    publish(&static_cast<double*>(I.p[pi ^ 1])[i], w_next);
                                 } else {
                                     const double x1 = om0 * dinv_i * w_next;
                                     publish(&static_cast<double*>(I.lvl_x[0])[i], x1 + om0 * dinv_i * acc_next);
@@ -2396,6 +2479,7 @@ __device__ __forceinline__ SolveResult solve(const Instance& I, const Launch& P,
                     x_zero = false;
                     zero_x = false;   // x[xi] now holds x + alpha ph
                     final_totals(part, tot2, sweep_next);
+                    prep.p_next = pi ^ 1;   // where this phase kept the next first direction
                     ++iterations;
                     if (sqrt(tot2[0]) <= inner_target) [[likely]] {
                         // r holds the true residual; a restart takes it as its

@@ -350,20 +350,26 @@ def test_invalid_arguments_raise(ctx):
 
 @pytest.mark.parametrize("level,precision,case", [(3, P.PLAIN_FP64, "literal"), (6, P.PLAIN_FP64, "literal"),
                                                    (4, P.PLAIN_FP64, "stiff"), (6, P.MIXED, "sigma1"),
-                                                   (4, P.MIXED, "stiff")])
+                                                   (4, P.MIXED, "stiff"), (5, P.PLAIN_FP64, "switch"),
+                                                   (5, P.MIXED, "switch")])
 def test_chain_launch_equals_separate_steps(ctx, level, precision, case):
     """psim_heat_chain (set_matrix + N steps in one launch, each step's last
     phase preparing the next step's right-hand side and first level-0 sweep)
     gives the same bits as psim_heat_set_matrix followed by N psim_heat_step
     launches, in both forms; the stiff inputs take several iterations per
-    step, so the prepared values are written while a step still iterates."""
+    step, so the prepared values are written while a step still iterates; at
+    L5 with conductance scale 1e8 the first steps end after one V-cycle and
+    the later ones need a full iteration, so a prepared step goes on past the
+    phase that prepares the next one."""
     announce("chained launch", level, "mixed" if precision == P.MIXED else "plain", case)
     if case == "stiff":
         m, seed, cond, ca, land, t_old, _ = heat_case(level, k_base=0.5 * 6371000.0 ** 2)
+    elif case == "switch":
+        m, seed, cond, ca, land, t_old, _ = heat_case(level, k_base=1e8)
     else:
         m, seed, cond, ca, land, t_old, _ = heat_case(level, sigma=1.0 if case == "sigma1" else 0.3)
-    tol = 1e-10 if precision == P.MIXED else 1e-11
-    steps = 6
+    tol = 1e-10 if precision == P.MIXED or case == "switch" else 1e-11
+    steps = 12 if case == "switch" else 6
     forcings = np.concatenate([P.heat_forcing(m, seed, k) for k in range(steps)])
     g = P.Graph(ctx, m)
     s = P.System(g, precision)
@@ -373,6 +379,9 @@ def test_chain_launch_equals_separate_steps(ctx, level, precision, case):
     assert all(i.device_ms > 0 for i in infos)
     if case == "stiff":
         assert all(i.iterations > 1 for i in infos), [i.iterations for i in infos]
+    if case == "switch":
+        v = [i.v_cycles for i in infos]
+        assert any(a == 1 and b > 1 for a, b in zip(v, v[1:])), v
     s2 = P.System(g, precision)
     s2.heat_set_matrix(m.entry_values(cond), ca)
     t = t_old.copy()
