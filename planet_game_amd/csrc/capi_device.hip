@@ -61,8 +61,11 @@ void launch(psim_system* s, const Launch& p) {
     // a group of more than one workgroup that fits one per CU runs the
     // full-register-file instance of the program
     const bool wide = cooperative && s->workgroups <= ctx->compute_units;
+    // a mixed system's solve in the full-register-file program: the kernel
+    // with that solve inlined
+    const bool mixed_solve = wide && (p.op & psim::dev::kOpSolve) && s->host.precision == 1;
     psim::hip_check(psim::dev::launch_program(s->d_inst, s->d_map, nullptr, p, s->workgroups, lds, ctx->stream,
-                                              cooperative, wide ? 1 : 3),
+                                              cooperative, mixed_solve ? psim::dev::kMixedProgram : wide ? 1 : 3),
                     "solver launch");
 }
 

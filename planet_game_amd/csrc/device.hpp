@@ -175,6 +175,9 @@ struct Launch {
 // for 1, 2 or 3 workgroups per CU (the register budget of 512, 256 or 168
 // VGPRs): 1 for a single instance whose group fits one workgroup per CU, 2
 // or 3 for batches.
+// kMixedProgram as `waves`: the full-register-file program with the
+// mixed-precision solve inlined, for one mixed system.
+constexpr int kMixedProgram = -1;
 hipError_t launch_program(const Instance* instances, const int32_t* block_instance, const Launch* launches,
                           const Launch& single, int32_t blocks, size_t dynamic_lds, hipStream_t stream,
                           bool cooperative, int waves = 3);

@@ -2885,11 +2885,14 @@ __device__ __noinline__ SyncState run_solve_call(const Instance* instance, const
 // group has at most one workgroup per CU, where occupancy buys nothing and
 // the register file is worth more: one wave per SIMD lets the compiler keep
 // the solve's state in registers (512 VGPRs) instead of scratch.
-template <int kWaves>
-__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves))) void program_kernel_t(
-    const Instance* instances, const int32_t* block_instance, const Launch* launches, Launch single) {
-    __shared__ Lds lds;
-    __shared__ Launch params;
+// The program's body. kMixedInline (single mixed systems in the
+// full-register-file program): the solve is inlined in the split form
+// instead of called out of line, so a launch does not save and restore the
+// called solve's registers in scratch memory (about 30 us per separate
+// step); a kernel of its own, so the plain program keeps its registers.
+template <int kWaves, bool kMixedInline>
+__device__ __forceinline__ void program_body(const Instance* instances, const int32_t* block_instance,
+                                             const Launch* launches, const Launch& single, Lds& lds, Launch& params) {
     // the workgroup's instance, uniform: its descriptor fields load as scalar
     // loads into SGPRs instead of per-lane vector loads
     const int index = __builtin_amdgcn_readfirstlane(block_instance[blockIdx.x]);
@@ -2948,7 +2951,9 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
         }
     }
     if (P.op & kOpSolve) {
-        if constexpr (kWaves == 1) {
+        if constexpr (kWaves == 1 && kMixedInline) {
+            run_solve<double, kWaves, 2>(I, P, g, G, lds, timeouts, parity);
+        } else if constexpr (kWaves == 1) {
             run_heat_solve<kWaves>(I, P, g, G, lds, timeouts, parity);
         } else {
             const SyncState st = run_solve_call<kWaves>(&I, P, g, G, lds, SyncState{timeouts, parity});
@@ -2974,6 +2979,22 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
     }
     if (timeouts != 0 && threadIdx.x == 0) atomicAdd(&grp.result->barrier_timeouts, timeouts);
     group_exit();
+}
+
+template <int kWaves>
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves))) void program_kernel_t(
+    const Instance* instances, const int32_t* block_instance, const Launch* launches, Launch single) {
+    __shared__ Lds lds;
+    __shared__ Launch params;
+    program_body<kWaves, false>(instances, block_instance, launches, single, lds, params);
+}
+
+// One mixed-precision system in the full-register-file program.
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(1))) void program_kernel_mixed(
+    const Instance* instances, const int32_t* block_instance, const Launch* launches, Launch single) {
+    __shared__ Lds lds;
+    __shared__ Launch params;
+    program_body<1, true>(instances, block_instance, launches, single, lds, params);
 }
 
 template __global__ void program_kernel_t<3>(const Instance*, const int32_t*, const Launch*, Launch);
@@ -3145,6 +3166,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kWaves)))
 
 // The kernels of one occupancy: 1, 2 or 3 workgroups per CU.
 const void* program_kernel(int waves) {
+    if (waves == kMixedProgram) return reinterpret_cast<const void*>(&program_kernel_mixed);
     if (waves == 1) return reinterpret_cast<const void*>(&program_kernel_t<1>);
     if (waves == 2) return reinterpret_cast<const void*>(&program_kernel_t<2>);
     return reinterpret_cast<const void*>(&program_kernel_t<3>);
