@@ -9,7 +9,10 @@ against scipy's sparse direct solve of the same system. The test requires a
 true relative residual of at most 1e-9 at tolerance 1e-10, and an error within
 kappa times that residual, with kappa bounded by Gershgorin's discs. Chains
 must equal the same steps launched one by one, bit for bit, and the assembly
-and SpMV must match the slot-order loops bit for bit.
+and SpMV must match the slot-order loops bit for bit. A server with three
+queues (the widened L5 graph of test_gpu_wide, L3 with isolated cells, and the
+one-cell graph) must serve interleaved sessions bit for bit as a stand-alone
+system does the same steps.
 """
 import types
 
@@ -19,7 +22,7 @@ import scipy.sparse.linalg as spla
 
 import planet_game_amd as P
 from test_gpu_parity import announce, bits_equal, product_mesh
-from test_gpu_wide import DT, STIFF, heat_system, kappa_bound, slot_loops
+from test_gpu_wide import DT, STIFF, heat_system, kappa_bound, slot_loops, wide_case
 
 pytestmark = pytest.mark.gpu
 
@@ -118,3 +121,42 @@ def test_small_graph_steps_and_chain(ctx, name, precision):
         assert true <= 1e-9 and err <= kappa_bound(A) * true * (1 + 1e-6) + 1e-14, k
         t = t_next
     assert bits_equal(chained, t)
+
+
+@pytest.mark.parametrize("precision", [P.MIXED, P.PLAIN_FP64])
+def test_wide_and_degenerate_graphs_served_equal_alone(ctx, precision):
+    cases = [wide_case(STIFF), case("l3_isolated"), case("one_cell")]
+    announce("served wide and degenerate graphs, precision", precision, [c[0].cells for c in cases])
+    steps = 3
+    graphs = [P.Graph(ctx, c[0]) for c in cases]
+    server = P.Server(ctx, graphs, precision=precision, memory_budget=8 << 30)
+    dev = []
+    for g, cond, ca, t_old, forcings in cases:
+        dev.append({"cond": P.DeviceArray.of(ctx, cond), "ca": P.DeviceArray.of(ctx, ca),
+                    "t0": P.DeviceArray.of(ctx, t_old),
+                    "f": [P.DeviceArray.of(ctx, forcings[k]) for k in range(steps)],
+                    "out": [P.DeviceArray(ctx, g.cells) for _ in range(steps)]})
+    requests = []
+    for step in range(steps):
+        for q, d in enumerate(dev):
+            r = P.ServeRequest()
+            r.arrival_s = 0.002 * len(requests)
+            r.graph, r.session = q, q
+            r.first, r.last = int(step == 0), int(step == steps - 1)
+            r.conductance_per_entry = d["cond"].ptr if step == 0 else None
+            r.capacity_area = d["ca"].ptr if step == 0 else None
+            r.temperature_old = (d["t0"] if step == 0 else d["out"][step - 1]).ptr
+            r.forcing_W_m2 = d["f"][step].ptr
+            r.temperature_new = d["out"][step].ptr
+            requests.append(r)
+    records = server.run(requests, DT, 1e-10, 1000)
+    for k, rec in enumerate(records):
+        assert rec.status == 0 and rec.relative_residual <= 1e-10, (k, rec.status, rec.relative_residual)
+    for q, ((g, cond, ca, t_old, forcings), d) in enumerate(zip(cases, dev)):
+        alone = P.System(graphs[q], precision)
+        alone.heat_set_matrix(cond, ca)
+        t = t_old
+        for step in range(steps):
+            t, info = alone.heat_step(t, forcings[step], DT, 1e-10, 1000)
+            assert info.status == 0
+            assert bits_equal(d["out"][step].download(), t), (q, step)
