@@ -75,6 +75,30 @@ def test_noalloc_matches_heat_step_twice(ctx, precision):
             assert err <= 1e-11, err
 
 
+@pytest.mark.parametrize("precision", [P.PLAIN_FP64, P.MIXED], ids=["fp64", "mixed"])
+def test_noalloc_on_a_wide_graph(ctx, precision):
+    """The widened L5 graph of test_gpu_wide (rows of 18, 11 and 8 entries on
+    its three levels, so the solver's general row loops), stiff conductances
+    (several iterations): the same bits and solve record as psim_heat_step."""
+    from test_gpu_wide import STIFF, wide_case
+    g, cond, ca, t_old, forcings = wide_case(STIFF)
+    announce("noalloc vs heat_step", "wide L5 graph", "precision", precision)
+    system = P.System(P.Graph(ctx, g), precision)
+    d_cond, d_ca, d_t0, d_f = (P.DeviceArray.of(ctx, a) for a in (cond, ca, t_old, forcings[0]))
+    system.heat_set_matrix_device(d_cond, d_ca, 2.63e6)
+    out_ref, out_na = P.DeviceArray(ctx, g.cells), P.DeviceArray(ctx, g.cells)
+    system.heat_step_device(d_t0.ptr, d_f.ptr, out_ref.ptr, 2.63e6, 1e-10, 1000)
+    ref_info = system.last_info()
+    info = P.SolveInfo()
+    status = P.lib.psim_heat_step_noalloc(system.handle, d_t0.ptr, d_f.ptr, C.c_double(2.63e6), C.c_double(1e-10),
+                                          1000, out_na.ptr, C.byref(info))
+    assert status == 0, P.lib.psim_last_error()
+    assert bits_equal(out_na.download(), out_ref.download())
+    for field in ("iterations", "restarts", "status", "v_cycles", "relative_residual"):
+        assert getattr(info, field) == getattr(ref_info, field), field
+    assert info.status == 0 and info.iterations > 1
+
+
 def test_noalloc_rejects_a_system_above_the_limit(ctx):
     announce("noalloc limit", "L7 fp64")
     mesh = P.Mesh(7)   # 163,842 cells: about 21 MB of FP64 workspace
