@@ -211,8 +211,8 @@ def cpu_baseline(mesh, cond, ca, t0, forcings, seconds, workers=1):
     return solves / elapsed, solves, chains, elapsed, final, kind
 
 
-def config0_section(P, ctx, seconds=3.0):
-    """Config 0's size (L4, 2,562 cells), the literal distributions: the
+def l4_chain_section(P, ctx, seconds=3.0):
+    """The L4 chain (2,562 cells), the literal distributions: the
     120-step chain on the GPU (plain FP64, one launch per pass) beside the
     reference CPU path on a bounded sample of the same chain."""
     mesh = P.Mesh(4)
@@ -265,7 +265,7 @@ def fv_bytes(fields, n, edges, directed):
 
 
 def fv_section(P, ctx, mesh, with_cpu, cpu_workers, repeats=50):
-    """Config 1's operator half: 64 seeded FP64 fields through the LSQ
+    """The operator sweeps beside the L6 chain: 64 seeded FP64 fields through the LSQ
     gradient and the edge-flux divergence at L6 (cell scalars and edge fluxes
     u - 0.5, as test_operator_determinism.cpp:15-26), one launch per operator
     for all 64 fields; HIP events on the library's stream around `repeats`
@@ -401,7 +401,7 @@ def mixed_section(P, ctx, mesh, graph, levels, with_oracle):
 def batched_section(P, ctx, meshes, graphs, hierarchy_levels, precision=None, autotune=True):
     """256 independent instances, L3:L4:L5:L6 = 64 each, own seeds, matrix
     assembled in the same launch; default allocation and the autotuner's.
-    Mixed precision (config 3) unless another precision is given."""
+    Mixed precision unless another precision is given."""
     precision = P.MIXED if precision is None else precision
     members = []
     for i, level in enumerate([3, 4, 5, 6] * 64):
@@ -1198,8 +1198,8 @@ def main():
         progress("CPU baseline and oracle checks")
         line["fv_sweeps"] = fv_section(P, ctx, mesh, True, best)
         progress("FV sweeps")
-        line["config0_l4"] = config0_section(P, ctx)
-        progress("config 0")
+        line["l4_chain"] = l4_chain_section(P, ctx)
+        progress("L4 chain")
     else:
         line["fv_sweeps"] = fv_section(P, ctx, mesh, False, 1)
     if not args.no_extras:

@@ -882,7 +882,7 @@ __device__ __forceinline__ void jacobi_phase(const Instance& I, const Values<VT>
 // 0, and wave_sum adds the lanes in its fixed order, so every workgroup computing a row
 // gets the same bits. A wave takes kCoarseRows rows at a time.
 //
-// Two steps: coarse_rows_issue loads the first round's row ids and, with
+// Two steps: coarse_rows_issue loads the first batch's row ids and, with
 // kEarly, the first kCoarseChunks columns of each of those rows for this
 // lane (independent of the right-hand side, so they are in flight while the
 // caller issues its own loads and the right-hand side is staged);
@@ -938,8 +938,8 @@ __device__ __forceinline__ void coarse_rows_finish(const Instance& I, const VT* 
     __syncthreads();
     const auto inv = gptr(static_cast<const double*>(I.cinv));
     for (int q0 = wave * kCoarseRows; q0 < count; q0 += 4 * kCoarseRows) {
-        const bool first_round = q0 == wave * kCoarseRows;
-        if (!first_round) coarse_row_ids<kEarly>(list, first, count, q0, cf.a);
+        const bool first_batch = q0 == wave * kCoarseRows;
+        if (!first_batch) coarse_row_ids<kEarly>(list, first, count, q0, cf.a);
         double acc[kCoarseRows];
 #pragma unroll
         for (int u = 0; u < kCoarseRows; ++u) acc[u] = 0.0;
@@ -948,7 +948,7 @@ __device__ __forceinline__ void coarse_rows_finish(const Instance& I, const VT* 
         // loads a valid element and is discarded); each lane still adds its
         // columns in ascending order
         int k0 = lane;
-        if (kEarly && first_round) {
+        if (kEarly && first_batch) {
             // the first columns, loaded by coarse_rows_issue
 #pragma unroll
             for (int u = 0; u < kCoarseRows; ++u)
@@ -1146,7 +1146,7 @@ __device__ __forceinline__ void merged_up_tile_early(const Instance& I, const Va
     const double* split_w = split ? so.w : static_cast<const double*>(I.dinv64[0]);
     const double* split_dot = split && so.dot != nullptr ? so.dot : static_cast<const double*>(I.dinv64[0]);
     const VT* dot_in = !split && dot_with != nullptr ? dot_with : rhs;
-    // round 1: ids, static row data and own inputs
+    // first memory round trip: ids, static row data and own inputs
     const int mq = threadIdx.x;
     const int m_id = ld(L.mid, tile * L.mid_stride + min(mq, L.mid_stride - 1));
     const int m = mq < L.mid_stride ? m_id : -1;
@@ -1172,8 +1172,8 @@ __device__ __forceinline__ void merged_up_tile_early(const Instance& I, const Va
     const double sw = fetch<kF>(&split_w[i]);
     const double sdot = fetch<kF>(&split_dot[i]);
     const double dv = static_cast<double>(fetch<kF>(&dot_in[i]));
-    // round 2: the level-(j+1) row (its id came in round 1) and the level-0
-    // gathers
+    // second round trip: the level-(j+1) row (its id came in the first) and
+    // the level-0 gathers
     const int mm = m >= 0 ? m : 0;
     int cm[kMaxWidth], am[kMaxWidth];
     VT omv[kMaxWidth];
@@ -1198,7 +1198,7 @@ __device__ __forceinline__ void merged_up_tile_early(const Instance& I, const Va
         else
             gv[s] = om * ld(dinv, cc) * fetch<kF>(&rhs[cc]);
     }
-    // round 3: the level-(j+1) gathers
+    // third round trip: the level-(j+1) gathers
     VT gm[kMaxWidth];
 #pragma unroll
     for (int s = 0; s < kMaxWidth; ++s) {

@@ -1,13 +1,13 @@
 """The measured configurations at their own sizes and tolerance, against the
 CPU oracle.
 
-- Config 1: the L6 120-step chain at tolerance 1e-10, each step's error
+- The L6 120-step chain at tolerance 1e-10, each step's error
   against the oracle solving that step (BiCGSTAB to 1e-14) from the same T_k;
   the single-launch chain equal bit for bit to the step-by-step launches;
   the trajectory against the oracle's own chain solved to 1e-14.
-- Config 3: the full 256-instance batch (64 each of L3-L6, own seeds), every
+- The full 256-instance batch (64 each of L3-L6, own seeds), every
   member against the FP64 oracle, mixed and plain.
-- Config 4: a seeded Poisson stream over L3-L6 (40:30:20:10), sessions of 1
+- A seeded Poisson stream over L3-L6 (40:30:20:10), sessions of 1
   and 12 steps sharing a small bank of matrices so that pooled systems are
   reused across sessions; every result bit-equal to its stand-alone run, and
   the sessions' last steps against the FP64 oracle.
@@ -72,19 +72,19 @@ def test_stiff_iterations_match_reference(ctx, level, tol, key):
     assert true_res <= tol, true_res
 
 
-# ---------------------------------------------------------------- config 1
+# ------------------------------------------------------- L6 120-step chain
 
 
 def test_l6_chain_each_step_against_oracle(ctx):
-    """Config 1 at the metric's tolerance: 120 chained L6 steps. Each GPU
+    """The L6 chain at the metric's tolerance: 120 chained steps. Each GPU
     step T_{k+1} is compared with the oracle's solve of step k from the
     GPU's own T_k (BiCGSTAB to 1e-14): the per-step error is bounded by
-    1e-12 relative L2, the north_star's parity bar. The single launch of the
+    1e-12 relative L2, the FP64 path's parity bar. The single launch of the
     whole chain (psim_heat_chain) equals the step-by-step launches bit for
     bit, and the trajectory stays within 120 x 1e-12 of the oracle's own
     chain solved to 1e-14."""
     level, steps = 6, 120
-    announce("config 1 chain", level, steps)
+    announce("L6 chain", level, steps)
     m, seed, cond, ca, land, t0, _ = heat_case(level)
     forcings = [P.heat_forcing(m, seed, k) for k in range(steps)]
     s = P.System(graph(ctx, level))
@@ -130,18 +130,18 @@ def test_l6_chain_each_step_against_oracle(ctx):
     assert drift <= steps * 1e-12, drift
 
 
-# ---------------------------------------------------------------- config 3
+# ------------------------------------------------ 256-instance batch
 
 
 @pytest.mark.parametrize("precision", [P.MIXED, P.PLAIN_FP64], ids=["mixed", "fp64"])
-def test_full_config3_batch_against_oracle(ctx, precision):
-    """Config 3 at its full size: 256 instances, 64 each of L3/L4/L5/L6 in the
+def test_full_256_instance_batch_against_oracle(ctx, precision):
+    """The batch at its full size: 256 instances, 64 each of L3/L4/L5/L6 in the
     bench's order and seeds, set_matrix + one solve in ONE launch. Mixed:
     tolerance 1e-10, every member <= 1e-8 relative L2 from the FP64 oracle;
     plain: tolerance 1e-13, every member <= 1e-12."""
     tol, bound = (TOL, 1e-8) if precision == P.MIXED else (1e-13, 1e-12)
     spec = [3, 4, 5, 6] * 64
-    announce("config 3 batch", len(spec), "precision", precision)
+    announce("256-instance batch", len(spec), "precision", precision)
     members = []
     for i, level in enumerate(spec):
         m = product_mesh(level)
@@ -169,7 +169,7 @@ def test_full_config3_batch_against_oracle(ctx, precision):
     print("worst error per level", {k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
 
 
-# ---------------------------------------------------------------- config 4
+# ------------------------------------------------- Poisson served stream
 
 
 class StreamSession:
@@ -202,7 +202,7 @@ def test_poisson_stream_equal_alone_and_oracle(ctx):
                                 d_cond=P.DeviceArray.of(ctx, m.entry_values(cond)),
                                 d_ca=P.DeviceArray.of(ctx, ca), d_t0=P.DeviceArray.of(ctx, t0)))
     count = 360
-    announce("config 4 stream", count, "requests")
+    announce("Poisson stream", count, "requests")
     times = np.cumsum(rng.exponential(1.0 / 2000.0, count))
     times -= times[0]
     sessions, live, requests, order = [], [], [], []

@@ -82,7 +82,7 @@ def test_fv_operators_bit_exact(ctx, level, dtype):
 
 
 def test_fv_sixty_four_fields_l6(ctx):
-    """Config 1's operator sweep: 64 seeded FP64 fields at L6 in one call,
+    """The L6 operator sweep: 64 seeded FP64 fields at L6 in one call,
     every field bit-identical to the oracle."""
     level, fields = 6, 64
     announce("fv 64 fields", level)
@@ -309,7 +309,7 @@ def test_mixed_precision_error_vs_fp64_oracle(ctx, level):
 
 
 def test_chained_steps_track_oracle(ctx):
-    """Config 1's form at L4: twelve monthly steps, each T feeding the next."""
+    """The chained form at L4: twelve monthly steps, each T feeding the next."""
     level = 4
     announce("chained steps", level)
     m, seed, cond, ca, land, t_old, _ = heat_case(level)
