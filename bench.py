@@ -952,10 +952,11 @@ def main():
     ap.add_argument("--workgroups", type=int, default=0, help="override the solver's workgroup count")
     ap.add_argument("--only-consumers", action="store_true",
                     help="run only the consumer measurements (tracer, ILU, advection, operator solves)")
-    ap.add_argument("--only", choices=["chain", "fv", "batched", "served", "kernels"],
+    ap.add_argument("--only", choices=["chain", "fv", "mixed", "batched", "served", "kernels"],
                     help="run only one section (profiling runs): the headline chain's census, warm-up and timed "
-                         "passes, the 64-field FV sweeps, the 256-instance batch, a short served run, or "
-                         "'kernels': the chain passes, then the batch, then the served run, in one process")
+                         "passes, the 64-field FV sweeps, the mixed-precision chain beside the plain one (no "
+                         "oracle), the 256-instance batch, a short served run, or 'kernels': the chain passes, "
+                         "then the batch, then the served run, in one process")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed-precision, batched and served measurements")
     args = ap.parse_args()
@@ -982,6 +983,10 @@ def main():
         ctx.synchronize()
         if args.only == "fv":
             print(json.dumps({"fv_sweeps": fv_section(P, ctx, mesh, False, 1)}))
+            return
+        if args.only == "mixed":
+            levels = [(lv["size"], lv["nnz"]) for lv in P.Hierarchy(mesh).levels]
+            print(json.dumps({"mixed_precision": mixed_section(P, ctx, mesh, P.Graph(ctx, mesh), levels, False)}))
             return
         meshes = {L: (mesh if L == level else P.Mesh(L)) for L in (3, 4, 5, 6)}
         graphs = {L: P.Graph(ctx, meshes[L]) for L in meshes}
