@@ -174,7 +174,11 @@ int psim_laplacian(psim_fv* fv, int dtype, uint64_t fields, const void* scalar, 
 /* ------------------------------------------- transport graph on device
  * TransportGraph (heat_transport.hpp:27-36) plus the multigrid hierarchy
  * structure built from it (GraphMultigrid's constructor,
- * heat_transport.cpp:550-553). Shared by every system on the graph. */
+ * heat_transport.cpp:550-553). Shared by every system on the graph.
+ * offset[0..n]: CSR row starts, offset[0] == 0, never decreasing, fewer
+ * than 2^31 entries; any row width (empty rows included). neighbour and
+ * weight may be NULL when the graph has no entries. Otherwise
+ * PSIM_INVALID_ARGUMENT. (psim_hierarchy_create checks its graph alike.) */
 typedef struct psim_graph psim_graph;
 int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area_m2, const uint64_t* offset,
                       const uint64_t* neighbour, const double* weight, const double* weight_sum, psim_graph** out);

@@ -275,14 +275,17 @@ int psim_device_copy(psim_ctx* ctx, void* dst, const void* src, uint64_t bytes) 
 int psim_graph_create(psim_ctx* ctx, uint64_t n, const double* area, const uint64_t* offset, const uint64_t* neighbour,
                       const double* weight, const double* weight_sum, psim_graph** out) {
     return guarded([&] {
-        if (ctx == nullptr || out == nullptr || !area || !offset || !neighbour || !weight || !weight_sum)
+        if (ctx == nullptr || out == nullptr || !area || !offset || !weight_sum)
             throw std::invalid_argument("null argument");
         if (n == 0 || n >= (1ULL << 30)) throw std::invalid_argument("graph size out of range");
+        // a graph without entries (isolated nodes only) may pass no neighbour
+        // or weight arrays
+        const uint64_t nnz = psim::checked_graph_entries(n, offset);
+        if (nnz > 0 && (!neighbour || !weight)) throw std::invalid_argument("null argument");
         psim::HostGraph g;
         g.n = static_cast<int32_t>(n);
         g.row.resize(n + 1);
         for (uint64_t i = 0; i <= n; ++i) g.row[i] = static_cast<int32_t>(offset[i]);
-        const uint64_t nnz = offset[n];
         g.col.resize(nnz);
         for (uint64_t k = 0; k < nnz; ++k) {
             if (neighbour[k] >= n) throw std::invalid_argument("graph neighbour out of range");

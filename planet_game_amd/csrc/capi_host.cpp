@@ -153,11 +153,12 @@ int psim_hierarchy_create(uint64_t n, const uint64_t* offset, const uint64_t* ne
     return guarded([&] {
         if (out == nullptr || offset == nullptr || weight_sum == nullptr) throw std::invalid_argument("null argument");
         if (n >= (1ULL << 30)) throw std::invalid_argument("graph too large");
+        const uint64_t nnz = psim::checked_graph_entries(n, offset);
+        if (nnz > 0 && (neighbour == nullptr || weight == nullptr)) throw std::invalid_argument("null argument");
         psim::HostGraph g;
         g.n = static_cast<int32_t>(n);
         g.row.resize(n + 1);
         for (uint64_t i = 0; i <= n; ++i) g.row[i] = static_cast<int32_t>(offset[i]);
-        const uint64_t nnz = offset[n];
         g.col.resize(nnz);
         g.weight.assign(weight, weight + nnz);
         for (uint64_t k = 0; k < nnz; ++k) {

@@ -244,6 +244,14 @@ void mid_lists(HostLevel& fine, const HostLevel& mid) {
 
 }  // namespace
 
+uint64_t checked_graph_entries(uint64_t n, const uint64_t* offset) {
+    if (offset[0] != 0) throw std::invalid_argument("graph offsets must start at 0");
+    for (uint64_t i = 0; i < n; ++i)
+        if (offset[i + 1] < offset[i]) throw std::invalid_argument("graph offsets must not decrease");
+    if (offset[n] >= (1ULL << 31)) throw std::invalid_argument("graph has too many entries");
+    return offset[n];
+}
+
 HostHierarchy build_hierarchy(const HostGraph& g) {
     if (static_cast<int32_t>(g.row.size()) != g.n + 1) throw std::invalid_argument("graph offsets do not match its size");
     HostHierarchy h;

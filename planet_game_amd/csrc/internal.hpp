@@ -154,6 +154,11 @@ struct HostHierarchy {
 // heat_transport.cpp:175-200, on the graph's unit-conductance weights.
 HostHierarchy build_hierarchy(const HostGraph& graph);
 
+// A caller's CSR offsets of n rows: offset[0] == 0, never decreasing, and
+// fewer than 2^31 entries in all (rows and entries are int32 on the
+// device). Throws std::invalid_argument otherwise; returns the entry count.
+uint64_t checked_graph_entries(uint64_t n, const uint64_t* offset);
+
 HostGraph graph_of(const HostMesh& mesh);
 
 // The agglomerated transport graph's groups (heat_transport.cpp:467-512):

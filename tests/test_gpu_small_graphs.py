@@ -14,6 +14,7 @@ queues (the widened L5 graph of test_gpu_wide, L3 with isolated cells, and the
 one-cell graph) must serve interleaved sessions bit for bit as a stand-alone
 system does the same steps.
 """
+import ctypes as C
 import types
 
 import numpy as np
@@ -160,3 +161,39 @@ def test_wide_and_degenerate_graphs_served_equal_alone(ctx, precision):
             t, info = alone.heat_step(t, forcings[step], DT, 1e-10, 1000)
             assert info.status == 0
             assert bits_equal(d["out"][step].download(), t), (q, step)
+
+
+def test_graph_of_isolated_nodes_from_null_arrays(ctx):
+    """psim_graph_create with no entries and NULL neighbour and weight
+    arrays: each node's heat step is its own scalar equation,
+    T' = (C A T + dt A S) / (C A), which one coarsest solve gives exactly
+    (relative 1e-15)."""
+    n = 7
+    announce("graph of isolated nodes from NULL arrays", n)
+    rng = np.random.default_rng(11)
+    area = 1.0e10 * (1.0 + rng.random(n))
+    ca = 1.0e7 * area
+    t_old = 250.0 + 50.0 * rng.random(n)
+    forcing = 100.0 * (rng.random(n) - 0.5)
+    f = C.CDLL(P.LIB_PATH).psim_graph_create
+    f.restype = C.c_int
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    handle = C.c_void_p()
+    offset = np.zeros(n + 1, np.uint64)
+    status = f(ctx.handle, C.c_uint64(n), ptr(area), ptr(offset), None, None, ptr(np.zeros(n)), C.byref(handle))
+    assert status == 0, P.lib.psim_last_error()
+    assert P.lib.psim_graph_depth(handle) == 1 and P.lib.psim_graph_entries(handle) == 0
+    s = C.c_void_p()
+    assert P.lib.psim_system_create(handle, P.PLAIN_FP64, C.byref(s)) == 0, P.lib.psim_last_error()
+    info = P.SolveInfo()
+    out = np.empty(n)
+    empty = np.zeros(1)
+    status = P.lib.psim_heat_set_matrix(s, ptr(empty), ptr(ca), C.c_double(DT), P.HOST)
+    assert status == 0, P.lib.psim_last_error()
+    status = P.lib.psim_heat_step(s, ptr(t_old), ptr(forcing), C.c_double(DT), C.c_double(1e-10), 1000, ptr(out),
+                                  C.byref(info), P.HOST)
+    assert status == 0 and info.status == 0, P.lib.psim_last_error()
+    expect = (ca * t_old + DT * area * forcing) / ca
+    assert np.max(np.abs(out - expect) / np.abs(expect)) <= 1e-15
+    P.lib.psim_system_destroy(s)
+    P.lib.psim_graph_destroy(handle)

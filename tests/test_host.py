@@ -96,6 +96,35 @@ def test_hierarchy_rejects_bad_graph():
     h = C.c_void_p()
     status = P.lib.psim_hierarchy_create(m.cells, m.offset, bad, m.weight, m.weight_sum, C.byref(h))
     assert status == 1 and b"out of range" in P.lib.psim_last_error()
+    # offsets: not starting at 0, decreasing, or past the entry arrays
+    for offset, message in ((m.offset + 1, b"start at 0"),
+                            (np.concatenate([m.offset[:5], m.offset[3:4], m.offset[6:]]), b"must not decrease")):
+        offset = np.ascontiguousarray(offset, np.uint64)
+        status = P.lib.psim_hierarchy_create(m.cells, offset, m.graph_neighbour, m.weight, m.weight_sum,
+                                             C.byref(h))
+        assert status == 1 and message in P.lib.psim_last_error(), message
+    status = _raw_hierarchy_create(m.cells, m.offset, None, None, m.weight_sum, h)
+    assert status == 1 and b"null argument" in P.lib.psim_last_error()
+
+
+def _raw_hierarchy_create(n, offset, neighbour, weight, weight_sum, h):
+    """psim_hierarchy_create without the binding's array types, so that NULL
+    arrays can be passed."""
+    f = C.CDLL(P.LIB_PATH).psim_hierarchy_create
+    f.restype = C.c_int
+    ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+    return f(C.c_uint64(n), ptr(offset), ptr(neighbour), ptr(weight), ptr(weight_sum), C.byref(h))
+
+
+def test_hierarchy_of_a_graph_without_entries():
+    """Isolated nodes only: no neighbour or weight arrays are needed, and
+    every node is its own coarsest row."""
+    n = 5
+    h = C.c_void_p()
+    status = _raw_hierarchy_create(n, np.zeros(n + 1, np.uint64), None, None, np.zeros(n), h)
+    assert status == 0, P.lib.psim_last_error()
+    assert P.lib.psim_hierarchy_depth(h) == 1
+    P.lib.psim_hierarchy_destroy(h)
 
 
 def test_no_gpu_is_a_device_error():
