@@ -391,7 +391,8 @@ int psim_implicit_transport_table(psim_graph* graph, double conductance_W_K, con
  * IncompleteLU (sim/core/math/incomplete_lu.hpp:14-43, .cpp:40-146), block
  * form, on the device: rows [block_start[b], block_start[b+1]) keep their
  * diagonal and their in-block off-diagonals of the CSR pattern
- * (offset[n+1], neighbour[offset[n]]) and are factored on their own.
+ * (offset[n+1], neighbour[offset[n]]; offsets checked as for
+ * psim_graph_create) and are factored on their own.
  * Factors and solves are bit-identical to the reference's. One workgroup
  * per block walks its rows' level schedule. */
 typedef struct psim_ilu psim_ilu;

@@ -596,6 +596,8 @@ int psim_ilu_create(psim_ctx* ctx, uint64_t n, const uint64_t* offset, const uin
         if (ctx == nullptr || offset == nullptr || neighbour == nullptr || block_start == nullptr || out == nullptr)
             throw std::invalid_argument("null argument");
         if (blocks == 0 || blocks > n) throw std::invalid_argument("incomplete LU: the pattern does not match");
+        if (n >= (1ULL << 30)) throw std::invalid_argument("incomplete LU: the pattern does not match");
+        (void)psim::checked_graph_entries(n, offset);   // rows are read by these offsets
         std::vector<int32_t> bs(blocks + 1);
         for (uint64_t b = 0; b <= blocks; ++b) {
             if (block_start[b] > n) throw std::invalid_argument("incomplete LU: the pattern does not match");

@@ -58,6 +58,11 @@ def test_ilu_errors(ctx):
         f.factor(bad, off)
     with pytest.raises(P.PsimError, match="pattern does not match"):
         P.Ilu(ctx, om.offset, om.g_neighbour, np.array([0, 5], np.uint64))
+    # offsets that would read rows past the neighbour array are refused first
+    decreasing = np.array(om.offset, np.uint64)
+    decreasing[5] = decreasing[3]
+    with pytest.raises(P.PsimError, match="must not decrease"):
+        P.Ilu(ctx, decreasing, om.g_neighbour, blocks_of(om.n))
     f.factor(diag, off)
     assert np.all(np.isfinite(f.solve(np.ones(om.n))))
 
